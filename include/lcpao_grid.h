@@ -1,0 +1,214 @@
+/*
+ * lcpao_grid.h -- C ABI of the MI355X (gfx950) real-space grid library.
+ *
+ * The library computes, for an LCPAO basis on a real-space grid,
+ *   H_{Ai,Bj,Rn} = GridVol * sum_p phi_Ai(p) V(g(p)) phi_Bj(p)     (grid Hamiltonian)
+ *   rho(g)       = sum_pairs sum_{p: g(p)=g} phi_A(p)^T D_pair phi_B(p)  (density)
+ *   S_A          = sum_pairs(A,.) sum_p V(g(p)) dphi_A(p)^T (D_pair phi_B(p)) (Force3)
+ * over the atom-pair overlap lists of the host code.  Every entry point is a
+ * drop-in for one routine of LCPAODFT.jl; the citation on each declaration
+ * names the reference routine (file:line, paths relative to the reference
+ * repository root) whose argument meaning and output semantics it keeps.
+ *
+ * Conventions
+ *  - All functions return 0 on success and a non-zero LCPAO_E* code on
+ *    failure; lcpao_grid_last_error() then describes the failure.  No C++
+ *    exception crosses this boundary.
+ *  - Host pointers are read during the call only; nothing is retained.
+ *  - Pointers named d_* are device pointers (hipMalloc'd on the context's
+ *    device); they let a caller keep V, D, H and rho resident in HBM.
+ *  - A context is bound to one device and one stream and is not
+ *    thread-safe.  Calls are synchronous on return.
+ *  - Index arrays use the reference's values: grid indices, cell indices and
+ *    overlap indices are 0-based; atom indices use geom->atom_base (1 for the
+ *    Julia arrays MPI_atom / MPI_natn, 0 for C callers).
+ */
+#ifndef LCPAO_GRID_H
+#define LCPAO_GRID_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    LCPAO_OK = 0,
+    LCPAO_EINVAL = 1,    /* bad argument / inconsistent geometry */
+    LCPAO_ERANGE = 2,    /* size beyond a supported maximum (e.g. NO > 32) */
+    LCPAO_EDEVICE = 3,   /* HIP runtime error, no device, or out of device memory */
+    LCPAO_ESTATE = 4,    /* call order (e.g. contraction before tables are set) */
+    LCPAO_EMESH = 5      /* radial mesh index out of range in table generation */
+};
+
+enum {
+    LCPAO_FP64 = 0,      /* plain FP64, the parity form */
+    LCPAO_FP32 = 1,      /* orbital products in FP32, pair partials summed in FP64 */
+    LCPAO_BF16X3 = 2     /* 3-term split-BF16 MFMA, FP32 accumulation, FP64 pair sums */
+};
+
+typedef struct lcpao_ctx lcpao_ctx;
+
+/* The rank-local part of the reference's UCell (UCell.jl:1-46, 210-294).
+ * Arrays indexed by atom have natom entries; arrays indexed by pair have
+ * npairs entries, in the rank-local pair order of split_system_grid
+ * (UCell.jl:76-132). */
+typedef struct {
+    int32_t natom;
+    int32_t atom_base;               /* 0 or 1: base of pair_atom / pair_natn values */
+    int32_t ngrid[3];                /* Ngrid1, Ngrid2, Ngrid3 (g = n1*N2*N3 + n2*N3 + n3) */
+    int32_t cpycell;                 /* CpyCell: cell index <-> (i,j,k) of Generation_ATV! */
+    double gridvol;                  /* GridVol */
+    const int32_t *no;               /* Total_NumOrbs[natom] */
+    const int32_t *gridn;            /* GridN_Atom[natom] */
+    const int32_t *const *grid_list; /* GridListAtom[atom][GridN_Atom[atom]] */
+    const int32_t *const *cell_list; /* CellListAtom[atom][...]; may be NULL unless tables are built on device */
+    int64_t npairs;                  /* MPI_size */
+    const int32_t *pair_atom;        /* MPI_atom[npairs] */
+    const int32_t *pair_natn;        /* MPI_natn[npairs] */
+    const int32_t *pair_ncn;         /* MPI_ncn[npairs] */
+    const int32_t *numolg;           /* MPI_NumOLG[npairs] */
+    const int32_t *const *olg1;      /* MPI_GListTAtoms1[pair][NumOLG] */
+    const int32_t *const *olg2;      /* MPI_GListTAtoms2[pair][NumOLG] */
+    /* Needed only for on-device table generation (may be NULL otherwise). */
+    const double *latvecs;           /* 3x3 row-major, rows are lattice vectors */
+    const double *origin;            /* Grid_Origin[3] */
+    const double *gxyz;              /* natom x 3 Cartesian positions (bohr) */
+    const int32_t *atom2spe;         /* natom, 0-based species */
+} lcpao_geom;
+
+/* Radial PAO data of one species (Read_PAO.jl:1-18): the mesh RV and the
+ * radial functions RWF[l][p] for l <= maxl, p < num_basis[l]. */
+typedef struct {
+    int32_t maxl;                    /* Spe_MaxL_Basis (<= 3) */
+    int32_t num_basis[4];            /* Spe_Num_Basis */
+    int32_t nmesh;                   /* Spe_Num_Mesh_PAO */
+    const double *rv;                /* Spe_PAO_RV[nmesh] */
+    const double *const *rwf;        /* rwf[l*4 + p] -> Spe_PAO_RWF[l+1][p+1][nmesh] */
+} lcpao_pao;
+
+/* Builds a context for one geometry: copies the lists to the device and
+ * prepares the work decomposition.  device < 0 selects the current device.
+ * Replaces the per-call setup inside Set_Hamiltonian! / Set_Density_Grid! /
+ * Force3 (Hamiltonian.jl:235-277, Set_Density_Grid.jl:60-109,
+ * Force.jl:407-451), which read the same UCell fields. */
+int lcpao_grid_create(lcpao_ctx **out, const lcpao_geom *geom, int device);
+
+/* Uploads the orbital tables Orbs_Grid.data[atom] (NO_A x GridN_A,
+ * column-major, orbital fastest) of PackedOrbitalsGrid
+ * (Set_Orbitals_Grid.jl:1-42). */
+int lcpao_grid_set_tables(lcpao_ctx *ctx, const double *const *phi);
+
+/* Uploads the gradient tables dOrbs_Grid[1..3].data[atom]
+ * (Set_dOrbitals_Grid.jl:1-20), same layout as phi. */
+int lcpao_grid_set_dtables(lcpao_ctx *ctx, const double *const *dx, const double *const *dy,
+                           const double *const *dz);
+
+/* Computes the tables on the device: Set_Orbitals_Grid!
+ * (Set_Orbitals_Grid.jl:45-277) and, with with_derivatives != 0, also
+ * Set_dOrbitals_Grid! (Set_dOrbitals_Grid.jl:23-362).  Needs the lattice
+ * fields of lcpao_geom. */
+int lcpao_grid_build_tables(lcpao_ctx *ctx, int32_t nspecies, const lcpao_pao *paos, int32_t with_derivatives);
+
+/* Copies the device tables back (NO_A x GridN_A per atom; which: 0 = phi,
+ * 1..3 = d/dx, d/dy, d/dz). */
+int lcpao_grid_get_tables(lcpao_ctx *ctx, int32_t which, double *const *out);
+
+/* Grid Hamiltonian, Calc_MatrixElements_dVH_Vxc_{off,on,nc}!
+ * (Hamiltonian.jl:235-380) with _Calc_Hamiltonian_Blocked! (:384-429):
+ * nch = 1, 2 or 4 channels V[c] (NN doubles each); H[c] receives the
+ * rank-local MPI_Hks[c] (MPI_Hsize doubles), overwritten and already scaled
+ * by GridVol. */
+int lcpao_grid_hamiltonian(lcpao_ctx *ctx, int32_t nch, const double *const *V, double *const *H,
+                           int32_t precision);
+
+/* Density, Set_Density_Grid_{nonpol,pol,nc}! (Set_Density_Grid.jl:60-240):
+ * nspin = 1, 2 or 4 global flat density matrices DM[s]; the rank slice
+ * starts at dm_offset (MPHks[rank]).  rho[s] (NN doubles) is overwritten
+ * with the rank-local partial sum (the caller still reduces over ranks). */
+int lcpao_grid_density(lcpao_ctx *ctx, int32_t nspin, const double *const *DM, int64_t dm_offset,
+                       double *const *rho, int32_t precision);
+
+/* Force3_{nospin,spin,nc} (Force.jl:407-566): for each channel c, the
+ * per-pair sums of _Calc_Force3_Blocked! (Force.jl:3321-3392) with V[c] and
+ * DM[c], weighted by weight[c] and GridVol, are ADDED into force
+ * (natom x 3, column-major like the Julia Matrix VpotForce).  Reference
+ * weights: nospin {4}; spin {2,2}; nc {2,2,4,-4}.  Needs the gradient
+ * tables. */
+int lcpao_grid_force3(lcpao_ctx *ctx, int32_t nch, const double *const *V, const double *const *DM,
+                      int64_t dm_offset, const double *weight, double *force, int32_t precision);
+
+/* Device-resident variants of the three contractions: same semantics, all
+ * array pointers are device pointers; d_force stays natom x 3 column-major
+ * and is accumulated into.  These are the entry points timed by the
+ * benchmark (inputs already in HBM). */
+int lcpao_grid_hamiltonian_dev(lcpao_ctx *ctx, int32_t nch, const double *const *d_V, double *const *d_H,
+                               int32_t precision);
+int lcpao_grid_density_dev(lcpao_ctx *ctx, int32_t nspin, const double *const *d_DM, int64_t dm_offset,
+                           double *const *d_rho, int32_t precision);
+int lcpao_grid_force3_dev(lcpao_ctx *ctx, int32_t nch, const double *const *d_V, const double *const *d_DM,
+                          int64_t dm_offset, const double *weight, double *d_force, int32_t precision);
+
+/* Makes the context launch on the caller's HIP stream (a hipStream_t passed
+ * as void*; NULL restores the context's own stream).  No counterpart in the
+ * reference (which has no GPU): it lets a caller order the contractions
+ * with its own device work. */
+int lcpao_grid_set_stream(lcpao_ctx *ctx, void *hip_stream);
+
+/* Per-kernel timing with HIP events recorded on the launch stream around
+ * every kernel of the contractions (no counterpart in the reference; its
+ * TimerOutputs labels are Set_Hamiltonian / Set_Density_Grid / Force3).
+ * enable != 0 starts recording (and clears the counters); kernel_stats
+ * returns the launch count and total milliseconds of one kernel id:
+ * 0 H units, 1 H reduce, 2 density fold, 3 density units, 4 density gather,
+ * 5 Force3 units, 6 Force3 reduce. */
+int lcpao_grid_profile(lcpao_ctx *ctx, int32_t enable);
+int lcpao_grid_kernel_stats(const lcpao_ctx *ctx, int32_t kernel_id, int64_t *launches, double *total_ms);
+
+/* Sizes of the context (rank-local H size, grid size, sums of the lists). */
+int lcpao_grid_sizes(const lcpao_ctx *ctx, int64_t *my_hsize, int64_t *nn, int64_t *total_olg,
+                     int64_t *total_gridn);
+
+/* Message of the last failure on this context (or of the last failed
+ * lcpao_grid_create when ctx is NULL). */
+const char *lcpao_grid_last_error(const lcpao_ctx *ctx);
+
+/* Frees the context and its device memory. */
+void lcpao_grid_destroy(lcpao_ctx *ctx);
+
+/* ------------------------------------------------------------------ */
+/* Host-side UCell list construction (UCell.jl:210-294): the neighbour  */
+/* search Get_FNAN/Trn_System (:520-672), Calc_AtomsGrid (:297-411),    */
+/* split_system_grid (:76-132) and Calc_AtomOverlap_Grid (:425-516).    */
+/* ------------------------------------------------------------------ */
+typedef struct lcpao_ucell lcpao_ucell;
+
+/* latvecs: 3x3 row-major (rows are lattice vectors); gxyz natom x 3;
+ * cut[natom] = Atoms_Cut1; no[natom] = Total_NumOrbs (may be NULL);
+ * nprocs/rank select the rank-local pair range. */
+int lcpao_ucell_build(lcpao_ucell **out, const double *latvecs, int32_t natom, const double *gxyz,
+                      const double *cut, const int32_t *no, const int32_t *ngrid, const double *origin,
+                      int32_t nprocs, int32_t rank, int32_t nthreads);
+
+/* Scalars of the built UCell. */
+int lcpao_ucell_scalars(const lcpao_ucell *u, int32_t *cpycell, int64_t *nloop, int64_t *npairs,
+                        int64_t *total_hsize, int64_t *my_hsize, int64_t *my_hoff, double *gridvol,
+                        int64_t *total_gridn, int64_t *total_olg);
+
+/* Copies the lists into caller arrays (sizes from lcpao_ucell_scalars; any
+ * pointer may be NULL to skip that list).  Neighbour lists are CSR by atom
+ * (nb_off[natom+1]); grid lists CSR by atom (gl_off[natom+1]); pair arrays
+ * have npairs entries with 0-based atoms; olg lists CSR by pair
+ * (olg_off[npairs+1]). */
+int lcpao_ucell_copy(const lcpao_ucell *u, int32_t *fnan, int64_t *nb_off, int32_t *natn, int32_t *ncn,
+                     int32_t *gridn, int64_t *gl_off, int32_t *glist, int32_t *clist, int32_t *pair_atom,
+                     int32_t *pair_natn, int32_t *pair_ncn, int64_t *pair_hoff, int32_t *numolg,
+                     int64_t *olg_off, int32_t *olg1, int32_t *olg2);
+
+const char *lcpao_ucell_last_error(void);
+void lcpao_ucell_destroy(lcpao_ucell *u);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LCPAO_GRID_H */

@@ -1,0 +1,1442 @@
+// lcpao_grid.hip -- MI355X (gfx950) kernels and C ABI for the LCPAO
+// real-space grid contractions: grid Hamiltonian, density and Force3.
+//
+// Work decomposition (all on the reference's atom-pair overlap lists):
+//  * H  : pair-major.  Each "canonical" pair P (the one of P=(A,B,Rn) and its
+//         transpose P'=(B,A,-Rn) with the lower index, when both are local)
+//         is cut into units of <= UNIT_PTS overlap points.  A unit gathers
+//         phi_A and V*phi_B in chunks of 32 points into LDS and accumulates
+//         the NO_A x NO_B block in 4x4 FP64 register tiles; units write
+//         partial blocks that a second kernel sums per pair in a fixed order
+//         (deterministic), scales by GridVol and writes H[P] and H[P']^T.
+//  * rho: atom-owned.  Work unit = (home atom A, chunk of RHO_CHUNK entries
+//         of A's grid list).  Each wave walks the (pair, sub-range) segments
+//         of A's canonical pairs inside the chunk; a lane owns one overlap
+//         point, the pair's folded density block D_P + D_P'^T is wave-uniform
+//         and read through the scalar path.  Per-wave LDS accumulators are
+//         summed in a fixed order; a gather kernel then sums the entries of
+//         every grid point (CSR by grid index) -- no atomics anywhere.
+//  * Force3: pair-major units over all local pairs, lane = overlap point,
+//         per-unit partial sums reduced per atom in a fixed order.
+// Units are launched in order of the grid index of their first point, so
+// the units in flight touch a few grid planes and the tables they gather
+// stay in the Infinity Cache.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "lcpao_grid.h"
+
+#define UNIT_PTS 2048     // overlap points per H / Force3 unit
+#define RHO_CHUNK 2048    // grid-list entries per density unit
+#define HK 32             // points per LDS chunk in the H kernel
+#define MAXNO 32          // largest orbital count per atom supported
+#define RHO_WAVES 4
+
+namespace {
+
+std::string g_create_err;
+
+struct HUnit { int32_t pair, lo, n, pad; int64_t part; };
+struct RhoUnit { int32_t atom, t0, nt, seg_beg, seg_end, pad; };
+struct Seg { int32_t pair, lo, hi, pad; };
+struct F3Unit { int32_t pair, lo, n, pad; };
+struct PairDev { int32_t a, b, noa, nob; int64_t hoff, olg; int64_t tab_a, tab_b; int32_t gl_a, partner; };
+
+struct SpeciesDev {
+    int32_t maxl, nmesh, no, pad;
+    int32_t num_basis[4];
+    const double *rv;
+    const double *rwf[4][16];
+};
+
+}  // namespace
+
+struct lcpao_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // geometry (host)
+    int natom = 0, cpy = 0, maxno = 0;
+    int ngrid[3] = {0, 0, 0};
+    int64_t nn = 0, total_gridn = 0, total_olg = 0, npair = 0, my_hsize = 0;
+    double gridvol = 0;
+    std::vector<int> no, gridn;
+    std::vector<int64_t> gl_off, tab_off;
+    std::vector<PairDev> pairs;
+    bool have_lattice = false;
+    double lat[9], origin[3];
+    std::vector<double> gxyz;
+    std::vector<int> atom2spe;
+    // device: lists and tables
+    int32_t *d_glist = nullptr, *d_clist = nullptr;
+    int64_t *d_gl_off = nullptr, *d_tab_off = nullptr;
+    int32_t *d_no = nullptr;
+    int32_t *d_olg1 = nullptr, *d_olg2 = nullptr;
+    PairDev *d_pairs = nullptr;
+    double *d_tab = nullptr, *d_dtab[3] = {nullptr, nullptr, nullptr};
+    bool tables = false, dtables = false;
+    // H
+    std::vector<HUnit> hunits;
+    HUnit *d_hunits = nullptr;
+    int32_t *d_horder = nullptr;
+    int32_t *d_hpair_ub = nullptr;  // canonical pair list + unit ranges
+    int32_t *d_hcanon = nullptr;
+    int64_t ncanon = 0, hpart_size = 0;
+    double *d_hpart = nullptr;
+    // rho
+    std::vector<RhoUnit> runits;
+    RhoUnit *d_runits = nullptr;
+    int32_t *d_rorder = nullptr;
+    Seg *d_segs = nullptr;
+    double *d_deff = nullptr, *d_rhoA = nullptr;
+    int64_t *d_ent_off = nullptr;
+    int64_t *d_ent = nullptr;
+    // Force3
+    std::vector<F3Unit> funits;
+    F3Unit *d_funits = nullptr;
+    int32_t *d_forder = nullptr;
+    int32_t *d_atom_ub = nullptr;
+    double *d_fpart = nullptr, *d_fsum = nullptr;
+    // staging for host-pointer entry points
+    double *d_stage = nullptr;
+    int64_t stage_size = 0;
+    int32_t *d_flag = nullptr;
+    // per-kernel event timing
+    bool own_stream = true;
+    bool profiling = false;
+    struct Pending { int id; hipEvent_t beg, end; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    int64_t prof_launches[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double prof_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+namespace {
+enum { KID_HAM = 0, KID_HRED = 1, KID_DEFF = 2, KID_RHO = 3, KID_RGATHER = 4, KID_F3 = 5, KID_F3RED = 6 };
+
+hipEvent_t take_event(lcpao_ctx *ctx) {
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+// Brackets one kernel launch with events when profiling is on.
+struct KTimer {
+    lcpao_ctx *ctx;
+    int id;
+    hipEvent_t beg = nullptr;
+    KTimer(lcpao_ctx *c, int i) : ctx(c), id(i) {
+        if (ctx->profiling) {
+            beg = take_event(ctx);
+            (void)hipEventRecord(beg, ctx->stream);
+        }
+    }
+    ~KTimer() {
+        if (ctx->profiling && beg) {
+            hipEvent_t end = take_event(ctx);
+            (void)hipEventRecord(end, ctx->stream);
+            ctx->pending.push_back({id, beg, end});
+        }
+    }
+};
+
+// After the stream has been synchronised: fold pending events into the counters.
+void collect_timings(lcpao_ctx *ctx) {
+    for (auto &p : ctx->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.beg, p.end) == hipSuccess) {
+            ctx->prof_ms[p.id] += ms;
+            ctx->prof_launches[p.id]++;
+        }
+        ctx->event_pool.push_back(p.beg);
+        ctx->event_pool.push_back(p.end);
+    }
+    ctx->pending.clear();
+}
+}  // namespace
+
+#define HIP_TRY(ctx, call)                                                                     \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                   \
+            return LCPAO_EDEVICE;                                                              \
+        }                                                                                      \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// Hamiltonian unit kernel, FP64.  One wave (64 lanes) per unit.  Lane tiling:
+// the NO_A x NO_B block padded to multiples of 4 is cut into ntile 4x4 tiles;
+// 64/ntile "slices" of lanes split the points of each LDS chunk.
+__global__ __launch_bounds__(64) void k_ham_f64(const HUnit *__restrict__ units, const int32_t *__restrict__ order,
+                                                int32_t nunits, const PairDev *__restrict__ pairs,
+                                                const double *__restrict__ tab, const int32_t *__restrict__ olg1,
+                                                const int32_t *__restrict__ olg2, const int32_t *__restrict__ glist,
+                                                const double *__restrict__ V, double *__restrict__ part) {
+    __shared__ double As[HK * MAXNO];
+    __shared__ double Bs[HK * MAXNO];
+    __shared__ int32_t s_t1[HK], s_t2[HK];
+    __shared__ double s_v[HK];
+    const int b = blockIdx.x;
+    if (b >= nunits) return;
+    const HUnit u = units[order[b]];
+    const PairDev P = pairs[u.pair];
+    const int lane = threadIdx.x;
+    const int noa = P.noa, nob = P.nob;
+    const int tia = (noa + 3) >> 2, tjb = (nob + 3) >> 2;
+    const int sa = tia * 4, sb = tjb * 4;
+    const int ntile = tia * tjb;
+    const int nslice = 64 / ntile;
+    const int slice = lane / ntile, tl = lane - slice * ntile;
+    const int ti = tl / tjb, tj = tl - ti * tjb;
+    const bool active = slice < nslice;
+    const double *tabA = tab + P.tab_a, *tabB = tab + P.tab_b;
+    const int32_t *L1 = olg1 + P.olg + u.lo, *L2 = olg2 + P.olg + u.lo;
+    const int32_t *glA = glist + P.gl_a;
+    double acc[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) acc[r][c] = 0.0;
+    for (int k0 = 0; k0 < u.n; k0 += HK) {
+        const int cnt = min(HK, u.n - k0);
+        if (lane < HK) {
+            if (lane < cnt) {
+                const int t1 = L1[k0 + lane];
+                s_t1[lane] = t1;
+                s_t2[lane] = L2[k0 + lane];
+                s_v[lane] = V[glA[t1]];
+            } else {
+                s_t1[lane] = -1;
+                s_t2[lane] = -1;
+                s_v[lane] = 0.0;
+            }
+        }
+        __syncthreads();
+        for (int f = lane; f < HK * sa; f += 64) {
+            const int k = f / sa, i = f - k * sa;
+            const int t1 = s_t1[k];
+            As[f] = (t1 >= 0 && i < noa) ? tabA[(int64_t)t1 * noa + i] : 0.0;
+        }
+        for (int f = lane; f < HK * sb; f += 64) {
+            const int k = f / sb, j = f - k * sb;
+            const int t2 = s_t2[k];
+            Bs[f] = (t2 >= 0 && j < nob) ? tabB[(int64_t)t2 * nob + j] * s_v[k] : 0.0;
+        }
+        __syncthreads();
+        if (active) {
+            for (int k = slice; k < HK; k += nslice) {
+                const double2 a01 = *reinterpret_cast<const double2 *>(&As[k * sa + 4 * ti]);
+                const double2 a23 = *reinterpret_cast<const double2 *>(&As[k * sa + 4 * ti + 2]);
+                const double2 b01 = *reinterpret_cast<const double2 *>(&Bs[k * sb + 4 * tj]);
+                const double2 b23 = *reinterpret_cast<const double2 *>(&Bs[k * sb + 4 * tj + 2]);
+                const double av[4] = {a01.x, a01.y, a23.x, a23.y};
+                const double bv[4] = {b01.x, b01.y, b23.x, b23.y};
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+#pragma unroll
+                    for (int c = 0; c < 4; c++) acc[r][c] = fma(av[r], bv[c], acc[r][c]);
+            }
+        }
+        __syncthreads();
+    }
+    // cross-slice reduction through LDS (As is free now: 64*16 doubles fit in HK*MAXNO)
+    double *red = As;
+    if (active) {
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) red[(slice * ntile + tl) * 16 + r * 4 + c] = acc[r][c];
+    }
+    __syncthreads();
+    double *out = part + u.part;
+    for (int e = lane; e < ntile * 16; e += 64) {
+        const int t = e >> 4, rc = e & 15;
+        const int r = rc >> 2, c = rc & 3;
+        const int i = (t / tjb) * 4 + r, j = (t % tjb) * 4 + c;
+        if (i < noa && j < nob) {
+            double s = 0.0;
+            for (int q = 0; q < nslice; q++) s += red[(q * ntile + t) * 16 + rc];
+            out[i * nob + j] = s;
+        }
+    }
+}
+
+// Sums the unit partials of every computed pair in unit order, scales by
+// GridVol and writes H[P] (and H[P']^T for a folded transpose partner).
+__global__ void k_ham_reduce(const int32_t *__restrict__ canon, const int32_t *__restrict__ ub, int64_t ncanon,
+                             const HUnit *__restrict__ units, const PairDev *__restrict__ pairs,
+                             const double *__restrict__ part, double gridvol, double *__restrict__ H) {
+    const int64_t c = blockIdx.x;
+    if (c >= ncanon) return;
+    const int p = canon[c];
+    const PairDev P = pairs[p];
+    const int ne = P.noa * P.nob;
+    const int u0 = ub[c], u1 = ub[c + 1];
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+        double s = 0.0;
+        for (int u = u0; u < u1; u++) s += part[units[u].part + e];
+        s *= gridvol;
+        H[P.hoff + e] = s;
+        if (P.partner >= 0 && P.partner != p) {
+            const PairDev Q = pairs[P.partner];
+            const int i = e / P.nob, j = e - i * P.nob;
+            H[Q.hoff + (int64_t)j * P.noa + i] = s;
+        }
+    }
+}
+
+// Folded density block of each computed pair: D_P + D_P'^T (or D_P alone).
+__global__ void k_deff(const int32_t *__restrict__ canon, int64_t ncanon, const PairDev *__restrict__ pairs,
+                       const double *__restrict__ D, double *__restrict__ Deff) {
+    const int64_t c = blockIdx.x;
+    if (c >= ncanon) return;
+    const int p = canon[c];
+    const PairDev P = pairs[p];
+    const int ne = P.noa * P.nob;
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+        double v = D[P.hoff + e];
+        if (P.partner >= 0 && P.partner != p) {
+            const PairDev Q = pairs[P.partner];
+            const int i = e / P.nob, j = e - i * P.nob;
+            v += D[Q.hoff + (int64_t)j * P.noa + i];
+        }
+        Deff[P.hoff + e] = v;
+    }
+}
+
+// Density per (atom, chunk) unit.  NB = compile-time bound on NO_B.
+template <int NB>
+__global__ __launch_bounds__(64 * RHO_WAVES) void k_rho_f64(
+    const RhoUnit *__restrict__ units, const int32_t *__restrict__ order, int32_t nunits,
+    const Seg *__restrict__ segs, const PairDev *__restrict__ pairs, const double *__restrict__ tab,
+    const int32_t *__restrict__ olg1, const int32_t *__restrict__ olg2, const double *__restrict__ Deff,
+    const int64_t *__restrict__ gl_off, double *__restrict__ rhoA) {
+    __shared__ double racc[RHO_WAVES][RHO_CHUNK];
+    const int b = blockIdx.x;
+    if (b >= nunits) return;
+    const RhoUnit u = units[order[b]];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int t = threadIdx.x; t < RHO_WAVES * RHO_CHUNK; t += blockDim.x) (&racc[0][0])[t] = 0.0;
+    __syncthreads();
+    for (int s = u.seg_beg + wave; s < u.seg_end; s += RHO_WAVES) {
+        const Seg sg = segs[s];
+        const PairDev P = pairs[__builtin_amdgcn_readfirstlane(sg.pair)];
+        const int noa = __builtin_amdgcn_readfirstlane(P.noa), nob = __builtin_amdgcn_readfirstlane(P.nob);
+        const double *D = Deff + P.hoff;
+        const double *tabA = tab + P.tab_a, *tabB = tab + P.tab_b;
+        const int32_t *L1 = olg1 + P.olg, *L2 = olg2 + P.olg;
+        for (int base = sg.lo; base < sg.hi; base += 64) {
+            const int idx = base + lane;
+            const bool ok = idx < sg.hi;
+            const int t1 = ok ? L1[idx] : -1;
+            const int t2 = ok ? L2[idx] : 0;
+            double pb[NB];
+            const double *colB = tabB + (int64_t)t2 * nob;
+#pragma unroll
+            for (int j = 0; j < NB; j++) pb[j] = (ok && j < nob) ? colB[j] : 0.0;
+            const double *colA = tabA + (int64_t)(ok ? t1 : 0) * noa;
+            double z = 0.0;
+            for (int i = 0; i < noa; i++) {
+                const double *Di = D + i * nob;
+                double y = 0.0;
+#pragma unroll
+                for (int j = 0; j < NB; j++)
+                    if (j < nob) y = fma(Di[j], pb[j], y);
+                z = fma(ok ? colA[i] : 0.0, y, z);
+            }
+            if (ok) racc[wave][t1 - u.t0] += z;
+        }
+    }
+    __syncthreads();
+    double *out = rhoA + gl_off[u.atom] + u.t0;
+    for (int t = threadIdx.x; t < u.nt; t += blockDim.x) {
+        double s = racc[0][t];
+#pragma unroll
+        for (int w = 1; w < RHO_WAVES; w++) s += racc[w][t];
+        out[t] = s;
+    }
+}
+
+// rho(g) = sum of the per-entry partials of the entries at g (CSR order).
+__global__ void k_rho_gather(const int64_t *__restrict__ ent_off, const int64_t *__restrict__ ent,
+                             const double *__restrict__ rhoA, int64_t nn, double *__restrict__ rho) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nn) return;
+    double s = 0.0;
+    for (int64_t e = ent_off[g]; e < ent_off[g + 1]; e++) s += rhoA[ent[e]];
+    rho[g] = s;
+}
+
+// Force3 units, FP64: lane = overlap point.
+template <int NB>
+__global__ __launch_bounds__(64) void k_f3_f64(const F3Unit *__restrict__ units, const int32_t *__restrict__ order,
+                                               int32_t nunits, const PairDev *__restrict__ pairs,
+                                               const double *__restrict__ tab, const double *__restrict__ dx,
+                                               const double *__restrict__ dy, const double *__restrict__ dz,
+                                               const int32_t *__restrict__ olg1, const int32_t *__restrict__ olg2,
+                                               const int32_t *__restrict__ glist, const double *__restrict__ V,
+                                               const double *__restrict__ D, double *__restrict__ fpart) {
+    const int b = blockIdx.x;
+    if (b >= nunits) return;
+    const int ui = order[b];
+    const F3Unit u = units[ui];
+    const PairDev P = pairs[__builtin_amdgcn_readfirstlane(u.pair)];
+    const int noa = __builtin_amdgcn_readfirstlane(P.noa), nob = __builtin_amdgcn_readfirstlane(P.nob);
+    const int lane = threadIdx.x;
+    const double *Dp = D + P.hoff;
+    const int32_t *L1 = olg1 + P.olg + u.lo, *L2 = olg2 + P.olg + u.lo;
+    const int32_t *glA = glist + P.gl_a;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int base = 0; base < u.n; base += 64) {
+        const int idx = base + lane;
+        const bool ok = idx < u.n;
+        const int t1 = ok ? L1[idx] : 0;
+        const int t2 = ok ? L2[idx] : 0;
+        double pb[NB];
+        const double *colB = tab + P.tab_b + (int64_t)t2 * nob;
+#pragma unroll
+        for (int j = 0; j < NB; j++) pb[j] = (ok && j < nob) ? colB[j] : 0.0;
+        const int64_t ca = P.tab_a + (int64_t)t1 * noa;
+        double lx = 0.0, ly = 0.0, lz = 0.0;
+        for (int i = 0; i < noa; i++) {
+            const double *Di = Dp + i * nob;
+            double y = 0.0;
+#pragma unroll
+            for (int j = 0; j < NB; j++)
+                if (j < nob) y = fma(Di[j], pb[j], y);
+            lx = fma(dx[ca + i], y, lx);
+            ly = fma(dy[ca + i], y, ly);
+            lz = fma(dz[ca + i], y, lz);
+        }
+        const double pot = ok ? V[glA[t1]] : 0.0;
+        sx = fma(pot, lx, sx);
+        sy = fma(pot, ly, sy);
+        sz = fma(pot, lz, sz);
+    }
+    // wave reduction in a fixed butterfly order
+    for (int off = 32; off >= 1; off >>= 1) {
+        sx += __shfl_xor(sx, off, 64);
+        sy += __shfl_xor(sy, off, 64);
+        sz += __shfl_xor(sz, off, 64);
+    }
+    if (lane == 0) {
+        fpart[3 * (int64_t)ui] = sx;
+        fpart[3 * (int64_t)ui + 1] = sy;
+        fpart[3 * (int64_t)ui + 2] = sz;
+    }
+}
+
+// Per-atom sums of Force3 unit partials (pairs are atom-major, units
+// consecutive per pair): fsum[a][xyz] += w * sum.
+__global__ void k_f3_reduce(const int32_t *__restrict__ atom_ub, int natom, const double *__restrict__ fpart,
+                            double w, double *__restrict__ fsum) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= natom) return;
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int u = atom_ub[a]; u < atom_ub[a + 1]; u++)
+        for (int x = 0; x < 3; x++) s[x] += fpart[3 * (int64_t)u + x];
+    for (int x = 0; x < 3; x++) fsum[3 * a + x] += w * s[x];
+}
+
+// force[a + natom*xyz] += fsum[a][xyz] * GridVol
+__global__ void k_f3_apply(int natom, const double *__restrict__ fsum, double gridvol, double *__restrict__ force) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= natom) return;
+    for (int x = 0; x < 3; x++) force[a + (int64_t)natom * x] += fsum[3 * a + x] * gridvol;
+}
+
+// ---------------------------------------------------------------------------
+// Table generation: Set_Orbitals_Grid! (Set_Orbitals_Grid.jl:45-277) and
+// Set_dOrbitals_Grid! (Set_dOrbitals_Grid.jl:23-362), one thread per grid
+// list entry.  Expression order follows the reference; compiled without FP
+// contraction so the tables match the host restatement to the last bits of
+// the math library.
+// ---------------------------------------------------------------------------
+struct TabGeom {
+    double glat[9], origin[3];
+    int32_t n2, n3, natom;
+};
+
+__device__ inline void xyz_to_spherical_d(double x, double y, double z, double &r, double &theta, double &phi) {
+    const double Min_r = 1e-14;
+    const double pi = 3.141592653589793;
+    const double dum = x * x + y * y;
+    r = sqrt(dum + z * z);
+    const double r1 = sqrt(dum);
+    if (Min_r <= r) {
+        const double sz = z > 0.0 ? 1.0 : (z < 0.0 ? -1.0 : z);
+        theta = acos(r < fabs(z) ? sz : z / r);
+        if (Min_r <= r1) {
+            const double sy = y > 0.0 ? 1.0 : (y < 0.0 ? -1.0 : y);
+            const double d2 = r1 < fabs(y) ? sy : y / r1;
+            phi = x >= 0.0 ? asin(d2) : pi - asin(d2);
+        } else {
+            phi = 0.0;
+        }
+    } else {
+        theta = 0.5 * pi;
+        phi = 0.0;
+    }
+}
+
+// Real spherical harmonics and their angular derivatives in the reference's
+// m order per l (Set_Orbitals_Grid.jl:243-267; Ylm_real.jl:1-162).
+__device__ inline void ylm_all(int l, double siQ, double coQ, double siP, double coP, double *Y, double *dQ,
+                               double *dP, bool deriv) {
+    const double pi = 3.141592653589793;
+    if (l == 0) {
+        Y[0] = 0.5 * sqrt(1.0 / pi);
+        if (deriv) { dQ[0] = 0.0; dP[0] = 0.0; }
+    } else if (l == 1) {
+        const double c = sqrt(3.0 / 4.0 / pi);
+        Y[0] = c * siQ * coP; Y[1] = c * siQ * siP; Y[2] = c * coQ;   // px, py, pz
+        if (deriv) {
+            dQ[0] = c * coQ * coP; dQ[1] = c * coQ * siP; dQ[2] = -c * siQ;
+            dP[0] = -c * siP; dP[1] = c * coP; dP[2] = 0.0;
+        }
+    } else if (l == 2) {
+        const double c15 = sqrt(15.0 / pi), c5 = sqrt(5.0 / pi);
+        // dz2, dx2-y2, dxy, dxz, dyz
+        Y[0] = 0.25 * c5 * (3 * (coQ * coQ) - 1);
+        Y[1] = 0.25 * c15 * (siQ * siQ) * (1 - 2 * (siP * siP));
+        Y[2] = 0.5 * c15 * (siQ * siQ) * siP * coP;
+        Y[3] = 0.5 * c15 * siQ * coQ * coP;
+        Y[4] = 0.5 * c15 * siQ * coQ * siP;
+        if (deriv) {
+            dQ[0] = -1.5 * c5 * siQ * coQ;
+            dQ[1] = 0.5 * c15 * siQ * coQ * (1 - 2 * (siP * siP));
+            dQ[2] = c15 * siQ * coQ * siP * coP;
+            dQ[3] = 0.5 * c15 * (1 - 2 * (siQ * siQ)) * coP;
+            dQ[4] = 0.5 * c15 * (1 - 2 * (siQ * siQ)) * siP;
+            dP[0] = 0.0;
+            dP[1] = -c15 * siQ * siP * coP;
+            dP[2] = 0.5 * c15 * siQ * (1 - 2 * (siP * siP));
+            dP[3] = -0.5 * c15 * coQ * siP;
+            dP[4] = 0.5 * c15 * coQ * coP;
+        }
+    } else {
+        const double c7 = sqrt(7.0 / pi), c21 = sqrt(21.0 / 2.0 / pi), c105 = sqrt(105.0 / pi), c35 = sqrt(35.0 / 2.0 / pi);
+        // z3, xz2, yz2, z(x2-y2), xyz, x(x2-3y2), y(3x2-y2)
+        Y[0] = 0.25 * c7 * (5 * (coQ * coQ * coQ) - 3 * coQ);
+        Y[1] = 0.25 * c21 * siQ * (5 * (coQ * coQ) - 1) * coP;
+        Y[2] = 0.25 * c21 * siQ * (5 * (coQ * coQ) - 1) * siP;
+        Y[3] = 0.25 * c105 * (siQ * siQ) * coQ * ((coP * coP) - (siP * siP));
+        Y[4] = 0.5 * c105 * (siQ * siQ) * coQ * siP * coP;
+        Y[5] = 0.25 * c35 * (siQ * siQ * siQ) * (4 * (coP * coP * coP) - 3 * coP);
+        Y[6] = 0.25 * c35 * (siQ * siQ * siQ) * (3 * siP - 4 * (siP * siP * siP));
+        if (deriv) {
+            dQ[0] = 0.25 * c7 * siQ * (-15 * (coQ * coQ) + 3);
+            dQ[1] = 0.25 * c21 * coP * coQ * (15 * (coQ * coQ) - 11);
+            dQ[2] = 0.25 * c21 * siP * coQ * (15 * (coQ * coQ) - 11);
+            dQ[3] = 0.25 * c105 * ((coP * coP) - (siP * siP)) * siQ * (2 * (coQ * coQ) - (siQ * siQ));
+            dQ[4] = 0.5 * c105 * coP * siP * siQ * (2 * (coQ * coQ) - (siQ * siQ));
+            dQ[5] = 0.75 * c35 * coP * coQ * (siQ * siQ) * (-3 + 4 * (coP * coP));
+            dQ[6] = 0.75 * c35 * (siQ * siQ) * coQ * siP * (3 - 4 * (siP * siP));
+            dP[0] = 0.0;
+            dP[1] = 0.25 * c21 * siP * (-5 * (coQ * coQ) + 1);
+            dP[2] = 0.25 * c21 * coP * (5 * (coQ * coQ) - 1);
+            dP[3] = -c105 * coP * coQ * siP * siQ;
+            dP[4] = 0.5 * c105 * coQ * siQ * ((coP * coP) - (siP * siP));
+            dP[5] = 0.75 * c35 * siP * (siQ * siQ) * (1 - 4 * (coP * coP));
+            dP[6] = 0.75 * c35 * coP * (siQ * siQ) * (1 - 4 * (siP * siP));
+        }
+    }
+}
+
+// The Ylm constants are formed as products with sqrt() of the literal
+// quotients, e.g. 0.5*sqrt(15/pi)*siQ*coQ*coP = ((0.5*c15)*siQ)*coQ*coP,
+// which is how the expressions above associate.
+__global__ void k_tables(int64_t nent, const int64_t *__restrict__ gl_off, const int32_t *__restrict__ glist,
+                         const int32_t *__restrict__ clist, const int64_t *__restrict__ tab_off,
+                         const double *__restrict__ gxyz, const int32_t *__restrict__ a2s,
+                         const double *__restrict__ atv, TabGeom tg, const SpeciesDev *__restrict__ spe,
+                         int deriv, double *__restrict__ tab, double *__restrict__ tx, double *__restrict__ ty,
+                         double *__restrict__ tz, int32_t *__restrict__ flag) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nent) return;
+    // owning atom by binary search over gl_off
+    int lo = 0, hi = tg.natom;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (gl_off[mid] <= e) lo = mid; else hi = mid;
+    }
+    const int a = lo;
+    const int t = (int)(e - gl_off[a]);
+    const SpeciesDev &S = spe[a2s[a]];
+    const int GNc = glist[e], GRc = clist[e];
+    const int n1 = GNc / (tg.n2 * tg.n3);
+    const int n2 = (GNc - n1 * tg.n2 * tg.n3) / tg.n3;
+    const int n3 = GNc - n1 * tg.n2 * tg.n3 - n2 * tg.n3;
+    const double Cx = n1 * tg.glat[0] + n2 * tg.glat[3] + n3 * tg.glat[6] + tg.origin[0];
+    const double Cy = n1 * tg.glat[1] + n2 * tg.glat[4] + n3 * tg.glat[7] + tg.origin[1];
+    const double Cz = n1 * tg.glat[2] + n2 * tg.glat[5] + n3 * tg.glat[8] + tg.origin[2];
+    double x = Cx + atv[3 * GRc] - gxyz[3 * a];
+    double y = Cy + atv[3 * GRc + 1] - gxyz[3 * a + 1];
+    double z = Cz + atv[3 * GRc + 2] - gxyz[3 * a + 2];
+    double R, theta, phi;
+    xyz_to_spherical_d(x, y, z, R, theta, phi);
+    const double Rmin = 1.0e-13;
+    if (deriv && R < Rmin) {  // Set_dOrbitals_Grid.jl:107-112
+        x = x + Rmin; y = y + Rmin; z = z + Rmin;
+        xyz_to_spherical_d(x, y, z, R, theta, phi);
+    }
+    const double *RV = S.rv;
+    const int nm = S.nmesh;
+    double RF[4][16], dRF[4][16];
+    bool outside = false;
+    if (RV[nm - 1] < R) {
+        outside = true;
+        for (int l = 0; l <= S.maxl; l++)
+            for (int p = 0; p < S.num_basis[l]; p++) { RF[l][p] = 0.0; dRF[l][p] = 0.0; }
+    } else if (R < RV[0]) {
+        // polynomial continuation inside the first mesh point (Set_Orbitals_Grid.jl:124-185)
+        const int m = 4;
+        const double rm = RV[m];
+        const double h1 = RV[m - 1] - RV[m - 2], h2 = RV[m] - RV[m - 1], h3 = RV[m + 1] - RV[m];
+        const double x1 = rm - RV[m - 1], x2 = rm - RV[m];
+        const double y1 = x1 / h2, y2 = x2 / h2, y12 = y1 * y1, y22 = y2 * y2;
+        double dum = h1 + h2;
+        const double dum1 = h1 / h2 / dum, dum2 = h2 / h1 / dum;
+        dum = h2 + h3;
+        const double dum3 = h2 / h3 / dum, dum4 = h3 / h2 / dum;
+        for (int l = 0; l <= S.maxl; l++)
+            for (int p = 0; p < S.num_basis[l]; p++) {
+                const double *W = S.rwf[l][p];
+                const double f1 = W[m - 2], f2 = W[m - 1], f3 = W[m], f4 = W[m + 1];
+                const double d = f3 - f2;
+                const double g1 = d * dum1 + (f2 - f1) * dum2;
+                const double g2 = (f4 - f3) * dum3 + d * dum4;
+                const double A = 3 * f2 + h2 * g1, B = 2 * f2 + h2 * g1;
+                const double Cc = 3 * f3 - h2 * g2, D = 2 * f3 - h2 * g2;
+                const double E = A + B * y2, F = Cc - D * y1;
+                const double f = y22 * E + y12 * F;
+                const double df = deriv ? (2 * y2 * E + y22 * B + 2.0 * y1 * F - y12 * D) / h2
+                                        : (2 * y2 / h2) * E + y22 * B / h2 + (2 * y1 / h2) * F - y12 * D / h2;
+                double ca, cb, cc, cd;
+                if (l == 0) { ca = 0.0; cb = 0.5 * df / rm; cc = 0.0; cd = f - cb * (rm * rm); }
+                else if (l == 1) { ca = (rm * df - f) / (2 * (rm * rm * rm)); cb = 0.0; cc = df - 3 * ca * (rm * rm); cd = 0.0; }
+                else { cb = (3 * f - rm * df) / (rm * rm); ca = (f - cb * (rm * rm)) / (rm * rm * rm); cc = 0.0; cd = 0.0; }
+                RF[l][p] = ca * (R * R * R) + cb * (R * R) + cc * R + cd;
+                dRF[l][p] = 3 * ca * (R * R) + 2 * cb * R + cc;
+            }
+    } else {
+        int mp_min = 1, mp_max = nm;  // 1-based, as in the reference's search
+        while (mp_max - mp_min != 1) {
+            const int mm = (mp_min + mp_max) / 2;
+            if (RV[mm - 1] < R) mp_min = mm; else mp_max = mm;
+        }
+        const int m = mp_max;
+        if (m < 3 || m + 1 > nm) {  // the reference would read past the mesh here
+            atomicOr(flag, 1);
+            return;
+        }
+        const double h1 = RV[m - 2] - RV[m - 3], h2 = RV[m - 1] - RV[m - 2], h3 = RV[m] - RV[m - 1];
+        const double x1 = R - RV[m - 2], x2 = R - RV[m - 1];
+        const double y1 = x1 / h2, y2 = x2 / h2, y12 = y1 * y1, y22 = y2 * y2;
+        double dum = h1 + h2;
+        const double dum1 = h1 / h2 / dum, dum2 = h2 / h1 / dum;
+        dum = h2 + h3;
+        const double dum3 = h2 / h3 / dum, dum4 = h3 / h2 / dum;
+        for (int l = 0; l <= S.maxl; l++)
+            for (int p = 0; p < S.num_basis[l]; p++) {
+                const double *W = S.rwf[l][p];
+                double f1 = W[m - 3], f4 = W[m];
+                const double f2 = W[m - 2], f3 = W[m - 1];
+                if (!deriv && m == nm - 1) f4 = f1;       // Set_Orbitals_Grid.jl:226-228
+                if (deriv && m == nm) f4 = f1;            // Set_dOrbitals_Grid.jl:230-232
+                const double d = f3 - f2;
+                const double g1 = d * dum1 + (f2 - f1) * dum2;
+                const double g2 = (f4 - f3) * dum3 + d * dum4;
+                if (!deriv) {
+                    RF[l][p] = (y22 * (3 * f2 + h2 * g1 + (2 * f2 + h2 * g1) * y2) +
+                                y12 * (3 * f3 - h2 * g2 - (2 * f3 - h2 * g2) * y1));
+                } else {
+                    const double A = 3 * f2 + h2 * g1, B = 2 * f2 + h2 * g1;
+                    const double Cc = 3 * f3 - h2 * g2, D = 2 * f3 - h2 * g2;
+                    const double E = A + B * y2, F = Cc - D * y1;
+                    RF[l][p] = y22 * E + y12 * F;
+                    dRF[l][p] = (2 * y2 * E + y22 * B + 2.0 * y1 * F - y12 * D) / h2;
+                }
+            }
+    }
+    const int NO = S.no;
+    const int64_t base = tab_off[a] + (int64_t)t * NO;
+    double siQ = 0.0, coQ = 0.0, siP = 0.0, coP = 0.0;
+    if (!outside) { siQ = sin(theta); coQ = cos(theta); siP = sin(phi); coP = cos(phi); }
+    double Y[7], dQ[7], dP[7];
+    int ist = 0;
+    if (!deriv) {
+        for (int l = 0; l <= S.maxl; l++) {
+            ylm_all(l, siQ, coQ, siP, coP, Y, dQ, dP, false);
+            for (int p = 0; p < S.num_basis[l]; p++)
+                for (int mi = 0; mi < 2 * l + 1; mi++) tab[base + ist++] = outside ? 0.0 : RF[l][p] * Y[mi];
+        }
+        return;
+    }
+    double dRx = 0, dRy = 0, dRz = 0, dQx = 0, dQy = 0, dQz = 0, dPx = 0, dPy = 0, dPz = 0;
+    if (!outside) {
+        dRx = siQ * coP; dRy = siQ * siP; dRz = coQ;
+        if (R > Rmin) {
+            dQx = coQ * coP / R; dQy = coQ * siP / R; dQz = -siQ / R;
+            dPx = -siP / R; dPy = coP / R; dPz = 0.0;
+        }
+    }
+    for (int l = 0; l <= S.maxl; l++) {
+        ylm_all(l, siQ, coQ, siP, coP, Y, dQ, dP, true);
+        for (int p = 0; p < S.num_basis[l]; p++)
+            for (int mi = 0; mi < 2 * l + 1; mi++) {
+                if (outside) { tx[base + ist] = 0.0; ty[base + ist] = 0.0; tz[base + ist] = 0.0; ist++; continue; }
+                const double dchidr = dRF[l][p] * Y[mi];
+                const double dchidtheta = RF[l][p] * dQ[mi];
+                const double dchidphi = RF[l][p] * dP[mi];
+                tx[base + ist] = -dRx * dchidr - dQx * dchidtheta - dPx * dchidphi;
+                ty[base + ist] = -dRy * dchidr - dQy * dchidtheta - dPy * dchidphi;
+                tz[base + ist] = -dRz * dchidr - dQz * dchidtheta - dPz * dchidphi;
+                ist++;
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+namespace {
+
+template <class T>
+int upload(lcpao_ctx *ctx, T **dst, const std::vector<T> &src) {
+    if (*dst) { (void)hipFree(*dst); *dst = nullptr; }
+    const size_t bytes = std::max<size_t>(1, src.size()) * sizeof(T);
+    HIP_TRY(ctx, hipMalloc(reinterpret_cast<void **>(dst), bytes));
+    if (!src.empty()) HIP_TRY(ctx, hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return LCPAO_OK;
+}
+
+int dev_alloc(lcpao_ctx *ctx, void **p, size_t bytes) {
+    if (*p) { (void)hipFree(*p); *p = nullptr; }
+    HIP_TRY(ctx, hipMalloc(p, std::max<size_t>(1, bytes)));
+    return LCPAO_OK;
+}
+
+// cell index -> (i,j,k) for the Generation_ATV! order (cell 0 = home, then
+// the (i,j,k) loop over [-C,C]^3 skipping (0,0,0))
+void cell_ijk(int cpy, int cell, int *ijk) {
+    if (cell == 0) { ijk[0] = ijk[1] = ijk[2] = 0; return; }
+    const int W = 2 * cpy + 1;
+    const int zero_lin = (cpy * W + cpy) * W + cpy;
+    const int lin = cell <= zero_lin ? cell - 1 : cell;
+    ijk[0] = lin / (W * W) - cpy;
+    ijk[1] = (lin / W) % W - cpy;
+    ijk[2] = lin % W - cpy;
+}
+
+int cell_index(int cpy, const int *ijk) {
+    if (ijk[0] == 0 && ijk[1] == 0 && ijk[2] == 0) return 0;
+    const int W = 2 * cpy + 1;
+    const int lin = ((ijk[0] + cpy) * W + (ijk[1] + cpy)) * W + (ijk[2] + cpy);
+    const int zero_lin = (cpy * W + cpy) * W + cpy;
+    return lin < zero_lin ? lin + 1 : lin;
+}
+
+int set_err(lcpao_ctx *ctx, int code, const std::string &m) {
+    ctx->err = m;
+    return code;
+}
+
+
+// Orders unit indices by the grid index of their first point.
+template <class GetKey>
+std::vector<int32_t> launch_order(size_t n, GetKey key) {
+    std::vector<int64_t> k(n);
+    for (size_t i = 0; i < n; i++) k[i] = key(i);
+    std::vector<int32_t> ord(n);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) { return k[x] < k[y]; });
+    return ord;
+}
+
+int build_work(lcpao_ctx *ctx, const lcpao_geom *g, const std::vector<int32_t> &o1, const std::vector<int32_t> &o2,
+               const std::vector<int32_t> &glist_h) {
+    const int64_t np = ctx->npair;
+    // transpose partner of each local pair: (B, A, -Rn)
+    std::vector<std::vector<std::pair<int64_t, int>>> by_atom(ctx->natom);
+    auto key = [](int b, int cell) { return (int64_t)b * 4000037LL + cell; };
+    for (int64_t p = 0; p < np; p++) by_atom[ctx->pairs[p].a].push_back({key(ctx->pairs[p].b, g->pair_ncn[p]), (int)p});
+    for (auto &v : by_atom) std::sort(v.begin(), v.end());
+    for (int64_t p = 0; p < np; p++) {
+        int ijk[3], nijk[3];
+        cell_ijk(ctx->cpy, g->pair_ncn[p], ijk);
+        for (int d = 0; d < 3; d++) nijk[d] = -ijk[d];
+        const int64_t k = key(ctx->pairs[p].a, cell_index(ctx->cpy, nijk));
+        const auto &v = by_atom[ctx->pairs[p].b];
+        auto it = std::lower_bound(v.begin(), v.end(), std::make_pair(k, -1));
+        ctx->pairs[p].partner = (it != v.end() && it->first == k) ? it->second : -1;
+    }
+    // computed pairs: partner absent or not smaller (the transpose is then filled)
+    std::vector<int32_t> canon;
+    for (int64_t p = 0; p < np; p++) {
+        const int q = ctx->pairs[p].partner;
+        if (q < 0 || q >= p) canon.push_back((int32_t)p);
+    }
+    ctx->ncanon = (int64_t)canon.size();
+    auto first_g = [&](int p, int lo) -> int64_t {
+        const PairDev &P = ctx->pairs[p];
+        if (g->numolg[p] == 0) return INT64_MAX;
+        return glist_h[P.gl_a + o1[P.olg + lo]];
+    };
+    // H units over computed pairs, contiguous per pair
+    ctx->hunits.clear();
+    std::vector<int32_t> hub(1, 0);
+    int64_t part = 0;
+    for (int p : canon) {
+        const PairDev &P = ctx->pairs[p];
+        const int n = g->numolg[p];
+        for (int lo = 0; lo < n; lo += UNIT_PTS) {
+            ctx->hunits.push_back({p, lo, std::min(UNIT_PTS, n - lo), 0, part});
+            part += (int64_t)P.noa * P.nob;
+        }
+        hub.push_back((int32_t)ctx->hunits.size());
+    }
+    ctx->hpart_size = part;
+    auto horder = launch_order(ctx->hunits.size(), [&](size_t i) { return first_g(ctx->hunits[i].pair, ctx->hunits[i].lo); });
+    // Force3 units over all pairs; pairs are grouped by home atom
+    ctx->funits.clear();
+    std::vector<int32_t> aub(ctx->natom + 1, 0);
+    for (int64_t p = 0; p < np; p++) {
+        if (p > 0 && ctx->pairs[p].a < ctx->pairs[p - 1].a) return set_err(ctx, LCPAO_EINVAL, "pairs must be grouped by home atom in ascending order");
+        const int n = g->numolg[p];
+        for (int lo = 0; lo < n; lo += UNIT_PTS) {
+            ctx->funits.push_back({(int32_t)p, lo, std::min(UNIT_PTS, n - lo), 0});
+            aub[ctx->pairs[p].a + 1]++;
+        }
+    }
+    for (int a = 0; a < ctx->natom; a++) aub[a + 1] += aub[a];
+    auto forder = launch_order(ctx->funits.size(), [&](size_t i) { return first_g(ctx->funits[i].pair, ctx->funits[i].lo); });
+    // density units: (atom, chunk) holding the segments of its computed pairs
+    ctx->runits.clear();
+    std::vector<Seg> segs;
+    std::vector<std::vector<int32_t>> apairs(ctx->natom);
+    for (int p : canon) apairs[ctx->pairs[p].a].push_back(p);
+    for (int a = 0; a < ctx->natom; a++) {
+        std::vector<int> cur(apairs[a].size(), 0);
+        for (int t0 = 0; t0 < ctx->gridn[a]; t0 += RHO_CHUNK) {
+            RhoUnit u;
+            u.atom = a; u.t0 = t0; u.nt = std::min(RHO_CHUNK, ctx->gridn[a] - t0);
+            u.seg_beg = (int32_t)segs.size(); u.pad = 0;
+            for (size_t k = 0; k < apairs[a].size(); k++) {
+                const int p = apairs[a][k];
+                const int32_t *L1 = o1.data() + ctx->pairs[p].olg;
+                const int n = g->numolg[p];
+                const int lo = cur[k];
+                int hi = lo;
+                while (hi < n && L1[hi] < t0 + u.nt) hi++;
+                if (hi > lo) segs.push_back({p, lo, hi, 0});
+                cur[k] = hi;
+            }
+            u.seg_end = (int32_t)segs.size();
+            ctx->runits.push_back(u);
+        }
+    }
+    auto rorder = launch_order(ctx->runits.size(), [&](size_t i) { return (int64_t)glist_h[ctx->gl_off[ctx->runits[i].atom] + ctx->runits[i].t0]; });
+    int rc;
+    if ((rc = upload(ctx, &ctx->d_hunits, ctx->hunits))) return rc;
+    if ((rc = upload(ctx, &ctx->d_horder, horder))) return rc;
+    if ((rc = upload(ctx, &ctx->d_hcanon, canon))) return rc;
+    if ((rc = upload(ctx, &ctx->d_hpair_ub, hub))) return rc;
+    if ((rc = upload(ctx, &ctx->d_funits, ctx->funits))) return rc;
+    if ((rc = upload(ctx, &ctx->d_forder, forder))) return rc;
+    if ((rc = upload(ctx, &ctx->d_atom_ub, aub))) return rc;
+    if ((rc = upload(ctx, &ctx->d_runits, ctx->runits))) return rc;
+    if ((rc = upload(ctx, &ctx->d_rorder, rorder))) return rc;
+    if ((rc = upload(ctx, &ctx->d_segs, segs))) return rc;
+    if ((rc = dev_alloc(ctx, (void **)&ctx->d_hpart, (size_t)std::max<int64_t>(1, part) * sizeof(double)))) return rc;
+    if ((rc = dev_alloc(ctx, (void **)&ctx->d_fpart, (size_t)std::max<size_t>(1, ctx->funits.size()) * 3 * sizeof(double)))) return rc;
+    if ((rc = dev_alloc(ctx, (void **)&ctx->d_fsum, (size_t)ctx->natom * 3 * sizeof(double)))) return rc;
+    if ((rc = dev_alloc(ctx, (void **)&ctx->d_deff, (size_t)std::max<int64_t>(1, ctx->my_hsize) * sizeof(double)))) return rc;
+    if ((rc = dev_alloc(ctx, (void **)&ctx->d_rhoA, (size_t)std::max<int64_t>(1, ctx->total_gridn) * sizeof(double)))) return rc;
+    // entries of each grid point (CSR by grid index, atoms ascending)
+    std::vector<int64_t> eoff(ctx->nn + 1, 0), ent(ctx->total_gridn);
+    for (int64_t e = 0; e < ctx->total_gridn; e++) eoff[glist_h[e] + 1]++;
+    for (int64_t q = 0; q < ctx->nn; q++) eoff[q + 1] += eoff[q];
+    {
+        std::vector<int64_t> fill(eoff.begin(), eoff.end() - 1);
+        for (int64_t e = 0; e < ctx->total_gridn; e++) ent[fill[glist_h[e]]++] = e;
+    }
+    if ((rc = upload(ctx, &ctx->d_ent_off, eoff))) return rc;
+    if ((rc = upload(ctx, &ctx->d_ent, ent))) return rc;
+    return LCPAO_OK;
+}
+
+int check_ctx(lcpao_ctx *ctx) {
+    if (!ctx) return LCPAO_EINVAL;
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, LCPAO_EDEVICE, "hipSetDevice failed");
+    return LCPAO_OK;
+}
+
+int stage(lcpao_ctx *ctx, int64_t ndoubles) {
+    if (ndoubles <= ctx->stage_size) return LCPAO_OK;
+    int rc = dev_alloc(ctx, (void **)&ctx->d_stage, (size_t)ndoubles * sizeof(double));
+    if (rc) return rc;
+    ctx->stage_size = ndoubles;
+    return LCPAO_OK;
+}
+
+int check_precision(lcpao_ctx *ctx, int precision) {
+    if (precision != LCPAO_FP64) return set_err(ctx, LCPAO_EINVAL, "precision form not available in this build");
+    return LCPAO_OK;
+}
+
+int nb_bound(int maxno) { return maxno <= 16 ? 16 : (maxno <= 20 ? 20 : 32); }
+
+int ham_dev(lcpao_ctx *ctx, int nch, const double *const *d_V, double *const *d_H, int precision) {
+    if (!ctx->tables) return set_err(ctx, LCPAO_ESTATE, "orbital tables are not set");
+    if (nch != 1 && nch != 2 && nch != 4) return set_err(ctx, LCPAO_EINVAL, "nch must be 1, 2 or 4");
+    int rc = check_precision(ctx, precision);
+    if (rc) return rc;
+    const int nu = (int)ctx->hunits.size();
+    for (int c = 0; c < nch; c++) {
+        if (!d_V[c] || !d_H[c]) return set_err(ctx, LCPAO_EINVAL, "null channel pointer");
+        if (nu > 0) {
+            KTimer kt(ctx, KID_HAM);
+            hipLaunchKernelGGL(k_ham_f64, dim3(nu), dim3(64), 0, ctx->stream, ctx->d_hunits, ctx->d_horder, nu,
+                               ctx->d_pairs, ctx->d_tab, ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c], ctx->d_hpart);
+        }
+        if (ctx->ncanon > 0) {
+            KTimer kt(ctx, KID_HRED);
+            hipLaunchKernelGGL(k_ham_reduce, dim3((unsigned)ctx->ncanon), dim3(128), 0, ctx->stream, ctx->d_hcanon,
+                               ctx->d_hpair_ub, ctx->ncanon, ctx->d_hunits, ctx->d_pairs, ctx->d_hpart, ctx->gridvol,
+                               d_H[c]);
+        }
+        HIP_TRY(ctx, hipGetLastError());
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    collect_timings(ctx);
+    return LCPAO_OK;
+}
+
+int rho_dev(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_offset, double *const *d_rho, int precision) {
+    if (!ctx->tables) return set_err(ctx, LCPAO_ESTATE, "orbital tables are not set");
+    if (nspin != 1 && nspin != 2 && nspin != 4) return set_err(ctx, LCPAO_EINVAL, "nspin must be 1, 2 or 4");
+    if (dm_offset < 0) return set_err(ctx, LCPAO_EINVAL, "negative dm_offset");
+    int rc = check_precision(ctx, precision);
+    if (rc) return rc;
+    const int nu = (int)ctx->runits.size();
+    const int nb = nb_bound(ctx->maxno);
+    for (int s = 0; s < nspin; s++) {
+        if (!d_DM[s] || !d_rho[s]) return set_err(ctx, LCPAO_EINVAL, "null spin pointer");
+        const double *D = d_DM[s] + dm_offset;
+        if (ctx->ncanon > 0) {
+            KTimer kt(ctx, KID_DEFF);
+            hipLaunchKernelGGL(k_deff, dim3((unsigned)ctx->ncanon), dim3(128), 0, ctx->stream, ctx->d_hcanon, ctx->ncanon,
+                               ctx->d_pairs, D, ctx->d_deff);
+        }
+        if (nu > 0) {
+            KTimer kt(ctx, KID_RHO);
+#define RHO_LAUNCH(NB)                                                                                         \
+    hipLaunchKernelGGL(k_rho_f64<NB>, dim3(nu), dim3(64 * RHO_WAVES), 0, ctx->stream, ctx->d_runits, ctx->d_rorder, \
+                       nu, ctx->d_segs, ctx->d_pairs, ctx->d_tab, ctx->d_olg1, ctx->d_olg2, ctx->d_deff,        \
+                       ctx->d_gl_off, ctx->d_rhoA)
+            if (nb == 16) RHO_LAUNCH(16);
+            else if (nb == 20) RHO_LAUNCH(20);
+            else RHO_LAUNCH(32);
+#undef RHO_LAUNCH
+        }
+        const unsigned gb = (unsigned)((ctx->nn + 255) / 256);
+        {
+            KTimer kt(ctx, KID_RGATHER);
+            hipLaunchKernelGGL(k_rho_gather, dim3(gb), dim3(256), 0, ctx->stream, ctx->d_ent_off, ctx->d_ent, ctx->d_rhoA,
+                               ctx->nn, d_rho[s]);
+        }
+        HIP_TRY(ctx, hipGetLastError());
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    collect_timings(ctx);
+    return LCPAO_OK;
+}
+
+int f3_dev(lcpao_ctx *ctx, int nch, const double *const *d_V, const double *const *d_DM, int64_t dm_offset,
+           const double *weight, double *d_force, int precision) {
+    if (!ctx->tables || !ctx->dtables) return set_err(ctx, LCPAO_ESTATE, "orbital and gradient tables must be set");
+    if (nch != 1 && nch != 2 && nch != 4) return set_err(ctx, LCPAO_EINVAL, "nch must be 1, 2 or 4");
+    if (!weight || !d_force) return set_err(ctx, LCPAO_EINVAL, "null weight or force pointer");
+    int rc = check_precision(ctx, precision);
+    if (rc) return rc;
+    const int nu = (int)ctx->funits.size();
+    const int nb = nb_bound(ctx->maxno);
+    HIP_TRY(ctx, hipMemsetAsync(ctx->d_fsum, 0, (size_t)ctx->natom * 3 * sizeof(double), ctx->stream));
+    for (int c = 0; c < nch; c++) {
+        if (!d_V[c] || !d_DM[c]) return set_err(ctx, LCPAO_EINVAL, "null channel pointer");
+        const double *D = d_DM[c] + dm_offset;
+        if (nu > 0) {
+            KTimer kt(ctx, KID_F3);
+#define F3_LAUNCH(NB)                                                                                             \
+    hipLaunchKernelGGL(k_f3_f64<NB>, dim3(nu), dim3(64), 0, ctx->stream, ctx->d_funits, ctx->d_forder, nu, ctx->d_pairs, \
+                       ctx->d_tab, ctx->d_dtab[0], ctx->d_dtab[1], ctx->d_dtab[2], ctx->d_olg1, ctx->d_olg2,          \
+                       ctx->d_glist, d_V[c], D, ctx->d_fpart)
+            if (nb == 16) F3_LAUNCH(16);
+            else if (nb == 20) F3_LAUNCH(20);
+            else F3_LAUNCH(32);
+#undef F3_LAUNCH
+        }
+        if (nu > 0) {
+            KTimer kt(ctx, KID_F3RED);
+            hipLaunchKernelGGL(k_f3_reduce, dim3((ctx->natom + 127) / 128), dim3(128), 0, ctx->stream, ctx->d_atom_ub,
+                               ctx->natom, ctx->d_fpart, weight[c], ctx->d_fsum);
+        }
+        HIP_TRY(ctx, hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_f3_apply, dim3((ctx->natom + 127) / 128), dim3(128), 0, ctx->stream, ctx->natom, ctx->d_fsum,
+                       ctx->gridvol, d_force);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    collect_timings(ctx);
+    return LCPAO_OK;
+}
+
+void free_all(lcpao_ctx *ctx) {
+    void *ptrs[] = {ctx->d_glist, ctx->d_clist, ctx->d_gl_off, ctx->d_tab_off, ctx->d_no, ctx->d_olg1, ctx->d_olg2,
+                    ctx->d_pairs, ctx->d_tab, ctx->d_dtab[0], ctx->d_dtab[1], ctx->d_dtab[2], ctx->d_hunits,
+                    ctx->d_horder, ctx->d_hpair_ub, ctx->d_hcanon, ctx->d_hpart, ctx->d_runits, ctx->d_rorder,
+                    ctx->d_segs, ctx->d_deff, ctx->d_rhoA, ctx->d_ent_off, ctx->d_ent, ctx->d_funits, ctx->d_forder,
+                    ctx->d_atom_ub, ctx->d_fpart, ctx->d_fsum, ctx->d_stage, ctx->d_flag};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (auto &p : ctx->pending) { (void)hipEventDestroy(p.beg); (void)hipEventDestroy(p.end); }
+    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    if (ctx->stream && ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int lcpao_grid_create(lcpao_ctx **out, const lcpao_geom *g, int device) {
+    g_create_err.clear();
+    if (!out || !g) { g_create_err = "null argument"; return LCPAO_EINVAL; }
+    *out = nullptr;
+    if (g->natom <= 0 || !g->no || !g->gridn || !g->grid_list || g->npairs < 0 ||
+        (g->npairs > 0 && (!g->pair_atom || !g->pair_natn || !g->pair_ncn || !g->numolg || !g->olg1 || !g->olg2)) ||
+        g->ngrid[0] <= 0 || g->ngrid[1] <= 0 || g->ngrid[2] <= 0 || g->cpycell < 0 ||
+        (g->atom_base != 0 && g->atom_base != 1)) {
+        g_create_err = "invalid geometry description";
+        return LCPAO_EINVAL;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { g_create_err = "no HIP device available"; return LCPAO_EDEVICE; }
+    auto *ctx = new lcpao_ctx();
+    if (device < 0) { if (hipGetDevice(&device) != hipSuccess) device = 0; }
+    if (device >= ndev) { g_create_err = "device index out of range"; delete ctx; return LCPAO_EDEVICE; }
+    ctx->device = device;
+    auto fail = [&](int code, const std::string &m) {
+        g_create_err = m.empty() ? ctx->err : m;
+        free_all(ctx);
+        delete ctx;
+        return code;
+    };
+    if (hipSetDevice(device) != hipSuccess) return fail(LCPAO_EDEVICE, "hipSetDevice failed");
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return fail(LCPAO_EDEVICE, "stream creation failed");
+    ctx->natom = g->natom;
+    ctx->cpy = g->cpycell;
+    for (int d = 0; d < 3; d++) ctx->ngrid[d] = g->ngrid[d];
+    ctx->nn = (int64_t)g->ngrid[0] * g->ngrid[1] * g->ngrid[2];
+    if (ctx->nn > INT32_MAX) return fail(LCPAO_ERANGE, "grid larger than 2^31 points");
+    ctx->gridvol = g->gridvol;
+    ctx->no.assign(g->no, g->no + g->natom);
+    ctx->gridn.assign(g->gridn, g->gridn + g->natom);
+    ctx->gl_off.assign(g->natom + 1, 0);
+    ctx->tab_off.assign(g->natom + 1, 0);
+    for (int a = 0; a < g->natom; a++) {
+        if (ctx->no[a] < 0 || ctx->no[a] > MAXNO) return fail(LCPAO_ERANGE, "orbital count per atom must be in [0, 32]");
+        if (ctx->gridn[a] < 0 || (ctx->gridn[a] > 0 && !g->grid_list[a])) return fail(LCPAO_EINVAL, "bad grid list");
+        ctx->maxno = std::max(ctx->maxno, ctx->no[a]);
+        ctx->gl_off[a + 1] = ctx->gl_off[a] + ctx->gridn[a];
+        ctx->tab_off[a + 1] = ctx->tab_off[a] + (int64_t)ctx->gridn[a] * ctx->no[a];
+    }
+    ctx->total_gridn = ctx->gl_off[g->natom];
+    std::vector<int32_t> glist(ctx->total_gridn), clist;
+    for (int a = 0; a < g->natom; a++)
+        for (int t = 0; t < ctx->gridn[a]; t++) {
+            const int32_t v = g->grid_list[a][t];
+            if (v < 0 || v >= ctx->nn) return fail(LCPAO_EINVAL, "grid index out of range");
+            glist[ctx->gl_off[a] + t] = v;
+        }
+    if (g->cell_list) {
+        const int ncell = (2 * g->cpycell + 1) * (2 * g->cpycell + 1) * (2 * g->cpycell + 1);
+        clist.resize(ctx->total_gridn);
+        for (int a = 0; a < g->natom; a++)
+            for (int t = 0; t < ctx->gridn[a]; t++) {
+                const int32_t v = g->cell_list[a][t];
+                if (v < 0 || v >= ncell) return fail(LCPAO_EINVAL, "cell index out of range");
+                clist[ctx->gl_off[a] + t] = v;
+            }
+    }
+    // pairs
+    ctx->npair = g->npairs;
+    ctx->pairs.resize(g->npairs);
+    int64_t hoff = 0, olg = 0;
+    const int ncell = (2 * g->cpycell + 1) * (2 * g->cpycell + 1) * (2 * g->cpycell + 1);
+    for (int64_t p = 0; p < g->npairs; p++) {
+        const int a = g->pair_atom[p] - g->atom_base, b = g->pair_natn[p] - g->atom_base;
+        if (a < 0 || a >= g->natom || b < 0 || b >= g->natom) return fail(LCPAO_EINVAL, "pair atom index out of range");
+        if (g->pair_ncn[p] < 0 || g->pair_ncn[p] >= ncell) return fail(LCPAO_EINVAL, "pair cell index out of range");
+        if (g->numolg[p] < 0 || (g->numolg[p] > 0 && (!g->olg1[p] || !g->olg2[p]))) return fail(LCPAO_EINVAL, "bad overlap list");
+        PairDev &P = ctx->pairs[p];
+        P.a = a; P.b = b; P.noa = ctx->no[a]; P.nob = ctx->no[b];
+        P.hoff = hoff; P.olg = olg; P.tab_a = ctx->tab_off[a]; P.tab_b = ctx->tab_off[b];
+        P.gl_a = (int32_t)ctx->gl_off[a]; P.partner = -1;
+        if (ctx->gl_off[a] > INT32_MAX) return fail(LCPAO_ERANGE, "grid lists longer than 2^31 entries");
+        hoff += (int64_t)P.noa * P.nob;
+        olg += g->numolg[p];
+    }
+    ctx->my_hsize = hoff;
+    ctx->total_olg = olg;
+    // overlap lists, each pair's list ordered by the home-atom index t1
+    std::vector<int32_t> o1(olg), o2(olg);
+    for (int64_t p = 0; p < g->npairs; p++) {
+        const int n = g->numolg[p];
+        const int32_t *L1 = g->olg1[p], *L2 = g->olg2[p];
+        const PairDev &P = ctx->pairs[p];
+        std::vector<int32_t> idx(n);
+        std::iota(idx.begin(), idx.end(), 0);
+        bool sorted = true;
+        for (int k = 0; k < n; k++) {
+            if (L1[k] < 0 || L1[k] >= ctx->gridn[P.a] || L2[k] < 0 || L2[k] >= ctx->gridn[P.b])
+                return fail(LCPAO_EINVAL, "overlap index out of range");
+            if (k > 0 && L1[k] < L1[k - 1]) sorted = false;
+        }
+        if (!sorted) std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return L1[x] < L1[y]; });
+        for (int k = 0; k < n; k++) { o1[P.olg + k] = L1[idx[k]]; o2[P.olg + k] = L2[idx[k]]; }
+    }
+    int rc;
+    if ((rc = upload(ctx, &ctx->d_glist, glist))) return fail(rc, "");
+    if (!clist.empty() && (rc = upload(ctx, &ctx->d_clist, clist))) return fail(rc, "");
+    if ((rc = upload(ctx, &ctx->d_gl_off, ctx->gl_off))) return fail(rc, "");
+    if ((rc = upload(ctx, &ctx->d_tab_off, ctx->tab_off))) return fail(rc, "");
+    if ((rc = upload(ctx, &ctx->d_no, ctx->no))) return fail(rc, "");
+    if ((rc = upload(ctx, &ctx->d_olg1, o1))) return fail(rc, "");
+    if ((rc = upload(ctx, &ctx->d_olg2, o2))) return fail(rc, "");
+    if ((rc = build_work(ctx, g, o1, o2, glist))) return fail(rc, "");
+    if ((rc = upload(ctx, &ctx->d_pairs, ctx->pairs))) return fail(rc, "");
+    if ((rc = dev_alloc(ctx, (void **)&ctx->d_flag, sizeof(int32_t)))) return fail(rc, "");
+    if (g->latvecs && g->origin && g->gxyz && g->atom2spe && g->cell_list) {
+        ctx->have_lattice = true;
+        std::memcpy(ctx->lat, g->latvecs, sizeof ctx->lat);
+        std::memcpy(ctx->origin, g->origin, sizeof ctx->origin);
+        ctx->gxyz.assign(g->gxyz, g->gxyz + 3 * (size_t)g->natom);
+        ctx->atom2spe.assign(g->atom2spe, g->atom2spe + g->natom);
+    }
+    *out = ctx;
+    return LCPAO_OK;
+}
+
+void lcpao_grid_destroy(lcpao_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    free_all(ctx);
+    delete ctx;
+}
+
+int lcpao_grid_set_stream(lcpao_ctx *ctx, void *hip_stream) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->own_stream) {
+        if (hip_stream) { (void)hipStreamDestroy(ctx->stream); ctx->stream = (hipStream_t)hip_stream; ctx->own_stream = false; }
+    } else {
+        if (hip_stream) ctx->stream = (hipStream_t)hip_stream;
+        else { HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)); ctx->own_stream = true; }
+    }
+    return LCPAO_OK;
+}
+
+int lcpao_grid_profile(lcpao_ctx *ctx, int32_t enable) {
+    if (!ctx) return LCPAO_EINVAL;
+    ctx->profiling = enable != 0;
+    for (int k = 0; k < 8; k++) { ctx->prof_launches[k] = 0; ctx->prof_ms[k] = 0.0; }
+    return LCPAO_OK;
+}
+
+int lcpao_grid_kernel_stats(const lcpao_ctx *ctx, int32_t kernel_id, int64_t *launches, double *total_ms) {
+    if (!ctx || kernel_id < 0 || kernel_id >= 8) return LCPAO_EINVAL;
+    if (launches) *launches = ctx->prof_launches[kernel_id];
+    if (total_ms) *total_ms = ctx->prof_ms[kernel_id];
+    return LCPAO_OK;
+}
+
+int lcpao_grid_sizes(const lcpao_ctx *ctx, int64_t *my_hsize, int64_t *nn, int64_t *total_olg, int64_t *total_gridn) {
+    if (!ctx) return LCPAO_EINVAL;
+    if (my_hsize) *my_hsize = ctx->my_hsize;
+    if (nn) *nn = ctx->nn;
+    if (total_olg) *total_olg = ctx->total_olg;
+    if (total_gridn) *total_gridn = ctx->total_gridn;
+    return LCPAO_OK;
+}
+
+static int upload_tables(lcpao_ctx *ctx, double **dst, const double *const *src) {
+    if (!src) return set_err(ctx, LCPAO_EINVAL, "null table array");
+    const int64_t total = ctx->tab_off[ctx->natom];
+    if (!*dst) {
+        int rc = dev_alloc(ctx, (void **)dst, (size_t)std::max<int64_t>(1, total) * sizeof(double));
+        if (rc) return rc;
+    }
+    for (int a = 0; a < ctx->natom; a++) {
+        const int64_t n = ctx->tab_off[a + 1] - ctx->tab_off[a];
+        if (n == 0) continue;
+        if (!src[a]) return set_err(ctx, LCPAO_EINVAL, "null table of an atom");
+        HIP_TRY(ctx, hipMemcpyAsync(*dst + ctx->tab_off[a], src[a], (size_t)n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return LCPAO_OK;
+}
+
+int lcpao_grid_set_tables(lcpao_ctx *ctx, const double *const *phi) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if ((rc = upload_tables(ctx, &ctx->d_tab, phi))) return rc;
+    ctx->tables = true;
+    return LCPAO_OK;
+}
+
+int lcpao_grid_set_dtables(lcpao_ctx *ctx, const double *const *dx, const double *const *dy, const double *const *dz) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if ((rc = upload_tables(ctx, &ctx->d_dtab[0], dx))) return rc;
+    if ((rc = upload_tables(ctx, &ctx->d_dtab[1], dy))) return rc;
+    if ((rc = upload_tables(ctx, &ctx->d_dtab[2], dz))) return rc;
+    ctx->dtables = true;
+    return LCPAO_OK;
+}
+
+int lcpao_grid_build_tables(lcpao_ctx *ctx, int32_t nspecies, const lcpao_pao *paos, int32_t with_derivatives) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!ctx->have_lattice || !ctx->d_clist) return set_err(ctx, LCPAO_ESTATE, "geometry lacks lattice, positions, species or cell lists");
+    if (nspecies <= 0 || !paos) return set_err(ctx, LCPAO_EINVAL, "no species");
+    for (int a = 0; a < ctx->natom; a++)
+        if (ctx->atom2spe[a] < 0 || ctx->atom2spe[a] >= nspecies) return set_err(ctx, LCPAO_EINVAL, "atom2spe out of range");
+    // radial data to the device
+    std::vector<SpeciesDev> sd(nspecies);
+    std::vector<double *> owned;
+    auto cleanup = [&]() { for (double *p : owned) (void)hipFree(p); };
+    for (int s = 0; s < nspecies; s++) {
+        const lcpao_pao &P = paos[s];
+        if (P.maxl < 0 || P.maxl > 3 || P.nmesh < 6 || !P.rv || !P.rwf) { cleanup(); return set_err(ctx, LCPAO_EINVAL, "bad PAO description"); }
+        SpeciesDev &S = sd[s];
+        std::memset(&S, 0, sizeof S);
+        S.maxl = P.maxl; S.nmesh = P.nmesh;
+        int no = 0;
+        for (int l = 0; l <= P.maxl; l++) {
+            if (P.num_basis[l] < 0 || P.num_basis[l] > 16) { cleanup(); return set_err(ctx, LCPAO_ERANGE, "multiplicity above 16"); }
+            S.num_basis[l] = P.num_basis[l];
+            no += P.num_basis[l] * (2 * l + 1);
+        }
+        S.no = no;
+        double *d = nullptr;
+        if (hipMalloc(&d, sizeof(double) * P.nmesh) != hipSuccess) { cleanup(); return set_err(ctx, LCPAO_EDEVICE, "hipMalloc failed"); }
+        owned.push_back(d);
+        (void)hipMemcpy(d, P.rv, sizeof(double) * P.nmesh, hipMemcpyHostToDevice);
+        S.rv = d;
+        for (int l = 0; l <= P.maxl; l++)
+            for (int p = 0; p < P.num_basis[l]; p++) {
+                const double *src = P.rwf[l * 4 + p];
+                if (!src) { cleanup(); return set_err(ctx, LCPAO_EINVAL, "null radial function"); }
+                double *w = nullptr;
+                if (hipMalloc(&w, sizeof(double) * P.nmesh) != hipSuccess) { cleanup(); return set_err(ctx, LCPAO_EDEVICE, "hipMalloc failed"); }
+                owned.push_back(w);
+                (void)hipMemcpy(w, src, sizeof(double) * P.nmesh, hipMemcpyHostToDevice);
+                S.rwf[l][p] = w;
+            }
+    }
+    for (int a = 0; a < ctx->natom; a++)
+        if (sd[ctx->atom2spe[a]].no != ctx->no[a]) { cleanup(); return set_err(ctx, LCPAO_EINVAL, "species orbital count differs from Total_NumOrbs"); }
+    SpeciesDev *d_sd = nullptr;
+    double *d_gxyz = nullptr, *d_atv = nullptr;
+    int32_t *d_a2s = nullptr;
+    const int C = ctx->cpy, W = 2 * C + 1, ncell = W * W * W;
+    std::vector<double> atv((size_t)ncell * 3, 0.0);
+    for (int c = 1; c < ncell; c++) {
+        int ijk[3];
+        cell_ijk(C, c, ijk);
+        const double di = ijk[0], dj = ijk[1], dk = ijk[2];
+        for (int x = 0; x < 3; x++) atv[3 * (size_t)c + x] = di * ctx->lat[x] + dj * ctx->lat[3 + x] + dk * ctx->lat[6 + x];
+    }
+    bool ok = hipMalloc(&d_sd, sizeof(SpeciesDev) * nspecies) == hipSuccess &&
+              hipMalloc(&d_gxyz, sizeof(double) * 3 * ctx->natom) == hipSuccess &&
+              hipMalloc(&d_atv, sizeof(double) * atv.size()) == hipSuccess &&
+              hipMalloc(&d_a2s, sizeof(int32_t) * ctx->natom) == hipSuccess;
+    if (ok) {
+        (void)hipMemcpy(d_sd, sd.data(), sizeof(SpeciesDev) * nspecies, hipMemcpyHostToDevice);
+        (void)hipMemcpy(d_gxyz, ctx->gxyz.data(), sizeof(double) * 3 * ctx->natom, hipMemcpyHostToDevice);
+        (void)hipMemcpy(d_atv, atv.data(), sizeof(double) * atv.size(), hipMemcpyHostToDevice);
+        (void)hipMemcpy(d_a2s, ctx->atom2spe.data(), sizeof(int32_t) * ctx->natom, hipMemcpyHostToDevice);
+    }
+    const int64_t total = ctx->tab_off[ctx->natom];
+    if (ok && !ctx->d_tab) ok = hipMalloc(&ctx->d_tab, sizeof(double) * std::max<int64_t>(1, total)) == hipSuccess;
+    if (ok && with_derivatives)
+        for (int x = 0; x < 3 && ok; x++)
+            if (!ctx->d_dtab[x]) ok = hipMalloc(&ctx->d_dtab[x], sizeof(double) * std::max<int64_t>(1, total)) == hipSuccess;
+    int result = LCPAO_OK;
+    if (!ok) result = set_err(ctx, LCPAO_EDEVICE, "device allocation for table generation failed");
+    if (ok) {
+        TabGeom tg;
+        for (int j = 0; j < 3; j++) {
+            tg.glat[j] = ctx->lat[j] / ctx->ngrid[0];
+            tg.glat[3 + j] = ctx->lat[3 + j] / ctx->ngrid[1];
+            tg.glat[6 + j] = ctx->lat[6 + j] / ctx->ngrid[2];
+            tg.origin[j] = ctx->origin[j];
+        }
+        tg.n2 = ctx->ngrid[1]; tg.n3 = ctx->ngrid[2]; tg.natom = ctx->natom;
+        (void)hipMemsetAsync(ctx->d_flag, 0, sizeof(int32_t), ctx->stream);
+        const int64_t nent = ctx->total_gridn;
+        const unsigned nb = (unsigned)((nent + 127) / 128);
+        if (nent > 0) {
+            hipLaunchKernelGGL(k_tables, dim3(nb), dim3(128), 0, ctx->stream, nent, ctx->d_gl_off, ctx->d_glist,
+                               ctx->d_clist, ctx->d_tab_off, d_gxyz, d_a2s, d_atv, tg, d_sd, 0, ctx->d_tab, nullptr,
+                               nullptr, nullptr, ctx->d_flag);
+            if (with_derivatives)
+                hipLaunchKernelGGL(k_tables, dim3(nb), dim3(128), 0, ctx->stream, nent, ctx->d_gl_off, ctx->d_glist,
+                                   ctx->d_clist, ctx->d_tab_off, d_gxyz, d_a2s, d_atv, tg, d_sd, 1, nullptr,
+                                   ctx->d_dtab[0], ctx->d_dtab[1], ctx->d_dtab[2], ctx->d_flag);
+        }
+        int32_t flag = 0;
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(&flag, ctx->d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) result = set_err(ctx, LCPAO_EDEVICE, std::string("table generation: ") + hipGetErrorString(e));
+        else if (flag) result = set_err(ctx, LCPAO_EMESH, "radial mesh index out of range (point too close to the mesh ends)");
+        else {
+            ctx->tables = true;
+            if (with_derivatives) ctx->dtables = true;
+        }
+    }
+    (void)hipFree(d_sd); (void)hipFree(d_gxyz); (void)hipFree(d_atv); (void)hipFree(d_a2s);
+    cleanup();
+    return result;
+}
+
+int lcpao_grid_get_tables(lcpao_ctx *ctx, int32_t which, double *const *out) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (which < 0 || which > 3 || !out) return set_err(ctx, LCPAO_EINVAL, "bad table selector");
+    const double *src = which == 0 ? ctx->d_tab : ctx->d_dtab[which - 1];
+    if (!src || (which == 0 && !ctx->tables) || (which > 0 && !ctx->dtables)) return set_err(ctx, LCPAO_ESTATE, "table not set");
+    for (int a = 0; a < ctx->natom; a++) {
+        const int64_t n = ctx->tab_off[a + 1] - ctx->tab_off[a];
+        if (n && out[a]) HIP_TRY(ctx, hipMemcpy(out[a], src + ctx->tab_off[a], (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    return LCPAO_OK;
+}
+
+int lcpao_grid_hamiltonian_dev(lcpao_ctx *ctx, int32_t nch, const double *const *d_V, double *const *d_H, int32_t precision) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!d_V || !d_H) return set_err(ctx, LCPAO_EINVAL, "null argument");
+    return ham_dev(ctx, nch, d_V, d_H, precision);
+}
+
+int lcpao_grid_density_dev(lcpao_ctx *ctx, int32_t nspin, const double *const *d_DM, int64_t dm_offset,
+                           double *const *d_rho, int32_t precision) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!d_DM || !d_rho) return set_err(ctx, LCPAO_EINVAL, "null argument");
+    return rho_dev(ctx, nspin, d_DM, dm_offset, d_rho, precision);
+}
+
+int lcpao_grid_force3_dev(lcpao_ctx *ctx, int32_t nch, const double *const *d_V, const double *const *d_DM,
+                          int64_t dm_offset, const double *weight, double *d_force, int32_t precision) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!d_V || !d_DM) return set_err(ctx, LCPAO_EINVAL, "null argument");
+    return f3_dev(ctx, nch, d_V, d_DM, dm_offset, weight, d_force, precision);
+}
+
+int lcpao_grid_hamiltonian(lcpao_ctx *ctx, int32_t nch, const double *const *V, double *const *H, int32_t precision) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!V || !H || (nch != 1 && nch != 2 && nch != 4)) return set_err(ctx, LCPAO_EINVAL, "bad channel arguments");
+    const int64_t per = ctx->nn + ctx->my_hsize;
+    if ((rc = stage(ctx, per * nch))) return rc;
+    const double *dV[4];
+    double *dH[4];
+    for (int c = 0; c < nch; c++) {
+        if (!V[c] || !H[c]) return set_err(ctx, LCPAO_EINVAL, "null channel pointer");
+        double *v = ctx->d_stage + per * c;
+        HIP_TRY(ctx, hipMemcpyAsync(v, V[c], (size_t)ctx->nn * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        dV[c] = v;
+        dH[c] = v + ctx->nn;
+    }
+    if ((rc = ham_dev(ctx, nch, dV, dH, precision))) return rc;
+    for (int c = 0; c < nch; c++)
+        HIP_TRY(ctx, hipMemcpyAsync(H[c], dH[c], (size_t)ctx->my_hsize * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return LCPAO_OK;
+}
+
+int lcpao_grid_density(lcpao_ctx *ctx, int32_t nspin, const double *const *DM, int64_t dm_offset, double *const *rho,
+                       int32_t precision) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!DM || !rho || (nspin != 1 && nspin != 2 && nspin != 4)) return set_err(ctx, LCPAO_EINVAL, "bad spin arguments");
+    if (dm_offset < 0) return set_err(ctx, LCPAO_EINVAL, "negative dm_offset");
+    const int64_t per = ctx->nn + ctx->my_hsize;
+    if ((rc = stage(ctx, per * nspin))) return rc;
+    const double *dD[4];
+    double *dR[4];
+    for (int s = 0; s < nspin; s++) {
+        if (!DM[s] || !rho[s]) return set_err(ctx, LCPAO_EINVAL, "null spin pointer");
+        double *d = ctx->d_stage + per * s;
+        HIP_TRY(ctx, hipMemcpyAsync(d, DM[s] + dm_offset, (size_t)ctx->my_hsize * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        dD[s] = d;
+        dR[s] = d + ctx->my_hsize;
+    }
+    if ((rc = rho_dev(ctx, nspin, dD, 0, dR, precision))) return rc;
+    for (int s = 0; s < nspin; s++)
+        HIP_TRY(ctx, hipMemcpyAsync(rho[s], dR[s], (size_t)ctx->nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return LCPAO_OK;
+}
+
+int lcpao_grid_force3(lcpao_ctx *ctx, int32_t nch, const double *const *V, const double *const *DM, int64_t dm_offset,
+                      const double *weight, double *force, int32_t precision) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!V || !DM || !weight || !force || (nch != 1 && nch != 2 && nch != 4)) return set_err(ctx, LCPAO_EINVAL, "bad arguments");
+    if (dm_offset < 0) return set_err(ctx, LCPAO_EINVAL, "negative dm_offset");
+    const int64_t per = ctx->nn + ctx->my_hsize;
+    if ((rc = stage(ctx, per * nch + 3 * (int64_t)ctx->natom))) return rc;
+    const double *dV[4], *dD[4];
+    for (int c = 0; c < nch; c++) {
+        if (!V[c] || !DM[c]) return set_err(ctx, LCPAO_EINVAL, "null channel pointer");
+        double *v = ctx->d_stage + per * c;
+        HIP_TRY(ctx, hipMemcpyAsync(v, V[c], (size_t)ctx->nn * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(v + ctx->nn, DM[c] + dm_offset, (size_t)ctx->my_hsize * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        dV[c] = v;
+        dD[c] = v + ctx->nn;
+    }
+    double *dF = ctx->d_stage + per * nch;
+    HIP_TRY(ctx, hipMemcpyAsync(dF, force, 3 * (size_t)ctx->natom * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = f3_dev(ctx, nch, dV, dD, 0, weight, dF, precision))) return rc;
+    HIP_TRY(ctx, hipMemcpyAsync(force, dF, 3 * (size_t)ctx->natom * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return LCPAO_OK;
+}
+
+const char *lcpao_grid_last_error(const lcpao_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
+
+}  // extern "C"

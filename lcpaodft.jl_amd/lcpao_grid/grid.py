@@ -1,0 +1,268 @@
+"""Host-side mirror of the reference's grid-path routines over the C ABI.
+
+The functions keep the reference's names and argument order, with the
+Julia mutation mark "!" dropped:
+
+  Set_Orbitals_Grid(pao, ucell)                       Set_Orbitals_Grid.jl:26-42
+  Set_dOrbitals_Grid(pao, ucell, Orbs_Grid)           Set_dOrbitals_Grid.jl:1-20
+  Calc_MatrixElements_dVH_Vxc_{off,on,nc}(ucell, Orbs_Grid, Vpot_Grid, MPI_Hks)
+                                                      Hamiltonian.jl:235-380
+  Set_Density_Grid(SpinPol, ucell, Orbs_Grid, DM, Density_Grid) and
+  Set_Density_Grid_{nonpol,pol,nc}(...)               Set_Density_Grid.jl:1-240
+  Force3_{nospin,spin}(pao, Orbs_Grid, Vpot_Grid, DM, ucell),
+  Force3_nc(pao, Orbs_Grid, dVHart_Grid, Vxc_Grid, DM, ucell)
+                                                      Force.jl:407-566
+
+Orbs_Grid is a GridContext: the tables live in device memory.  DM is the
+global flat density matrix per spin (the layout of DM before
+Set_DM2DM_Vec, Convert_Matrix.jl:27-62); Force3 takes it flat as well.
+MPI_Hks / Density_Grid are filled in place (numpy arrays).  Errors raise
+GridError.  There is no CPU fallback.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import GridError, check
+
+WEIGHTS_FORCE3 = {1: (4.0,), 2: (2.0, 2.0), 4: (2.0, 2.0, 4.0, -4.0)}
+
+
+def _ptrs(arrs, t=_lib.dp):
+    return (t * len(arrs))(*[a.ctypes.data_as(t) if a is not None else None for a in arrs])
+
+
+def _prec(p):
+    if isinstance(p, str):
+        if p not in _lib.PRECISIONS:
+            raise ValueError("precision must be one of %s" % list(_lib.PRECISIONS))
+        return _lib.PRECISIONS[p]
+    return int(p)
+
+
+class GridContext:
+    """Device context of one geometry (lists, tables, work decomposition)."""
+
+    def __init__(self, ucell, device=0, with_lattice=True):
+        L = _lib.load()
+        u = ucell
+        self.ucell = u
+        self._keep = []
+        keep = self._keep.append
+        no = np.ascontiguousarray(u.Total_NumOrbs, np.int32)
+        gridn = np.ascontiguousarray(u.GridN_Atom, np.int32)
+        glist = np.ascontiguousarray(u.glist, np.int32)
+        clist = np.ascontiguousarray(u.clist, np.int32)
+        olg1 = np.ascontiguousarray(u.olg1, np.int32)
+        olg2 = np.ascontiguousarray(u.olg2, np.int32)
+        for x in (no, gridn, glist, clist, olg1, olg2):
+            keep(x)
+        isz = glist.itemsize
+        base_g, base_c = glist.ctypes.data, clist.ctypes.data
+        gptr = (_lib.ip * u.Natom)(*[C.cast(base_g + int(u.gl_off[a]) * isz, _lib.ip) for a in range(u.Natom)])
+        cptr = (_lib.ip * u.Natom)(*[C.cast(base_c + int(u.gl_off[a]) * isz, _lib.ip) for a in range(u.Natom)])
+        b1, b2 = olg1.ctypes.data, olg2.ctypes.data
+        P = u.MPI_size
+        o1p = (_lib.ip * max(P, 1))(*[C.cast(b1 + int(u.olg_off[p]) * isz, _lib.ip) for p in range(P)])
+        o2p = (_lib.ip * max(P, 1))(*[C.cast(b2 + int(u.olg_off[p]) * isz, _lib.ip) for p in range(P)])
+        pa = np.ascontiguousarray(u.MPI_atom, np.int32)
+        pn = np.ascontiguousarray(u.MPI_natn, np.int32)
+        pc = np.ascontiguousarray(u.MPI_ncn, np.int32)
+        nolg = np.ascontiguousarray(u.MPI_NumOLG, np.int32)
+        lat = np.ascontiguousarray(u.Latvecs, np.float64)
+        org = np.ascontiguousarray(u.Grid_Origin, np.float64)
+        gx = np.ascontiguousarray(u.Gxyz, np.float64)
+        a2s = np.ascontiguousarray(u.atom2spe, np.int32)
+        for x in (gptr, cptr, o1p, o2p, pa, pn, pc, nolg, lat, org, gx, a2s):
+            keep(x)
+        g = _lib.Geom()
+        g.natom, g.atom_base = u.Natom, 0
+        g.ngrid[:] = list(u.Ngrid)
+        g.cpycell, g.gridvol = u.CpyCell, u.GridVol
+        g.no, g.gridn = no.ctypes.data_as(_lib.ip), gridn.ctypes.data_as(_lib.ip)
+        g.grid_list, g.cell_list = gptr, cptr
+        g.npairs = P
+        g.pair_atom, g.pair_natn, g.pair_ncn = pa.ctypes.data_as(_lib.ip), pn.ctypes.data_as(_lib.ip), pc.ctypes.data_as(_lib.ip)
+        g.numolg, g.olg1, g.olg2 = nolg.ctypes.data_as(_lib.ip), o1p, o2p
+        if with_lattice:
+            g.latvecs, g.origin, g.gxyz = lat.ctypes.data_as(_lib.dp), org.ctypes.data_as(_lib.dp), gx.ctypes.data_as(_lib.dp)
+            g.atom2spe = a2s.ctypes.data_as(_lib.ip)
+        self.h = C.c_void_p()
+        rc = L.lcpao_grid_create(C.byref(self.h), C.byref(g), int(device))
+        if rc != 0:
+            raise GridError(rc, L.lcpao_grid_last_error(None).decode())
+        self._keep = []  # the library copied everything
+        self.L = L
+        hs, nn, to, tg = (C.c_int64() for _ in range(4))
+        L.lcpao_grid_sizes(self.h, C.byref(hs), C.byref(nn), C.byref(to), C.byref(tg))
+        self.my_hsize, self.nn, self.total_olg, self.total_gridn = hs.value, nn.value, to.value, tg.value
+        self.natom = u.Natom
+        self.no = no.copy()
+        self.gridn = gridn.copy()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.lcpao_grid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        check(rc, self.h)
+
+    # tables -----------------------------------------------------------------
+    def set_tables(self, tabs):
+        """tabs[a]: (GridN_a, NO_a) C-order == Julia NO_a x GridN_a column-major."""
+        tabs = [np.ascontiguousarray(t, np.float64) for t in tabs]
+        self._check(self.L.lcpao_grid_set_tables(self.h, _ptrs(tabs)))
+
+    def set_dtables(self, dx, dy, dz):
+        dx, dy, dz = ([np.ascontiguousarray(t, np.float64) for t in d] for d in (dx, dy, dz))
+        self._check(self.L.lcpao_grid_set_dtables(self.h, _ptrs(dx), _ptrs(dy), _ptrs(dz)))
+
+    def build_tables(self, paos, with_derivatives=False):
+        """Tables generated on the device from the radial data of each species."""
+        arr = (_lib.Pao * len(paos))()
+        keep = []
+        for s, p in enumerate(paos):
+            arr[s].maxl = p.Spe_MaxL_Basis
+            nb = list(p.Spe_Num_Basis) + [0] * (4 - len(p.Spe_Num_Basis))
+            arr[s].num_basis[:] = nb
+            arr[s].nmesh = p.Spe_Num_Mesh_PAO
+            rv = np.ascontiguousarray(p.Spe_PAO_RV)
+            keep.append(rv)
+            arr[s].rv = rv.ctypes.data_as(_lib.dp)
+            rw = (_lib.dp * 16)()
+            for l in range(p.Spe_MaxL_Basis + 1):
+                for q in range(p.Spe_Num_Basis[l]):
+                    w = np.ascontiguousarray(p.Spe_PAO_RWF[l][q])
+                    keep.append(w)
+                    rw[l * 4 + q] = w.ctypes.data_as(_lib.dp)
+            keep.append(rw)
+            arr[s].rwf = C.cast(rw, _lib.dpp)
+        self._check(self.L.lcpao_grid_build_tables(self.h, len(paos), arr, int(bool(with_derivatives))))
+
+    def get_tables(self, which=0):
+        out = [np.zeros((int(self.gridn[a]), int(self.no[a]))) for a in range(self.natom)]
+        self._check(self.L.lcpao_grid_get_tables(self.h, which, _ptrs(out)))
+        return out
+
+    # contractions on host arrays ----------------------------------------------
+    def hamiltonian(self, V, precision="fp64"):
+        V = [np.ascontiguousarray(v, np.float64) for v in V]
+        H = [np.zeros(self.my_hsize) for _ in V]
+        self._check(self.L.lcpao_grid_hamiltonian(self.h, len(V), _ptrs(V), _ptrs(H), _prec(precision)))
+        return H
+
+    def density(self, DM, dm_offset=0, precision="fp64"):
+        DM = [np.ascontiguousarray(d, np.float64) for d in DM]
+        R = [np.zeros(self.nn) for _ in DM]
+        self._check(self.L.lcpao_grid_density(self.h, len(DM), _ptrs(DM), int(dm_offset), _ptrs(R), _prec(precision)))
+        return R
+
+    def force3(self, V, DM, weights=None, dm_offset=0, precision="fp64", force=None):
+        V = [np.ascontiguousarray(v, np.float64) for v in V]
+        DM = [np.ascontiguousarray(d, np.float64) for d in DM]
+        w = np.ascontiguousarray(weights if weights is not None else WEIGHTS_FORCE3[len(V)], np.float64)
+        F = np.zeros((3, self.natom)) if force is None else np.ascontiguousarray(force.T)  # column-major natom x 3
+        self._check(self.L.lcpao_grid_force3(self.h, len(V), _ptrs(V), _ptrs(DM), int(dm_offset),
+                                             w.ctypes.data_as(_lib.dp), F.ctypes.data_as(_lib.dp), _prec(precision)))
+        return F.T.copy()
+
+    # contractions on device arrays (torch tensors, inputs resident in HBM) ------
+    @staticmethod
+    def _dptrs(ts):
+        return (_lib.dp * len(ts))(*[C.cast(t.data_ptr(), _lib.dp) for t in ts])
+
+    def hamiltonian_dev(self, dV, dH, precision="fp64"):
+        self._check(self.L.lcpao_grid_hamiltonian_dev(self.h, len(dV), self._dptrs(dV), self._dptrs(dH), _prec(precision)))
+
+    def density_dev(self, dDM, drho, dm_offset=0, precision="fp64"):
+        self._check(self.L.lcpao_grid_density_dev(self.h, len(dDM), self._dptrs(dDM), int(dm_offset), self._dptrs(drho),
+                                                  _prec(precision)))
+
+    def force3_dev(self, dV, dDM, dforce, weights=None, dm_offset=0, precision="fp64"):
+        w = np.ascontiguousarray(weights if weights is not None else WEIGHTS_FORCE3[len(dV)], np.float64)
+        self._check(self.L.lcpao_grid_force3_dev(self.h, len(dV), self._dptrs(dV), self._dptrs(dDM), int(dm_offset),
+                                                 w.ctypes.data_as(_lib.dp), C.cast(dforce.data_ptr(), _lib.dp),
+                                                 _prec(precision)))
+
+
+# ----------------------------------------------------------------------------
+# reference-named entry points
+# ----------------------------------------------------------------------------
+def Set_Orbitals_Grid(pao, ucell, device=0):
+    """Set_Orbitals_Grid (Set_Orbitals_Grid.jl:26-42): tables computed on the device."""
+    ctx = GridContext(ucell, device)
+    ctx.build_tables(pao, with_derivatives=False)
+    return ctx
+
+
+def Set_dOrbitals_Grid(pao, ucell, Orbs_Grid):
+    """Set_dOrbitals_Grid (Set_dOrbitals_Grid.jl:1-20): gradient tables into the same context."""
+    Orbs_Grid.build_tables(pao, with_derivatives=True)
+    return Orbs_Grid
+
+
+def _hks(Orbs_Grid, Vpot_Grid, MPI_Hks, nch, precision):
+    if len(Vpot_Grid) < nch or len(MPI_Hks) < nch:
+        raise ValueError("need %d channels" % nch)
+    H = Orbs_Grid.hamiltonian(Vpot_Grid[:nch], precision)
+    for c in range(nch):
+        MPI_Hks[c][:] = H[c]
+
+
+def Calc_MatrixElements_dVH_Vxc_off(ucell, Orbs_Grid, Vpot_Grid, MPI_Hks, precision="fp64"):
+    _hks(Orbs_Grid, Vpot_Grid, MPI_Hks, 1, precision)
+
+
+def Calc_MatrixElements_dVH_Vxc_on(ucell, Orbs_Grid, Vpot_Grid, MPI_Hks, precision="fp64"):
+    _hks(Orbs_Grid, Vpot_Grid, MPI_Hks, 2, precision)
+
+
+def Calc_MatrixElements_dVH_Vxc_nc(ucell, Orbs_Grid, Vpot_Grid, MPI_Hks, precision="fp64"):
+    _hks(Orbs_Grid, Vpot_Grid, MPI_Hks, 4, precision)
+
+
+def _dens(ucell, Orbs_Grid, DM, Density_Grid, nspin, precision):
+    R = Orbs_Grid.density(DM[:nspin], ucell.MPHks_rank, precision)
+    for s in range(nspin):
+        Density_Grid[s][:] = R[s]
+
+
+def Set_Density_Grid_nonpol(ucell, Orbs_Grid, DM, Density_Grid, precision="fp64"):
+    _dens(ucell, Orbs_Grid, DM, Density_Grid, 1, precision)
+
+
+def Set_Density_Grid_pol(ucell, Orbs_Grid, DM, Density_Grid, precision="fp64"):
+    _dens(ucell, Orbs_Grid, DM, Density_Grid, 2, precision)
+
+
+def Set_Density_Grid_nc(ucell, Orbs_Grid, DM, Density_Grid, precision="fp64"):
+    _dens(ucell, Orbs_Grid, DM, Density_Grid, 4, precision)
+
+
+def Set_Density_Grid(SpinPol, ucell, Orbs_Grid, DM, Density_Grid, precision="fp64"):
+    """Dispatcher of Set_Density_Grid.jl:1-9."""
+    {"off": Set_Density_Grid_nonpol, "on": Set_Density_Grid_pol, "nc": Set_Density_Grid_nc}[SpinPol](
+        ucell, Orbs_Grid, DM, Density_Grid, precision)
+
+
+def Force3_nospin(pao, Orbs_Grid, Vpot_Grid, DM, ucell, precision="fp64"):
+    """VpotForce (natom x 3) = 4*GridVol*sum (Force.jl:407-451); the gradient
+    tables must be in Orbs_Grid (Set_dOrbitals_Grid)."""
+    return Orbs_Grid.force3([Vpot_Grid[0]], [DM[0]], WEIGHTS_FORCE3[1], ucell.MPHks_rank, precision)
+
+
+def Force3_spin(pao, Orbs_Grid, Vpot_Grid, DM, ucell, precision="fp64"):
+    return Orbs_Grid.force3(list(Vpot_Grid[:2]), list(DM[:2]), WEIGHTS_FORCE3[2], ucell.MPHks_rank, precision)
+
+
+def Force3_nc(pao, Orbs_Grid, dVHart_Grid, Vxc_Grid, DM, ucell, precision="fp64"):
+    V = [dVHart_Grid + Vxc_Grid[0], dVHart_Grid + Vxc_Grid[1], Vxc_Grid[2], Vxc_Grid[3]]
+    return Orbs_Grid.force3(V, list(DM[:4]), WEIGHTS_FORCE3[4], ucell.MPHks_rank, precision)

@@ -1,0 +1,265 @@
+"""ctypes front end of the CPU oracle (oracle/lcpao_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+
+The oracle restates the reference (LCPAODFT.jl) operation by operation; see
+the header of lcpao_oracle.c for what pins each function.
+"""
+import ctypes as C
+import glob
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblcpao_oracle.so")
+PAO_DIR = os.path.join(os.path.dirname(HERE), "lcpaodft.jl_amd", "data", "pao")
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_lp = C.POINTER(C.c_int64)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_last_error.restype = C.c_char_p
+        for name in ["orc_pao_rv", "orc_pao_rwf", "orc_geom_atv"]:
+            getattr(L, name).restype = _dp
+        for name in ["orc_geom_fnan", "orc_geom_natn", "orc_geom_ncn", "orc_geom_atv_ijk", "orc_geom_gridn",
+                     "orc_geom_glist", "orc_geom_clist", "orc_geom_pair_atom", "orc_geom_pair_slot",
+                     "orc_geom_pair_natn", "orc_geom_pair_ncn", "orc_geom_numolg", "orc_geom_olg1",
+                     "orc_geom_olg2"]:
+            getattr(L, name).restype = _ip
+        for name in ["orc_geom_nb_off", "orc_geom_gl_off", "orc_geom_pair_hoff", "orc_geom_mpi_hsize",
+                     "orc_geom_olg_off"]:
+            getattr(L, name).restype = _lp
+        L.orc_ylm_real.restype = C.c_double
+        L.orc_ylm_real.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double]
+        L.orc_dylm_dtheta.restype = C.c_double
+        L.orc_dylm_dtheta.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double]
+        L.orc_dylm_dphi.restype = C.c_double
+        L.orc_dylm_dphi.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError("oracle: " + lib().orc_last_error().decode())
+
+
+def _arr(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+def _dptrs(arrs):
+    return (_dp * len(arrs))(*[a.ctypes.data_as(_dp) for a in arrs])
+
+
+class Pao:
+    """PAO radial data for one species (Read_PAO.jl:114-331)."""
+
+    def __init__(self, symbol, cutoff, basis, pao_dir=PAO_DIR):
+        path = os.path.join(pao_dir, "%s%.1f.pao" % (symbol, cutoff))
+        self.h = C.c_void_p()
+        _check(lib().orc_pao_read(path.encode(), C.c_double(cutoff), basis.encode(), C.byref(self.h)))
+        maxl, tno, nmesh = C.c_int(), C.c_int(), C.c_int()
+        nb = (C.c_int * 4)()
+        cut = C.c_double()
+        lib().orc_pao_info(self.h, C.byref(maxl), nb, C.byref(tno), C.byref(nmesh), C.byref(cut))
+        self.maxl, self.num_basis, self.total_orbs = maxl.value, list(nb)[: maxl.value + 1], tno.value
+        self.nmesh, self.cut = nmesh.value, cut.value
+        self.rv = _arr(lib().orc_pao_rv(self.h), self.nmesh, np.float64)
+
+    def rwf(self, l, mul):
+        return _arr(lib().orc_pao_rwf(self.h, l, mul), self.nmesh, np.float64)
+
+    def __del__(self):
+        try:
+            lib().orc_pao_free(self.h)
+        except Exception:
+            pass
+
+
+class Geom:
+    """UCell lists built by the oracle (UCell.jl:210-294)."""
+
+    def __init__(self, latvecs, gxyz, cut, no, ngrid, origin=(0.0, 0.0, 0.0), nprocs=1, rank=0):
+        L = lib()
+        lat = np.ascontiguousarray(latvecs, dtype=np.float64).reshape(3, 3)
+        gx = np.ascontiguousarray(gxyz, dtype=np.float64).reshape(-1, 3)
+        natom = gx.shape[0]
+        cut = np.ascontiguousarray(cut, dtype=np.float64)
+        no = np.ascontiguousarray(no, dtype=np.int32)
+        ng = np.ascontiguousarray(ngrid, dtype=np.int32)
+        org = np.ascontiguousarray(origin, dtype=np.float64)
+        self.h = C.c_void_p()
+        _check(L.orc_ucell_build(lat.ctypes.data_as(_dp), natom, gx.ctypes.data_as(_dp), cut.ctypes.data_as(_dp),
+                                 no.ctypes.data_as(_ip), ng.ctypes.data_as(_ip), org.ctypes.data_as(_dp),
+                                 nprocs, rank, C.byref(self.h)))
+        cpy = C.c_int()
+        nloop, npair, ths, mhs, mho, nolg, ngt = (C.c_int64() for _ in range(7))
+        gv = C.c_double()
+        L.orc_geom_scalars(self.h, C.byref(cpy), C.byref(nloop), C.byref(npair), C.byref(ths), C.byref(mhs),
+                           C.byref(mho), C.byref(gv), C.byref(nolg), C.byref(ngt))
+        self.natom, self.latvecs, self.gxyz, self.cut, self.no = natom, lat, gx, cut, no
+        self.ngrid, self.origin = tuple(int(x) for x in ng), org
+        self.cpycell, self.nloop, self.npair = cpy.value, nloop.value, npair.value
+        self.total_hsize, self.my_hsize, self.my_hoff = ths.value, mhs.value, mho.value
+        self.gridvol = gv.value
+        ncell = (2 * self.cpycell + 1) ** 3
+        self.fnan = _arr(L.orc_geom_fnan(self.h), natom, np.int32)
+        self.nb_off = _arr(L.orc_geom_nb_off(self.h), natom + 1, np.int64)
+        nnb = int(self.nb_off[-1])
+        self.natn = _arr(L.orc_geom_natn(self.h), nnb, np.int32)
+        self.ncn = _arr(L.orc_geom_ncn(self.h), nnb, np.int32)
+        self.atv = _arr(L.orc_geom_atv(self.h), ncell * 3, np.float64).reshape(-1, 3)
+        self.atv_ijk = _arr(L.orc_geom_atv_ijk(self.h), ncell * 3, np.int32).reshape(-1, 3)
+        self.gridn = _arr(L.orc_geom_gridn(self.h), natom, np.int32)
+        self.gl_off = _arr(L.orc_geom_gl_off(self.h), natom + 1, np.int64)
+        self.glist = _arr(L.orc_geom_glist(self.h), ngt.value, np.int32)
+        self.clist = _arr(L.orc_geom_clist(self.h), ngt.value, np.int32)
+        self.pair_atom = _arr(L.orc_geom_pair_atom(self.h), npair.value, np.int32)
+        self.pair_slot = _arr(L.orc_geom_pair_slot(self.h), npair.value, np.int32)
+        self.pair_natn = _arr(L.orc_geom_pair_natn(self.h), npair.value, np.int32)
+        self.pair_ncn = _arr(L.orc_geom_pair_ncn(self.h), npair.value, np.int32)
+        self.pair_hoff = _arr(L.orc_geom_pair_hoff(self.h), npair.value + 1, np.int64)
+        self.mpi_hsize = _arr(L.orc_geom_mpi_hsize(self.h), nprocs, np.int64)
+        self.numolg = _arr(L.orc_geom_numolg(self.h), npair.value, np.int32)
+        self.olg_off = _arr(L.orc_geom_olg_off(self.h), npair.value + 1, np.int64)
+        self.olg1 = _arr(L.orc_geom_olg1(self.h), nolg.value, np.int32)
+        self.olg2 = _arr(L.orc_geom_olg2(self.h), nolg.value, np.int32)
+        self.nn = int(np.prod(self.ngrid))
+
+    def overlap_bruteforce(self, q):
+        cap = int(self.gridn.max())
+        t1 = np.zeros(cap, np.int32)
+        t2 = np.zeros(cap, np.int32)
+        n = C.c_int64()
+        _check(lib().orc_overlap_bruteforce(self.h, C.c_int64(q), t1.ctypes.data_as(_ip), t2.ctypes.data_as(_ip),
+                                            C.c_int64(cap), C.byref(n)))
+        return t1[: n.value].copy(), t2[: n.value].copy()
+
+    def __del__(self):
+        try:
+            lib().orc_geom_free(self.h)
+        except Exception:
+            pass
+
+
+def _tables_alloc(geom):
+    return [np.zeros((int(geom.gridn[a]), int(geom.no[a])), dtype=np.float64) for a in range(geom.natom)]
+
+
+def orbitals(geom, paos, atom2spe):
+    """Set_Orbitals_Grid! tables; tab[a] has shape (GridN_a, NO_a) (row = point)."""
+    tabs = _tables_alloc(geom)
+    hs = (C.c_void_p * len(paos))(*[p.h for p in paos])
+    a2s = np.ascontiguousarray(atom2spe, dtype=np.int32)
+    _check(lib().orc_orbitals(geom.h, hs, a2s.ctypes.data_as(_ip), _dptrs(tabs)))
+    return tabs
+
+
+def dorbitals(geom, paos, atom2spe):
+    """Set_dOrbitals_Grid! tables (x, y, z), each a list like orbitals()."""
+    tx, ty, tz = _tables_alloc(geom), _tables_alloc(geom), _tables_alloc(geom)
+    hs = (C.c_void_p * len(paos))(*[p.h for p in paos])
+    a2s = np.ascontiguousarray(atom2spe, dtype=np.int32)
+    _check(lib().orc_dorbitals(geom.h, hs, a2s.ctypes.data_as(_ip), _dptrs(tx), _dptrs(ty), _dptrs(tz)))
+    return tx, ty, tz
+
+
+def hamiltonian(geom, tabs, V, with_abs=False):
+    """Calc_MatrixElements_dVH_Vxc_*! : list of channels -> list of H (my_hsize)."""
+    V = [np.ascontiguousarray(v, dtype=np.float64) for v in V]
+    H = [np.zeros(geom.my_hsize) for _ in V]
+    A = [np.zeros(geom.my_hsize) for _ in V] if with_abs else None
+    _check(lib().orc_hamiltonian(geom.h, _dptrs(tabs), len(V), _dptrs(V), _dptrs(H), _dptrs(A) if A else None))
+    return (H, A) if with_abs else H
+
+
+def density(geom, tabs, DM, with_abs=False):
+    """Set_Density_Grid_*! : list of spins (global flat DM) -> list of rho (NN)."""
+    DM = [np.ascontiguousarray(d, dtype=np.float64) for d in DM]
+    R = [np.zeros(geom.nn) for _ in DM]
+    A = [np.zeros(geom.nn) for _ in DM] if with_abs else None
+    _check(lib().orc_density(geom.h, _dptrs(tabs), len(DM), _dptrs(DM), _dptrs(R), _dptrs(A) if A else None))
+    return (R, A) if with_abs else R
+
+
+def force3_sums(geom, tabs, dtabs, V, DM):
+    """Per-atom sums of _Calc_Force3_Blocked! for one channel: (natom, 3)."""
+    S = np.zeros((geom.natom, 3))
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    DM = np.ascontiguousarray(DM, dtype=np.float64)
+    _check(lib().orc_force3_sums(geom.h, _dptrs(tabs), _dptrs(dtabs[0]), _dptrs(dtabs[1]), _dptrs(dtabs[2]),
+                                 V.ctypes.data_as(_dp), DM.ctypes.data_as(_dp), S.ctypes.data_as(_dp)))
+    return S
+
+
+FORCE3_WEIGHTS = {1: (4.0,), 2: (2.0, 2.0), 4: (2.0, 2.0, 4.0, -4.0)}
+
+
+def force3(geom, tabs, dtabs, Vs, DMs):
+    """Force3_{nospin,spin,nc} (Force.jl:407-566): returns VpotForce (natom, 3).
+
+    For nc, Vs = [dVH+Vxc1, dVH+Vxc2, Vxc3, Vxc4] as the reference forms them."""
+    w = FORCE3_WEIGHTS[len(Vs)]
+    F = np.zeros((geom.natom, 3))
+    sums = [force3_sums(geom, tabs, dtabs, Vs[c], DMs[c]) for c in range(len(Vs))]
+    # per pair the reference forms (w1*s1 + w2*s2 + ...)*GridVol; summed per atom here
+    acc = sum(w[c] * sums[c] for c in range(len(Vs)))
+    F += acc * geom.gridvol
+    return F
+
+
+def bruteforce(geom, tabs, V, DM):
+    H = np.zeros(geom.my_hsize)
+    rho = np.zeros(geom.nn)
+    n = C.c_int64()
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    DM = np.ascontiguousarray(DM, dtype=np.float64)
+    _check(lib().orc_bruteforce(geom.h, _dptrs(tabs), V.ctypes.data_as(_dp), DM.ctypes.data_as(_dp),
+                                H.ctypes.data_as(_dp), rho.ctypes.data_as(_dp), C.byref(n)))
+    return H, rho, n.value
+
+
+def blas_path():
+    import numpy
+    cands = glob.glob(os.path.join(os.path.dirname(numpy.__file__), "..", "numpy.libs", "libscipy_openblas64_*.so"))
+    return cands[0] if cands else None
+
+
+def cpu_baseline(geom, tabs, V, DM, nworkers=1, use_blas=True, pair_limit=0):
+    """Reference-algorithm restatement (C + OpenBLAS DGEMM) of one H + rho assembly.
+
+    Returns (seconds, pairs_done, H, rho)."""
+    if use_blas:
+        p = blas_path()
+        if p is None:
+            use_blas = False
+        else:
+            _check(lib().orc_blas_load(p.encode()))
+    H = np.zeros(geom.my_hsize)
+    rho = np.zeros(geom.nn)
+    secs = C.c_double()
+    done = C.c_int64()
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    DM = np.ascontiguousarray(DM, dtype=np.float64)
+    _check(lib().orc_cpu_baseline(geom.h, _dptrs(tabs), V.ctypes.data_as(_dp), DM.ctypes.data_as(_dp),
+                                  H.ctypes.data_as(_dp), rho.ctypes.data_as(_dp), nworkers, int(bool(use_blas)),
+                                  C.c_int64(pair_limit), C.byref(secs), C.byref(done)))
+    return secs.value, done.value, H, rho

@@ -1,0 +1,141 @@
+"""GPU parity of the HIP library (through the C ABI) against the CPU oracle.
+
+Cases: Si2 primitive (every reference branch incl. the R=0 continuation,
+periodic self-images), Si8 cubic, NaCl8 (mixed NO 13/14 and cutoffs 7/9),
+rank splits, 2- and 4-channel forms.  Bar: FP64 entries within
+1e-12 * max(|ref|, sum|terms|) (tests/tolerance.py).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from lcpao_grid import systems as S
+from lcpao_grid.grid import GridContext, WEIGHTS_FORCE3
+from tolerance import within, worst_ratio
+
+pytestmark = pytest.mark.gpu
+
+
+class Case:
+    def __init__(self, struct, nprocs=1, rank=0, nwaves=12, nocc=None, seed=0):
+        self.st = struct
+        self.u = struct.ucell(nprocs, rank)
+        self.g = O.Geom(struct.Latvecs, struct.Gxyz, struct.cutoffs(), struct.norbs(), struct.Ngrid, (0, 0, 0), nprocs, rank)
+        self.opaos = [O.Pao(*S.BASES[s]) for s in struct.species_names]
+        self.tabs = O.orbitals(self.g, self.opaos, struct.atom2spe)
+        full = struct.ucell()  # global DM needs every pair
+        V, DM = S.inputs(struct, full, nwaves=nwaves, nocc=nocc or max(2, int(full.Total_NumOrbs.sum()) // 8), seed=seed)
+        self.V, self.DM = V, DM
+        rng = S.SplitMix64(seed + 99)
+        self.V2 = V + 0.01 * rng.normal(V.size)
+        self.DM2 = DM * 0.5 + 0.001 * rng.normal(DM.size)
+        self.ctx = GridContext(self.u)
+
+
+_cache = {}
+
+
+def case(name):
+    if name not in _cache:
+        mk = {
+            "si2": lambda: Case(S.si2(24)),
+            "si8": lambda: Case(S.si(1, 20)),
+            "nacl8": lambda: Case(S.nacl(1, 24), nwaves=16, seed=3),
+            "si2_r1of3": lambda: Case(S.si2(24), nprocs=3, rank=1),
+            "nacl8_r0of2": lambda: Case(S.nacl(1, 24), nprocs=2, rank=0, seed=3),
+        }[name]
+        _cache[name] = mk()
+    return _cache[name]
+
+
+CASES = ["si2", "si8", "nacl8", "si2_r1of3", "nacl8_r0of2"]
+
+
+@pytest.mark.parametrize("name", ["si2", "nacl8"])
+def test_tables_on_device_match_oracle(name):
+    c = case(name)
+    ctx = GridContext(c.u)
+    ctx.build_tables(c.st.paos(), with_derivatives=True)
+    got = ctx.get_tables(0)
+    dx, dy, dz = O.dorbitals(c.g, c.opaos, c.st.atom2spe)
+    for a in range(c.u.Natom):
+        scale = np.abs(c.tabs[a]).max()
+        assert np.max(np.abs(got[a] - c.tabs[a])) <= 1e-14 * scale
+    for which, ref in ((1, dx), (2, dy), (3, dz)):
+        got = ctx.get_tables(which)
+        for a in range(c.u.Natom):
+            scale = np.abs(ref[a]).max()
+            assert np.max(np.abs(got[a] - ref[a])) <= 1e-13 * scale
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_hamiltonian_fp64(name):
+    c = case(name)
+    c.ctx.set_tables(c.tabs)
+    H = c.ctx.hamiltonian([c.V])[0]
+    Href, M = O.hamiltonian(c.g, c.tabs, [c.V], with_abs=True)
+    ok, msg = within(H, Href[0], M[0])
+    print(name, "H worst ratio", worst_ratio(H, Href[0], M[0]))
+    assert ok, msg
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_density_fp64(name):
+    c = case(name)
+    c.ctx.set_tables(c.tabs)
+    rho = c.ctx.density([c.DM], dm_offset=c.u.MPHks_rank)[0]
+    Rref, M = O.density(c.g, c.tabs, [c.DM], with_abs=True)
+    ok, msg = within(rho, Rref[0], M[0])
+    print(name, "rho worst ratio", worst_ratio(rho, Rref[0], M[0]))
+    assert ok, msg
+
+
+@pytest.mark.parametrize("name", ["si2", "nacl8"])
+def test_two_and_four_channels(name):
+    c = case(name)
+    c.ctx.set_tables(c.tabs)
+    Vs = [c.V, c.V2, 0.3 * c.V, -0.2 * c.V2]
+    H = c.ctx.hamiltonian(Vs)
+    Href, M = O.hamiltonian(c.g, c.tabs, Vs, with_abs=True)
+    for k in range(4):
+        ok, msg = within(H[k], Href[k], M[k])
+        assert ok, msg
+    Ds = [c.DM, c.DM2]
+    R = c.ctx.density(Ds)
+    Rref, MR = O.density(c.g, c.tabs, Ds, with_abs=True)
+    for k in range(2):
+        ok, msg = within(R[k], Rref[k], MR[k])
+        assert ok, msg
+
+
+def _force_bound(c, dt, Vs, Ds):
+    absd = [[np.abs(x) for x in d] for d in dt]
+    atabs = [np.abs(t) for t in c.tabs]
+    w = WEIGHTS_FORCE3[len(Vs)]
+    return sum(abs(w[k]) * O.force3_sums(c.g, atabs, absd, np.abs(Vs[k]), np.abs(Ds[k])) for k in range(len(Vs))) * c.g.gridvol
+
+
+@pytest.mark.parametrize("name", ["si2", "si8", "nacl8", "nacl8_r0of2"])
+def test_force3_fp64(name):
+    c = case(name)
+    dt = O.dorbitals(c.g, c.opaos, c.st.atom2spe)
+    c.ctx.set_tables(c.tabs)
+    c.ctx.set_dtables(*dt)
+    for Vs, Ds in (([c.V], [c.DM]), ([c.V, c.V2], [c.DM, c.DM2])):
+        F = c.ctx.force3(Vs, Ds, dm_offset=c.u.MPHks_rank)
+        Fref = O.force3(c.g, c.tabs, dt, Vs, Ds)
+        ok, msg = within(F, Fref, _force_bound(c, dt, Vs, Ds))
+        assert ok, msg
+
+
+def test_errors_are_reported():
+    c = case("si2")
+    ctx = GridContext(c.u)
+    from lcpao_grid import GridError
+    with pytest.raises(GridError):
+        ctx.hamiltonian([c.V])          # tables not set
+    ctx.set_tables(c.tabs)
+    with pytest.raises(GridError):
+        ctx.hamiltonian([c.V] * 3)      # nch must be 1, 2 or 4
+    with pytest.raises(GridError):
+        ctx.force3([c.V], [c.DM])       # gradient tables not set
