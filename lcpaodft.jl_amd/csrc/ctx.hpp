@@ -1,0 +1,162 @@
+// ctx.hpp -- context of the grid library shared by its translation units:
+// work-unit records, the lcpao_ctx structure, per-kernel event timing and
+// the HIP error-check macro.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "lcpao_grid.h"
+
+#define UNIT_PTS 2048     // overlap points per H / Force3 unit
+#define RHO_CHUNK 2048    // grid-list entries per density unit
+#define HK 32             // points per LDS chunk in the H kernel
+#define MAXNO 32          // largest orbital count per atom supported
+#define RHO_WAVES 4
+
+namespace lcpao {
+
+struct HUnit { int32_t pair, lo, n, pad; int64_t part; };
+struct RhoUnit { int32_t atom, t0, nt, seg_beg, seg_end, pad; };
+struct Seg { int32_t pair, lo, hi, pad; };
+struct F3Unit { int32_t pair, lo, n, pad; };
+struct PairDev { int32_t a, b, noa, nob; int64_t hoff, olg; int64_t tab_a, tab_b; int32_t gl_a, partner; };
+
+struct SpeciesDev {
+    int32_t maxl, nmesh, no, pad;
+    int32_t num_basis[4];
+    const double *rv;
+    const double *rwf[4][16];
+};
+
+// Block-dense structure of the split-precision forms (built on first use).
+struct BlockPlan;
+
+}  // namespace lcpao
+
+using namespace lcpao;
+
+struct lcpao_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // geometry (host)
+    int natom = 0, cpy = 0, maxno = 0;
+    int ngrid[3] = {0, 0, 0};
+    int64_t nn = 0, total_gridn = 0, total_olg = 0, npair = 0, my_hsize = 0;
+    double gridvol = 0;
+    std::vector<int> no, gridn;
+    std::vector<int64_t> gl_off, tab_off;
+    std::vector<PairDev> pairs;
+    bool have_lattice = false;
+    double lat[9], origin[3];
+    std::vector<double> gxyz;
+    std::vector<int> atom2spe;
+    // device: lists and tables
+    int32_t *d_glist = nullptr, *d_clist = nullptr;
+    int64_t *d_gl_off = nullptr, *d_tab_off = nullptr;
+    int32_t *d_no = nullptr;
+    int32_t *d_olg1 = nullptr, *d_olg2 = nullptr;
+    PairDev *d_pairs = nullptr;
+    double *d_tab = nullptr, *d_dtab[3] = {nullptr, nullptr, nullptr};
+    bool tables = false, dtables = false;
+    // H
+    std::vector<HUnit> hunits;
+    HUnit *d_hunits = nullptr;
+    int32_t *d_horder = nullptr;
+    int32_t *d_hpair_ub = nullptr;  // canonical pair list + unit ranges
+    int32_t *d_hcanon = nullptr;
+    int64_t ncanon = 0, hpart_size = 0;
+    double *d_hpart = nullptr;
+    // rho
+    std::vector<RhoUnit> runits;
+    RhoUnit *d_runits = nullptr;
+    int32_t *d_rorder = nullptr;
+    Seg *d_segs = nullptr;
+    double *d_deff = nullptr, *d_rhoA = nullptr;
+    int64_t *d_ent_off = nullptr;
+    int64_t *d_ent = nullptr;
+    // Force3
+    std::vector<F3Unit> funits;
+    F3Unit *d_funits = nullptr;
+    int32_t *d_forder = nullptr;
+    int32_t *d_atom_ub = nullptr;
+    double *d_fpart = nullptr, *d_fsum = nullptr;
+    // staging for host-pointer entry points
+    double *d_stage = nullptr;
+    int64_t stage_size = 0;
+    int32_t *d_flag = nullptr;
+    // split-precision forms
+    lcpao::BlockPlan *plan = nullptr;
+    float *d_tab32 = nullptr, *d_dtab32[3] = {nullptr, nullptr, nullptr};
+    float *d_deff32 = nullptr;
+    std::vector<int32_t> h_glist, h_clist;  // host copies for the block plan
+    std::vector<int32_t> pair_ncn, numolg, canon;
+    // per-kernel event timing
+    bool own_stream = true;
+    bool profiling = false;
+    struct Pending { int id; hipEvent_t beg, end; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    int64_t prof_launches[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double prof_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+enum { KID_HAM = 0, KID_HRED = 1, KID_DEFF = 2, KID_RHO = 3, KID_RGATHER = 4, KID_F3 = 5, KID_F3RED = 6 };
+
+inline hipEvent_t take_event(lcpao_ctx *ctx) {
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+// Brackets one kernel launch with events when profiling is on.
+struct KTimer {
+    lcpao_ctx *ctx;
+    int id;
+    hipEvent_t beg = nullptr;
+    KTimer(lcpao_ctx *c, int i) : ctx(c), id(i) {
+        if (ctx->profiling) {
+            beg = take_event(ctx);
+            (void)hipEventRecord(beg, ctx->stream);
+        }
+    }
+    ~KTimer() {
+        if (ctx->profiling && beg) {
+            hipEvent_t end = take_event(ctx);
+            (void)hipEventRecord(end, ctx->stream);
+            ctx->pending.push_back({id, beg, end});
+        }
+    }
+};
+
+// After the stream has been synchronised: fold pending events into the counters.
+inline void collect_timings(lcpao_ctx *ctx) {
+    for (auto &p : ctx->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.beg, p.end) == hipSuccess) {
+            ctx->prof_ms[p.id] += ms;
+            ctx->prof_launches[p.id]++;
+        }
+        ctx->event_pool.push_back(p.beg);
+        ctx->event_pool.push_back(p.end);
+    }
+    ctx->pending.clear();
+}
+
+#define HIP_TRY(ctx, call)                                                                     \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                   \
+            return LCPAO_EDEVICE;                                                              \
+        }                                                                                      \
+    } while (0)
+

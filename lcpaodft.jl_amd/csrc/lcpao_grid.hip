@@ -32,150 +32,12 @@
 #include <string>
 #include <vector>
 
-#include "lcpao_grid.h"
-
-#define UNIT_PTS 2048     // overlap points per H / Force3 unit
-#define RHO_CHUNK 2048    // grid-list entries per density unit
-#define HK 32             // points per LDS chunk in the H kernel
-#define MAXNO 32          // largest orbital count per atom supported
-#define RHO_WAVES 4
+#include "ctx.hpp"
+#include "split.hpp"
 
 namespace {
-
 std::string g_create_err;
-
-struct HUnit { int32_t pair, lo, n, pad; int64_t part; };
-struct RhoUnit { int32_t atom, t0, nt, seg_beg, seg_end, pad; };
-struct Seg { int32_t pair, lo, hi, pad; };
-struct F3Unit { int32_t pair, lo, n, pad; };
-struct PairDev { int32_t a, b, noa, nob; int64_t hoff, olg; int64_t tab_a, tab_b; int32_t gl_a, partner; };
-
-struct SpeciesDev {
-    int32_t maxl, nmesh, no, pad;
-    int32_t num_basis[4];
-    const double *rv;
-    const double *rwf[4][16];
-};
-
 }  // namespace
-
-struct lcpao_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    // geometry (host)
-    int natom = 0, cpy = 0, maxno = 0;
-    int ngrid[3] = {0, 0, 0};
-    int64_t nn = 0, total_gridn = 0, total_olg = 0, npair = 0, my_hsize = 0;
-    double gridvol = 0;
-    std::vector<int> no, gridn;
-    std::vector<int64_t> gl_off, tab_off;
-    std::vector<PairDev> pairs;
-    bool have_lattice = false;
-    double lat[9], origin[3];
-    std::vector<double> gxyz;
-    std::vector<int> atom2spe;
-    // device: lists and tables
-    int32_t *d_glist = nullptr, *d_clist = nullptr;
-    int64_t *d_gl_off = nullptr, *d_tab_off = nullptr;
-    int32_t *d_no = nullptr;
-    int32_t *d_olg1 = nullptr, *d_olg2 = nullptr;
-    PairDev *d_pairs = nullptr;
-    double *d_tab = nullptr, *d_dtab[3] = {nullptr, nullptr, nullptr};
-    bool tables = false, dtables = false;
-    // H
-    std::vector<HUnit> hunits;
-    HUnit *d_hunits = nullptr;
-    int32_t *d_horder = nullptr;
-    int32_t *d_hpair_ub = nullptr;  // canonical pair list + unit ranges
-    int32_t *d_hcanon = nullptr;
-    int64_t ncanon = 0, hpart_size = 0;
-    double *d_hpart = nullptr;
-    // rho
-    std::vector<RhoUnit> runits;
-    RhoUnit *d_runits = nullptr;
-    int32_t *d_rorder = nullptr;
-    Seg *d_segs = nullptr;
-    double *d_deff = nullptr, *d_rhoA = nullptr;
-    int64_t *d_ent_off = nullptr;
-    int64_t *d_ent = nullptr;
-    // Force3
-    std::vector<F3Unit> funits;
-    F3Unit *d_funits = nullptr;
-    int32_t *d_forder = nullptr;
-    int32_t *d_atom_ub = nullptr;
-    double *d_fpart = nullptr, *d_fsum = nullptr;
-    // staging for host-pointer entry points
-    double *d_stage = nullptr;
-    int64_t stage_size = 0;
-    int32_t *d_flag = nullptr;
-    // per-kernel event timing
-    bool own_stream = true;
-    bool profiling = false;
-    struct Pending { int id; hipEvent_t beg, end; };
-    std::vector<Pending> pending;
-    std::vector<hipEvent_t> event_pool;
-    int64_t prof_launches[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    double prof_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-};
-
-namespace {
-enum { KID_HAM = 0, KID_HRED = 1, KID_DEFF = 2, KID_RHO = 3, KID_RGATHER = 4, KID_F3 = 5, KID_F3RED = 6 };
-
-hipEvent_t take_event(lcpao_ctx *ctx) {
-    if (!ctx->event_pool.empty()) {
-        hipEvent_t e = ctx->event_pool.back();
-        ctx->event_pool.pop_back();
-        return e;
-    }
-    hipEvent_t e = nullptr;
-    (void)hipEventCreate(&e);
-    return e;
-}
-
-// Brackets one kernel launch with events when profiling is on.
-struct KTimer {
-    lcpao_ctx *ctx;
-    int id;
-    hipEvent_t beg = nullptr;
-    KTimer(lcpao_ctx *c, int i) : ctx(c), id(i) {
-        if (ctx->profiling) {
-            beg = take_event(ctx);
-            (void)hipEventRecord(beg, ctx->stream);
-        }
-    }
-    ~KTimer() {
-        if (ctx->profiling && beg) {
-            hipEvent_t end = take_event(ctx);
-            (void)hipEventRecord(end, ctx->stream);
-            ctx->pending.push_back({id, beg, end});
-        }
-    }
-};
-
-// After the stream has been synchronised: fold pending events into the counters.
-void collect_timings(lcpao_ctx *ctx) {
-    for (auto &p : ctx->pending) {
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, p.beg, p.end) == hipSuccess) {
-            ctx->prof_ms[p.id] += ms;
-            ctx->prof_launches[p.id]++;
-        }
-        ctx->event_pool.push_back(p.beg);
-        ctx->event_pool.push_back(p.end);
-    }
-    ctx->pending.clear();
-}
-}  // namespace
-
-#define HIP_TRY(ctx, call)                                                                     \
-    do {                                                                                       \
-        hipError_t e_ = (call);                                                                \
-        if (e_ != hipSuccess) {                                                                \
-            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                   \
-            return LCPAO_EDEVICE;                                                              \
-        }                                                                                      \
-    } while (0)
 
 // ---------------------------------------------------------------------------
 // Kernels
@@ -278,6 +140,68 @@ __global__ __launch_bounds__(64) void k_ham_f64(const HUnit *__restrict__ units,
     }
 }
 
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+// Hamiltonian unit kernel on the FP64 matrix cores (v_mfma_f64_16x16x4f64).
+// One wave per unit.  The NO_A x NO_B block is held as 2x2 MFMA tiles
+// (rows/cols padded to 32).  Lane l supplies A[l%16][l/16] = phi_A row of
+// point l/16 and B[l/16][l%16] = V*phi_B column, gathered straight from the
+// tables; the 4 points of a step are consecutive overlap points.
+__global__ __launch_bounds__(64) void k_ham_mfma64(const HUnit *__restrict__ units, const int32_t *__restrict__ order,
+                                                   int32_t nunits, const PairDev *__restrict__ pairs,
+                                                   const double *__restrict__ tab, const int32_t *__restrict__ olg1,
+                                                   const int32_t *__restrict__ olg2, const int32_t *__restrict__ glist,
+                                                   const double *__restrict__ V, double *__restrict__ part) {
+    const int b = blockIdx.x;
+    if (b >= nunits) return;
+    const HUnit u = units[order[b]];
+    const PairDev P = pairs[__builtin_amdgcn_readfirstlane(u.pair)];
+    const int noa = __builtin_amdgcn_readfirstlane(P.noa), nob = __builtin_amdgcn_readfirstlane(P.nob);
+    const int lane = threadIdx.x;
+    const int r16 = lane & 15, kq = lane >> 4;
+    const double *tabA = tab + P.tab_a, *tabB = tab + P.tab_b;
+    const int32_t *L1 = olg1 + P.olg + u.lo, *L2 = olg2 + P.olg + u.lo;
+    const int32_t *glA = glist + P.gl_a;
+    d4_t c00 = {0, 0, 0, 0}, c01 = c00, c10 = c00, c11 = c00;
+    const bool ra0 = r16 < noa, ra1 = 16 + r16 < noa, rb0 = r16 < nob, rb1 = 16 + r16 < nob;
+    for (int g0 = 0; g0 < u.n; g0 += 64) {
+        // indices and potential of 64 points, one per lane
+        const int gi = g0 + lane;
+        const bool gok = gi < u.n;
+        const int t1g = gok ? L1[gi] : 0;
+        const int t2g = gok ? L2[gi] : 0;
+        const double vg = gok ? V[glA[t1g]] : 0.0;
+        const int nstep = min(16, (u.n - g0 + 3) >> 2);
+#pragma unroll 4
+        for (int st = 0; st < nstep; st++) {
+            const int src = 4 * st + kq;
+            const int t1 = __shfl(t1g, src, 64);
+            const int t2 = __shfl(t2g, src, 64);
+            const double v = __shfl(vg, src, 64);  // 0 for points past the end
+            const double *ca = tabA + (int64_t)t1 * noa, *cb = tabB + (int64_t)t2 * nob;
+            const double a0 = ra0 ? ca[r16] : 0.0, a1 = ra1 ? ca[16 + r16] : 0.0;
+            const double b0 = rb0 ? cb[r16] * v : 0.0, b1 = rb1 ? cb[16 + r16] * v : 0.0;
+            c00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, c00, 0, 0, 0);
+            c01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, c01, 0, 0, 0);
+            c10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, c10, 0, 0, 0);
+            c11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, c11, 0, 0, 0);
+        }
+    }
+    // v_mfma_f64_16x16x4f64 keeps D[kq + 4v][r16] in lane (16 kq + r16), register v
+    double *out = part + u.part;
+    const d4_t *cs[4] = {&c00, &c01, &c10, &c11};
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int rt = t >> 1, ct = t & 1;
+        const int col = ct * 16 + r16;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = rt * 16 + kq + 4 * v;
+            if (row < noa && col < nob) out[row * nob + col] = (*cs[t])[v];
+        }
+    }
+}
+
 // Sums the unit partials of every computed pair in unit order, scales by
 // GridVol and writes H[P] (and H[P']^T for a folded transpose partner).
 __global__ void k_ham_reduce(const int32_t *__restrict__ canon, const int32_t *__restrict__ ub, int64_t ncanon,
@@ -303,8 +227,9 @@ __global__ void k_ham_reduce(const int32_t *__restrict__ canon, const int32_t *_
 }
 
 // Folded density block of each computed pair: D_P + D_P'^T (or D_P alone).
+template <typename T>
 __global__ void k_deff(const int32_t *__restrict__ canon, int64_t ncanon, const PairDev *__restrict__ pairs,
-                       const double *__restrict__ D, double *__restrict__ Deff) {
+                       const double *__restrict__ D, T *__restrict__ Deff) {
     const int64_t c = blockIdx.x;
     if (c >= ncanon) return;
     const int p = canon[c];
@@ -317,16 +242,75 @@ __global__ void k_deff(const int32_t *__restrict__ canon, int64_t ncanon, const 
             const int i = e / P.nob, j = e - i * P.nob;
             v += D[Q.hoff + (int64_t)j * P.noa + i];
         }
-        Deff[P.hoff + e] = v;
+        Deff[P.hoff + e] = (T)v;
     }
 }
 
-// Density per (atom, chunk) unit.  NB = compile-time bound on NO_B.
-template <int NB>
-__global__ __launch_bounds__(64 * RHO_WAVES) void k_rho_f64(
+// ---- density: point work of one segment -------------------------------------
+// A lane owns one overlap point (two per iteration).  D (the folded pair
+// block) is wave-uniform: with NA, NB known at compile time its loads are
+// batched scalar loads whose values feed v_fma_f64 as SGPR operands.
+template <typename T, int NA, int NB>
+__device__ __forceinline__ void rho_points(const T *__restrict__ D, const T *__restrict__ tabA,
+                                           const T *__restrict__ tabB, const int32_t *__restrict__ L1,
+                                           const int32_t *__restrict__ L2, int lo, int hi, int t0, int lane,
+                                           double *__restrict__ racc) {
+    for (int base = lo; base < hi; base += 128) {
+        const int i0 = base + lane, i1 = base + 64 + lane;
+        const bool ok0 = i0 < hi, ok1 = i1 < hi;
+        const int t1a = L1[ok0 ? i0 : lo], t2a = L2[ok0 ? i0 : lo];
+        const int t1b = L1[ok1 ? i1 : lo], t2b = L2[ok1 ? i1 : lo];
+        const T *cb0 = tabB + (int64_t)t2a * NB, *cb1 = tabB + (int64_t)t2b * NB;
+        const T *ca0 = tabA + (int64_t)t1a * NA, *ca1 = tabA + (int64_t)t1b * NA;
+        T b0[NB], b1[NB];
+#pragma unroll
+        for (int j = 0; j < NB; j++) { b0[j] = cb0[j]; b1[j] = cb1[j]; }
+        T z0 = 0, z1 = 0;
+#pragma unroll
+        for (int i = 0; i < NA; i++) {
+            T y0 = 0, y1 = 0;
+#pragma unroll
+            for (int j = 0; j < NB; j++) {
+                const T d = D[i * NB + j];
+                y0 = fma(d, b0[j], y0);
+                y1 = fma(d, b1[j], y1);
+            }
+            z0 = fma(ca0[i], y0, z0);
+            z1 = fma(ca1[i], y1, z1);
+        }
+        if (ok0) racc[t1a - t0] += (double)z0;
+        if (ok1) racc[t1b - t0] += (double)z1;
+    }
+}
+
+// Runtime-size fallback (any NO <= MAXNO).
+template <typename T>
+__device__ void rho_points_any(const T *__restrict__ D, const T *__restrict__ tabA, const T *__restrict__ tabB,
+                               const int32_t *__restrict__ L1, const int32_t *__restrict__ L2, int lo, int hi,
+                               int t0, int lane, double *__restrict__ racc, int noa, int nob) {
+    for (int base = lo; base < hi; base += 64) {
+        const int idx = base + lane;
+        const bool ok = idx < hi;
+        const int t1 = L1[ok ? idx : lo], t2 = L2[ok ? idx : lo];
+        const T *cb = tabB + (int64_t)t2 * nob, *ca = tabA + (int64_t)t1 * noa;
+        T z = 0;
+        for (int i = 0; i < noa; i++) {
+            T y = 0;
+            for (int j = 0; j < nob; j++) y = fma(D[i * nob + j], cb[j], y);
+            z = fma(ca[i], y, z);
+        }
+        if (ok) racc[t1 - t0] += (double)z;
+    }
+}
+
+#define NO_CASES(X) X(13, 13) X(13, 14) X(14, 13) X(14, 14) X(19, 19) X(13, 19) X(19, 13) X(14, 19) X(19, 14)
+
+// Density per (atom, chunk) unit; T = table/D element type (double or float).
+template <typename T>
+__global__ __launch_bounds__(64 * RHO_WAVES) void k_rho(
     const RhoUnit *__restrict__ units, const int32_t *__restrict__ order, int32_t nunits,
-    const Seg *__restrict__ segs, const PairDev *__restrict__ pairs, const double *__restrict__ tab,
-    const int32_t *__restrict__ olg1, const int32_t *__restrict__ olg2, const double *__restrict__ Deff,
+    const Seg *__restrict__ segs, const PairDev *__restrict__ pairs, const T *__restrict__ tab,
+    const int32_t *__restrict__ olg1, const int32_t *__restrict__ olg2, const T *__restrict__ Deff,
     const int64_t *__restrict__ gl_off, double *__restrict__ rhoA) {
     __shared__ double racc[RHO_WAVES][RHO_CHUNK];
     const int b = blockIdx.x;
@@ -339,38 +323,25 @@ __global__ __launch_bounds__(64 * RHO_WAVES) void k_rho_f64(
         const Seg sg = segs[s];
         const PairDev P = pairs[__builtin_amdgcn_readfirstlane(sg.pair)];
         const int noa = __builtin_amdgcn_readfirstlane(P.noa), nob = __builtin_amdgcn_readfirstlane(P.nob);
-        const double *D = Deff + P.hoff;
-        const double *tabA = tab + P.tab_a, *tabB = tab + P.tab_b;
+        const T *D = Deff + P.hoff;
+        const T *tabA = tab + P.tab_a, *tabB = tab + P.tab_b;
         const int32_t *L1 = olg1 + P.olg, *L2 = olg2 + P.olg;
-        for (int base = sg.lo; base < sg.hi; base += 64) {
-            const int idx = base + lane;
-            const bool ok = idx < sg.hi;
-            const int t1 = ok ? L1[idx] : -1;
-            const int t2 = ok ? L2[idx] : 0;
-            double pb[NB];
-            const double *colB = tabB + (int64_t)t2 * nob;
-#pragma unroll
-            for (int j = 0; j < NB; j++) pb[j] = (ok && j < nob) ? colB[j] : 0.0;
-            const double *colA = tabA + (int64_t)(ok ? t1 : 0) * noa;
-            double z = 0.0;
-            for (int i = 0; i < noa; i++) {
-                const double *Di = D + i * nob;
-                double y = 0.0;
-#pragma unroll
-                for (int j = 0; j < NB; j++)
-                    if (j < nob) y = fma(Di[j], pb[j], y);
-                z = fma(ok ? colA[i] : 0.0, y, z);
-            }
-            if (ok) racc[wave][t1 - u.t0] += z;
-        }
+#define RHO_CASE(A, B)                                                                                 \
+    if (noa == A && nob == B) {                                                                        \
+        rho_points<T, A, B>(D, tabA, tabB, L1, L2, sg.lo, sg.hi, u.t0, lane, racc[wave]);              \
+        continue;                                                                                      \
+    }
+        NO_CASES(RHO_CASE)
+#undef RHO_CASE
+        rho_points_any<T>(D, tabA, tabB, L1, L2, sg.lo, sg.hi, u.t0, lane, racc[wave], noa, nob);
     }
     __syncthreads();
     double *out = rhoA + gl_off[u.atom] + u.t0;
     for (int t = threadIdx.x; t < u.nt; t += blockDim.x) {
-        double s = racc[0][t];
+        double acc = racc[0][t];
 #pragma unroll
-        for (int w = 1; w < RHO_WAVES; w++) s += racc[w][t];
-        out[t] = s;
+        for (int w = 1; w < RHO_WAVES; w++) acc += racc[w][t];
+        out[t] = acc;
     }
 }
 
@@ -384,15 +355,75 @@ __global__ void k_rho_gather(const int64_t *__restrict__ ent_off, const int64_t 
     rho[g] = s;
 }
 
-// Force3 units, FP64: lane = overlap point.
-template <int NB>
-__global__ __launch_bounds__(64) void k_f3_f64(const F3Unit *__restrict__ units, const int32_t *__restrict__ order,
-                                               int32_t nunits, const PairDev *__restrict__ pairs,
-                                               const double *__restrict__ tab, const double *__restrict__ dx,
-                                               const double *__restrict__ dy, const double *__restrict__ dz,
-                                               const int32_t *__restrict__ olg1, const int32_t *__restrict__ olg2,
-                                               const int32_t *__restrict__ glist, const double *__restrict__ V,
-                                               const double *__restrict__ D, double *__restrict__ fpart) {
+// ---- Force3: point work of one unit (lane = overlap point) --------------------
+template <typename T, int NA, int NB>
+__device__ __forceinline__ void f3_points(const T *__restrict__ D, const T *__restrict__ tabA,
+                                          const T *__restrict__ tabB, const T *__restrict__ dx,
+                                          const T *__restrict__ dy, const T *__restrict__ dz,
+                                          const int32_t *__restrict__ L1, const int32_t *__restrict__ L2,
+                                          const int32_t *__restrict__ glA, const double *__restrict__ V, int n,
+                                          int lane, double &sx, double &sy, double &sz) {
+    for (int base = 0; base < n; base += 64) {
+        const int idx = base + lane;
+        const bool ok = idx < n;
+        const int t1 = L1[ok ? idx : 0], t2 = L2[ok ? idx : 0];
+        const T *cb = tabB + (int64_t)t2 * NB;
+        const int64_t ca = (int64_t)t1 * NA;
+        T bv[NB];
+#pragma unroll
+        for (int j = 0; j < NB; j++) bv[j] = cb[j];
+        T lx = 0, ly = 0, lz = 0;
+#pragma unroll
+        for (int i = 0; i < NA; i++) {
+            T y = 0;
+#pragma unroll
+            for (int j = 0; j < NB; j++) y = fma(D[i * NB + j], bv[j], y);
+            lx = fma(dx[ca + i], y, lx);
+            ly = fma(dy[ca + i], y, ly);
+            lz = fma(dz[ca + i], y, lz);
+        }
+        const double pot = ok ? V[glA[t1]] : 0.0;
+        sx = fma(pot, (double)lx, sx);
+        sy = fma(pot, (double)ly, sy);
+        sz = fma(pot, (double)lz, sz);
+    }
+}
+
+template <typename T>
+__device__ void f3_points_any(const T *__restrict__ D, const T *__restrict__ tabA, const T *__restrict__ tabB,
+                              const T *__restrict__ dx, const T *__restrict__ dy, const T *__restrict__ dz,
+                              const int32_t *__restrict__ L1, const int32_t *__restrict__ L2,
+                              const int32_t *__restrict__ glA, const double *__restrict__ V, int n, int lane,
+                              double &sx, double &sy, double &sz, int noa, int nob) {
+    for (int base = 0; base < n; base += 64) {
+        const int idx = base + lane;
+        const bool ok = idx < n;
+        const int t1 = L1[ok ? idx : 0], t2 = L2[ok ? idx : 0];
+        const T *cb = tabB + (int64_t)t2 * nob;
+        const int64_t ca = (int64_t)t1 * noa;
+        T lx = 0, ly = 0, lz = 0;
+        for (int i = 0; i < noa; i++) {
+            T y = 0;
+            for (int j = 0; j < nob; j++) y = fma(D[i * nob + j], cb[j], y);
+            lx = fma(dx[ca + i], y, lx);
+            ly = fma(dy[ca + i], y, ly);
+            lz = fma(dz[ca + i], y, lz);
+        }
+        const double pot = ok ? V[glA[t1]] : 0.0;
+        sx = fma(pot, (double)lx, sx);
+        sy = fma(pot, (double)ly, sy);
+        sz = fma(pot, (double)lz, sz);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_f3(const F3Unit *__restrict__ units, const int32_t *__restrict__ order,
+                                           int32_t nunits, const PairDev *__restrict__ pairs,
+                                           const T *__restrict__ tab, const T *__restrict__ dx,
+                                           const T *__restrict__ dy, const T *__restrict__ dz,
+                                           const int32_t *__restrict__ olg1, const int32_t *__restrict__ olg2,
+                                           const int32_t *__restrict__ glist, const double *__restrict__ V,
+                                           const T *__restrict__ D, double *__restrict__ fpart) {
     const int b = blockIdx.x;
     if (b >= nunits) return;
     const int ui = order[b];
@@ -400,38 +431,22 @@ __global__ __launch_bounds__(64) void k_f3_f64(const F3Unit *__restrict__ units,
     const PairDev P = pairs[__builtin_amdgcn_readfirstlane(u.pair)];
     const int noa = __builtin_amdgcn_readfirstlane(P.noa), nob = __builtin_amdgcn_readfirstlane(P.nob);
     const int lane = threadIdx.x;
-    const double *Dp = D + P.hoff;
+    const T *Dp = D + P.hoff;
     const int32_t *L1 = olg1 + P.olg + u.lo, *L2 = olg2 + P.olg + u.lo;
     const int32_t *glA = glist + P.gl_a;
+    const T *tA = tab + P.tab_a, *tB = tab + P.tab_b;
+    const T *ax = dx + P.tab_a, *ay = dy + P.tab_a, *az = dz + P.tab_a;
     double sx = 0.0, sy = 0.0, sz = 0.0;
-    for (int base = 0; base < u.n; base += 64) {
-        const int idx = base + lane;
-        const bool ok = idx < u.n;
-        const int t1 = ok ? L1[idx] : 0;
-        const int t2 = ok ? L2[idx] : 0;
-        double pb[NB];
-        const double *colB = tab + P.tab_b + (int64_t)t2 * nob;
-#pragma unroll
-        for (int j = 0; j < NB; j++) pb[j] = (ok && j < nob) ? colB[j] : 0.0;
-        const int64_t ca = P.tab_a + (int64_t)t1 * noa;
-        double lx = 0.0, ly = 0.0, lz = 0.0;
-        for (int i = 0; i < noa; i++) {
-            const double *Di = Dp + i * nob;
-            double y = 0.0;
-#pragma unroll
-            for (int j = 0; j < NB; j++)
-                if (j < nob) y = fma(Di[j], pb[j], y);
-            lx = fma(dx[ca + i], y, lx);
-            ly = fma(dy[ca + i], y, ly);
-            lz = fma(dz[ca + i], y, lz);
-        }
-        const double pot = ok ? V[glA[t1]] : 0.0;
-        sx = fma(pot, lx, sx);
-        sy = fma(pot, ly, sy);
-        sz = fma(pot, lz, sz);
+    bool done = false;
+#define F3_CASE(A, B)                                                                              \
+    if (!done && noa == A && nob == B) {                                                           \
+        f3_points<T, A, B>(Dp, tA, tB, ax, ay, az, L1, L2, glA, V, u.n, lane, sx, sy, sz);         \
+        done = true;                                                                               \
     }
-    // wave reduction in a fixed butterfly order
-    for (int off = 32; off >= 1; off >>= 1) {
+    NO_CASES(F3_CASE)
+#undef F3_CASE
+    if (!done) f3_points_any<T>(Dp, tA, tB, ax, ay, az, L1, L2, glA, V, u.n, lane, sx, sy, sz, noa, nob);
+    for (int off = 32; off >= 1; off >>= 1) {  // fixed butterfly order
         sx += __shfl_xor(sx, off, 64);
         sy += __shfl_xor(sy, off, 64);
         sz += __shfl_xor(sz, off, 64);
@@ -789,7 +804,8 @@ int build_work(lcpao_ctx *ctx, const lcpao_geom *g, const std::vector<int32_t> &
         ctx->pairs[p].partner = (it != v.end() && it->first == k) ? it->second : -1;
     }
     // computed pairs: partner absent or not smaller (the transpose is then filled)
-    std::vector<int32_t> canon;
+    std::vector<int32_t> &canon = ctx->canon;
+    canon.clear();
     for (int64_t p = 0; p < np; p++) {
         const int q = ctx->pairs[p].partner;
         if (q < 0 || q >= p) canon.push_back((int32_t)p);
@@ -898,23 +914,24 @@ int stage(lcpao_ctx *ctx, int64_t ndoubles) {
 }
 
 int check_precision(lcpao_ctx *ctx, int precision) {
-    if (precision != LCPAO_FP64) return set_err(ctx, LCPAO_EINVAL, "precision form not available in this build");
+    if (precision != LCPAO_FP64 && precision != LCPAO_FP32 && precision != LCPAO_BF16X3)
+        return set_err(ctx, LCPAO_EINVAL, "precision must be LCPAO_FP64, LCPAO_FP32 or LCPAO_BF16X3");
     return LCPAO_OK;
 }
 
-int nb_bound(int maxno) { return maxno <= 16 ? 16 : (maxno <= 20 ? 20 : 32); }
 
 int ham_dev(lcpao_ctx *ctx, int nch, const double *const *d_V, double *const *d_H, int precision) {
     if (!ctx->tables) return set_err(ctx, LCPAO_ESTATE, "orbital tables are not set");
     if (nch != 1 && nch != 2 && nch != 4) return set_err(ctx, LCPAO_EINVAL, "nch must be 1, 2 or 4");
     int rc = check_precision(ctx, precision);
     if (rc) return rc;
+    if (precision != LCPAO_FP64) return split_hamiltonian(ctx, nch, d_V, d_H, precision);
     const int nu = (int)ctx->hunits.size();
     for (int c = 0; c < nch; c++) {
         if (!d_V[c] || !d_H[c]) return set_err(ctx, LCPAO_EINVAL, "null channel pointer");
         if (nu > 0) {
             KTimer kt(ctx, KID_HAM);
-            hipLaunchKernelGGL(k_ham_f64, dim3(nu), dim3(64), 0, ctx->stream, ctx->d_hunits, ctx->d_horder, nu,
+            hipLaunchKernelGGL(k_ham_mfma64, dim3(nu), dim3(64), 0, ctx->stream, ctx->d_hunits, ctx->d_horder, nu,
                                ctx->d_pairs, ctx->d_tab, ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c], ctx->d_hpart);
         }
         if (ctx->ncanon > 0) {
@@ -936,26 +953,31 @@ int rho_dev(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_off
     if (dm_offset < 0) return set_err(ctx, LCPAO_EINVAL, "negative dm_offset");
     int rc = check_precision(ctx, precision);
     if (rc) return rc;
+    const bool f32 = precision != LCPAO_FP64;
+    if (f32 && (rc = fp32_tables(ctx, false))) return rc;
     const int nu = (int)ctx->runits.size();
-    const int nb = nb_bound(ctx->maxno);
     for (int s = 0; s < nspin; s++) {
         if (!d_DM[s] || !d_rho[s]) return set_err(ctx, LCPAO_EINVAL, "null spin pointer");
         const double *D = d_DM[s] + dm_offset;
         if (ctx->ncanon > 0) {
             KTimer kt(ctx, KID_DEFF);
-            hipLaunchKernelGGL(k_deff, dim3((unsigned)ctx->ncanon), dim3(128), 0, ctx->stream, ctx->d_hcanon, ctx->ncanon,
-                               ctx->d_pairs, D, ctx->d_deff);
+            if (f32)
+                hipLaunchKernelGGL(k_deff<float>, dim3((unsigned)ctx->ncanon), dim3(128), 0, ctx->stream, ctx->d_hcanon,
+                                   ctx->ncanon, ctx->d_pairs, D, ctx->d_deff32);
+            else
+                hipLaunchKernelGGL(k_deff<double>, dim3((unsigned)ctx->ncanon), dim3(128), 0, ctx->stream, ctx->d_hcanon,
+                                   ctx->ncanon, ctx->d_pairs, D, ctx->d_deff);
         }
         if (nu > 0) {
             KTimer kt(ctx, KID_RHO);
-#define RHO_LAUNCH(NB)                                                                                         \
-    hipLaunchKernelGGL(k_rho_f64<NB>, dim3(nu), dim3(64 * RHO_WAVES), 0, ctx->stream, ctx->d_runits, ctx->d_rorder, \
-                       nu, ctx->d_segs, ctx->d_pairs, ctx->d_tab, ctx->d_olg1, ctx->d_olg2, ctx->d_deff,        \
-                       ctx->d_gl_off, ctx->d_rhoA)
-            if (nb == 16) RHO_LAUNCH(16);
-            else if (nb == 20) RHO_LAUNCH(20);
-            else RHO_LAUNCH(32);
-#undef RHO_LAUNCH
+            if (f32)
+                hipLaunchKernelGGL(k_rho<float>, dim3(nu), dim3(64 * RHO_WAVES), 0, ctx->stream, ctx->d_runits,
+                                   ctx->d_rorder, nu, ctx->d_segs, ctx->d_pairs, ctx->d_tab32, ctx->d_olg1, ctx->d_olg2,
+                                   ctx->d_deff32, ctx->d_gl_off, ctx->d_rhoA);
+            else
+                hipLaunchKernelGGL(k_rho<double>, dim3(nu), dim3(64 * RHO_WAVES), 0, ctx->stream, ctx->d_runits,
+                                   ctx->d_rorder, nu, ctx->d_segs, ctx->d_pairs, ctx->d_tab, ctx->d_olg1, ctx->d_olg2,
+                                   ctx->d_deff, ctx->d_gl_off, ctx->d_rhoA);
         }
         const unsigned gb = (unsigned)((ctx->nn + 255) / 256);
         {
@@ -977,22 +999,24 @@ int f3_dev(lcpao_ctx *ctx, int nch, const double *const *d_V, const double *cons
     if (!weight || !d_force) return set_err(ctx, LCPAO_EINVAL, "null weight or force pointer");
     int rc = check_precision(ctx, precision);
     if (rc) return rc;
+    const bool f32 = precision != LCPAO_FP64;
+    if (f32 && (rc = fp32_tables(ctx, true))) return rc;
     const int nu = (int)ctx->funits.size();
-    const int nb = nb_bound(ctx->maxno);
     HIP_TRY(ctx, hipMemsetAsync(ctx->d_fsum, 0, (size_t)ctx->natom * 3 * sizeof(double), ctx->stream));
     for (int c = 0; c < nch; c++) {
         if (!d_V[c] || !d_DM[c]) return set_err(ctx, LCPAO_EINVAL, "null channel pointer");
         const double *D = d_DM[c] + dm_offset;
+        if (f32) launch_to_f32(ctx->stream, ctx->my_hsize, D, ctx->d_deff32);
         if (nu > 0) {
             KTimer kt(ctx, KID_F3);
-#define F3_LAUNCH(NB)                                                                                             \
-    hipLaunchKernelGGL(k_f3_f64<NB>, dim3(nu), dim3(64), 0, ctx->stream, ctx->d_funits, ctx->d_forder, nu, ctx->d_pairs, \
-                       ctx->d_tab, ctx->d_dtab[0], ctx->d_dtab[1], ctx->d_dtab[2], ctx->d_olg1, ctx->d_olg2,          \
-                       ctx->d_glist, d_V[c], D, ctx->d_fpart)
-            if (nb == 16) F3_LAUNCH(16);
-            else if (nb == 20) F3_LAUNCH(20);
-            else F3_LAUNCH(32);
-#undef F3_LAUNCH
+            if (f32)
+                hipLaunchKernelGGL(k_f3<float>, dim3(nu), dim3(64), 0, ctx->stream, ctx->d_funits, ctx->d_forder, nu,
+                                   ctx->d_pairs, ctx->d_tab32, ctx->d_dtab32[0], ctx->d_dtab32[1], ctx->d_dtab32[2],
+                                   ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c], ctx->d_deff32, ctx->d_fpart);
+            else
+                hipLaunchKernelGGL(k_f3<double>, dim3(nu), dim3(64), 0, ctx->stream, ctx->d_funits, ctx->d_forder, nu,
+                                   ctx->d_pairs, ctx->d_tab, ctx->d_dtab[0], ctx->d_dtab[1], ctx->d_dtab[2],
+                                   ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c], D, ctx->d_fpart);
         }
         if (nu > 0) {
             KTimer kt(ctx, KID_F3RED);
@@ -1017,6 +1041,10 @@ void free_all(lcpao_ctx *ctx) {
                     ctx->d_atom_ub, ctx->d_fpart, ctx->d_fsum, ctx->d_stage, ctx->d_flag};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    invalidate_split_tables(ctx);
+    if (ctx->d_deff32) (void)hipFree(ctx->d_deff32);
+    plan_free(ctx->plan);
+    ctx->plan = nullptr;
     for (auto &p : ctx->pending) { (void)hipEventDestroy(p.beg); (void)hipEventDestroy(p.end); }
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     if (ctx->stream && ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
@@ -1123,6 +1151,8 @@ int lcpao_grid_create(lcpao_ctx **out, const lcpao_geom *g, int device) {
         if (!sorted) std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return L1[x] < L1[y]; });
         for (int k = 0; k < n; k++) { o1[P.olg + k] = L1[idx[k]]; o2[P.olg + k] = L2[idx[k]]; }
     }
+    ctx->pair_ncn.assign(g->pair_ncn, g->pair_ncn + g->npairs);
+    ctx->numolg.assign(g->numolg, g->numolg + g->npairs);
     int rc;
     if ((rc = upload(ctx, &ctx->d_glist, glist))) return fail(rc, "");
     if (!clist.empty() && (rc = upload(ctx, &ctx->d_clist, clist))) return fail(rc, "");
@@ -1132,6 +1162,8 @@ int lcpao_grid_create(lcpao_ctx **out, const lcpao_geom *g, int device) {
     if ((rc = upload(ctx, &ctx->d_olg1, o1))) return fail(rc, "");
     if ((rc = upload(ctx, &ctx->d_olg2, o2))) return fail(rc, "");
     if ((rc = build_work(ctx, g, o1, o2, glist))) return fail(rc, "");
+    ctx->h_glist = std::move(glist);
+    ctx->h_clist = std::move(clist);
     if ((rc = upload(ctx, &ctx->d_pairs, ctx->pairs))) return fail(rc, "");
     if ((rc = dev_alloc(ctx, (void **)&ctx->d_flag, sizeof(int32_t)))) return fail(rc, "");
     if (g->latvecs && g->origin && g->gxyz && g->atom2spe && g->cell_list) {
@@ -1208,6 +1240,7 @@ static int upload_tables(lcpao_ctx *ctx, double **dst, const double *const *src)
 int lcpao_grid_set_tables(lcpao_ctx *ctx, const double *const *phi) {
     int rc = check_ctx(ctx);
     if (rc) return rc;
+    invalidate_split_tables(ctx);
     if ((rc = upload_tables(ctx, &ctx->d_tab, phi))) return rc;
     ctx->tables = true;
     return LCPAO_OK;
@@ -1216,6 +1249,7 @@ int lcpao_grid_set_tables(lcpao_ctx *ctx, const double *const *phi) {
 int lcpao_grid_set_dtables(lcpao_ctx *ctx, const double *const *dx, const double *const *dy, const double *const *dz) {
     int rc = check_ctx(ctx);
     if (rc) return rc;
+    invalidate_split_tables(ctx);
     if ((rc = upload_tables(ctx, &ctx->d_dtab[0], dx))) return rc;
     if ((rc = upload_tables(ctx, &ctx->d_dtab[1], dy))) return rc;
     if ((rc = upload_tables(ctx, &ctx->d_dtab[2], dz))) return rc;
@@ -1302,6 +1336,7 @@ int lcpao_grid_build_tables(lcpao_ctx *ctx, int32_t nspecies, const lcpao_pao *p
             tg.origin[j] = ctx->origin[j];
         }
         tg.n2 = ctx->ngrid[1]; tg.n3 = ctx->ngrid[2]; tg.natom = ctx->natom;
+        invalidate_split_tables(ctx);
         (void)hipMemsetAsync(ctx->d_flag, 0, sizeof(int32_t), ctx->stream);
         const int64_t nent = ctx->total_gridn;
         const unsigned nb = (unsigned)((nent + 127) / 128);

@@ -1,0 +1,181 @@
+# LCPAOGridHIP.jl -- drop-in methods for LCPAODFT.jl's real-space grid path,
+# calling liblcpao_grid.so (MI355X / gfx950) through `ccall`.
+#
+# Meant to be `include`d inside `module LCPAODFT` after the reference's own
+# definitions: each method below has the reference's exact signature and
+# replaces its body.  Untested here: Julia is not available in the build
+# environment; the C ABI underneath is tested from Python (tests/).
+#
+# Replaced methods (reference file:line):
+#   Calc_MatrixElements_dVH_Vxc_{off,on,nc}!   Hamiltonian.jl:235,280,327
+#   Set_Density_Grid_{nonpol,pol,nc}!          Set_Density_Grid.jl:60,112,172
+#   Force3_{nospin,spin}, Force3_nc            Force.jl:407,454,506
+# Force3 receives the nested DM_Vec in the reference; the wrapper flattens it
+# back to the global flat layout (Convert_Matrix.jl:1-24 order).
+
+const LIBGRID = get(ENV, "LCPAO_GRID_LIB", joinpath(@__DIR__, "..", "lcpao_grid", "liblcpao_grid.so"))
+const GRID_PRECISION = Ref{Int32}(0)   # 0 FP64, 1 FP32, 2 split-BF16x3
+
+struct LcpaoGeom
+    natom::Int32
+    atom_base::Int32
+    ngrid::NTuple{3,Int32}
+    cpycell::Int32
+    gridvol::Float64
+    no::Ptr{Int32}
+    gridn::Ptr{Int32}
+    grid_list::Ptr{Ptr{Int32}}
+    cell_list::Ptr{Ptr{Int32}}
+    npairs::Int64
+    pair_atom::Ptr{Int32}
+    pair_natn::Ptr{Int32}
+    pair_ncn::Ptr{Int32}
+    numolg::Ptr{Int32}
+    olg1::Ptr{Ptr{Int32}}
+    olg2::Ptr{Ptr{Int32}}
+    latvecs::Ptr{Float64}
+    origin::Ptr{Float64}
+    gxyz::Ptr{Float64}
+    atom2spe::Ptr{Int32}
+end
+
+mutable struct GridHandle
+    ptr::Ptr{Cvoid}
+    orbs::Any       # the Orbs_Grid whose tables were uploaded
+    dorbs::Any
+end
+
+const GRID_CTX = IdDict{Any,GridHandle}()
+
+function _grid_check(rc, ctx)
+    rc == 0 && return nothing
+    msg = unsafe_string(ccall((:lcpao_grid_last_error, LIBGRID), Cstring, (Ptr{Cvoid},), ctx))
+    error("lcpao_grid error $rc: $msg")
+end
+
+function _grid_context(ucell::UCell)
+    h = get(GRID_CTX, ucell, nothing)
+    h === nothing || return h
+    sg = ucell.system_grid
+    glp = [pointer(v) for v in ucell.GridListAtom]
+    clp = [pointer(v) for v in ucell.CellListAtom]
+    o1p = [pointer(v) for v in ucell.MPI_GListTAtoms1]
+    o2p = [pointer(v) for v in ucell.MPI_GListTAtoms2]
+    lat = Matrix{Float64}(transpose(sg.Latvecs))       # row-major 3x3
+    gxyz = reduce(vcat, sg.Gxyz)
+    a2s = Int32.(sg.atom2spe .- 1)
+    no = Int32.(sg.Total_NumOrbs)
+    ctxref = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve glp clp o1p o2p lat gxyz a2s no ucell begin
+        g = LcpaoGeom(sg.Natom, 1, Int32.(sg.Ngrid), sg.CpyCell, sg.GridVol, pointer(no),
+                      pointer(ucell.GridN_Atom), pointer(glp), pointer(clp), sg.MPI_size,
+                      pointer(sg.MPI_atom), pointer(sg.MPI_natn), pointer(sg.MPI_ncn),
+                      pointer(ucell.MPI_NumOLG), pointer(o1p), pointer(o2p),
+                      pointer(lat), pointer(sg.Grid_Origin), pointer(gxyz), pointer(a2s))
+        rc = ccall((:lcpao_grid_create, LIBGRID), Cint, (Ref{Ptr{Cvoid}}, Ref{LcpaoGeom}, Cint), ctxref, g, -1)
+        _grid_check(rc, C_NULL)
+    end
+    h = GridHandle(ctxref[], nothing, nothing)
+    finalizer(x -> ccall((:lcpao_grid_destroy, LIBGRID), Cvoid, (Ptr{Cvoid},), x.ptr), h)
+    GRID_CTX[ucell] = h
+    return h
+end
+
+function _grid_tables!(h::GridHandle, Orbs_Grid)
+    h.orbs === Orbs_Grid && return nothing
+    ptrs = [pointer(m) for m in Orbs_Grid.data]
+    GC.@preserve Orbs_Grid ptrs begin
+        _grid_check(ccall((:lcpao_grid_set_tables, LIBGRID), Cint, (Ptr{Cvoid}, Ptr{Ptr{Float64}}), h.ptr, ptrs), h.ptr)
+    end
+    h.orbs = Orbs_Grid
+    return nothing
+end
+
+function _grid_hks!(ucell, Orbs_Grid, Vpot_Grid, MPI_Hks, nch)
+    h = _grid_context(ucell)
+    _grid_tables!(h, Orbs_Grid)
+    vp = [pointer(Vpot_Grid[c]) for c in 1:nch]
+    hp = [pointer(MPI_Hks[c]) for c in 1:nch]
+    GC.@preserve Vpot_Grid MPI_Hks vp hp begin
+        rc = ccall((:lcpao_grid_hamiltonian, LIBGRID), Cint,
+                   (Ptr{Cvoid}, Int32, Ptr{Ptr{Float64}}, Ptr{Ptr{Float64}}, Int32), h.ptr, nch, vp, hp, GRID_PRECISION[])
+        _grid_check(rc, h.ptr)
+    end
+    return nothing
+end
+
+Calc_MatrixElements_dVH_Vxc_off!(ucell::UCell, Orbs_Grid, Vpot_Grid::Vector{Vector{Float64}}, MPI_Hks) =
+    _grid_hks!(ucell, Orbs_Grid, Vpot_Grid, MPI_Hks, 1)
+Calc_MatrixElements_dVH_Vxc_on!(ucell::UCell, Orbs_Grid, Vpot_Grid::Vector{Vector{Float64}}, MPI_Hks) =
+    _grid_hks!(ucell, Orbs_Grid, Vpot_Grid, MPI_Hks, 2)
+Calc_MatrixElements_dVH_Vxc_nc!(ucell::UCell, Orbs_Grid, Vpot_Grid::Vector{Vector{Float64}}, MPI_Hks) =
+    _grid_hks!(ucell, Orbs_Grid, Vpot_Grid, MPI_Hks, 4)
+
+function _grid_density!(ucell, Orbs_Grid, DM, Density_Grid, nspin)
+    comm = MPI.COMM_WORLD
+    myrank = MPI.Comm_rank(comm)
+    h = _grid_context(ucell)
+    _grid_tables!(h, Orbs_Grid)
+    dp = [pointer(DM[s]) for s in 1:nspin]
+    rp = [pointer(Density_Grid[s]) for s in 1:nspin]
+    offset = Int64(ucell.system_grid.MPHks[myrank+1])
+    GC.@preserve DM Density_Grid dp rp begin
+        rc = ccall((:lcpao_grid_density, LIBGRID), Cint,
+                   (Ptr{Cvoid}, Int32, Ptr{Ptr{Float64}}, Int64, Ptr{Ptr{Float64}}, Int32),
+                   h.ptr, nspin, dp, offset, rp, GRID_PRECISION[])
+        _grid_check(rc, h.ptr)
+    end
+    for s in 1:nspin
+        MPI.Allreduce!(Density_Grid[s], MPI.SUM, comm)   # Set_Density_Grid.jl:108
+    end
+    return nothing
+end
+
+Set_Density_Grid_nonpol!(ucell::UCell, Orbs_Grid, DM, Density_Grid) = _grid_density!(ucell, Orbs_Grid, DM, Density_Grid, 1)
+Set_Density_Grid_pol!(ucell::UCell, Orbs_Grid, DM, Density_Grid) = _grid_density!(ucell, Orbs_Grid, DM, Density_Grid, 2)
+Set_Density_Grid_nc!(ucell::UCell, Orbs_Grid, DM, Density_Grid) = _grid_density!(ucell, Orbs_Grid, DM, Density_Grid, 4)
+
+# DM_Vec[s][atom][Rn][ist][jst] -> global flat DM[s] (Convert_Matrix.jl:1-24)
+_flat_dm(DM_Vec, ucell) = Set_DM_Vec2DM(DM_Vec, ucell.system_grid)
+
+function _grid_force3(pao, Orbs_Grid, Vs, DM_Vec, ucell, weights)
+    comm = MPI.COMM_WORLD
+    myrank = MPI.Comm_rank(comm)
+    h = _grid_context(ucell)
+    _grid_tables!(h, Orbs_Grid)
+    if h.dorbs === nothing
+        dOrbs = Set_dOrbitals_Grid(pao, ucell)         # Force.jl:427
+        dx = [pointer(m) for m in dOrbs[1].data]
+        dy = [pointer(m) for m in dOrbs[2].data]
+        dz = [pointer(m) for m in dOrbs[3].data]
+        GC.@preserve dOrbs dx dy dz begin
+            rc = ccall((:lcpao_grid_set_dtables, LIBGRID), Cint,
+                       (Ptr{Cvoid}, Ptr{Ptr{Float64}}, Ptr{Ptr{Float64}}, Ptr{Ptr{Float64}}), h.ptr, dx, dy, dz)
+            _grid_check(rc, h.ptr)
+        end
+        h.dorbs = dOrbs
+    end
+    DM = _flat_dm(DM_Vec, ucell)
+    nch = length(Vs)
+    VpotForce = zeros(Float64, ucell.system_grid.Natom, 3)
+    vp = [pointer(Vs[c]) for c in 1:nch]
+    dp = [pointer(DM[c]) for c in 1:nch]
+    w = Float64.(weights)
+    offset = Int64(ucell.system_grid.MPHks[myrank+1])
+    GC.@preserve Vs DM vp dp w VpotForce begin
+        rc = ccall((:lcpao_grid_force3, LIBGRID), Cint,
+                   (Ptr{Cvoid}, Int32, Ptr{Ptr{Float64}}, Ptr{Ptr{Float64}}, Int64, Ptr{Float64}, Ptr{Float64}, Int32),
+                   h.ptr, nch, vp, dp, offset, w, VpotForce, GRID_PRECISION[])
+        _grid_check(rc, h.ptr)
+    end
+    MPI.Allreduce!(VpotForce, MPI.SUM, comm)          # Force.jl:447
+    return VpotForce
+end
+
+Force3_nospin(pao::Vector{PAO}, Orbs_Grid, Vpot_Grid, DM, ucell::UCell) =
+    _grid_force3(pao, Orbs_Grid, [Vpot_Grid[1]], DM, ucell, (4.0,))
+Force3_spin(pao::Vector{PAO}, Orbs_Grid, Vpot_Grid, DM, ucell::UCell) =
+    _grid_force3(pao, Orbs_Grid, [Vpot_Grid[1], Vpot_Grid[2]], DM, ucell, (2.0, 2.0))
+Force3_nc(pao::Vector{PAO}, Orbs_Grid, dVHart_Grid, Vxc_Grid, DM, ucell::UCell) =
+    _grid_force3(pao, Orbs_Grid, [dVHart_Grid + Vxc_Grid[1], dVHart_Grid + Vxc_Grid[2], Vxc_Grid[3], Vxc_Grid[4]],
+                 DM, ucell, (2.0, 2.0, 4.0, -4.0))

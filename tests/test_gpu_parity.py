@@ -139,3 +139,61 @@ def test_errors_are_reported():
         ctx.hamiltonian([c.V] * 3)      # nch must be 1, 2 or 4
     with pytest.raises(GridError):
         ctx.force3([c.V], [c.DM])       # gradient tables not set
+
+
+# ---- split-precision forms --------------------------------------------------
+# Tolerances, relative to the entry's own sum of absolute terms (see
+# tests/tolerance.py): bf16x3 keeps ~16 bits of each factor (hi + lo), so
+# 1e-4; FP32 products keep 24 bits, so 1e-5 (accumulation over up to 512
+# points per block in FP32, then FP64 across blocks).
+SPLIT_RTOL = {"bf16x3": 1e-4, "fp32": 1e-5}
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "fp32"])
+@pytest.mark.parametrize("name", ["si2", "si8", "nacl8", "si2_r1of3", "nacl8_r0of2"])
+def test_hamiltonian_split(name, prec):
+    c = case(name)
+    c.ctx.set_tables(c.tabs)
+    H = c.ctx.hamiltonian([c.V, c.V2], prec)
+    Href, M = O.hamiltonian(c.g, c.tabs, [c.V, c.V2], with_abs=True)
+    for k in range(2):
+        r = worst_ratio(H[k], Href[k], M[k])
+        print(name, prec, "H worst ratio", r)
+        assert r <= SPLIT_RTOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "fp32"])
+@pytest.mark.parametrize("name", ["si2", "nacl8", "si2_r1of3"])
+def test_density_split(name, prec):
+    c = case(name)
+    c.ctx.set_tables(c.tabs)
+    R = c.ctx.density([c.DM, c.DM2], dm_offset=c.u.MPHks_rank, precision=prec)
+    Rref, M = O.density(c.g, c.tabs, [c.DM, c.DM2], with_abs=True)
+    for k in range(2):
+        r = worst_ratio(R[k], Rref[k], M[k])
+        print(name, prec, "rho worst ratio", r)
+        assert r <= 1e-5
+
+
+@pytest.mark.parametrize("name", ["si8", "nacl8"])
+def test_force3_split(name):
+    c = case(name)
+    dt = O.dorbitals(c.g, c.opaos, c.st.atom2spe)
+    c.ctx.set_tables(c.tabs)
+    c.ctx.set_dtables(*dt)
+    F = c.ctx.force3([c.V], [c.DM], dm_offset=c.u.MPHks_rank, precision="fp32")
+    Fref = O.force3(c.g, c.tabs, dt, [c.V], [c.DM])
+    r = worst_ratio(F, Fref, _force_bound(c, dt, [c.V], [c.DM]))
+    print(name, "F3 fp32 worst ratio", r)
+    assert r <= 1e-5
+
+
+def test_split_tables_follow_table_updates():
+    """A second set_tables call invalidates the split-precision tiles."""
+    c = case("si2")
+    c.ctx.set_tables(c.tabs)
+    H1 = c.ctx.hamiltonian([c.V], "bf16x3")[0]
+    c.ctx.set_tables([2.0 * t for t in c.tabs])
+    H2 = c.ctx.hamiltonian([c.V], "bf16x3")[0]
+    c.ctx.set_tables(c.tabs)
+    assert np.allclose(H2, 4.0 * H1, rtol=1e-3, atol=1e-12)
