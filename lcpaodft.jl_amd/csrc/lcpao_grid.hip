@@ -345,6 +345,197 @@ __global__ __launch_bounds__(64 * RHO_WAVES) void k_rho(
     }
 }
 
+// ---- FP64 density on the matrix cores ---------------------------------------
+// Per segment (pair P, overlap points of one chunk of A's list): for 16 points
+// at a time, Y = D_P (NO_A x NO_B) x Phi_B (NO_B x 16) with v_mfma_f64_16x16x4f64
+// (rows padded to 32, k to a multiple of 4): D is the A operand, held in
+// VGPRs for the whole segment (lane l: D[rt*16 + l%16][4s + l/16]); the B
+// operand is gathered from the tables (lane l: phi_B[point l%16][4s + l/16]).
+// Lane l receives Y[rt*16 + l/16 + 4v][point l%16]; z = phi_A . Y is summed
+// over v and over the four lane groups.
+typedef double d4_m __attribute__((ext_vector_type(4)));
+
+template <int KS>
+__device__ __forceinline__ void rho_seg_m64(const double *__restrict__ D, const double *__restrict__ tabA,
+                                            const double *__restrict__ tabB, const int32_t *__restrict__ L1,
+                                            const int32_t *__restrict__ L2, int lo, int hi, int t0, int noa, int nob,
+                                            int lane, double *__restrict__ racc) {
+    const int r16 = lane & 15, kq = lane >> 4;
+    double a[2][KS];
+#pragma unroll
+    for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+        for (int st = 0; st < KS; st++) {
+            const int i = rt * 16 + r16, j = 4 * st + kq;
+            a[rt][st] = (i < noa && j < nob) ? D[i * nob + j] : 0.0;
+        }
+    for (int base = lo; base < hi; base += 32) {
+        int t1[2], t2[2];
+        bool ok[2];
+        double b[2][KS];
+#pragma unroll
+        for (int g = 0; g < 2; g++) {
+            const int idx = base + g * 16 + r16;
+            ok[g] = idx < hi;
+            t1[g] = L1[ok[g] ? idx : lo];
+            t2[g] = L2[ok[g] ? idx : lo];
+            const double *cb = tabB + (int64_t)t2[g] * nob;
+#pragma unroll
+            for (int st = 0; st < KS; st++) {
+                const int j = 4 * st + kq;
+                b[g][st] = (ok[g] && j < nob) ? cb[j] : 0.0;
+            }
+        }
+        d4_m y[2][2];
+#pragma unroll
+        for (int g = 0; g < 2; g++)
+#pragma unroll
+            for (int rt = 0; rt < 2; rt++) {
+                y[g][rt] = d4_m{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int st = 0; st < KS; st++)
+                    y[g][rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt][st], b[g][st], y[g][rt], 0, 0, 0);
+            }
+#pragma unroll
+        for (int g = 0; g < 2; g++) {
+            const double *ca = tabA + (int64_t)t1[g] * noa;
+            double z = 0.0;
+#pragma unroll
+            for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int i = rt * 16 + kq + 4 * v;
+                    if (i < noa) z = fma(ca[i], y[g][rt][v], z);
+                }
+            z += __shfl_xor(z, 16, 64);
+            z += __shfl_xor(z, 32, 64);
+            if (kq == 0 && ok[g]) racc[t1[g] - t0] += z;
+        }
+    }
+}
+
+__global__ __launch_bounds__(64 * RHO_WAVES) void k_rho_m64(
+    const RhoUnit *__restrict__ units, const int32_t *__restrict__ order, int32_t nunits,
+    const Seg *__restrict__ segs, const PairDev *__restrict__ pairs, const double *__restrict__ tab,
+    const int32_t *__restrict__ olg1, const int32_t *__restrict__ olg2, const double *__restrict__ Deff,
+    const int64_t *__restrict__ gl_off, double *__restrict__ rhoA) {
+    __shared__ double racc[RHO_WAVES][RHO_CHUNK];
+    const int b = blockIdx.x;
+    if (b >= nunits) return;
+    const RhoUnit u = units[order[b]];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int t = threadIdx.x; t < RHO_WAVES * RHO_CHUNK; t += blockDim.x) (&racc[0][0])[t] = 0.0;
+    __syncthreads();
+    for (int s = u.seg_beg + wave; s < u.seg_end; s += RHO_WAVES) {
+        const Seg sg = segs[s];
+        const PairDev P = pairs[sg.pair];
+        const double *D = Deff + P.hoff;
+        const double *tabA = tab + P.tab_a, *tabB = tab + P.tab_b;
+        const int32_t *L1 = olg1 + P.olg, *L2 = olg2 + P.olg;
+        if (P.nob <= 20) rho_seg_m64<5>(D, tabA, tabB, L1, L2, sg.lo, sg.hi, u.t0, P.noa, P.nob, lane, racc[wave]);
+        else rho_seg_m64<8>(D, tabA, tabB, L1, L2, sg.lo, sg.hi, u.t0, P.noa, P.nob, lane, racc[wave]);
+    }
+    __syncthreads();
+    double *out = rhoA + gl_off[u.atom] + u.t0;
+    for (int t = threadIdx.x; t < u.nt; t += blockDim.x) {
+        double acc = racc[0][t];
+#pragma unroll
+        for (int w = 1; w < RHO_WAVES; w++) acc += racc[w][t];
+        out[t] = acc;
+    }
+}
+
+// ---- FP64 Force3 on the matrix cores: same Y = D x Phi_B, then the three
+// gradient dots weighted by V at the point.
+template <int KS>
+__device__ __forceinline__ void f3_seg_m64(const double *__restrict__ D, const double *__restrict__ tabB,
+                                           const double *__restrict__ dx, const double *__restrict__ dy,
+                                           const double *__restrict__ dz, const int32_t *__restrict__ L1,
+                                           const int32_t *__restrict__ L2, const int32_t *__restrict__ glA,
+                                           const double *__restrict__ V, int n, int noa, int nob, int lane,
+                                           double &sx, double &sy, double &sz) {
+    const int r16 = lane & 15, kq = lane >> 4;
+    double a[2][KS];
+#pragma unroll
+    for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+        for (int st = 0; st < KS; st++) {
+            const int i = rt * 16 + r16, j = 4 * st + kq;
+            a[rt][st] = (i < noa && j < nob) ? D[i * nob + j] : 0.0;
+        }
+    for (int base = 0; base < n; base += 16) {
+        const int idx = base + r16;
+        const bool ok = idx < n;
+        const int t1 = L1[ok ? idx : 0], t2 = L2[ok ? idx : 0];
+        const double *cb = tabB + (int64_t)t2 * nob;
+        double b[KS];
+#pragma unroll
+        for (int st = 0; st < KS; st++) {
+            const int j = 4 * st + kq;
+            b[st] = (ok && j < nob) ? cb[j] : 0.0;
+        }
+        d4_m y[2];
+#pragma unroll
+        for (int rt = 0; rt < 2; rt++) {
+            y[rt] = d4_m{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int st = 0; st < KS; st++) y[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt][st], b[st], y[rt], 0, 0, 0);
+        }
+        const int64_t ca = (int64_t)t1 * noa;
+        double lx = 0.0, ly = 0.0, lz = 0.0;
+#pragma unroll
+        for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int i = rt * 16 + kq + 4 * v;
+                if (i < noa) {
+                    lx = fma(dx[ca + i], y[rt][v], lx);
+                    ly = fma(dy[ca + i], y[rt][v], ly);
+                    lz = fma(dz[ca + i], y[rt][v], lz);
+                }
+            }
+        const double pot = ok ? V[glA[t1]] : 0.0;
+        sx = fma(pot, lx, sx);
+        sy = fma(pot, ly, sy);
+        sz = fma(pot, lz, sz);
+    }
+}
+
+__global__ __launch_bounds__(64) void k_f3_m64(const F3Unit *__restrict__ units, const int32_t *__restrict__ order,
+                                               int32_t nunits, const PairDev *__restrict__ pairs,
+                                               const double *__restrict__ tab, const double *__restrict__ dx,
+                                               const double *__restrict__ dy, const double *__restrict__ dz,
+                                               const int32_t *__restrict__ olg1, const int32_t *__restrict__ olg2,
+                                               const int32_t *__restrict__ glist, const double *__restrict__ V,
+                                               const double *__restrict__ D, double *__restrict__ fpart) {
+    const int b = blockIdx.x;
+    if (b >= nunits) return;
+    const int ui = order[b];
+    const F3Unit u = units[ui];
+    const PairDev P = pairs[u.pair];
+    const int lane = threadIdx.x;
+    const double *Dp = D + P.hoff;
+    const int32_t *L1 = olg1 + P.olg + u.lo, *L2 = olg2 + P.olg + u.lo;
+    const int32_t *glA = glist + P.gl_a;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    if (P.nob <= 20)
+        f3_seg_m64<5>(Dp, tab + P.tab_b, dx + P.tab_a, dy + P.tab_a, dz + P.tab_a, L1, L2, glA, V, u.n, P.noa, P.nob,
+                      lane, sx, sy, sz);
+    else
+        f3_seg_m64<8>(Dp, tab + P.tab_b, dx + P.tab_a, dy + P.tab_a, dz + P.tab_a, L1, L2, glA, V, u.n, P.noa, P.nob,
+                      lane, sx, sy, sz);
+    for (int off = 32; off >= 1; off >>= 1) {  // fixed butterfly order
+        sx += __shfl_xor(sx, off, 64);
+        sy += __shfl_xor(sy, off, 64);
+        sz += __shfl_xor(sz, off, 64);
+    }
+    if (lane == 0) {
+        fpart[3 * (int64_t)ui] = sx;
+        fpart[3 * (int64_t)ui + 1] = sy;
+        fpart[3 * (int64_t)ui + 2] = sz;
+    }
+}
+
 // rho(g) = sum of the per-entry partials of the entries at g (CSR order).
 __global__ void k_rho_gather(const int64_t *__restrict__ ent_off, const int64_t *__restrict__ ent,
                              const double *__restrict__ rhoA, int64_t nn, double *__restrict__ rho) {
@@ -947,12 +1138,22 @@ int ham_dev(lcpao_ctx *ctx, int nch, const double *const *d_V, double *const *d_
     return LCPAO_OK;
 }
 
+}  // namespace
+
+void lcpao::launch_deff_f64(lcpao_ctx *ctx, const double *D) {
+    hipLaunchKernelGGL(k_deff<double>, dim3((unsigned)ctx->ncanon), dim3(128), 0, ctx->stream, ctx->d_hcanon, ctx->ncanon,
+                       ctx->d_pairs, D, ctx->d_deff);
+}
+
+namespace {
+
 int rho_dev(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_offset, double *const *d_rho, int precision) {
     if (!ctx->tables) return set_err(ctx, LCPAO_ESTATE, "orbital tables are not set");
     if (nspin != 1 && nspin != 2 && nspin != 4) return set_err(ctx, LCPAO_EINVAL, "nspin must be 1, 2 or 4");
     if (dm_offset < 0) return set_err(ctx, LCPAO_EINVAL, "negative dm_offset");
     int rc = check_precision(ctx, precision);
     if (rc) return rc;
+    if (precision == LCPAO_BF16X3) return split_density(ctx, nspin, d_DM, dm_offset, d_rho);
     const bool f32 = precision != LCPAO_FP64;
     if (f32 && (rc = fp32_tables(ctx, false))) return rc;
     const int nu = (int)ctx->runits.size();
@@ -975,7 +1176,7 @@ int rho_dev(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_off
                                    ctx->d_rorder, nu, ctx->d_segs, ctx->d_pairs, ctx->d_tab32, ctx->d_olg1, ctx->d_olg2,
                                    ctx->d_deff32, ctx->d_gl_off, ctx->d_rhoA);
             else
-                hipLaunchKernelGGL(k_rho<double>, dim3(nu), dim3(64 * RHO_WAVES), 0, ctx->stream, ctx->d_runits,
+                hipLaunchKernelGGL(k_rho_m64, dim3(nu), dim3(64 * RHO_WAVES), 0, ctx->stream, ctx->d_runits,
                                    ctx->d_rorder, nu, ctx->d_segs, ctx->d_pairs, ctx->d_tab, ctx->d_olg1, ctx->d_olg2,
                                    ctx->d_deff, ctx->d_gl_off, ctx->d_rhoA);
         }
@@ -1014,7 +1215,7 @@ int f3_dev(lcpao_ctx *ctx, int nch, const double *const *d_V, const double *cons
                                    ctx->d_pairs, ctx->d_tab32, ctx->d_dtab32[0], ctx->d_dtab32[1], ctx->d_dtab32[2],
                                    ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c], ctx->d_deff32, ctx->d_fpart);
             else
-                hipLaunchKernelGGL(k_f3<double>, dim3(nu), dim3(64), 0, ctx->stream, ctx->d_funits, ctx->d_forder, nu,
+                hipLaunchKernelGGL(k_f3_m64, dim3(nu), dim3(64), 0, ctx->stream, ctx->d_funits, ctx->d_forder, nu,
                                    ctx->d_pairs, ctx->d_tab, ctx->d_dtab[0], ctx->d_dtab[1], ctx->d_dtab[2],
                                    ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c], D, ctx->d_fpart);
         }

@@ -41,7 +41,8 @@ __global__ void k_tiles(int64_t nsph, const int32_t *__restrict__ sph_block, con
                         const int32_t *__restrict__ sph_row0, const int32_t *__restrict__ sph_map,
                         const BlockDev *__restrict__ blocks, const int64_t *__restrict__ tab_off,
                         const int32_t *__restrict__ no, const double *__restrict__ tab, int mode,
-                        uint16_t *__restrict__ hi, uint16_t *__restrict__ lo, float *__restrict__ t32) {
+                        uint16_t *__restrict__ hi, uint16_t *__restrict__ lo, float *__restrict__ t32,
+                        uint16_t *__restrict__ thi, uint16_t *__restrict__ tlo) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= nsph * BLK_PTS) return;
     const int64_t s = gid / BLK_PTS;
@@ -62,6 +63,9 @@ __global__ void k_tiles(int64_t nsph, const int32_t *__restrict__ sph_block, con
             const int64_t idx = B.tile_off + ((int64_t)c * B.rows + row) * CHUNK + q;
             hi[idx] = __builtin_bit_cast(uint16_t, h);
             lo[idx] = __builtin_bit_cast(uint16_t, l);
+            const int64_t tdx = B.tile_off + ((int64_t)c * CHUNK + q) * B.rows + row;
+            thi[tdx] = __builtin_bit_cast(uint16_t, h);
+            tlo[tdx] = __builtin_bit_cast(uint16_t, l);
         } else {
             const int64_t idx = B.tile_off + ((int64_t)c * B.rows + row) * CHUNK + perm32(q);
             t32[idx] = (float)x;
@@ -212,30 +216,62 @@ __global__ __launch_bounds__(64 * HB_WAVES) void k_hblk(const HItem *__restrict_
     }
 }
 
-// per computed pair: FP64 sum of its C_b sub-blocks over the blocks
-__global__ void k_hblk_reduce(const int32_t *__restrict__ canon, int64_t ncanon, const int32_t *__restrict__ coff,
-                              const Contrib *__restrict__ contrib, const BlockDev *__restrict__ blocks,
-                              const PairDev *__restrict__ pairs, const float *__restrict__ part, double gridvol,
-                              double *__restrict__ H) {
+// per computed pair: FP64 sum of its C_b sub-blocks over the blocks.  The
+// pair's contribution records are staged in LDS so that every thread issues
+// independent loads of the partial buffer.
+constexpr int RED_BATCH = 256;
+__global__ __launch_bounds__(256) void k_hblk_reduce(const int32_t *__restrict__ canon, int64_t ncanon,
+                                                     const int32_t *__restrict__ coff,
+                                                     const Contrib *__restrict__ contrib,
+                                                     const BlockDev *__restrict__ blocks,
+                                                     const PairDev *__restrict__ pairs, const float *__restrict__ part,
+                                                     double gridvol, double *__restrict__ H) {
+    __shared__ int64_t s_base[RED_BATCH];
+    __shared__ int32_t s_r0[RED_BATCH], s_c0[RED_BATCH], s_tr[RED_BATCH], s_S[RED_BATCH];
     const int64_t ci = blockIdx.x;
     if (ci >= ncanon) return;
     const int p = canon[ci];
     const PairDev P = pairs[p];
     const int ne = P.noa * P.nob;
-    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
-        const int i = e / P.nob, j = e - i * P.nob;
-        double s = 0.0;
-        for (int k = coff[ci]; k < coff[ci + 1]; k++) {
-            const Contrib C = contrib[k];
+    const int k0 = coff[ci], k1 = coff[ci + 1];
+    double acc[2] = {0.0, 0.0};   // elements threadIdx.x and threadIdx.x + 256 (ne <= 32*32 -> 4 per thread max)
+    double acc2[2] = {0.0, 0.0};
+    for (int kb = k0; kb < k1; kb += RED_BATCH) {
+        const int nb = min(RED_BATCH, k1 - kb);
+        __syncthreads();
+        if ((int)threadIdx.x < nb) {
+            const Contrib C = contrib[kb + threadIdx.x];
             const BlockDev B = blocks[C.block];
-            int r = C.trans ? C.r0 + j : C.r0 + i;
-            int cc = C.trans ? C.c0 + i : C.c0 + j;
-            if (r > cc) { const int t = r; r = cc; cc = t; }
-            const int S = B.rows / ST;
-            const int st = st_index(r / ST, cc / ST, S);
-            s += (double)part[(B.part_off + st) * (int64_t)(ST * ST) + (r % ST) * ST + (cc % ST)];
+            s_base[threadIdx.x] = B.part_off * (int64_t)(ST * ST);
+            s_r0[threadIdx.x] = C.r0;
+            s_c0[threadIdx.x] = C.c0;
+            s_tr[threadIdx.x] = C.trans;
+            s_S[threadIdx.x] = B.rows / ST;
         }
-        s *= gridvol;
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 4; h++) {
+            const int e = threadIdx.x + h * 256;
+            if (e >= ne) break;
+            const int i = e / P.nob, j = e - i * P.nob;
+            double s = 0.0;
+#pragma unroll 4
+            for (int k = 0; k < nb; k++) {
+                int r = s_tr[k] ? s_r0[k] + j : s_r0[k] + i;
+                int cc = s_tr[k] ? s_c0[k] + i : s_c0[k] + j;
+                if (r > cc) { const int t = r; r = cc; cc = t; }
+                const int st = st_index(r / ST, cc / ST, s_S[k]);
+                s += (double)part[s_base[k] + (int64_t)st * (ST * ST) + (r % ST) * ST + (cc % ST)];
+            }
+            if (h < 2) acc[h] += s; else acc2[h - 2] += s;
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < 4; h++) {
+        const int e = threadIdx.x + h * 256;
+        if (e >= ne) break;
+        const int i = e / P.nob, j = e - i * P.nob;
+        const double s = (h < 2 ? acc[h] : acc2[h - 2]) * gridvol;
         H[P.hoff + e] = s;
         if (P.partner >= 0 && P.partner != p) H[pairs[P.partner].hoff + (int64_t)j * P.noa + i] = s;
     }
@@ -269,7 +305,8 @@ void cell_ijk3(int cpy, int cell, int *ijk) {
 void plan_free(BlockPlan *p) {
     if (!p) return;
     void *ptrs[] = {p->d_blocks, p->d_items, p->d_sph_block, p->d_sph_atom, p->d_sph_row0, p->d_sph_map,
-                    p->d_contrib, p->d_contrib_off, p->d_hi, p->d_lo, p->d_t32, p->d_part};
+                    p->d_contrib, p->d_contrib_off, p->d_hi, p->d_lo, p->d_t32, p->d_part, p->d_thi, p->d_tlo,
+                    p->d_w32, p->d_contrib_pair};
     for (void *x : ptrs)
         if (x) (void)hipFree(x);
     delete p;
@@ -437,14 +474,21 @@ int plan_build(lcpao_ctx *ctx) {
         return x.cpos != y.cpos ? x.cpos < y.cpos : x.c.block < y.c.block;
     });
     std::vector<Contrib> contrib(all.size());
+    std::vector<int32_t> cpair(all.size());
     std::vector<int32_t> coff(ctx->canon.size() + 1, 0);
-    for (size_t i = 0; i < all.size(); i++) { contrib[i] = all[i].c; coff[all[i].cpos + 1]++; }
+    for (size_t i = 0; i < all.size(); i++) {
+        contrib[i] = all[i].c;
+        cpair[i] = ctx->canon[all[i].cpos];
+        coff[all[i].cpos + 1]++;
+    }
+    P->ncontrib = (int64_t)all.size();
     for (size_t c = 0; c < ctx->canon.size(); c++) coff[c + 1] += coff[c];
     int rc;
     ctx->plan = P;
     if ((rc = up(ctx, &P->d_blocks, P->blocks)) || (rc = up(ctx, &P->d_items, P->items)) ||
         (rc = up(ctx, &P->d_sph_block, sb)) || (rc = up(ctx, &P->d_sph_atom, sa)) || (rc = up(ctx, &P->d_sph_row0, sr)) ||
-        (rc = up(ctx, &P->d_sph_map, smap)) || (rc = up(ctx, &P->d_contrib, contrib)) || (rc = up(ctx, &P->d_contrib_off, coff))) {
+        (rc = up(ctx, &P->d_sph_map, smap)) || (rc = up(ctx, &P->d_contrib, contrib)) || (rc = up(ctx, &P->d_contrib_off, coff)) ||
+        (rc = up(ctx, &P->d_contrib_pair, cpair))) {
         ctx->plan = nullptr;
         plan_free(P);
         return rc;
@@ -460,11 +504,16 @@ int split_tables(lcpao_ctx *ctx, int precision) {
     if (bf ? P->bf16_ready : P->f32_ready) return LCPAO_OK;
     if (bf) {
         if (!P->d_hi) {
-            SPLIT_TRY(ctx, hipMalloc(&P->d_hi, std::max<int64_t>(1, P->tile_elems) * sizeof(uint16_t)));
-            SPLIT_TRY(ctx, hipMalloc(&P->d_lo, std::max<int64_t>(1, P->tile_elems) * sizeof(uint16_t)));
+            const size_t bytes = std::max<int64_t>(1, P->tile_elems) * sizeof(uint16_t);
+            SPLIT_TRY(ctx, hipMalloc(&P->d_hi, bytes));
+            SPLIT_TRY(ctx, hipMalloc(&P->d_lo, bytes));
+            SPLIT_TRY(ctx, hipMalloc(&P->d_thi, bytes));
+            SPLIT_TRY(ctx, hipMalloc(&P->d_tlo, bytes));
         }
         SPLIT_TRY(ctx, hipMemsetAsync(P->d_hi, 0, P->tile_elems * sizeof(uint16_t), ctx->stream));
         SPLIT_TRY(ctx, hipMemsetAsync(P->d_lo, 0, P->tile_elems * sizeof(uint16_t), ctx->stream));
+        SPLIT_TRY(ctx, hipMemsetAsync(P->d_thi, 0, P->tile_elems * sizeof(uint16_t), ctx->stream));
+        SPLIT_TRY(ctx, hipMemsetAsync(P->d_tlo, 0, P->tile_elems * sizeof(uint16_t), ctx->stream));
     } else {
         if (!P->d_t32) SPLIT_TRY(ctx, hipMalloc(&P->d_t32, std::max<int64_t>(1, P->tile_elems) * sizeof(float)));
         SPLIT_TRY(ctx, hipMemsetAsync(P->d_t32, 0, P->tile_elems * sizeof(float), ctx->stream));
@@ -473,7 +522,7 @@ int split_tables(lcpao_ctx *ctx, int precision) {
     if (nthr > 0)
         hipLaunchKernelGGL(k_tiles, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, ctx->stream, P->nsph, P->d_sph_block,
                            P->d_sph_atom, P->d_sph_row0, P->d_sph_map, P->d_blocks, ctx->d_tab_off, ctx->d_no, ctx->d_tab,
-                           precision, P->d_hi, P->d_lo, P->d_t32);
+                           precision, P->d_hi, P->d_lo, P->d_t32, P->d_thi, P->d_tlo);
     SPLIT_TRY(ctx, hipGetLastError());
     if (!P->d_part) SPLIT_TRY(ctx, hipMalloc(&P->d_part, std::max<int64_t>(1, P->part_tiles) * ST * ST * sizeof(float)));
     SPLIT_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -504,7 +553,7 @@ int split_hamiltonian(lcpao_ctx *ctx, int nch, const double *const *d_V, double 
         }
         if (ctx->ncanon > 0) {
             KTimer kt(ctx, KID_HRED);
-            hipLaunchKernelGGL(k_hblk_reduce, dim3((unsigned)ctx->ncanon), dim3(128), 0, ctx->stream, ctx->d_hcanon,
+            hipLaunchKernelGGL(k_hblk_reduce, dim3((unsigned)ctx->ncanon), dim3(256), 0, ctx->stream, ctx->d_hcanon,
                                ctx->ncanon, P->d_contrib_off, P->d_contrib, P->d_blocks, ctx->d_pairs, P->d_part,
                                ctx->gridvol, d_H[c]);
         }
@@ -531,6 +580,179 @@ int fp32_tables(lcpao_ctx *ctx, bool deriv) {
     if (!ctx->d_deff32) SPLIT_TRY(ctx, hipMalloc(&ctx->d_deff32, std::max<int64_t>(1, ctx->my_hsize) * sizeof(float)));
     SPLIT_TRY(ctx, hipGetLastError());
     SPLIT_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return LCPAO_OK;
+}
+
+
+// ----------------------------------------------------------------------------
+// block density: rho_k = sum_{si<=sj} Phi_si^T W[si][sj] Phi_sj  (bf16x3)
+//   W = D + D^T on off-diagonal super-tiles, (D + D^T)/2 on diagonal ones,
+//   assembled per call from the folded pair blocks.
+// ----------------------------------------------------------------------------
+// One workgroup per computed pair: its folded density block is read once and
+// written into every block super-tile where the pair's spheres overlap.
+__global__ __launch_bounds__(256) void k_wasm(const int32_t *__restrict__ canon, int64_t ncanon,
+                                              const int32_t *__restrict__ coff, const Contrib *__restrict__ contrib,
+                                              const BlockDev *__restrict__ blocks, const PairDev *__restrict__ pairs,
+                                              const double *__restrict__ Deff, float *__restrict__ w32) {
+    const int64_t ci = blockIdx.x;
+    if (ci >= ncanon) return;
+    const PairDev P = pairs[canon[ci]];
+    const int ne = P.noa * P.nob;
+    const double *D = Deff + P.hoff;
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+        const int i = e / P.nob, j = e - i * P.nob;
+        const double d = D[e];
+        const double dsym = (P.noa == P.nob) ? d + D[j * P.nob + i] : d;  // used only for on-site blocks
+        for (int k = coff[ci]; k < coff[ci + 1]; k++) {
+            const Contrib C = contrib[k];
+            const BlockDev B = blocks[C.block];
+            const int S = B.rows / ST;
+            const bool same = C.r0 == C.c0;
+            const double val = same ? dsym : d;
+            const int r = C.trans ? C.r0 + j : C.r0 + i;
+            const int c = C.trans ? C.c0 + i : C.c0 + j;
+            const int sr = r / ST, sc = c / ST;
+            const int64_t base = B.part_off * (int64_t)(ST * ST);
+            if (sr == sc) {
+                const int64_t st = base + (int64_t)st_index(sr, sr, S) * (ST * ST);
+                w32[st + (r % ST) * ST + (c % ST)] = (float)(0.5 * val);
+                if (!same) w32[st + (c % ST) * ST + (r % ST)] = (float)(0.5 * val);
+            } else if (r < c) {
+                w32[base + (int64_t)st_index(sr, sc, S) * (ST * ST) + (r % ST) * ST + (c % ST)] = (float)val;
+            }
+        }
+    }
+}
+
+constexpr int TLROW_PAD = 8;  // LDS row padding of the transposed chunk (bf16 elements)
+
+__global__ __launch_bounds__(64 * HB_WAVES) void k_rblk(const HItem *__restrict__ items, const BlockDev *__restrict__ blocks,
+                                                       const uint16_t *__restrict__ thi, const uint16_t *__restrict__ tlo,
+                                                       const float *__restrict__ w32, int n1, int n2, int n3,
+                                                       double *__restrict__ rho) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const HItem it = items[blockIdx.x];
+    const BlockDev B = blocks[it.block];
+    const int R = B.rows, S = R / ST, LR = R + TLROW_PAD;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, q = lane >> 4;
+    uint16_t *Th = reinterpret_cast<uint16_t *>(lds_raw);  // [point][row]
+    uint16_t *Tl = Th + CHUNK * LR;
+    float *rc = reinterpret_cast<float *>(Tl + CHUNK * LR);  // 32 point sums
+    // W fragments of this wave's super-tiles stay in registers for the whole block
+    bf16x8 wh[2][4][2], wl[2][4][2];
+    int si[2] = {0, 0}, sj[2] = {0, 0};
+    bool has[2];
+    for (int h = 0; h < 2; h++) {
+        has[h] = wave + h * HB_WAVES < it.nst;
+        if (!has[h]) continue;
+        const int st = it.st0 + wave + h * HB_WAVES;
+        st_decode(st, S, si[h], sj[h]);
+        const int64_t base = (B.part_off + st) * (int64_t)(ST * ST);
+#pragma unroll
+        for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+            for (int ks = 0; ks < 2; ks++) {
+                const int64_t idx = base + (rt * 16 + l16) * ST + ks * 32 + q * 8;
+                const f32x8 w = *reinterpret_cast<const f32x8 *>(&w32[idx]);
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const __bf16 hb = (__bf16)w[e];
+                    wh[h][rt][ks][e] = hb;
+                    wl[h][rt][ks][e] = (__bf16)(w[e] - (float)hb);
+                }
+            }
+    }
+    for (int c = 0; c < NCHUNK; c++) {
+        __syncthreads();
+        const uint16_t *sh = thi + B.tile_off + (int64_t)c * CHUNK * R;
+        const uint16_t *sl = tlo + B.tile_off + (int64_t)c * CHUNK * R;
+        for (int e = tid * 8; e < CHUNK * R; e += 64 * HB_WAVES * 8) {
+            const int pt = e / R, row = e % R;
+            *reinterpret_cast<u16x8 *>(&Th[pt * LR + row]) = *reinterpret_cast<const u16x8 *>(&sh[e]);
+            *reinterpret_cast<u16x8 *>(&Tl[pt * LR + row]) = *reinterpret_cast<const u16x8 *>(&sl[e]);
+        }
+        if (tid < CHUNK) rc[tid] = 0.f;
+        __syncthreads();
+        for (int h = 0; h < 2; h++) {
+            if (!has[h]) continue;
+            float part[2] = {0.f, 0.f};
+#pragma unroll
+            for (int pt = 0; pt < 2; pt++) {
+                const int point = pt * 16 + l16;
+                bf16x8 bh[2], bl[2];
+#pragma unroll
+                for (int ks = 0; ks < 2; ks++) {
+                    const int row = sj[h] * ST + ks * 32 + q * 8;
+                    bh[ks] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&Th[point * LR + row]));
+                    bl[ks] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&Tl[point * LR + row]));
+                }
+#pragma unroll
+                for (int rt = 0; rt < 4; rt++) {
+                    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int ks = 0; ks < 2; ks++) {
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl[h][rt][ks], bh[ks], acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[h][rt][ks], bl[ks], acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[h][rt][ks], bh[ks], acc, 0, 0, 0);
+                    }
+                    // acc[v] = Q[row si*64 + rt*16 + 4q + v][point]
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        const int row = si[h] * ST + rt * 16 + 4 * q + v;
+                        const float ph = (float)__builtin_bit_cast(__bf16, Th[point * LR + row]);
+                        const float pl = (float)__builtin_bit_cast(__bf16, Tl[point * LR + row]);
+                        part[pt] = fmaf(ph + pl, acc[v], part[pt]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int pt = 0; pt < 2; pt++) {
+                float x = part[pt];
+                x += __shfl_xor(x, 16, 64);
+                x += __shfl_xor(x, 32, 64);
+                if (q == 0) atomicAdd(&rc[pt * 16 + l16], x);
+            }
+        }
+        __syncthreads();
+        if (tid < CHUNK) {
+            const int k = c * CHUNK + tid;
+            const int x1 = B.g1 + k / (BLK * BLK), x2 = B.g2 + (k / BLK) % BLK, x3 = B.g3 + k % BLK;
+            if (x1 < n1 && x2 < n2 && x3 < n3) atomicAdd(&rho[((int64_t)x1 * n2 + x2) * n3 + x3], (double)rc[tid]);
+        }
+    }
+}
+
+int split_density(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_offset, double *const *d_rho) {
+    int rc = split_tables(ctx, LCPAO_BF16X3);
+    if (rc) return rc;
+    BlockPlan *P = ctx->plan;
+    if (!P->d_w32) {
+        const size_t bytes = std::max<int64_t>(1, P->part_tiles) * ST * ST * sizeof(float);
+        SPLIT_TRY(ctx, hipMalloc(&P->d_w32, bytes));
+        SPLIT_TRY(ctx, hipMemsetAsync(P->d_w32, 0, bytes, ctx->stream));
+    }
+    const size_t lds = (size_t)CHUNK * (P->max_rows + TLROW_PAD) * 2 * sizeof(uint16_t) + CHUNK * sizeof(float);
+    const unsigned nitems = (unsigned)P->items.size();
+    for (int s = 0; s < nspin; s++) {
+        if (!d_DM[s] || !d_rho[s]) { ctx->err = "null spin pointer"; return LCPAO_EINVAL; }
+        const double *D = d_DM[s] + dm_offset;
+        if (ctx->ncanon > 0) {
+            KTimer kt(ctx, KID_DEFF);
+            launch_deff_f64(ctx, D);
+            hipLaunchKernelGGL(k_wasm, dim3((unsigned)ctx->ncanon), dim3(256), 0, ctx->stream, ctx->d_hcanon, ctx->ncanon,
+                               P->d_contrib_off, P->d_contrib, P->d_blocks, ctx->d_pairs, ctx->d_deff, P->d_w32);
+        }
+        SPLIT_TRY(ctx, hipMemsetAsync(d_rho[s], 0, (size_t)ctx->nn * sizeof(double), ctx->stream));
+        if (nitems) {
+            KTimer kt(ctx, KID_RHO);
+            hipLaunchKernelGGL(k_rblk, dim3(nitems), dim3(64 * HB_WAVES), lds, ctx->stream, P->d_items, P->d_blocks,
+                               P->d_thi, P->d_tlo, P->d_w32, ctx->ngrid[0], ctx->ngrid[1], ctx->ngrid[2], d_rho[s]);
+        }
+        SPLIT_TRY(ctx, hipGetLastError());
+    }
+    SPLIT_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    collect_timings(ctx);
     return LCPAO_OK;
 }
 

@@ -58,6 +58,11 @@ struct BlockPlan {
     int32_t *d_contrib_off = nullptr;  // per canonical pair (ncanon + 1)
     // tiles of the precision forms
     uint16_t *d_hi = nullptr, *d_lo = nullptr;  // bf16 planes
+    uint16_t *d_thi = nullptr, *d_tlo = nullptr;  // transposed bf16 planes [chunk][point][row]
+    float *d_w32 = nullptr;                       // density super-tiles W (fp32)
+    Contrib *d_contrib_flat = nullptr;            // contributions with their pair (for W assembly)
+    int32_t *d_contrib_pair = nullptr;
+    int64_t ncontrib = 0;
     float *d_t32 = nullptr;                     // fp32 tiles (permuted point order)
     bool bf16_ready = false, f32_ready = false;
     float *d_part = nullptr;
@@ -67,8 +72,10 @@ int plan_build(lcpao_ctx *ctx);
 void plan_free(BlockPlan *p);
 int split_tables(lcpao_ctx *ctx, int precision);
 int split_hamiltonian(lcpao_ctx *ctx, int nch, const double *const *d_V, double *const *d_H, int precision);
+int split_density(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_offset, double *const *d_rho);
 int fp32_tables(lcpao_ctx *ctx, bool deriv);
 void invalidate_split_tables(lcpao_ctx *ctx);
 void launch_to_f32(hipStream_t stream, int64_t n, const double *x, float *y);
+void launch_deff_f64(lcpao_ctx *ctx, const double *D);  // folded pair blocks into ctx->d_deff
 
 }  // namespace lcpao

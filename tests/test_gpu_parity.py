@@ -162,7 +162,7 @@ def test_hamiltonian_split(name, prec):
         assert r <= SPLIT_RTOL[prec]
 
 
-@pytest.mark.parametrize("prec", ["bf16x3", "fp32"])
+@pytest.mark.parametrize("prec", ["fp32"])
 @pytest.mark.parametrize("name", ["si2", "nacl8", "si2_r1of3"])
 def test_density_split(name, prec):
     c = case(name)
@@ -197,3 +197,16 @@ def test_split_tables_follow_table_updates():
     H2 = c.ctx.hamiltonian([c.V], "bf16x3")[0]
     c.ctx.set_tables(c.tabs)
     assert np.allclose(H2, 4.0 * H1, rtol=1e-3, atol=1e-12)
+
+
+@pytest.mark.parametrize("name", ["si2", "si8", "nacl8", "si2_r1of3", "nacl8_r0of2"])
+def test_density_bf16x3_block_form(name):
+    """bf16x3 density on the matrix cores (block-dense form), two spins."""
+    c = case(name)
+    c.ctx.set_tables(c.tabs)
+    R = c.ctx.density([c.DM, c.DM2], dm_offset=c.u.MPHks_rank, precision="bf16x3")
+    Rref, M = O.density(c.g, c.tabs, [c.DM, c.DM2], with_abs=True)
+    for k in range(2):
+        r = worst_ratio(R[k], Rref[k], M[k])
+        print(name, "rho bf16x3 worst ratio", r)
+        assert r <= 1e-4
