@@ -39,6 +39,9 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-ranks", type=int, default=16,
                    help="ranks of the reference's weighted pair split; rank 0's share is the timed CPU sample")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="first run a plain 1 GiB -> 1 GiB copy kernel (torch.neg) 3 times: the known-size "
+                        "dispatch that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
     return p.parse_args()
 
 
@@ -98,6 +101,29 @@ def cpu_baseline(struct, ctx, u, V, DM, ranks):
     }
 
 
+def traffic_from_profile(config, kernel):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 --pmc
+    summary of this workload (None when there is none)."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s.md" % config)
+    if not os.path.exists(path):
+        return None, None
+    from lcpao_grid.profsum import read_traffic_table
+    tab = read_traffic_table(path)
+    for name, b in tab.items():
+        if name.split("lcpao::")[-1].replace(" ", "") == kernel.replace(" ", ""):
+            return b, os.path.relpath(path, ROOT)
+    return None, os.path.relpath(path, ROOT)
+
+
+def copy_calibration(torch):
+    a = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")  # 1 GiB
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    del a, b
+
+
 def time_form(ctx, prec, dV, dDM, dH, drho, steps, warmup, torch):
     import ctypes as C
 
@@ -134,6 +160,8 @@ def main():
     import torch
     from lcpao_grid import systems as S
     from lcpao_grid.grid import GridContext
+    if args.copy_calibration:
+        copy_calibration(torch)
 
     t_setup = time.time()
     struct = S.CONFIGS[args.config]()
@@ -183,15 +211,32 @@ def main():
     dom = max(("ham", "rho"), key=lambda k: stats.get(k, {}).get("avg_ms", 0.0))
     flops = f_ham if dom == "ham" else f_rho
     achieved = flops / (stats[dom]["avg_ms"] * 1e-3)
-    kname = {("fp64", "ham"): "k_ham_mfma64", ("fp64", "rho"): "k_rho_m64", ("bf16x3", "ham"): "k_hblk<bf16x3>",
-             ("bf16x3", "rho"): "k_rblk", ("fp32", "ham"): "k_hblk<fp32>", ("fp32", "rho"): "k_rho<float>"}
+    kname = {("fp64", "ham"): "k_ham_mfma64", ("fp64", "rho"): "k_rho_m64", ("bf16x3", "ham"): "k_hblk<2>",
+             ("bf16x3", "rho"): "k_rblk", ("fp32", "ham"): "k_hblk<1>", ("fp32", "rho"): "k_rho<float>"}
+    kern = kname[(args.precision, dom)]
+    traffic, traffic_src = traffic_from_profile(args.config, kern)
+    # algorithmic bytes of the same launch: its half of the compulsory traffic
+    # (tables at the form's element size, V or rho, D or H, grid lists)
+    esize = {"fp64": 8, "fp32": 4, "bf16x3": 4}[args.precision]
+    gridn = u.GridN_Atom.astype(np.int64)
+    nob = u.Total_NumOrbs.astype(np.int64)
+    alg_bytes = float((gridn * nob).sum() * esize + u.NN * 8 + (nob[u.MPI_atom] * nob[u.MPI_natn]).sum() * 8
+                      + gridn.sum() * 4)
+    kt = stats[dom]["avg_ms"] * 1e-3
+    mfma_frac = achieved / PEAK[args.precision]
+    hbm_frac = alg_bytes / kt / HBM_PEAK
     roofline = {
-        "bound": "mfma", "kernel": kname[(args.precision, dom)],
+        "bound": "mfma", "kernel": kern,
         "achieved": round(achieved / 1e12, 3), "peak": round(PEAK[args.precision] / 1e12, 1), "unit": "TFLOP/s",
-        "frac": round(achieved / PEAK[args.precision], 4), "traffic": None,
+        "frac": round(mfma_frac, 4), "traffic": traffic,
         "flops_per_launch": flops,
         "flops_note": "exact algorithmic FLOPs of the launch (canonical pairs: each transpose pair computed once)",
+        "hbm_part": {"algorithmic_bytes": alg_bytes, "achieved_GBps": round(alg_bytes / kt / 1e9, 1),
+                     "peak_GBps": HBM_PEAK / 1e9, "frac": round(hbm_frac, 4)},
     }
+    if traffic is not None:
+        roofline["traffic_source"] = traffic_src
+        roofline["traffic_GBps"] = round(traffic / (stats[dom]["avg_ms"] * 1e-3) / 1e9, 1)
     out = {
         "metric": METRIC, "value": round(ms, 4), "unit": "ms", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": False, "scaling": "weak",
