@@ -109,10 +109,15 @@ def traffic_from_profile(config, kernel):
         return None, None
     from lcpao_grid.profsum import read_traffic_table
     tab = read_traffic_table(path)
-    for name, b in tab.items():
-        if name.split("lcpao::")[-1].replace(" ", "") == kernel.replace(" ", ""):
-            return b, os.path.relpath(path, ROOT)
-    return None, os.path.relpath(path, ROOT)
+
+    def norm(n):
+        return n.split("lcpao::")[-1].replace(" ", "")
+
+    if "<" in kernel:
+        hits = [b for n, b in tab.items() if norm(n) == norm(kernel)]
+    else:  # every template instance of the kernel (one launch per row class)
+        hits = [b for n, b in tab.items() if norm(n).split("<")[0] == kernel]
+    return (sum(hits) if hits else None), os.path.relpath(path, ROOT)
 
 
 def copy_calibration(torch):
@@ -211,8 +216,8 @@ def main():
     dom = max(("ham", "rho"), key=lambda k: stats.get(k, {}).get("avg_ms", 0.0))
     flops = f_ham if dom == "ham" else f_rho
     achieved = flops / (stats[dom]["avg_ms"] * 1e-3)
-    kname = {("fp64", "ham"): "k_ham_mfma64", ("fp64", "rho"): "k_rho_m64", ("bf16x3", "ham"): "k_hblk<2>",
-             ("bf16x3", "rho"): "k_rblk", ("fp32", "ham"): "k_hblk<1>", ("fp32", "rho"): "k_rho<float>"}
+    kname = {("fp64", "ham"): "k_ham_mfma64", ("fp64", "rho"): "k_rho_m64", ("bf16x3", "ham"): "k_hblk_bf",
+             ("bf16x3", "rho"): "k_rblk_bf", ("fp32", "ham"): "k_hblk_f32", ("fp32", "rho"): "k_rho<float>"}
     kern = kname[(args.precision, dom)]
     traffic, traffic_src = traffic_from_profile(args.config, kern)
     # algorithmic bytes of the same launch: its half of the compulsory traffic

@@ -14,6 +14,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
 
 #define SPLIT_TRY(ctx, call)                                                        \
     do {                                                                            \
@@ -24,8 +25,7 @@ typedef float f32x8 __attribute__((ext_vector_type(8)));
         }                                                                           \
     } while (0)
 
-constexpr int MAX_ROWS = 768;    // R limit (LDS: 2*R*40*2 B)
-constexpr int LROW = 40;         // LDS row stride of the bf16 planes (32 + 8 pad)
+constexpr int MAX_ROWS = 640;    // R limit (bf16x3 H: four R x 32 bf16 planes in 160 KB of LDS)
 constexpr int LROW32 = 36;       // LDS row stride of fp32 tiles (32 + 4 pad)
 
 __device__ __forceinline__ int st_index(int si, int sj, int S) { return si * S - (si * (si - 1)) / 2 + (sj - si); }
@@ -33,6 +33,16 @@ __device__ __forceinline__ int st_index(int si, int sj, int S) { return si * S -
 // position of point q (0..31) inside a fp32 chunk row: lane group q%4 of the
 // 16x16x4 layout reads its 8 points contiguously
 __host__ __device__ __forceinline__ int perm32(int q) { return (q & 3) * 8 + (q >> 2); }
+
+// element offset, inside a block's tiles, of (row, point 32 c + k) in the two
+// bf16 fragment orders
+__host__ __device__ __forceinline__ int64_t rowfrag_index(int c, int row, int k, int R) {
+    return ((((int64_t)c * (R / 16) + row / 16) * 64 + (k / 8) * 16 + row % 16) * 8) + k % 8;
+}
+__host__ __device__ __forceinline__ int64_t ptfrag_index(int c, int row, int k, int R) {
+    const int rr = row % 32;
+    return ((((int64_t)c * 2 + k / 16) * (R / 32) + row / 32) * 64 + (rr / 8) * 16 + k % 16) * 8 + rr % 8;
+}
 
 // ----------------------------------------------------------------------------
 // tile construction (once per table set)
@@ -60,10 +70,10 @@ __global__ void k_tiles(int64_t nsph, const int32_t *__restrict__ sph_block, con
             const float xf = (float)x;
             const __bf16 h = (__bf16)xf;
             const __bf16 l = (__bf16)(xf - (float)h);
-            const int64_t idx = B.tile_off + ((int64_t)c * B.rows + row) * CHUNK + q;
+            const int64_t idx = B.tile_off + rowfrag_index(c, (int)row, q, B.rows);
             hi[idx] = __builtin_bit_cast(uint16_t, h);
             lo[idx] = __builtin_bit_cast(uint16_t, l);
-            const int64_t tdx = B.tile_off + ((int64_t)c * CHUNK + q) * B.rows + row;
+            const int64_t tdx = B.tile_off + ptfrag_index(c, (int)row, q, B.rows);
             thi[tdx] = __builtin_bit_cast(uint16_t, h);
             tlo[tdx] = __builtin_bit_cast(uint16_t, l);
         } else {
@@ -85,84 +95,209 @@ void launch_to_f32(hipStream_t stream, int64_t n, const double *x, float *y) {
 // ----------------------------------------------------------------------------
 // block Hamiltonian: C_b = Phi diag(V) Phi^T, upper 64x64 super-tiles
 // ----------------------------------------------------------------------------
-template <int MODE>
-__device__ __forceinline__ void supertile(const uint16_t *__restrict__ Ahi, const uint16_t *__restrict__ Alo,
-                                          const float *__restrict__ A32, const float *__restrict__ Vs, int si, int sj,
-                                          int l16, int q, f32x4 *acc) {
-    if constexpr (MODE == LCPAO_BF16X3) {
-        bf16x8 ah[4], al[4];
-#pragma unroll
-        for (int rt = 0; rt < 4; rt++) {
-            const int row = si * ST + rt * 16 + l16;
-            ah[rt] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&Ahi[row * LROW + q * 8]));
-            al[rt] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&Alo[row * LROW + q * 8]));
-        }
-#pragma unroll
-        for (int ct = 0; ct < 4; ct++) {
-            const int row = sj * ST + ct * 16 + l16;
-            const bf16x8 xh = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&Ahi[row * LROW + q * 8]));
-            const bf16x8 xl = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&Alo[row * LROW + q * 8]));
-            bf16x8 bh, bl;
-#pragma unroll
-            for (int e = 0; e < 8; e++) {
-                const float x = ((float)xh[e] + (float)xl[e]) * Vs[q * 8 + e];
-                const __bf16 h = (__bf16)x;
-                bh[e] = h;
-                bl[e] = (__bf16)(x - (float)h);
-            }
-#pragma unroll
-            for (int rt = 0; rt < 4; rt++) {
-                f32x4 c = acc[rt * 4 + ct];
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rt], bh, c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt], bl, c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt], bh, c, 0, 0, 0);
-                acc[rt * 4 + ct] = c;
-            }
-        }
-    } else {
-        f32x8 a[4];
-#pragma unroll
-        for (int rt = 0; rt < 4; rt++) {
-            const int row = si * ST + rt * 16 + l16;
-            a[rt] = *reinterpret_cast<const f32x8 *>(&A32[row * LROW32 + q * 8]);
-        }
-#pragma unroll
-        for (int ct = 0; ct < 4; ct++) {
-            const int row = sj * ST + ct * 16 + l16;
-            f32x8 b = *reinterpret_cast<const f32x8 *>(&A32[row * LROW32 + q * 8]);
-#pragma unroll
-            for (int e = 0; e < 8; e++) b[e] *= Vs[q * 8 + e];
-#pragma unroll
-            for (int rt = 0; rt < 4; rt++) {
-                f32x4 c = acc[rt * 4 + ct];
-#pragma unroll
-                for (int s = 0; s < 8; s++) c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt][s], b[s], c, 0, 0, 0);
-                acc[rt * 4 + ct] = c;
-            }
-        }
-    }
-}
-
 __device__ __forceinline__ void st_decode(int st, int S, int &si, int &sj) {
     si = 0;
     while (st >= S - si) { st -= S - si; si++; }
     sj = si + st;
 }
 
-template <int MODE>
-__global__ __launch_bounds__(64 * HB_WAVES) void k_hblk(const HItem *__restrict__ items, const BlockDev *__restrict__ blocks,
-                                                       const uint16_t *__restrict__ hi, const uint16_t *__restrict__ lo,
-                                                       const float *__restrict__ t32, const double *__restrict__ V,
-                                                       int n1, int n2, int n3, float *__restrict__ part) {
+__device__ __forceinline__ float bf_bits_to_f32(uint16_t u) { return __uint_as_float((uint32_t)u << 16); }
+
+// ---- bf16x3 form ------------------------------------------------------------
+// One workgroup per item (a block and up to 16 of its super-tiles, two per
+// wave, accumulators in registers over all 512 points).  Per 32-point chunk
+// the LDS holds four row-fragment planes: Phi hi, Phi lo, (V Phi) hi and
+// (V Phi) lo, each R x 32 bf16; V Phi is formed once per element while the
+// chunk is staged.  Every operand read is one ds_read_b128 at lane * 16 bytes
+// of a 1 KB fragment.  The next chunk's fragments are loaded into registers
+// before the current chunk is multiplied, so HBM latency overlaps the MFMAs.
+constexpr int FRAG = 512;                          // bf16 elements of one fragment
+constexpr int MAXF = MAX_ROWS / 16 / HB_WAVES;     // fragments per wave and plane (R = MAX_ROWS)
+// NF: fragments per wave and plane for the item class, ceil(R / 16 / HB_WAVES)
+
+// L2 warm-up: one dword of every 128-byte line of a chunk of a block's hi and
+// lo planes (32 R bf16 each, R / 2 lines per plane).  The values are kept
+// until the chunk is staged (by then the loads have completed in order) and
+// only then released, so the touch never stalls the wave.
+struct Touch {
+    uint32_t a, b;
+};
+__device__ __forceinline__ Touch touch_chunk(const uint16_t *__restrict__ hi, const uint16_t *__restrict__ lo,
+                                             int64_t off, int R, int tid) {
+    Touch t{0u, 0u};
+    const int half = R / 2;
+    int x = tid;
+    if (x < R) t.a = *reinterpret_cast<const uint32_t *>((x < half ? hi : lo) + off + (int64_t)(x < half ? x : x - half) * 64);
+    x += 64 * HB_WAVES;
+    if (x < R) t.b = *reinterpret_cast<const uint32_t *>((x < half ? hi : lo) + off + (int64_t)(x < half ? x : x - half) * 64);
+    return t;
+}
+__device__ __forceinline__ void release(const Touch &t) { asm volatile("" ::"v"(t.a), "v"(t.b)); }
+
+inline int nfrag(int rows) { return (rows / 16 + HB_WAVES - 1) / HB_WAVES; }
+
+__device__ __forceinline__ void tile_bf16x3(const u16x8 *__restrict__ PH, const u16x8 *__restrict__ PL,
+                                            const u16x8 *__restrict__ VH, const u16x8 *__restrict__ VL, int si,
+                                            int sj, int lane, f32x4 *acc) {
+    bf16x8 ah[4], al[4];
+#pragma unroll
+    for (int rt = 0; rt < 4; rt++) {
+        ah[rt] = __builtin_bit_cast(bf16x8, PH[(si * 4 + rt) * 64 + lane]);
+        al[rt] = __builtin_bit_cast(bf16x8, PL[(si * 4 + rt) * 64 + lane]);
+    }
+#pragma unroll
+    for (int ct = 0; ct < 4; ct++) {
+        const bf16x8 bh = __builtin_bit_cast(bf16x8, VH[(sj * 4 + ct) * 64 + lane]);
+        const bf16x8 bl = __builtin_bit_cast(bf16x8, VL[(sj * 4 + ct) * 64 + lane]);
+#pragma unroll
+        for (int rt = 0; rt < 4; rt++) {
+            f32x4 c = acc[rt * 4 + ct];
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rt], bh, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt], bl, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt], bh, c, 0, 0, 0);
+            acc[rt * 4 + ct] = c;
+        }
+    }
+}
+
+template <int NF>
+__global__ __launch_bounds__(64 * HB_WAVES) void k_hblk_bf(const HItem *__restrict__ items,
+                                                          const BlockDev *__restrict__ blocks,
+                                                          const uint16_t *__restrict__ hi,
+                                                          const uint16_t *__restrict__ lo,
+                                                          const double *__restrict__ V, int n1, int n2, int n3,
+                                                          float *__restrict__ part) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const HItem it = items[blockIdx.x];
+    const BlockDev B = blocks[it.block];
+    const int R = B.rows, S = R / ST, G = R / 16;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, q = lane >> 4;
+    u16x8 *PH = reinterpret_cast<u16x8 *>(lds_raw);
+    u16x8 *PL = PH + G * 64;
+    u16x8 *VH = PL + G * 64;
+    u16x8 *VL = VH + G * 64;
+    float *Vb = reinterpret_cast<float *>(VL + G * 64);  // V at the block's 512 points
+    {
+        const int k = tid;  // 512 threads, one point each
+        const int x1 = B.g1 + k / (BLK * BLK), x2 = B.g2 + (k / BLK) % BLK, x3 = B.g3 + k % BLK;
+        Vb[k] = (x1 < n1 && x2 < n2 && x3 < n3) ? (float)V[((int64_t)x1 * n2 + x2) * n3 + x3] : 0.f;
+    }
+    const bool has0 = wave < it.nst, has1 = wave + HB_WAVES < it.nst;
+    int si0 = 0, sj0 = 0, si1 = 0, sj1 = 0;
+    if (has0) st_decode(it.st0 + wave, S, si0, sj0);
+    if (has1) st_decode(it.st0 + wave + HB_WAVES, S, si1, sj1);
+    f32x4 acc0[16], acc1[16];
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+        acc0[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+        acc1[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+    u16x8 fh[NF], fl[NF];
+    // chunk c: fragments g = wave + 8 f of both planes
+    auto prefetch = [&](int c) {
+        const uint16_t *sh = hi + B.tile_off + (int64_t)c * G * FRAG;
+        const uint16_t *sl = lo + B.tile_off + (int64_t)c * G * FRAG;
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            const int g = wave + f * HB_WAVES;
+            if (g < G) {
+                fh[f] = *reinterpret_cast<const u16x8 *>(sh + (g * 64 + lane) * 8);
+                fl[f] = *reinterpret_cast<const u16x8 *>(sl + (g * 64 + lane) * 8);
+            }
+        }
+    };
+    // chunk c is touched two iterations before it is staged (c - 2), loaded
+    // into registers one iteration before (c - 1); touches alternate between
+    // two register slots so that no register copy waits on a load in flight
+    auto step = [&](int c, Touch &slot) {
+        __syncthreads();
+        // V at this lane's eight points k = 32 c + 8 q + e
+        const f32x8 vv = *reinterpret_cast<const f32x8 *>(&Vb[c * CHUNK + 8 * q]);
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            const int g = wave + f * HB_WAVES;
+            if (g < G) {
+                const int at = g * 64 + lane;
+                PH[at] = fh[f];
+                PL[at] = fl[f];
+                u16x8 xh, xl;
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const float x = (bf_bits_to_f32(fh[f][e]) + bf_bits_to_f32(fl[f][e])) * vv[e];
+                    const __bf16 h = (__bf16)x;
+                    xh[e] = __builtin_bit_cast(uint16_t, h);
+                    xl[e] = __builtin_bit_cast(uint16_t, (__bf16)(x - (float)h));
+                }
+                VH[at] = xh;
+                VL[at] = xl;
+            }
+        }
+        release(slot);
+        __syncthreads();
+        if (c + 1 < NCHUNK) prefetch(c + 1);
+        if (c + 2 < NCHUNK) slot = touch_chunk(hi, lo, B.tile_off + (int64_t)(c + 2) * G * FRAG, R, tid);
+        if (has0) tile_bf16x3(PH, PL, VH, VL, si0, sj0, lane, acc0);
+        if (has1) tile_bf16x3(PH, PL, VH, VL, si1, sj1, lane, acc1);
+    };
+    Touch t0{0u, 0u}, t1 = touch_chunk(hi, lo, B.tile_off + (int64_t)G * FRAG, R, tid);
+    prefetch(0);
+    for (int c = 0; c < NCHUNK; c += 2) {
+        step(c, t0);
+        step(c + 1, t1);
+    }
+    // lane holds D[4q + v][l16] of each 16x16 tile
+    const int l16 = lane & 15;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        if (h == 0 ? !has0 : !has1) continue;
+        const int st = it.st0 + wave + h * HB_WAVES;
+        float *out = part + (B.part_off + st) * (int64_t)(ST * ST);
+        const f32x4 *acc = h == 0 ? acc0 : acc1;
+#pragma unroll
+        for (int t = 0; t < 16; t++) {
+            const int rt = t >> 2, ct = t & 3;
+#pragma unroll
+            for (int v = 0; v < 4; v++) out[(rt * 16 + 4 * q + v) * ST + ct * 16 + l16] = acc[t][v];
+        }
+    }
+}
+
+// ---- fp32 form --------------------------------------------------------------
+__device__ __forceinline__ void tile_f32(const float *__restrict__ A32, const float *__restrict__ Vs, int si, int sj,
+                                         int l16, int q, f32x4 *acc) {
+    f32x8 a[4];
+#pragma unroll
+    for (int rt = 0; rt < 4; rt++) {
+        const int row = si * ST + rt * 16 + l16;
+        a[rt] = *reinterpret_cast<const f32x8 *>(&A32[row * LROW32 + q * 8]);
+    }
+#pragma unroll
+    for (int ct = 0; ct < 4; ct++) {
+        const int row = sj * ST + ct * 16 + l16;
+        f32x8 b = *reinterpret_cast<const f32x8 *>(&A32[row * LROW32 + q * 8]);
+#pragma unroll
+        for (int e = 0; e < 8; e++) b[e] *= Vs[q * 8 + e];
+#pragma unroll
+        for (int rt = 0; rt < 4; rt++) {
+            f32x4 c = acc[rt * 4 + ct];
+#pragma unroll
+            for (int s = 0; s < 8; s++) c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt][s], b[s], c, 0, 0, 0);
+            acc[rt * 4 + ct] = c;
+        }
+    }
+}
+
+__global__ __launch_bounds__(64 * HB_WAVES) void k_hblk_f32(const HItem *__restrict__ items,
+                                                           const BlockDev *__restrict__ blocks,
+                                                           const float *__restrict__ t32, const double *__restrict__ V,
+                                                           int n1, int n2, int n3, float *__restrict__ part) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const HItem it = items[blockIdx.x];
     const BlockDev B = blocks[it.block];
     const int R = B.rows, S = R / ST;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, q = lane >> 4;
-    uint16_t *Ahi = reinterpret_cast<uint16_t *>(lds_raw);
-    uint16_t *Alo = Ahi + R * LROW;
     float *A32 = reinterpret_cast<float *>(lds_raw);
-    float *Vs = MODE == LCPAO_BF16X3 ? reinterpret_cast<float *>(Alo + R * LROW) : A32 + R * LROW32;
+    float *Vs = A32 + R * LROW32;
     const bool has0 = wave < it.nst, has1 = wave + HB_WAVES < it.nst;
     int si0 = 0, sj0 = 0, si1 = 0, sj1 = 0;
     if (has0) st_decode(it.st0 + wave, S, si0, sj0);
@@ -175,32 +310,21 @@ __global__ __launch_bounds__(64 * HB_WAVES) void k_hblk(const HItem *__restrict_
     }
     for (int c = 0; c < NCHUNK; c++) {
         __syncthreads();
-        if constexpr (MODE == LCPAO_BF16X3) {
-            const uint16_t *sh = hi + B.tile_off + (int64_t)c * R * CHUNK;
-            const uint16_t *sl = lo + B.tile_off + (int64_t)c * R * CHUNK;
-            for (int e = tid * 8; e < R * CHUNK; e += 64 * HB_WAVES * 8) {
-                const int row = e / CHUNK, col = e % CHUNK;
-                *reinterpret_cast<u16x8 *>(&Ahi[row * LROW + col]) = *reinterpret_cast<const u16x8 *>(&sh[e]);
-                *reinterpret_cast<u16x8 *>(&Alo[row * LROW + col]) = *reinterpret_cast<const u16x8 *>(&sl[e]);
-            }
-        } else {
-            const float *s32 = t32 + B.tile_off + (int64_t)c * R * CHUNK;
-            for (int e = tid * 4; e < R * CHUNK; e += 64 * HB_WAVES * 4) {
-                const int row = e / CHUNK, col = e % CHUNK;
-                *reinterpret_cast<f32x4 *>(&A32[row * LROW32 + col]) = *reinterpret_cast<const f32x4 *>(&s32[e]);
-            }
+        const float *s32 = t32 + B.tile_off + (int64_t)c * R * CHUNK;
+        for (int e = tid * 4; e < R * CHUNK; e += 64 * HB_WAVES * 4) {
+            const int row = e / CHUNK, col = e % CHUNK;
+            *reinterpret_cast<f32x4 *>(&A32[row * LROW32 + col]) = *reinterpret_cast<const f32x4 *>(&s32[e]);
         }
         if (tid < CHUNK) {
             const int k = c * CHUNK + tid;
             const int x1 = B.g1 + k / (BLK * BLK), x2 = B.g2 + (k / BLK) % BLK, x3 = B.g3 + k % BLK;
             const float v = (x1 < n1 && x2 < n2 && x3 < n3) ? (float)V[((int64_t)x1 * n2 + x2) * n3 + x3] : 0.f;
-            Vs[MODE == LCPAO_BF16X3 ? tid : perm32(tid)] = v;
+            Vs[perm32(tid)] = v;
         }
         __syncthreads();
-        if (has0) supertile<MODE>(Ahi, Alo, A32, Vs, si0, sj0, l16, q, acc0);
-        if (has1) supertile<MODE>(Ahi, Alo, A32, Vs, si1, sj1, l16, q, acc1);
+        if (has0) tile_f32(A32, Vs, si0, sj0, l16, q, acc0);
+        if (has1) tile_f32(A32, Vs, si1, sj1, l16, q, acc1);
     }
-    // lane holds D[4q + v][l16] of each 16x16 tile
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         if (h == 0 ? !has0 : !has1) continue;
@@ -306,7 +430,7 @@ void plan_free(BlockPlan *p) {
     if (!p) return;
     void *ptrs[] = {p->d_blocks, p->d_items, p->d_sph_block, p->d_sph_atom, p->d_sph_row0, p->d_sph_map,
                     p->d_contrib, p->d_contrib_off, p->d_hi, p->d_lo, p->d_t32, p->d_part, p->d_thi, p->d_tlo,
-                    p->d_w32, p->d_contrib_pair};
+                    p->d_rowinfo, p->d_spair, p->d_contrib_pair};
     for (void *x : ptrs)
         if (x) (void)hipFree(x);
     delete p;
@@ -369,7 +493,7 @@ int plan_build(lcpao_ctx *ctx) {
     }
     P->blocks.resize(NB);
     std::vector<int64_t> sph_first(NB + 1, 0);
-    int64_t toff = 0, poff = 0;
+    int64_t toff = 0, poff = 0, spoff = 0, roff = 0;
     for (int64_t b = 0; b < NB; b++) {
         int r = 0;
         rows0[b].resize(sph[b].size());
@@ -383,9 +507,13 @@ int plan_build(lcpao_ctx *ctx) {
         P->max_rows = std::max(P->max_rows, R);
         const int S = R / ST, nst = S * (S + 1) / 2;
         const int b1 = (int)(b / ((int64_t)P->nb[1] * P->nb[2])), b2 = (int)((b / P->nb[2]) % P->nb[1]), b3 = (int)(b % P->nb[2]);
-        P->blocks[b] = {toff, poff, R, b1 * BLK, b2 * BLK, b3 * BLK};
+        const int ns = (int)sph[b].size();
+        if (roff + R > INT32_MAX) { ctx->err = "too many grid blocks for the split-precision plan"; delete P; return LCPAO_ERANGE; }
+        P->blocks[b] = {toff, poff, spoff, R, b1 * BLK, b2 * BLK, b3 * BLK, ns, (int32_t)roff};
         toff += (int64_t)R * BLK_PTS;
         poff += nst;
+        spoff += (int64_t)ns * ns;
+        roff += R;
         for (int st0 = 0; st0 < nst; st0 += HB_WAVES * ST_PER_WAVE)
             P->items.push_back({(int32_t)b, st0, std::min(HB_WAVES * ST_PER_WAVE, nst - st0), 0});
         sph_first[b + 1] = sph_first[b] + (int64_t)sph[b].size();
@@ -393,6 +521,22 @@ int plan_build(lcpao_ctx *ctx) {
     P->tile_elems = toff;
     P->part_tiles = poff;
     P->nsph = sph_first[NB];
+    // items grouped by row count (one launch per group, LDS sized for its R)
+    std::stable_sort(P->items.begin(), P->items.end(),
+                     [&](const HItem &x, const HItem &y) { return P->blocks[x.block].rows < P->blocks[y.block].rows; });
+    for (size_t i = 0; i < P->items.size(); i++) {
+        const int R = P->blocks[P->items[i].block].rows;
+        if (P->classes.empty() || P->classes.back().rows != R) P->classes.push_back({R, (int32_t)i, 0});
+        P->classes.back().count++;
+    }
+    // row descriptors: slot * 32 + orbital (-1 for padding rows)
+    std::vector<int16_t> rowinfo(std::max<int64_t>(1, roff), -1);
+    for (int64_t b = 0; b < NB; b++)
+        for (size_t x = 0; x < sph[b].size(); x++)
+            for (int i = 0; i < ctx->no[sph[b][x].first]; i++)
+                rowinfo[P->blocks[b].row_off + rows0[b][x] + i] = (int16_t)(x * 32 + i);
+    if (ctx->my_hsize > INT32_MAX) { ctx->err = "pair blocks exceed the split-precision index range"; delete P; return LCPAO_ERANGE; }
+    std::vector<SlotPair> spairs(std::max<int64_t>(1, spoff), SlotPair{-1, 0});
     // sphere tables and point maps
     std::vector<int32_t> sb(P->nsph), sa(P->nsph), sr(P->nsph), smap((size_t)P->nsph * BLK_PTS, -1);
     std::vector<uint64_t> mask((size_t)P->nsph * (BLK_PTS / 64), 0);
@@ -423,7 +567,7 @@ int plan_build(lcpao_ctx *ctx) {
     }
     std::vector<int32_t> canon_pos(ctx->npair, -1);
     for (size_t c = 0; c < ctx->canon.size(); c++) canon_pos[ctx->canon[c]] = (int32_t)c;
-    struct CRec { int32_t cpos; Contrib c; };
+    struct CRec { int32_t cpos; Contrib c; int32_t x, y; };
     std::vector<std::vector<CRec>> recs(nthreads);
     std::vector<std::vector<int64_t>> olg_count(nthreads, std::vector<int64_t>(ctx->npair, 0));
     {
@@ -453,7 +597,7 @@ int plan_build(lcpao_ctx *ctx) {
                             if (pxy >= 0 && canon_pos[pxy] >= 0) pc = pxy;
                             else if (pyx >= 0 && canon_pos[pyx] >= 0) { pc = pyx; trans = 1; }
                             if (pc < 0) continue;
-                            recs[w].push_back({canon_pos[pc], {(int32_t)b, rows0[b][x], rows0[b][y], trans}});
+                            recs[w].push_back({canon_pos[pc], {(int32_t)b, rows0[b][x], rows0[b][y], trans}, (int32_t)x, (int32_t)y});
                         }
                 }
             });
@@ -480,6 +624,9 @@ int plan_build(lcpao_ctx *ctx) {
         contrib[i] = all[i].c;
         cpair[i] = ctx->canon[all[i].cpos];
         coff[all[i].cpos + 1]++;
+        const BlockDev &B = P->blocks[all[i].c.block];
+        const PairDev &Q = ctx->pairs[cpair[i]];
+        spairs[B.sp_off + (int64_t)all[i].x * B.nslot + all[i].y] = SlotPair{(int32_t)Q.hoff, Q.nob | (all[i].c.trans << 6)};
     }
     P->ncontrib = (int64_t)all.size();
     for (size_t c = 0; c < ctx->canon.size(); c++) coff[c + 1] += coff[c];
@@ -488,7 +635,8 @@ int plan_build(lcpao_ctx *ctx) {
     if ((rc = up(ctx, &P->d_blocks, P->blocks)) || (rc = up(ctx, &P->d_items, P->items)) ||
         (rc = up(ctx, &P->d_sph_block, sb)) || (rc = up(ctx, &P->d_sph_atom, sa)) || (rc = up(ctx, &P->d_sph_row0, sr)) ||
         (rc = up(ctx, &P->d_sph_map, smap)) || (rc = up(ctx, &P->d_contrib, contrib)) || (rc = up(ctx, &P->d_contrib_off, coff)) ||
-        (rc = up(ctx, &P->d_contrib_pair, cpair))) {
+        (rc = up(ctx, &P->d_contrib_pair, cpair)) || (rc = up(ctx, &P->d_rowinfo, rowinfo)) ||
+        (rc = up(ctx, &P->d_spair, spairs))) {
         ctx->plan = nullptr;
         plan_free(P);
         return rc;
@@ -535,21 +683,35 @@ int split_hamiltonian(lcpao_ctx *ctx, int nch, const double *const *d_V, double 
     if (rc) return rc;
     BlockPlan *P = ctx->plan;
     const bool bf = precision == LCPAO_BF16X3;
-    const size_t lds = bf ? (size_t)P->max_rows * LROW * 2 * sizeof(uint16_t) + CHUNK * sizeof(float)
-                          : (size_t)P->max_rows * LROW32 * sizeof(float) + CHUNK * sizeof(float);
-    const unsigned nitems = (unsigned)P->items.size();
     for (int c = 0; c < nch; c++) {
         if (!d_V[c] || !d_H[c]) { ctx->err = "null channel pointer"; return LCPAO_EINVAL; }
-        if (nitems) {
+        if (!P->items.empty()) {
             KTimer kt(ctx, KID_HAM);
-            if (bf)
-                hipLaunchKernelGGL(k_hblk<LCPAO_BF16X3>, dim3(nitems), dim3(64 * HB_WAVES), lds, ctx->stream, P->d_items,
-                                   P->d_blocks, P->d_hi, P->d_lo, P->d_t32, d_V[c], ctx->ngrid[0], ctx->ngrid[1],
-                                   ctx->ngrid[2], P->d_part);
-            else
-                hipLaunchKernelGGL(k_hblk<LCPAO_FP32>, dim3(nitems), dim3(64 * HB_WAVES), lds, ctx->stream, P->d_items,
-                                   P->d_blocks, P->d_hi, P->d_lo, P->d_t32, d_V[c], ctx->ngrid[0], ctx->ngrid[1],
-                                   ctx->ngrid[2], P->d_part);
+            for (const ItemClass &ic : P->classes) {
+                if (bf) {
+                    const size_t lds = (size_t)4 * ic.rows * CHUNK * sizeof(uint16_t) + BLK_PTS * sizeof(float);
+                    auto kern = k_hblk_bf<MAXF>;
+                    switch (nfrag(ic.rows)) {
+                        case 1: kern = k_hblk_bf<1>; break;
+                        case 2: kern = k_hblk_bf<2>; break;
+                        case 3: kern = k_hblk_bf<3>; break;
+                        case 4: kern = k_hblk_bf<4>; break;
+                        default: break;
+                    }
+                    SPLIT_TRY(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                    hipLaunchKernelGGL(kern, dim3((unsigned)ic.count), dim3(64 * HB_WAVES), lds, ctx->stream,
+                                       P->d_items + ic.first, P->d_blocks, P->d_hi, P->d_lo, d_V[c], ctx->ngrid[0],
+                                       ctx->ngrid[1], ctx->ngrid[2], P->d_part);
+                } else {
+                    const size_t lds = (size_t)ic.rows * LROW32 * sizeof(float) + CHUNK * sizeof(float);
+                    SPLIT_TRY(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(k_hblk_f32),
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                    hipLaunchKernelGGL(k_hblk_f32, dim3((unsigned)ic.count), dim3(64 * HB_WAVES), lds, ctx->stream,
+                                       P->d_items + ic.first, P->d_blocks, P->d_t32, d_V[c], ctx->ngrid[0],
+                                       ctx->ngrid[1], ctx->ngrid[2], P->d_part);
+                }
+            }
         }
         if (ctx->ncanon > 0) {
             KTimer kt(ctx, KID_HRED);
@@ -587,105 +749,152 @@ int fp32_tables(lcpao_ctx *ctx, bool deriv) {
 // ----------------------------------------------------------------------------
 // block density: rho_k = sum_{si<=sj} Phi_si^T W[si][sj] Phi_sj  (bf16x3)
 //   W = D + D^T on off-diagonal super-tiles, (D + D^T)/2 on diagonal ones,
-//   assembled per call from the folded pair blocks.
+//   gathered per item from the folded pair blocks through the block's
+//   slot-pair table (no W plane in HBM).
 // ----------------------------------------------------------------------------
-// One workgroup per computed pair: its folded density block is read once and
-// written into every block super-tile where the pair's spheres overlap.
-__global__ __launch_bounds__(256) void k_wasm(const int32_t *__restrict__ canon, int64_t ncanon,
-                                              const int32_t *__restrict__ coff, const Contrib *__restrict__ contrib,
-                                              const BlockDev *__restrict__ blocks, const PairDev *__restrict__ pairs,
-                                              const double *__restrict__ Deff, float *__restrict__ w32) {
-    const int64_t ci = blockIdx.x;
-    if (ci >= ncanon) return;
-    const PairDev P = pairs[canon[ci]];
-    const int ne = P.noa * P.nob;
-    const double *D = Deff + P.hoff;
-    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
-        const int i = e / P.nob, j = e - i * P.nob;
-        const double d = D[e];
-        const double dsym = (P.noa == P.nob) ? d + D[j * P.nob + i] : d;  // used only for on-site blocks
-        for (int k = coff[ci]; k < coff[ci + 1]; k++) {
-            const Contrib C = contrib[k];
-            const BlockDev B = blocks[C.block];
-            const int S = B.rows / ST;
-            const bool same = C.r0 == C.c0;
-            const double val = same ? dsym : d;
-            const int r = C.trans ? C.r0 + j : C.r0 + i;
-            const int c = C.trans ? C.c0 + i : C.c0 + j;
-            const int sr = r / ST, sc = c / ST;
-            const int64_t base = B.part_off * (int64_t)(ST * ST);
-            if (sr == sc) {
-                const int64_t st = base + (int64_t)st_index(sr, sr, S) * (ST * ST);
-                w32[st + (r % ST) * ST + (c % ST)] = (float)(0.5 * val);
-                if (!same) w32[st + (c % ST) * ST + (r % ST)] = (float)(0.5 * val);
-            } else if (r < c) {
-                w32[base + (int64_t)st_index(sr, sc, S) * (ST * ST) + (r % ST) * ST + (c % ST)] = (float)val;
-            }
-        }
-    }
+__global__ void k_fill_f64(int64_t n, double *__restrict__ x, double v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
 }
 
-constexpr int TLROW_PAD = 8;  // LDS row padding of the transposed chunk (bf16 elements)
+constexpr int RB_BYTES_PER_ROW = 2 * CHUNK * 2;   // LDS bytes per row: hi + lo point fragments
 
-__global__ __launch_bounds__(64 * HB_WAVES) void k_rblk(const HItem *__restrict__ items, const BlockDev *__restrict__ blocks,
-                                                       const uint16_t *__restrict__ thi, const uint16_t *__restrict__ tlo,
-                                                       const float *__restrict__ w32, int n1, int n2, int n3,
-                                                       double *__restrict__ rho) {
+template <int NF>
+__global__ __launch_bounds__(64 * HB_WAVES) void k_rblk_bf(const HItem *__restrict__ items,
+                                                          const BlockDev *__restrict__ blocks,
+                                                          const uint16_t *__restrict__ thi,
+                                                          const uint16_t *__restrict__ tlo,
+                                                          const int16_t *__restrict__ rowinfo,
+                                                          const SlotPair *__restrict__ spair,
+                                                          const double *__restrict__ Deff, int n1, int n2, int n3,
+                                                          double *__restrict__ rho) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const HItem it = items[blockIdx.x];
     const BlockDev B = blocks[it.block];
-    const int R = B.rows, S = R / ST, LR = R + TLROW_PAD;
+    const int R = B.rows, S = R / ST, G = R / 16, G32 = R / 32;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, q = lane >> 4;
-    uint16_t *Th = reinterpret_cast<uint16_t *>(lds_raw);  // [point][row]
-    uint16_t *Tl = Th + CHUNK * LR;
-    float *rc = reinterpret_cast<float *>(Tl + CHUNK * LR);  // 32 point sums
-    // W fragments of this wave's super-tiles stay in registers for the whole block
+    u16x8 *TH = reinterpret_cast<u16x8 *>(lds_raw);
+    u16x8 *TL = TH + G * 64;
+    float *rc = reinterpret_cast<float *>(TL + G * 64);  // 32 point sums
+    // W fragments of this wave's super-tiles (A operand: W[16 rt + l16][32 ks + 8 q + e])
     bf16x8 wh[2][4][2], wl[2][4][2];
     int si[2] = {0, 0}, sj[2] = {0, 0};
     bool has[2];
+    const int16_t *rinfo = rowinfo + B.row_off;
+    const SlotPair *sp = spair + B.sp_off;
+    // the first two chunks are fetched into L2 while W is gathered
+    Touch t0 = touch_chunk(thi, tlo, B.tile_off, R, tid);
+    Touch t1 = touch_chunk(thi, tlo, B.tile_off + (int64_t)G * FRAG, R, tid);
+    // W of each super-tile is gathered element by element (row t of the tile
+    // in iteration t, column = lane) into a per-wave LDS image laid out as the
+    // A-operand fragments [rt * 2 + ks][hi, lo][lane][8], then read back into
+    // registers.  The chunk buffers are not in use yet, so the image uses them.
+    uint16_t *img = reinterpret_cast<uint16_t *>(lds_raw) + wave * (8 * 2 * FRAG);
+#pragma unroll
     for (int h = 0; h < 2; h++) {
         has[h] = wave + h * HB_WAVES < it.nst;
-        if (!has[h]) continue;
-        const int st = it.st0 + wave + h * HB_WAVES;
-        st_decode(st, S, si[h], sj[h]);
-        const int64_t base = (B.part_off + st) * (int64_t)(ST * ST);
+        if (has[h]) {
+            st_decode(it.st0 + wave + h * HB_WAVES, S, si[h], sj[h]);
+            const bool diag = si[h] == sj[h];
+            const int cinfo = rinfo[sj[h] * ST + lane];
+            const int rowv = rinfo[si[h] * ST + lane];  // row t's descriptor sits in lane t
+            const int cb = cinfo >> 5, cj = cinfo & 31;
+            const int cfrag = lane / 32, cq = (lane % 32) / 8, ce = lane % 8;
+            // 16 rows at a time, in three branch-free phases so that the loads
+            // of a phase issue back to back: slot pairs, then D, then the split
+#pragma unroll 1
+            for (int t0 = 0; t0 < ST; t0 += 16) {
+                SlotPair pv[16];
 #pragma unroll
-        for (int rt = 0; rt < 4; rt++)
+                for (int u = 0; u < 16; u++) {
+                    const int rinf = __builtin_amdgcn_readlane(rowv, t0 + u);
+                    const int ra = rinf >> 5;
+                    const bool valid = rinf >= 0 && cinfo >= 0;
+                    pv[u] = sp[valid ? min(ra, cb) * B.nslot + max(ra, cb) : 0];
+                }
+                double d1[16], d2[16];
 #pragma unroll
-            for (int ks = 0; ks < 2; ks++) {
-                const int64_t idx = base + (rt * 16 + l16) * ST + ks * 32 + q * 8;
-                const f32x8 w = *reinterpret_cast<const f32x8 *>(&w32[idx]);
+                for (int u = 0; u < 16; u++) {
+                    const int rinf = __builtin_amdgcn_readlane(rowv, t0 + u);
+                    const int ra = rinf >> 5, ri = rinf & 31;
+                    const bool sw = ra > cb;  // lower triangle of a diagonal super-tile: W is symmetric
+                    const int i = sw ? cj : ri, j = sw ? ri : cj;
+                    const bool ok = rinf >= 0 && cinfo >= 0 && pv[u].hoff >= 0;
+                    const int ld = pv[u].meta & 63;
+                    const int x1 = (pv[u].meta >> 6) ? j * ld + i : i * ld + j;
+                    d1[u] = Deff[ok ? pv[u].hoff + x1 : 0];
+                    d2[u] = Deff[ok ? pv[u].hoff + j * ld + i : 0];
+                }
 #pragma unroll
-                for (int e = 0; e < 8; e++) {
-                    const __bf16 hb = (__bf16)w[e];
-                    wh[h][rt][ks][e] = hb;
-                    wl[h][rt][ks][e] = (__bf16)(w[e] - (float)hb);
+                for (int u = 0; u < 16; u++) {
+                    const int rinf = __builtin_amdgcn_readlane(rowv, t0 + u);
+                    const int ra = rinf >> 5;
+                    const bool ok = rinf >= 0 && cinfo >= 0 && pv[u].hoff >= 0;
+                    // D is finite, so masking by products keeps the loads unconditional
+                    const double m1 = ok ? (diag ? 0.5 : 1.0) : 0.0;
+                    const double m2 = ra == cb ? m1 : 0.0;
+                    const float w = (float)(d1[u] * m1 + d2[u] * m2);
+                    const __bf16 hb = (__bf16)w;
+                    const int t = t0 + u;
+                    const int frag = (t / 16) * 2 + cfrag, at = ((cq * 16 + t % 16) * 8) + ce;
+                    img[(frag * 2 + 0) * FRAG + at] = __builtin_bit_cast(uint16_t, hb);
+                    img[(frag * 2 + 1) * FRAG + at] = __builtin_bit_cast(uint16_t, (__bf16)(w - (float)hb));
                 }
             }
-    }
-    for (int c = 0; c < NCHUNK; c++) {
-        __syncthreads();
-        const uint16_t *sh = thi + B.tile_off + (int64_t)c * CHUNK * R;
-        const uint16_t *sl = tlo + B.tile_off + (int64_t)c * CHUNK * R;
-        for (int e = tid * 8; e < CHUNK * R; e += 64 * HB_WAVES * 8) {
-            const int pt = e / R, row = e % R;
-            *reinterpret_cast<u16x8 *>(&Th[pt * LR + row]) = *reinterpret_cast<const u16x8 *>(&sh[e]);
-            *reinterpret_cast<u16x8 *>(&Tl[pt * LR + row]) = *reinterpret_cast<const u16x8 *>(&sl[e]);
         }
+        __syncthreads();
+        if (has[h]) {
+            const u16x8 *im = reinterpret_cast<const u16x8 *>(img);
+#pragma unroll
+            for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+                for (int ks = 0; ks < 2; ks++) {
+                    wh[h][rt][ks] = __builtin_bit_cast(bf16x8, im[((rt * 2 + ks) * 2 + 0) * 64 + lane]);
+                    wl[h][rt][ks] = __builtin_bit_cast(bf16x8, im[((rt * 2 + ks) * 2 + 1) * 64 + lane]);
+                }
+        }
+        __syncthreads();
+    }
+    u16x8 fh[NF], fl[NF];
+    auto prefetch = [&](int c) {
+        const uint16_t *sh = thi + B.tile_off + (int64_t)c * G * FRAG;
+        const uint16_t *sl = tlo + B.tile_off + (int64_t)c * G * FRAG;
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            const int g = wave + f * HB_WAVES;
+            if (g < G) {
+                fh[f] = *reinterpret_cast<const u16x8 *>(sh + (g * 64 + lane) * 8);
+                fl[f] = *reinterpret_cast<const u16x8 *>(sl + (g * 64 + lane) * 8);
+            }
+        }
+    };
+    auto step = [&](int c, Touch &slot) {
+        __syncthreads();
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            const int g = wave + f * HB_WAVES;
+            if (g < G) {
+                TH[g * 64 + lane] = fh[f];
+                TL[g * 64 + lane] = fl[f];
+            }
+        }
+        release(slot);
         if (tid < CHUNK) rc[tid] = 0.f;
         __syncthreads();
+        if (c + 1 < NCHUNK) prefetch(c + 1);
+        if (c + 2 < NCHUNK) slot = touch_chunk(thi, tlo, B.tile_off + (int64_t)(c + 2) * G * FRAG, R, tid);
+#pragma unroll
         for (int h = 0; h < 2; h++) {
             if (!has[h]) continue;
-            float part[2] = {0.f, 0.f};
-#pragma unroll
+#pragma unroll 1
             for (int pt = 0; pt < 2; pt++) {
-                const int point = pt * 16 + l16;
+                float part = 0.f;
                 bf16x8 bh[2], bl[2];
 #pragma unroll
                 for (int ks = 0; ks < 2; ks++) {
-                    const int row = sj[h] * ST + ks * 32 + q * 8;
-                    bh[ks] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&Th[point * LR + row]));
-                    bl[ks] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&Tl[point * LR + row]));
+                    const int fi = pt * G32 + sj[h] * 2 + ks;
+                    bh[ks] = __builtin_bit_cast(bf16x8, TH[fi * 64 + lane]);
+                    bl[ks] = __builtin_bit_cast(bf16x8, TL[fi * 64 + lane]);
                 }
 #pragma unroll
                 for (int rt = 0; rt < 4; rt++) {
@@ -696,22 +905,19 @@ __global__ __launch_bounds__(64 * HB_WAVES) void k_rblk(const HItem *__restrict_
                         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[h][rt][ks], bl[ks], acc, 0, 0, 0);
                         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[h][rt][ks], bh[ks], acc, 0, 0, 0);
                     }
-                    // acc[v] = Q[row si*64 + rt*16 + 4q + v][point]
+                    // acc[v] = Q[row si*64 + 16 rt + 4 q + v][point 16 pt + l16]; Phi at the
+                    // same four rows and point: 8 contiguous bytes of a point fragment
+                    const int fi = pt * G32 + si[h] * 2 + (rt >> 1);
+                    const int off = (fi * 64 + (2 * (rt & 1) + (q >> 1)) * 16 + l16) * 8 + 4 * (q & 1);
+                    const u16x4 ph = *reinterpret_cast<const u16x4 *>(reinterpret_cast<const uint16_t *>(TH) + off);
+                    const u16x4 pl = *reinterpret_cast<const u16x4 *>(reinterpret_cast<const uint16_t *>(TL) + off);
 #pragma unroll
-                    for (int v = 0; v < 4; v++) {
-                        const int row = si[h] * ST + rt * 16 + 4 * q + v;
-                        const float ph = (float)__builtin_bit_cast(__bf16, Th[point * LR + row]);
-                        const float pl = (float)__builtin_bit_cast(__bf16, Tl[point * LR + row]);
-                        part[pt] = fmaf(ph + pl, acc[v], part[pt]);
-                    }
+                    for (int v = 0; v < 4; v++)
+                        part = fmaf(bf_bits_to_f32(ph[v]) + bf_bits_to_f32(pl[v]), acc[v], part);
                 }
-            }
-#pragma unroll
-            for (int pt = 0; pt < 2; pt++) {
-                float x = part[pt];
-                x += __shfl_xor(x, 16, 64);
-                x += __shfl_xor(x, 32, 64);
-                if (q == 0) atomicAdd(&rc[pt * 16 + l16], x);
+                part += __shfl_xor(part, 16, 64);
+                part += __shfl_xor(part, 32, 64);
+                if (q == 0) atomicAdd(&rc[pt * 16 + l16], part);
             }
         }
         __syncthreads();
@@ -720,6 +926,11 @@ __global__ __launch_bounds__(64 * HB_WAVES) void k_rblk(const HItem *__restrict_
             const int x1 = B.g1 + k / (BLK * BLK), x2 = B.g2 + (k / BLK) % BLK, x3 = B.g3 + k % BLK;
             if (x1 < n1 && x2 < n2 && x3 < n3) atomicAdd(&rho[((int64_t)x1 * n2 + x2) * n3 + x3], (double)rc[tid]);
         }
+    };
+    prefetch(0);
+    for (int c = 0; c < NCHUNK; c += 2) {
+        step(c, t0);
+        step(c + 1, t1);
     }
 }
 
@@ -727,27 +938,35 @@ int split_density(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t 
     int rc = split_tables(ctx, LCPAO_BF16X3);
     if (rc) return rc;
     BlockPlan *P = ctx->plan;
-    if (!P->d_w32) {
-        const size_t bytes = std::max<int64_t>(1, P->part_tiles) * ST * ST * sizeof(float);
-        SPLIT_TRY(ctx, hipMalloc(&P->d_w32, bytes));
-        SPLIT_TRY(ctx, hipMemsetAsync(P->d_w32, 0, bytes, ctx->stream));
-    }
-    const size_t lds = (size_t)CHUNK * (P->max_rows + TLROW_PAD) * 2 * sizeof(uint16_t) + CHUNK * sizeof(float);
-    const unsigned nitems = (unsigned)P->items.size();
     for (int s = 0; s < nspin; s++) {
         if (!d_DM[s] || !d_rho[s]) { ctx->err = "null spin pointer"; return LCPAO_EINVAL; }
         const double *D = d_DM[s] + dm_offset;
         if (ctx->ncanon > 0) {
             KTimer kt(ctx, KID_DEFF);
             launch_deff_f64(ctx, D);
-            hipLaunchKernelGGL(k_wasm, dim3((unsigned)ctx->ncanon), dim3(256), 0, ctx->stream, ctx->d_hcanon, ctx->ncanon,
-                               P->d_contrib_off, P->d_contrib, P->d_blocks, ctx->d_pairs, ctx->d_deff, P->d_w32);
         }
-        SPLIT_TRY(ctx, hipMemsetAsync(d_rho[s], 0, (size_t)ctx->nn * sizeof(double), ctx->stream));
-        if (nitems) {
+        if (ctx->nn > 0)
+            hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((ctx->nn + 255) / 256)), dim3(256), 0, ctx->stream, ctx->nn,
+                               d_rho[s], 0.0);
+        if (!P->items.empty()) {
             KTimer kt(ctx, KID_RHO);
-            hipLaunchKernelGGL(k_rblk, dim3(nitems), dim3(64 * HB_WAVES), lds, ctx->stream, P->d_items, P->d_blocks,
-                               P->d_thi, P->d_tlo, P->d_w32, ctx->ngrid[0], ctx->ngrid[1], ctx->ngrid[2], d_rho[s]);
+            for (const ItemClass &ic : P->classes) {
+                const size_t lds = std::max<size_t>((size_t)ic.rows * RB_BYTES_PER_ROW + CHUNK * sizeof(float),
+                                                    (size_t)HB_WAVES * 8 * 2 * FRAG * sizeof(uint16_t));
+                auto kern = k_rblk_bf<MAXF>;
+                switch (nfrag(ic.rows)) {
+                    case 1: kern = k_rblk_bf<1>; break;
+                    case 2: kern = k_rblk_bf<2>; break;
+                    case 3: kern = k_rblk_bf<3>; break;
+                    case 4: kern = k_rblk_bf<4>; break;
+                    default: break;
+                }
+                SPLIT_TRY(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                hipLaunchKernelGGL(kern, dim3((unsigned)ic.count), dim3(64 * HB_WAVES), lds, ctx->stream,
+                                   P->d_items + ic.first, P->d_blocks, P->d_thi, P->d_tlo, P->d_rowinfo, P->d_spair,
+                                   ctx->d_deff, ctx->ngrid[0], ctx->ngrid[1], ctx->ngrid[2], d_rho[s]);
+            }
         }
         SPLIT_TRY(ctx, hipGetLastError());
     }

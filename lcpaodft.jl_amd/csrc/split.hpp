@@ -1,14 +1,22 @@
 // split.hpp -- block-dense split-precision forms of the grid contractions.
 //
 // The home grid is cut into 8x8x8-point blocks.  For each block the sphere
-// images (atom, cell) that cover any of its points are listed and their
-// orbitals concatenated into R rows (padded to a multiple of 64).  The
-// block's orbital values are stored once per table set as "tiles":
-// [chunk of 32 points][row][32 points] in the element type of the form
-// (bf16 hi/lo pairs, or fp32).  The Hamiltonian of a block is then one
-// dense product C_b = Phi_b diag(V) Phi_b^T on the matrix cores, kept in
-// 64x64 super-tiles of the upper triangle; every computed pair sums its
-// sub-blocks of C_b over the blocks in FP64.
+// images (atom, cell) that cover any of its points are listed ("slots") and
+// their orbitals concatenated into R rows (padded to a multiple of 64).  The
+// block's orbital values are stored once per table set as "tiles" in the
+// element type of the form:
+//  * bf16x3: hi and lo bf16 planes, each in two fragment orders (below), so
+//    that every matrix-core operand is one contiguous 16-byte read per lane;
+//  * fp32: [chunk of 32 points][row][32 points, permuted].
+// The Hamiltonian of a block is one dense product C_b = Phi_b diag(V) Phi_b^T
+// on the matrix cores, kept in 64x64 super-tiles of the upper triangle; every
+// computed pair sums its sub-blocks of C_b over the blocks in FP64.
+//
+// bf16 fragment orders (per block, chunk c of 32 points, lane l = 16 q + m):
+//  * row fragments  [c][g = row/16][l][8]: Phi[16 g + m][32 c + 8 q + e]
+//    (the A/B operand of v_mfma_f32_16x16x32_bf16 for 16 rows x 32 points);
+//  * point fragments [c][pt][g = row/32][l][8]: Phi[32 g + 8 q + e][32 c + 16 pt + m]
+//    (the B operand for 32 rows x 16 points).
 #pragma once
 #include "ctx.hpp"
 
@@ -25,12 +33,29 @@ constexpr int ST_PER_WAVE = 2;
 struct BlockDev {
     int64_t tile_off;   // element offset of the block's tiles
     int64_t part_off;   // super-tile offset of the block's partial C
+    int64_t sp_off;     // offset of the block's slot-pair table (nslot x nslot)
     int32_t rows;       // R (multiple of 64)
     int32_t g1, g2, g3; // first grid point of the block
+    int32_t nslot;      // sphere images covering the block
+    int32_t row_off;    // offset of the block's row descriptors (R entries)
+};
+
+// Folded density block of the pair formed by two slots a <= b of a block:
+// W element (orbital i of a, orbital j of b) is Deff[hoff + i*ld + j], or
+// Deff[hoff + j*ld + i] when the computed pair is oriented (b, a).  hoff < 0:
+// the two spheres form no computed pair.
+struct SlotPair {
+    int32_t hoff;
+    int32_t meta;       // ld | trans << 6
 };
 
 struct HItem {
     int32_t block, st0, nst, pad;
+};
+
+// items of one row count R, launched together (LDS sized for R)
+struct ItemClass {
+    int32_t rows, first, count;
 };
 
 // one (pair, block) sub-block of C_b: element (i, j) of H[pair] is
@@ -43,7 +68,8 @@ struct BlockPlan {
     int nb[3] = {0, 0, 0};
     int64_t nblocks = 0;
     std::vector<BlockDev> blocks;
-    std::vector<HItem> items;
+    std::vector<HItem> items;        // sorted by rows
+    std::vector<ItemClass> classes;
     int64_t tile_elems = 0;    // total tile elements (per hi/lo plane)
     int64_t part_tiles = 0;    // total super-tiles
     int32_t max_rows = 0;
@@ -57,10 +83,10 @@ struct BlockPlan {
     Contrib *d_contrib = nullptr;
     int32_t *d_contrib_off = nullptr;  // per canonical pair (ncanon + 1)
     // tiles of the precision forms
-    uint16_t *d_hi = nullptr, *d_lo = nullptr;  // bf16 planes
-    uint16_t *d_thi = nullptr, *d_tlo = nullptr;  // transposed bf16 planes [chunk][point][row]
-    float *d_w32 = nullptr;                       // density super-tiles W (fp32)
-    Contrib *d_contrib_flat = nullptr;            // contributions with their pair (for W assembly)
+    uint16_t *d_hi = nullptr, *d_lo = nullptr;    // bf16 planes, row fragments
+    uint16_t *d_thi = nullptr, *d_tlo = nullptr;  // bf16 planes, point fragments
+    int16_t *d_rowinfo = nullptr;                 // per block row: slot * 32 + orbital, -1 for padding
+    SlotPair *d_spair = nullptr;
     int32_t *d_contrib_pair = nullptr;
     int64_t ncontrib = 0;
     float *d_t32 = nullptr;                     // fp32 tiles (permuted point order)
