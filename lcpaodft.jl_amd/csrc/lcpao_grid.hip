@@ -227,9 +227,11 @@ __global__ void k_ham_reduce(const int32_t *__restrict__ canon, const int32_t *_
 }
 
 // Folded density block of each computed pair: D_P + D_P'^T (or D_P alone).
+// sym_self != 0: an on-site pair (its own transpose) gets D_P + D_P^T, the
+// form the block-dense density reads (one load per W element).
 template <typename T>
 __global__ void k_deff(const int32_t *__restrict__ canon, int64_t ncanon, const PairDev *__restrict__ pairs,
-                       const double *__restrict__ D, T *__restrict__ Deff) {
+                       const double *__restrict__ D, T *__restrict__ Deff, int sym_self = 0) {
     const int64_t c = blockIdx.x;
     if (c >= ncanon) return;
     const int p = canon[c];
@@ -241,6 +243,9 @@ __global__ void k_deff(const int32_t *__restrict__ canon, int64_t ncanon, const 
             const PairDev Q = pairs[P.partner];
             const int i = e / P.nob, j = e - i * P.nob;
             v += D[Q.hoff + (int64_t)j * P.noa + i];
+        } else if (sym_self && P.partner == p) {
+            const int i = e / P.nob, j = e - i * P.nob;
+            v += D[P.hoff + (int64_t)j * P.nob + i];
         }
         Deff[P.hoff + e] = (T)v;
     }
@@ -1140,9 +1145,9 @@ int ham_dev(lcpao_ctx *ctx, int nch, const double *const *d_V, double *const *d_
 
 }  // namespace
 
-void lcpao::launch_deff_f64(lcpao_ctx *ctx, const double *D) {
+void lcpao::launch_deff_f64(lcpao_ctx *ctx, const double *D, int sym_self) {
     hipLaunchKernelGGL(k_deff<double>, dim3((unsigned)ctx->ncanon), dim3(128), 0, ctx->stream, ctx->d_hcanon, ctx->ncanon,
-                       ctx->d_pairs, D, ctx->d_deff);
+                       ctx->d_pairs, D, ctx->d_deff, sym_self);
 }
 
 namespace {

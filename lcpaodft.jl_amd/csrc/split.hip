@@ -124,12 +124,14 @@ struct Touch {
 };
 __device__ __forceinline__ Touch touch_chunk(const uint16_t *__restrict__ hi, const uint16_t *__restrict__ lo,
                                              int64_t off, int R, int tid) {
-    Touch t{0u, 0u};
+    // always exactly two loads per thread (indices past the chunk repeat line
+    // 0), so that the compiler can count them when it places its waits
+    Touch t;
     const int half = R / 2;
-    int x = tid;
-    if (x < R) t.a = *reinterpret_cast<const uint32_t *>((x < half ? hi : lo) + off + (int64_t)(x < half ? x : x - half) * 64);
-    x += 64 * HB_WAVES;
-    if (x < R) t.b = *reinterpret_cast<const uint32_t *>((x < half ? hi : lo) + off + (int64_t)(x < half ? x : x - half) * 64);
+    int x = tid < R ? tid : 0;
+    t.a = *reinterpret_cast<const uint32_t *>((x < half ? hi : lo) + off + (int64_t)(x < half ? x : x - half) * 64);
+    x = tid + 64 * HB_WAVES < R ? tid + 64 * HB_WAVES : 0;
+    t.b = *reinterpret_cast<const uint32_t *>((x < half ? hi : lo) + off + (int64_t)(x < half ? x : x - half) * 64);
     return t;
 }
 __device__ __forceinline__ void release(const Touch &t) { asm volatile("" ::"v"(t.a), "v"(t.b)); }
@@ -198,12 +200,10 @@ __global__ __launch_bounds__(64 * HB_WAVES) void k_hblk_bf(const HItem *__restri
         const uint16_t *sh = hi + B.tile_off + (int64_t)c * G * FRAG;
         const uint16_t *sl = lo + B.tile_off + (int64_t)c * G * FRAG;
 #pragma unroll
-        for (int f = 0; f < NF; f++) {
-            const int g = wave + f * HB_WAVES;
-            if (g < G) {
-                fh[f] = *reinterpret_cast<const u16x8 *>(sh + (g * 64 + lane) * 8);
-                fl[f] = *reinterpret_cast<const u16x8 *>(sl + (g * 64 + lane) * 8);
-            }
+        for (int f = 0; f < NF; f++) {  // unconditional (clamped) loads: see touch_chunk
+            const int g = min(wave + f * HB_WAVES, G - 1);
+            fh[f] = *reinterpret_cast<const u16x8 *>(sh + (g * 64 + lane) * 8);
+            fl[f] = *reinterpret_cast<const u16x8 *>(sl + (g * 64 + lane) * 8);
         }
     };
     // chunk c is touched two iterations before it is staged (c - 2), loaded
@@ -213,34 +213,39 @@ __global__ __launch_bounds__(64 * HB_WAVES) void k_hblk_bf(const HItem *__restri
         __syncthreads();
         // V at this lane's eight points k = 32 c + 8 q + e
         const f32x8 vv = *reinterpret_cast<const f32x8 *>(&Vb[c * CHUNK + 8 * q]);
+        // branch-free: a wave past the last fragment rewrites fragment G - 1
+        // with the same values (no branch keeps the compiler's load counts exact)
 #pragma unroll
         for (int f = 0; f < NF; f++) {
-            const int g = wave + f * HB_WAVES;
-            if (g < G) {
-                const int at = g * 64 + lane;
-                PH[at] = fh[f];
-                PL[at] = fl[f];
-                u16x8 xh, xl;
+            const int g = min(wave + f * HB_WAVES, G - 1);
+            const int at = g * 64 + lane;
+            PH[at] = fh[f];
+            PL[at] = fl[f];
+            u16x8 xh, xl;
 #pragma unroll
-                for (int e = 0; e < 8; e++) {
-                    const float x = (bf_bits_to_f32(fh[f][e]) + bf_bits_to_f32(fl[f][e])) * vv[e];
-                    const __bf16 h = (__bf16)x;
-                    xh[e] = __builtin_bit_cast(uint16_t, h);
-                    xl[e] = __builtin_bit_cast(uint16_t, (__bf16)(x - (float)h));
-                }
-                VH[at] = xh;
-                VL[at] = xl;
+            for (int e = 0; e < 8; e++) {
+                const float x = (bf_bits_to_f32(fh[f][e]) + bf_bits_to_f32(fl[f][e])) * vv[e];
+                const __bf16 h = (__bf16)x;
+                xh[e] = __builtin_bit_cast(uint16_t, h);
+                xl[e] = __builtin_bit_cast(uint16_t, (__bf16)(x - (float)h));
             }
+            VH[at] = xh;
+            VL[at] = xl;
         }
         release(slot);
         __syncthreads();
-        if (c + 1 < NCHUNK) prefetch(c + 1);
-        if (c + 2 < NCHUNK) slot = touch_chunk(hi, lo, B.tile_off + (int64_t)(c + 2) * G * FRAG, R, tid);
+        // past the last chunk these reload chunk 15 (cache hits), keeping the
+        // number of loads in flight the same in every step
+        prefetch(min(c + 1, NCHUNK - 1));
+        slot = touch_chunk(hi, lo, B.tile_off + (int64_t)min(c + 2, NCHUNK - 1) * G * FRAG, R, tid);
         if (has0) tile_bf16x3(PH, PL, VH, VL, si0, sj0, lane, acc0);
         if (has1) tile_bf16x3(PH, PL, VH, VL, si1, sj1, lane, acc1);
     };
     Touch t0{0u, 0u}, t1 = touch_chunk(hi, lo, B.tile_off + (int64_t)G * FRAG, R, tid);
     prefetch(0);
+    // fully unrolled: with no loop back-edge the compiler's waits count the
+    // loads exactly and never wait for a touch issued in the step before
+#pragma unroll
     for (int c = 0; c < NCHUNK; c += 2) {
         step(c, t0);
         step(c + 1, t1);
@@ -510,6 +515,7 @@ int plan_build(lcpao_ctx *ctx) {
         const int ns = (int)sph[b].size();
         if (roff + R > INT32_MAX) { ctx->err = "too many grid blocks for the split-precision plan"; delete P; return LCPAO_ERANGE; }
         P->blocks[b] = {toff, poff, spoff, R, b1 * BLK, b2 * BLK, b3 * BLK, ns, (int32_t)roff};
+        P->max_slots = std::max(P->max_slots, ns);
         toff += (int64_t)R * BLK_PTS;
         poff += nst;
         spoff += (int64_t)ns * ns;
@@ -758,6 +764,7 @@ __global__ void k_fill_f64(int64_t n, double *__restrict__ x, double v) {
 }
 
 constexpr int RB_BYTES_PER_ROW = 2 * CHUNK * 2;   // LDS bytes per row: hi + lo point fragments
+constexpr int RB_IMAGE_BYTES = HB_WAVES * 8 * 2 * 512 * 2;  // per-wave W images (16 KB each)
 
 template <int NF>
 __global__ __launch_bounds__(64 * HB_WAVES) void k_rblk_bf(const HItem *__restrict__ items,
@@ -775,70 +782,76 @@ __global__ __launch_bounds__(64 * HB_WAVES) void k_rblk_bf(const HItem *__restri
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, q = lane >> 4;
     u16x8 *TH = reinterpret_cast<u16x8 *>(lds_raw);
     u16x8 *TL = TH + G * 64;
-    float *rc = reinterpret_cast<float *>(TL + G * 64);  // 32 point sums
+    // LDS: [0, RB_IMAGE_BYTES) the W images, later the chunk planes; then the
+    // block's point sums, row descriptors and slot-pair table
+    float *rc = reinterpret_cast<float *>(lds_raw + RB_IMAGE_BYTES);
+    int16_t *s_rinfo = reinterpret_cast<int16_t *>(rc + BLK_PTS);
+    SlotPair *s_sp = reinterpret_cast<SlotPair *>(s_rinfo + MAX_ROWS);
+    rc[tid] = 0.f;  // 512 threads, 512 points
+    for (int x = tid; x < R; x += 64 * HB_WAVES) s_rinfo[x] = rowinfo[B.row_off + x];
+    for (int x = tid; x < B.nslot * B.nslot; x += 64 * HB_WAVES) s_sp[x] = spair[B.sp_off + x];
     // W fragments of this wave's super-tiles (A operand: W[16 rt + l16][32 ks + 8 q + e])
     bf16x8 wh[2][4][2], wl[2][4][2];
     int si[2] = {0, 0}, sj[2] = {0, 0};
     bool has[2];
-    const int16_t *rinfo = rowinfo + B.row_off;
-    const SlotPair *sp = spair + B.sp_off;
     // the first two chunks are fetched into L2 while W is gathered
     Touch t0 = touch_chunk(thi, tlo, B.tile_off, R, tid);
     Touch t1 = touch_chunk(thi, tlo, B.tile_off + (int64_t)G * FRAG, R, tid);
-    // W of each super-tile is gathered element by element (row t of the tile
-    // in iteration t, column = lane) into a per-wave LDS image laid out as the
-    // A-operand fragments [rt * 2 + ks][hi, lo][lane][8], then read back into
-    // registers.  The chunk buffers are not in use yet, so the image uses them.
+    __syncthreads();
+    // W of each super-tile is gathered into a per-wave LDS image laid out as
+    // the A-operand fragments [rt * 2 + ks][hi, lo][lane][8], then read back
+    // into registers.  lane = (column group cg = lane / 8, row rr = lane % 8):
+    // the lane fills W[8 t8 + rr][8 cg .. 8 cg + 7], one 16-byte piece of a
+    // fragment per plane.  W element = mask * Deff[one index]: on-site blocks
+    // hold D + D^T, so every element is a single load; 32 loads per batch.
     uint16_t *img = reinterpret_cast<uint16_t *>(lds_raw) + wave * (8 * 2 * FRAG);
+    const int rr = lane & 7, cg = lane >> 3;
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         has[h] = wave + h * HB_WAVES < it.nst;
         if (has[h]) {
             st_decode(it.st0 + wave + h * HB_WAVES, S, si[h], sj[h]);
-            const bool diag = si[h] == sj[h];
-            const int cinfo = rinfo[sj[h] * ST + lane];
-            const int rowv = rinfo[si[h] * ST + lane];  // row t's descriptor sits in lane t
-            const int cb = cinfo >> 5, cj = cinfo & 31;
-            const int cfrag = lane / 32, cq = (lane % 32) / 8, ce = lane % 8;
-            // 16 rows at a time, in three branch-free phases so that the loads
-            // of a phase issue back to back: slot pairs, then D, then the split
+            const float wdiag = si[h] == sj[h] ? 0.5f : 1.f;
+            int ci[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) ci[e] = s_rinfo[sj[h] * ST + cg * 8 + e];
 #pragma unroll 1
-            for (int t0 = 0; t0 < ST; t0 += 16) {
-                SlotPair pv[16];
+            for (int tb = 0; tb < 8; tb += 4) {
+                double dv[4][8];
+                uint32_t okbits = 0;
 #pragma unroll
-                for (int u = 0; u < 16; u++) {
-                    const int rinf = __builtin_amdgcn_readlane(rowv, t0 + u);
-                    const int ra = rinf >> 5;
-                    const bool valid = rinf >= 0 && cinfo >= 0;
-                    pv[u] = sp[valid ? min(ra, cb) * B.nslot + max(ra, cb) : 0];
-                }
-                double d1[16], d2[16];
-#pragma unroll
-                for (int u = 0; u < 16; u++) {
-                    const int rinf = __builtin_amdgcn_readlane(rowv, t0 + u);
+                for (int r4 = 0; r4 < 4; r4++) {
+                    const int rinf = s_rinfo[si[h] * ST + (tb + r4) * 8 + rr];
                     const int ra = rinf >> 5, ri = rinf & 31;
-                    const bool sw = ra > cb;  // lower triangle of a diagonal super-tile: W is symmetric
-                    const int i = sw ? cj : ri, j = sw ? ri : cj;
-                    const bool ok = rinf >= 0 && cinfo >= 0 && pv[u].hoff >= 0;
-                    const int ld = pv[u].meta & 63;
-                    const int x1 = (pv[u].meta >> 6) ? j * ld + i : i * ld + j;
-                    d1[u] = Deff[ok ? pv[u].hoff + x1 : 0];
-                    d2[u] = Deff[ok ? pv[u].hoff + j * ld + i : 0];
+#pragma unroll
+                    for (int e = 0; e < 8; e++) {
+                        const int cb = ci[e] >> 5, cj = ci[e] & 31;
+                        const bool valid = rinf >= 0 && ci[e] >= 0;
+                        const SlotPair P = s_sp[valid ? min(ra, cb) * B.nslot + max(ra, cb) : 0];
+                        const bool sw = ra > cb;  // lower triangle of a diagonal super-tile: W is symmetric
+                        const int i = sw ? cj : ri, j = sw ? ri : cj;
+                        const bool ok = valid && P.hoff >= 0;
+                        const int ld = P.meta & 63;
+                        const int x1 = (P.meta >> 6) ? j * ld + i : i * ld + j;
+                        dv[r4][e] = Deff[ok ? P.hoff + x1 : 0];  // unconditional: loads issue back to back
+                        okbits |= (ok ? 1u : 0u) << (r4 * 8 + e);
+                    }
                 }
 #pragma unroll
-                for (int u = 0; u < 16; u++) {
-                    const int rinf = __builtin_amdgcn_readlane(rowv, t0 + u);
-                    const int ra = rinf >> 5;
-                    const bool ok = rinf >= 0 && cinfo >= 0 && pv[u].hoff >= 0;
-                    // D is finite, so masking by products keeps the loads unconditional
-                    const double m1 = ok ? (diag ? 0.5 : 1.0) : 0.0;
-                    const double m2 = ra == cb ? m1 : 0.0;
-                    const float w = (float)(d1[u] * m1 + d2[u] * m2);
-                    const __bf16 hb = (__bf16)w;
-                    const int t = t0 + u;
-                    const int frag = (t / 16) * 2 + cfrag, at = ((cq * 16 + t % 16) * 8) + ce;
-                    img[(frag * 2 + 0) * FRAG + at] = __builtin_bit_cast(uint16_t, hb);
-                    img[(frag * 2 + 1) * FRAG + at] = __builtin_bit_cast(uint16_t, (__bf16)(w - (float)hb));
+                for (int r4 = 0; r4 < 4; r4++) {
+                    u16x8 wh8, wl8;
+#pragma unroll
+                    for (int e = 0; e < 8; e++) {
+                        // D is finite, so masking by a product keeps the load unconditional
+                        const float w = (float)dv[r4][e] * (((okbits >> (r4 * 8 + e)) & 1u) ? wdiag : 0.f);
+                        const __bf16 hb = (__bf16)w;
+                        wh8[e] = __builtin_bit_cast(uint16_t, hb);
+                        wl8[e] = __builtin_bit_cast(uint16_t, (__bf16)(w - (float)hb));
+                    }
+                    const int row = (tb + r4) * 8 + rr;
+                    const int frag = (row / 16) * 2 + cg / 4, at = ((cg % 4) * 16 + row % 16) * 8;
+                    *reinterpret_cast<u16x8 *>(&img[(frag * 2 + 0) * FRAG + at]) = wh8;
+                    *reinterpret_cast<u16x8 *>(&img[(frag * 2 + 1) * FRAG + at]) = wl8;
                 }
             }
         }
@@ -860,35 +873,32 @@ __global__ __launch_bounds__(64 * HB_WAVES) void k_rblk_bf(const HItem *__restri
         const uint16_t *sh = thi + B.tile_off + (int64_t)c * G * FRAG;
         const uint16_t *sl = tlo + B.tile_off + (int64_t)c * G * FRAG;
 #pragma unroll
-        for (int f = 0; f < NF; f++) {
-            const int g = wave + f * HB_WAVES;
-            if (g < G) {
-                fh[f] = *reinterpret_cast<const u16x8 *>(sh + (g * 64 + lane) * 8);
-                fl[f] = *reinterpret_cast<const u16x8 *>(sl + (g * 64 + lane) * 8);
-            }
+        for (int f = 0; f < NF; f++) {  // unconditional (clamped) loads: see touch_chunk
+            const int g = min(wave + f * HB_WAVES, G - 1);
+            fh[f] = *reinterpret_cast<const u16x8 *>(sh + (g * 64 + lane) * 8);
+            fl[f] = *reinterpret_cast<const u16x8 *>(sl + (g * 64 + lane) * 8);
         }
     };
     auto step = [&](int c, Touch &slot) {
         __syncthreads();
 #pragma unroll
-        for (int f = 0; f < NF; f++) {
-            const int g = wave + f * HB_WAVES;
-            if (g < G) {
-                TH[g * 64 + lane] = fh[f];
-                TL[g * 64 + lane] = fl[f];
-            }
+        for (int f = 0; f < NF; f++) {  // branch-free, as in k_hblk_bf
+            const int g = min(wave + f * HB_WAVES, G - 1);
+            TH[g * 64 + lane] = fh[f];
+            TL[g * 64 + lane] = fl[f];
         }
         release(slot);
-        if (tid < CHUNK) rc[tid] = 0.f;
         __syncthreads();
-        if (c + 1 < NCHUNK) prefetch(c + 1);
-        if (c + 2 < NCHUNK) slot = touch_chunk(thi, tlo, B.tile_off + (int64_t)(c + 2) * G * FRAG, R, tid);
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            if (!has[h]) continue;
+        prefetch(min(c + 1, NCHUNK - 1));
+        slot = touch_chunk(thi, tlo, B.tile_off + (int64_t)min(c + 2, NCHUNK - 1) * G * FRAG, R, tid);
+        // per 16-point group: a lane's partial sum over both super-tiles; the
+        // four lanes of a point then add into its LDS sum (no shuffles)
 #pragma unroll 1
-            for (int pt = 0; pt < 2; pt++) {
-                float part = 0.f;
+        for (int pt = 0; pt < 2; pt++) {
+            float part = 0.f;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                if (!has[h]) continue;
                 bf16x8 bh[2], bl[2];
 #pragma unroll
                 for (int ks = 0; ks < 2; ks++) {
@@ -915,22 +925,23 @@ __global__ __launch_bounds__(64 * HB_WAVES) void k_rblk_bf(const HItem *__restri
                     for (int v = 0; v < 4; v++)
                         part = fmaf(bf_bits_to_f32(ph[v]) + bf_bits_to_f32(pl[v]), acc[v], part);
                 }
-                part += __shfl_xor(part, 16, 64);
-                part += __shfl_xor(part, 32, 64);
-                if (q == 0) atomicAdd(&rc[pt * 16 + l16], part);
             }
-        }
-        __syncthreads();
-        if (tid < CHUNK) {
-            const int k = c * CHUNK + tid;
-            const int x1 = B.g1 + k / (BLK * BLK), x2 = B.g2 + (k / BLK) % BLK, x3 = B.g3 + k % BLK;
-            if (x1 < n1 && x2 < n2 && x3 < n3) atomicAdd(&rho[((int64_t)x1 * n2 + x2) * n3 + x3], (double)rc[tid]);
+            if (has[0]) atomicAdd(&rc[c * CHUNK + pt * 16 + l16], part);
         }
     };
     prefetch(0);
+    // fully unrolled: with no loop back-edge the compiler's waits count the
+    // loads exactly and never wait for a touch issued in the step before
+#pragma unroll
     for (int c = 0; c < NCHUNK; c += 2) {
         step(c, t0);
         step(c + 1, t1);
+    }
+    __syncthreads();
+    {
+        const int k = tid;
+        const int x1 = B.g1 + k / (BLK * BLK), x2 = B.g2 + (k / BLK) % BLK, x3 = B.g3 + k % BLK;
+        if (x1 < n1 && x2 < n2 && x3 < n3) atomicAdd(&rho[((int64_t)x1 * n2 + x2) * n3 + x3], (double)rc[k]);
     }
 }
 
@@ -943,7 +954,7 @@ int split_density(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t 
         const double *D = d_DM[s] + dm_offset;
         if (ctx->ncanon > 0) {
             KTimer kt(ctx, KID_DEFF);
-            launch_deff_f64(ctx, D);
+            launch_deff_f64(ctx, D, 1);
         }
         if (ctx->nn > 0)
             hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((ctx->nn + 255) / 256)), dim3(256), 0, ctx->stream, ctx->nn,
@@ -951,8 +962,11 @@ int split_density(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t 
         if (!P->items.empty()) {
             KTimer kt(ctx, KID_RHO);
             for (const ItemClass &ic : P->classes) {
-                const size_t lds = std::max<size_t>((size_t)ic.rows * RB_BYTES_PER_ROW + CHUNK * sizeof(float),
-                                                    (size_t)HB_WAVES * 8 * 2 * FRAG * sizeof(uint16_t));
+                // chunk planes and W images share the first RB_IMAGE_BYTES; the
+                // block's point sums, row descriptors and slot pairs follow
+                static_assert(MAX_ROWS * RB_BYTES_PER_ROW <= RB_IMAGE_BYTES, "chunk planes exceed the image area");
+                const size_t lds = RB_IMAGE_BYTES + BLK_PTS * sizeof(float) + MAX_ROWS * sizeof(int16_t) +
+                                   (size_t)P->max_slots * P->max_slots * sizeof(SlotPair);
                 auto kern = k_rblk_bf<MAXF>;
                 switch (nfrag(ic.rows)) {
                     case 1: kern = k_rblk_bf<1>; break;

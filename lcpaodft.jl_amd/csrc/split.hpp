@@ -73,6 +73,7 @@ struct BlockPlan {
     int64_t tile_elems = 0;    // total tile elements (per hi/lo plane)
     int64_t part_tiles = 0;    // total super-tiles
     int32_t max_rows = 0;
+    int32_t max_slots = 0;
     // sphere entries flattened: per (block, sphere)
     int64_t nsph = 0;
     // device
@@ -102,6 +103,7 @@ int split_density(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t 
 int fp32_tables(lcpao_ctx *ctx, bool deriv);
 void invalidate_split_tables(lcpao_ctx *ctx);
 void launch_to_f32(hipStream_t stream, int64_t n, const double *x, float *y);
-void launch_deff_f64(lcpao_ctx *ctx, const double *D);  // folded pair blocks into ctx->d_deff
+// folded pair blocks into ctx->d_deff (sym_self: on-site blocks as D + D^T)
+void launch_deff_f64(lcpao_ctx *ctx, const double *D, int sym_self);
 
 }  // namespace lcpao
