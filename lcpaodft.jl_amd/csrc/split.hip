@@ -765,6 +765,8 @@ __global__ void k_fill_f64(int64_t n, double *__restrict__ x, double v) {
 
 constexpr int RB_BYTES_PER_ROW = 2 * CHUNK * 2;   // LDS bytes per row: hi + lo point fragments
 constexpr int RB_IMAGE_BYTES = HB_WAVES * 8 * 2 * 512 * 2;  // per-wave W images (16 KB each)
+constexpr int RB_SUMS_OFFSET = MAX_ROWS * RB_BYTES_PER_ROW;  // per-wave point sums, after the chunk planes
+static_assert(RB_SUMS_OFFSET + HB_WAVES * BLK_PTS * 4 <= RB_IMAGE_BYTES, "point sums exceed the image area");
 
 template <int NF>
 __global__ __launch_bounds__(64 * HB_WAVES) void k_rblk_bf(const HItem *__restrict__ items,
@@ -782,12 +784,11 @@ __global__ __launch_bounds__(64 * HB_WAVES) void k_rblk_bf(const HItem *__restri
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, q = lane >> 4;
     u16x8 *TH = reinterpret_cast<u16x8 *>(lds_raw);
     u16x8 *TL = TH + G * 64;
-    // LDS: [0, RB_IMAGE_BYTES) the W images, later the chunk planes; then the
-    // block's point sums, row descriptors and slot-pair table
-    float *rc = reinterpret_cast<float *>(lds_raw + RB_IMAGE_BYTES);
-    int16_t *s_rinfo = reinterpret_cast<int16_t *>(rc + BLK_PTS);
+    // LDS: [0, RB_IMAGE_BYTES) the W images, later the chunk planes and the
+    // per-wave point sums; then the block's row descriptors and slot pairs
+    float *rcw = reinterpret_cast<float *>(lds_raw + RB_SUMS_OFFSET);  // [wave][512 points]
+    int16_t *s_rinfo = reinterpret_cast<int16_t *>(lds_raw + RB_IMAGE_BYTES);
     SlotPair *s_sp = reinterpret_cast<SlotPair *>(s_rinfo + MAX_ROWS);
-    rc[tid] = 0.f;  // 512 threads, 512 points
     for (int x = tid; x < R; x += 64 * HB_WAVES) s_rinfo[x] = rowinfo[B.row_off + x];
     for (int x = tid; x < B.nslot * B.nslot; x += 64 * HB_WAVES) s_sp[x] = spair[B.sp_off + x];
     // W fragments of this wave's super-tiles (A operand: W[16 rt + l16][32 ks + 8 q + e])
@@ -926,7 +927,13 @@ __global__ __launch_bounds__(64 * HB_WAVES) void k_rblk_bf(const HItem *__restri
                         part = fmaf(bf_bits_to_f32(ph[v]) + bf_bits_to_f32(pl[v]), acc[v], part);
                 }
             }
-            if (has[0]) atomicAdd(&rc[c * CHUNK + pt * 16 + l16], part);
+            // sum over the four lanes of each point (lanes 16 apart) with the
+            // gfx950 lane swaps: after both, every lane holds its point's sum
+            const auto s32 = __builtin_amdgcn_permlane32_swap(__float_as_uint(part), __float_as_uint(part), false, false);
+            part = __uint_as_float(s32[0]) + __uint_as_float(s32[1]);
+            const auto s16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(part), __float_as_uint(part), false, false);
+            part = __uint_as_float(s16[0]) + __uint_as_float(s16[1]);
+            if (q == 0) rcw[wave * BLK_PTS + c * CHUNK + pt * 16 + l16] = has[0] ? part : 0.f;
         }
     };
     prefetch(0);
@@ -940,8 +947,11 @@ __global__ __launch_bounds__(64 * HB_WAVES) void k_rblk_bf(const HItem *__restri
     __syncthreads();
     {
         const int k = tid;
+        float sum = 0.f;
+#pragma unroll
+        for (int w = 0; w < HB_WAVES; w++) sum += rcw[w * BLK_PTS + k];
         const int x1 = B.g1 + k / (BLK * BLK), x2 = B.g2 + (k / BLK) % BLK, x3 = B.g3 + k % BLK;
-        if (x1 < n1 && x2 < n2 && x3 < n3) atomicAdd(&rho[((int64_t)x1 * n2 + x2) * n3 + x3], (double)rc[k]);
+        if (x1 < n1 && x2 < n2 && x3 < n3) atomicAdd(&rho[((int64_t)x1 * n2 + x2) * n3 + x3], (double)sum);
     }
 }
 
@@ -962,10 +972,10 @@ int split_density(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t 
         if (!P->items.empty()) {
             KTimer kt(ctx, KID_RHO);
             for (const ItemClass &ic : P->classes) {
-                // chunk planes and W images share the first RB_IMAGE_BYTES; the
-                // block's point sums, row descriptors and slot pairs follow
+                // W images, then chunk planes and point sums, share the first
+                // RB_IMAGE_BYTES; the block's row descriptors and slot pairs follow
                 static_assert(MAX_ROWS * RB_BYTES_PER_ROW <= RB_IMAGE_BYTES, "chunk planes exceed the image area");
-                const size_t lds = RB_IMAGE_BYTES + BLK_PTS * sizeof(float) + MAX_ROWS * sizeof(int16_t) +
+                const size_t lds = RB_IMAGE_BYTES + MAX_ROWS * sizeof(int16_t) +
                                    (size_t)P->max_slots * P->max_slots * sizeof(SlotPair);
                 auto kern = k_rblk_bf<MAXF>;
                 switch (nfrag(ic.rows)) {
