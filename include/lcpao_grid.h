@@ -165,6 +165,19 @@ int lcpao_grid_set_stream(lcpao_ctx *ctx, void *hip_stream);
 int lcpao_grid_profile(lcpao_ctx *ctx, int32_t enable);
 int lcpao_grid_kernel_stats(const lcpao_ctx *ctx, int32_t kernel_id, int64_t *launches, double *total_ms);
 
+/* Autotuner (no counterpart in the reference).  Some contractions have
+ * several kernels that give the same result within the form's tolerance;
+ * by default (variant -1) the first call on a context times every candidate
+ * once on the context's own geometry and keeps the fastest.  op:
+ *   LCPAO_TUNE_RHO64  FP64 density  (0 matrix-core kernel, 1 vector-FMA kernel)
+ *   LCPAO_TUNE_F364   FP64 Force3   (0 matrix-core kernel, 1 vector-FMA kernel)
+ * lcpao_grid_tune fixes a variant (or -1 to re-tune at the next call);
+ * lcpao_grid_tuned reports the choice and, when candidate_ms is not NULL,
+ * the measured milliseconds of up to ncand candidates. */
+enum { LCPAO_TUNE_RHO64 = 0, LCPAO_TUNE_F364 = 1 };
+int lcpao_grid_tune(lcpao_ctx *ctx, int32_t op, int32_t variant);
+int lcpao_grid_tuned(const lcpao_ctx *ctx, int32_t op, int32_t *variant, float *candidate_ms, int32_t ncand);
+
 /* Sizes of the context (rank-local H size, grid size, sums of the lists). */
 int lcpao_grid_sizes(const lcpao_ctx *ctx, int64_t *my_hsize, int64_t *nn, int64_t *total_olg,
                      int64_t *total_gridn);

@@ -100,9 +100,17 @@ struct lcpao_ctx {
     struct Pending { int id; hipEvent_t beg, end; };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> event_pool;
+    // kernel variants chosen by the autotuner (-1: time the candidates on the
+    // first call with this geometry), see lcpao_grid_tune
+    int32_t variant[4] = {-1, -1, -1, -1};
+    float variant_ms[4][4] = {};
     int64_t prof_launches[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     double prof_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
+
+// tunable operations and their candidate counts
+enum { TUNE_RHO64 = 0, TUNE_F364 = 1, TUNE_NOPS = 2 };
+constexpr int TUNE_NCAND[TUNE_NOPS] = {2, 2};
 
 enum { KID_HAM = 0, KID_HRED = 1, KID_DEFF = 2, KID_RHO = 3, KID_RGATHER = 4, KID_F3 = 5, KID_F3RED = 6 };
 

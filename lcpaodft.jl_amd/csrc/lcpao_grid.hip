@@ -202,6 +202,11 @@ __global__ __launch_bounds__(64) void k_ham_mfma64(const HUnit *__restrict__ uni
     }
 }
 
+__global__ void k_fill_zero(int64_t n, double *__restrict__ x) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0.0;
+}
+
 // Sums the unit partials of every computed pair in unit order, scales by
 // GridVol and writes H[P] (and H[P']^T for a folded transpose partner).
 __global__ void k_ham_reduce(const int32_t *__restrict__ canon, const int32_t *__restrict__ ub, int64_t ncanon,
@@ -1152,6 +1157,35 @@ void lcpao::launch_deff_f64(lcpao_ctx *ctx, const double *D, int sym_self) {
 
 namespace {
 
+// Runs `launch(variant)` once per candidate (after one untimed run) with
+// events on the context stream and keeps the fastest; the caller's real
+// launch follows, so every candidate writes only scratch that it overwrites.
+template <class F>
+int autotune(lcpao_ctx *ctx, int op, F launch) {
+    if (ctx->variant[op] >= 0) return LCPAO_OK;
+    hipEvent_t e0, e1;
+    HIP_TRY(ctx, hipEventCreate(&e0));
+    HIP_TRY(ctx, hipEventCreate(&e1));
+    int best = 0;
+    float best_ms = 0.f;
+    for (int v = 0; v < TUNE_NCAND[op]; v++) {
+        launch(v);
+        HIP_TRY(ctx, hipEventRecord(e0, ctx->stream));
+        launch(v);
+        HIP_TRY(ctx, hipEventRecord(e1, ctx->stream));
+        HIP_TRY(ctx, hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_TRY(ctx, hipEventElapsedTime(&ms, e0, e1));
+        ctx->variant_ms[op][v] = ms;
+        if (v == 0 || ms < best_ms) { best = v; best_ms = ms; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIP_TRY(ctx, hipGetLastError());
+    ctx->variant[op] = best;
+    return LCPAO_OK;
+}
+
 int rho_dev(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_offset, double *const *d_rho, int precision) {
     if (!ctx->tables) return set_err(ctx, LCPAO_ESTATE, "orbital tables are not set");
     if (nspin != 1 && nspin != 2 && nspin != 4) return set_err(ctx, LCPAO_EINVAL, "nspin must be 1, 2 or 4");
@@ -1174,6 +1208,18 @@ int rho_dev(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_off
                 hipLaunchKernelGGL(k_deff<double>, dim3((unsigned)ctx->ncanon), dim3(128), 0, ctx->stream, ctx->d_hcanon,
                                    ctx->ncanon, ctx->d_pairs, D, ctx->d_deff);
         }
+        // FP64 candidates: 0 = matrix cores (k_rho_m64), 1 = vector FMA (k_rho<double>)
+        auto rho64 = [&](int v) {
+            if (v == 0)
+                hipLaunchKernelGGL(k_rho_m64, dim3(nu), dim3(64 * RHO_WAVES), 0, ctx->stream, ctx->d_runits,
+                                   ctx->d_rorder, nu, ctx->d_segs, ctx->d_pairs, ctx->d_tab, ctx->d_olg1, ctx->d_olg2,
+                                   ctx->d_deff, ctx->d_gl_off, ctx->d_rhoA);
+            else
+                hipLaunchKernelGGL(k_rho<double>, dim3(nu), dim3(64 * RHO_WAVES), 0, ctx->stream, ctx->d_runits,
+                                   ctx->d_rorder, nu, ctx->d_segs, ctx->d_pairs, ctx->d_tab, ctx->d_olg1, ctx->d_olg2,
+                                   ctx->d_deff, ctx->d_gl_off, ctx->d_rhoA);
+        };
+        if (nu > 0 && !f32 && (rc = autotune(ctx, TUNE_RHO64, rho64))) return rc;
         if (nu > 0) {
             KTimer kt(ctx, KID_RHO);
             if (f32)
@@ -1181,9 +1227,7 @@ int rho_dev(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_off
                                    ctx->d_rorder, nu, ctx->d_segs, ctx->d_pairs, ctx->d_tab32, ctx->d_olg1, ctx->d_olg2,
                                    ctx->d_deff32, ctx->d_gl_off, ctx->d_rhoA);
             else
-                hipLaunchKernelGGL(k_rho_m64, dim3(nu), dim3(64 * RHO_WAVES), 0, ctx->stream, ctx->d_runits,
-                                   ctx->d_rorder, nu, ctx->d_segs, ctx->d_pairs, ctx->d_tab, ctx->d_olg1, ctx->d_olg2,
-                                   ctx->d_deff, ctx->d_gl_off, ctx->d_rhoA);
+                rho64(ctx->variant[TUNE_RHO64]);
         }
         const unsigned gb = (unsigned)((ctx->nn + 255) / 256);
         {
@@ -1208,11 +1252,25 @@ int f3_dev(lcpao_ctx *ctx, int nch, const double *const *d_V, const double *cons
     const bool f32 = precision != LCPAO_FP64;
     if (f32 && (rc = fp32_tables(ctx, true))) return rc;
     const int nu = (int)ctx->funits.size();
-    HIP_TRY(ctx, hipMemsetAsync(ctx->d_fsum, 0, (size_t)ctx->natom * 3 * sizeof(double), ctx->stream));
+    if (ctx->natom > 0)
+        hipLaunchKernelGGL(k_fill_zero, dim3((ctx->natom * 3 + 255) / 256), dim3(256), 0, ctx->stream,
+                           (int64_t)ctx->natom * 3, ctx->d_fsum);
     for (int c = 0; c < nch; c++) {
         if (!d_V[c] || !d_DM[c]) return set_err(ctx, LCPAO_EINVAL, "null channel pointer");
         const double *D = d_DM[c] + dm_offset;
         if (f32) launch_to_f32(ctx->stream, ctx->my_hsize, D, ctx->d_deff32);
+        // FP64 candidates: 0 = matrix cores (k_f3_m64), 1 = vector FMA (k_f3<double>)
+        auto f364 = [&](int v) {
+            if (v == 0)
+                hipLaunchKernelGGL(k_f3_m64, dim3(nu), dim3(64), 0, ctx->stream, ctx->d_funits, ctx->d_forder, nu,
+                                   ctx->d_pairs, ctx->d_tab, ctx->d_dtab[0], ctx->d_dtab[1], ctx->d_dtab[2],
+                                   ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c], D, ctx->d_fpart);
+            else
+                hipLaunchKernelGGL(k_f3<double>, dim3(nu), dim3(64), 0, ctx->stream, ctx->d_funits, ctx->d_forder, nu,
+                                   ctx->d_pairs, ctx->d_tab, ctx->d_dtab[0], ctx->d_dtab[1], ctx->d_dtab[2],
+                                   ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c], D, ctx->d_fpart);
+        };
+        if (nu > 0 && !f32 && (rc = autotune(ctx, TUNE_F364, f364))) return rc;
         if (nu > 0) {
             KTimer kt(ctx, KID_F3);
             if (f32)
@@ -1220,9 +1278,7 @@ int f3_dev(lcpao_ctx *ctx, int nch, const double *const *d_V, const double *cons
                                    ctx->d_pairs, ctx->d_tab32, ctx->d_dtab32[0], ctx->d_dtab32[1], ctx->d_dtab32[2],
                                    ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c], ctx->d_deff32, ctx->d_fpart);
             else
-                hipLaunchKernelGGL(k_f3_m64, dim3(nu), dim3(64), 0, ctx->stream, ctx->d_funits, ctx->d_forder, nu,
-                                   ctx->d_pairs, ctx->d_tab, ctx->d_dtab[0], ctx->d_dtab[1], ctx->d_dtab[2],
-                                   ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c], D, ctx->d_fpart);
+                f364(ctx->variant[TUNE_F364]);
         }
         if (nu > 0) {
             KTimer kt(ctx, KID_F3RED);
@@ -1407,6 +1463,22 @@ int lcpao_grid_profile(lcpao_ctx *ctx, int32_t enable) {
     if (!ctx) return LCPAO_EINVAL;
     ctx->profiling = enable != 0;
     for (int k = 0; k < 8; k++) { ctx->prof_launches[k] = 0; ctx->prof_ms[k] = 0.0; }
+    return LCPAO_OK;
+}
+
+int lcpao_grid_tune(lcpao_ctx *ctx, int32_t op, int32_t variant) {
+    if (!ctx) return LCPAO_EINVAL;
+    if (op < 0 || op >= TUNE_NOPS) return set_err(ctx, LCPAO_EINVAL, "unknown tunable operation");
+    if (variant < -1 || variant >= TUNE_NCAND[op]) return set_err(ctx, LCPAO_EINVAL, "variant out of range");
+    ctx->variant[op] = variant;
+    return LCPAO_OK;
+}
+
+int lcpao_grid_tuned(const lcpao_ctx *ctx, int32_t op, int32_t *variant, float *candidate_ms, int32_t ncand) {
+    if (!ctx || !variant || op < 0 || op >= TUNE_NOPS) return LCPAO_EINVAL;
+    *variant = ctx->variant[op];
+    for (int v = 0; v < ncand && v < TUNE_NCAND[op]; v++)
+        if (candidate_ms) candidate_ms[v] = ctx->variant_ms[op][v];
     return LCPAO_OK;
 }
 

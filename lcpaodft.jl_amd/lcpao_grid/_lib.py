@@ -39,7 +39,7 @@ EXPORTS = [
     "lcpao_grid_create", "lcpao_grid_set_tables", "lcpao_grid_set_dtables", "lcpao_grid_build_tables",
     "lcpao_grid_get_tables", "lcpao_grid_hamiltonian", "lcpao_grid_density", "lcpao_grid_force3",
     "lcpao_grid_hamiltonian_dev", "lcpao_grid_density_dev", "lcpao_grid_force3_dev", "lcpao_grid_set_stream",
-    "lcpao_grid_profile", "lcpao_grid_kernel_stats", "lcpao_grid_sizes",
+    "lcpao_grid_profile", "lcpao_grid_kernel_stats", "lcpao_grid_tune", "lcpao_grid_tuned", "lcpao_grid_sizes",
     "lcpao_grid_last_error", "lcpao_grid_destroy", "lcpao_ucell_build", "lcpao_ucell_scalars",
     "lcpao_ucell_copy", "lcpao_ucell_last_error", "lcpao_ucell_destroy",
 ]
@@ -78,6 +78,8 @@ def load():
     L.lcpao_grid_set_stream.argtypes = [vp, vp]
     L.lcpao_grid_profile.argtypes = [vp, C.c_int32]
     L.lcpao_grid_kernel_stats.argtypes = [vp, C.c_int32, lp, dp]
+    L.lcpao_grid_tune.argtypes = [vp, C.c_int32, C.c_int32]
+    L.lcpao_grid_tuned.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_float), C.c_int32]
     L.lcpao_grid_last_error.argtypes = [vp]
     L.lcpao_grid_last_error.restype = C.c_char_p
     L.lcpao_grid_destroy.argtypes = [vp]

@@ -179,6 +179,20 @@ class GridContext:
     def _dptrs(ts):
         return (_lib.dp * len(ts))(*[C.cast(t.data_ptr(), _lib.dp) for t in ts])
 
+    TUNE_OPS = {"rho64": 0, "f364": 1}
+
+    def tune(self, op, variant=-1):
+        """Fix the kernel variant of a tunable operation ("rho64", "f364"), or
+        -1 to let the next call time the candidates and keep the fastest."""
+        self._check(self.L.lcpao_grid_tune(self.h, self.TUNE_OPS[op], variant))
+
+    def tuned(self, op):
+        """(chosen variant or -1, measured ms of each candidate)."""
+        v = C.c_int32()
+        ms = (C.c_float * 4)()
+        self._check(self.L.lcpao_grid_tuned(self.h, self.TUNE_OPS[op], C.byref(v), ms, 4))
+        return v.value, [round(float(x), 4) for x in ms[:2]]
+
     def hamiltonian_dev(self, dV, dH, precision="fp64"):
         self._check(self.L.lcpao_grid_hamiltonian_dev(self.h, len(dV), self._dptrs(dV), self._dptrs(dH), _prec(precision)))
 

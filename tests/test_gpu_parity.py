@@ -210,3 +210,37 @@ def test_density_bf16x3_block_form(name):
         r = worst_ratio(R[k], Rref[k], M[k])
         print(name, "rho bf16x3 worst ratio", r)
         assert r <= 1e-4
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("name", ["si2", "nacl8", "si2_r1of3"])
+def test_fp64_kernel_variants(name, variant):
+    """Every candidate the autotuner may pick meets the FP64 bar."""
+    c = case(name)
+    ctx = GridContext(c.u)
+    ctx.set_tables(c.tabs)
+    ctx.tune("rho64", variant)
+    rho = ctx.density([c.DM], dm_offset=c.u.MPHks_rank)[0]
+    Rref, M = O.density(c.g, c.tabs, [c.DM], with_abs=True)
+    ok, msg = within(rho, Rref[0], M[0])
+    assert ok, msg
+    assert ctx.tuned("rho64")[0] == variant
+    if name == "si2_r1of3":
+        return
+    dt = O.dorbitals(c.g, c.opaos, c.st.atom2spe)
+    ctx.set_dtables(*dt)
+    ctx.tune("f364", variant)
+    F = ctx.force3([c.V], [c.DM], dm_offset=c.u.MPHks_rank)
+    Fref = O.force3(c.g, c.tabs, dt, [c.V], [c.DM])
+    ok, msg = within(F, Fref, _force_bound(c, dt, [c.V], [c.DM]))
+    assert ok, msg
+
+
+def test_autotuner_picks_a_timed_candidate():
+    c = case("si8")
+    ctx = GridContext(c.u)
+    ctx.set_tables(c.tabs)
+    assert ctx.tuned("rho64")[0] == -1
+    ctx.density([c.DM])
+    v, ms = ctx.tuned("rho64")
+    assert v in (0, 1) and all(m > 0 for m in ms) and ms[v] == min(ms)

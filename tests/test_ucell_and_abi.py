@@ -96,3 +96,10 @@ def test_create_without_device_reports_error():
     assert rc != 0 and L.lcpao_grid_last_error(None)
     rc = L.lcpao_grid_create(ctypes.byref(h), None, 0)
     assert rc == _lib.LCPAO_EINVAL
+
+
+def test_tune_rejects_null_context():
+    L = _lib.load()
+    assert L.lcpao_grid_tune(None, 0, 0) == _lib.LCPAO_EINVAL
+    v = ctypes.c_int32()
+    assert L.lcpao_grid_tuned(None, 0, ctypes.byref(v), None, 0) == _lib.LCPAO_EINVAL
