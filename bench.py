@@ -33,7 +33,10 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
-    p.add_argument("--config", default="gaas216", choices=["si2", "si64", "gaas216", "si511"])
+    p.add_argument("--config", default=None, choices=["si2", "si64", "gaas216", "si511"],
+                   help="workload (default gaas216 for --op assembly, si511 for --op force3)")
+    p.add_argument("--op", default="assembly", choices=["assembly", "force3"],
+                   help="assembly: grid H + rho (the headline metric); force3: the force contraction")
     p.add_argument("--precision", default="bf16x3", choices=["fp64", "fp32", "bf16x3"],
                    help="headline form (all three forms are timed)")
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -159,6 +162,88 @@ def time_form(ctx, prec, dV, dDM, dH, drho, steps, warmup, torch):
     return (t1 - t0) * 1e3 / steps, stats
 
 
+def force3_flops(u):
+    """Algorithmic FLOPs of one Force3 contraction (all local pairs):
+    sum NumOLG (2 NO_A NO_B + 6 NO_A + 6)."""
+    no = u.Total_NumOrbs.astype(np.int64)
+    A, B = no[u.MPI_atom], no[u.MPI_natn]
+    n = u.MPI_NumOLG.astype(np.int64)
+    return float((n * (2 * A * B + 6 * A + 6)).sum())
+
+
+def bench_force3(args):
+    import torch
+    from lcpao_grid import systems as S
+    from lcpao_grid.grid import GridContext
+    cfg = args.config or "si511"
+    t_setup = time.time()
+    struct = S.CONFIGS[cfg]()
+    u = struct.ucell()
+    V, DM = S.inputs(struct, u)
+    ctx = GridContext(u, device=0)
+    ctx.build_tables(struct.paos(), with_derivatives=True)
+    dev = torch.device("cuda:0")
+    dV = torch.from_numpy(V).to(dev)
+    dDM = torch.from_numpy(DM).to(dev)
+    stream = torch.cuda.current_stream(dev)
+    ctx.L.lcpao_grid_set_stream(ctx.h, stream.cuda_stream)
+    forms = ["fp32", "fp64"] if args.precision != "fp64" else ["fp64", "fp32"]
+    res, timing = {}, {}
+    for f in forms:
+        dF = torch.zeros((3, u.Natom), dtype=torch.float64, device=dev)
+        ctx.force3_dev([dV], [dDM], dF, precision=f)  # builds per-form data, autotunes
+        torch.cuda.synchronize()
+        res[f] = dF.cpu().numpy().T.copy()
+    setup_s = time.time() - t_setup
+    import ctypes as C
+    for f in forms:
+        dF = torch.zeros((3, u.Natom), dtype=torch.float64, device=dev)
+        for _ in range(args.warmup):
+            ctx.force3_dev([dV], [dDM], dF, precision=f)
+        torch.cuda.synchronize()
+        ctx.L.lcpao_grid_profile(ctx.h, 1)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.force3_dev([dV], [dDM], dF, precision=f)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        stats = {}
+        for name, kid in (("f3", 5), ("f3_reduce", 6)):
+            n, tot = C.c_int64(), C.c_double()
+            ctx.L.lcpao_grid_kernel_stats(ctx.h, kid, C.byref(n), C.byref(tot))
+            if n.value:
+                stats[name] = {"launches": n.value, "avg_ms": round(tot.value / n.value, 4)}
+        ctx.L.lcpao_grid_profile(ctx.h, 0)
+        timing[f] = {"ms": round(ms, 4), "kernels": stats}
+    head = forms[0]
+    flops = force3_flops(u)
+    kt = timing[head]["kernels"]["f3"]["avg_ms"] * 1e-3
+    peak = PEAK[head]
+    out = {
+        "metric": "ms per Force3 contraction (dV/dphi force, nospin)", "value": timing[head]["ms"], "unit": "ms",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": timing[head]["ms"],
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32 products, f64 pair sums"}[head],
+        "data": "synthetic (seeded plane-wave V_eff, random PSD density matrix)",
+        "config": {"workload": "%s: Force3 (1 channel), %d atoms, grid %s, %d pairs, %d overlap points"
+                               % (cfg, struct.Natom, "x".join(map(str, struct.Ngrid)), u.MPI_size, u.olg1.size),
+                   "precision": head},
+        "forms_ms": {f: timing[f]["ms"] for f in forms},
+        "kernels": {f: timing[f]["kernels"] for f in forms},
+        "max_abs_dF_vs_fp64_Ha_per_bohr": float(np.abs(res[head] - res["fp64"]).max()),
+        "max_abs_F_Ha_per_bohr": float(np.abs(res["fp64"]).max()),
+        "autotune": {"f364": {"variant": ctx.tuned("f364")[0], "candidate_ms": ctx.tuned("f364")[1]}},
+        "setup_s": round(setup_s, 2),
+        "roofline": {"bound": "mfma" if head != "fp32" else "valu", "kernel": "k_f3<float>" if head == "fp32" else "k_f3",
+                     "achieved": round(flops / kt / 1e12, 3), "peak": round(peak / 1e12, 1), "unit": "TFLOP/s",
+                     "frac": round(flops / kt / peak, 4), "traffic": None, "flops_per_launch": flops},
+        "cpu_baseline": {"value": None, "reason": "not timed for Force3: the CPU restatement would need the "
+                                                  "31 GB of FP64 tables copied to the host"},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -170,6 +255,9 @@ def main():
     from lcpao_grid.grid import GridContext
     if args.copy_calibration:
         copy_calibration(torch)
+    if args.op == "force3":
+        return bench_force3(args)
+    args.config = args.config or "gaas216"
 
     t_setup = time.time()
     struct = S.CONFIGS[args.config]()

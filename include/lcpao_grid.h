@@ -110,6 +110,12 @@ int lcpao_grid_set_dtables(lcpao_ctx *ctx, const double *const *dx, const double
  * fields of lcpao_geom. */
 int lcpao_grid_build_tables(lcpao_ctx *ctx, int32_t nspecies, const lcpao_pao *paos, int32_t with_derivatives);
 
+/* Replaces the atom positions (natom x 3, bohr) used by the next
+ * lcpao_grid_build_tables; the grid and overlap lists are kept.  No
+ * counterpart in the reference (which rebuilds UCell per geometry): it
+ * serves finite-difference checks of Force3 with fixed lists. */
+int lcpao_grid_set_positions(lcpao_ctx *ctx, const double *gxyz);
+
 /* Copies the device tables back (NO_A x GridN_A per atom; which: 0 = phi,
  * 1..3 = d/dx, d/dy, d/dz). */
 int lcpao_grid_get_tables(lcpao_ctx *ctx, int32_t which, double *const *out);

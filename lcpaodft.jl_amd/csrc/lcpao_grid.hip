@@ -1535,6 +1535,15 @@ int lcpao_grid_set_dtables(lcpao_ctx *ctx, const double *const *dx, const double
     return LCPAO_OK;
 }
 
+int lcpao_grid_set_positions(lcpao_ctx *ctx, const double *gxyz) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!gxyz) return set_err(ctx, LCPAO_EINVAL, "null positions");
+    if (!ctx->have_lattice) return set_err(ctx, LCPAO_ESTATE, "geometry lacks lattice and positions");
+    ctx->gxyz.assign(gxyz, gxyz + 3 * (size_t)ctx->natom);
+    return LCPAO_OK;
+}
+
 int lcpao_grid_build_tables(lcpao_ctx *ctx, int32_t nspecies, const lcpao_pao *paos, int32_t with_derivatives) {
     int rc = check_ctx(ctx);
     if (rc) return rc;

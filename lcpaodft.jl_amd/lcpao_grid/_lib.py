@@ -37,7 +37,7 @@ class Pao(C.Structure):
 # every extern "C" symbol declared in include/lcpao_grid.h
 EXPORTS = [
     "lcpao_grid_create", "lcpao_grid_set_tables", "lcpao_grid_set_dtables", "lcpao_grid_build_tables",
-    "lcpao_grid_get_tables", "lcpao_grid_hamiltonian", "lcpao_grid_density", "lcpao_grid_force3",
+    "lcpao_grid_set_positions", "lcpao_grid_get_tables", "lcpao_grid_hamiltonian", "lcpao_grid_density", "lcpao_grid_force3",
     "lcpao_grid_hamiltonian_dev", "lcpao_grid_density_dev", "lcpao_grid_force3_dev", "lcpao_grid_set_stream",
     "lcpao_grid_profile", "lcpao_grid_kernel_stats", "lcpao_grid_tune", "lcpao_grid_tuned", "lcpao_grid_sizes",
     "lcpao_grid_last_error", "lcpao_grid_destroy", "lcpao_ucell_build", "lcpao_ucell_scalars",
@@ -67,6 +67,7 @@ def load():
     L.lcpao_grid_set_tables.argtypes = [vp, dpp]
     L.lcpao_grid_set_dtables.argtypes = [vp, dpp, dpp, dpp]
     L.lcpao_grid_build_tables.argtypes = [vp, C.c_int32, C.POINTER(Pao), C.c_int32]
+    L.lcpao_grid_set_positions.argtypes = [vp, dp]
     L.lcpao_grid_get_tables.argtypes = [vp, C.c_int32, dpp]
     L.lcpao_grid_hamiltonian.argtypes = [vp, C.c_int32, dpp, dpp, C.c_int32]
     L.lcpao_grid_density.argtypes = [vp, C.c_int32, dpp, C.c_int64, dpp, C.c_int32]

@@ -147,6 +147,11 @@ class GridContext:
             arr[s].rwf = C.cast(rw, _lib.dpp)
         self._check(self.L.lcpao_grid_build_tables(self.h, len(paos), arr, int(bool(with_derivatives))))
 
+    def set_positions(self, gxyz):
+        """Atom positions (natom x 3, bohr) for the next build_tables; lists kept."""
+        g = np.ascontiguousarray(gxyz, np.float64).reshape(-1, 3)
+        self._check(self.L.lcpao_grid_set_positions(self.h, g.ctypes.data_as(_lib.dp)))
+
     def get_tables(self, which=0):
         out = [np.zeros((int(self.gridn[a]), int(self.no[a]))) for a in range(self.natom)]
         self._check(self.L.lcpao_grid_get_tables(self.h, which, _ptrs(out)))
