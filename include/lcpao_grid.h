@@ -202,6 +202,35 @@ void lcpao_grid_destroy(lcpao_ctx *ctx);
 /* ------------------------------------------------------------------ */
 typedef struct lcpao_ucell lcpao_ucell;
 
+/* ---- batched form (no counterpart in the reference) ------------------------
+ * The split-BF16x3 grid H (1 channel) and density (1 spin) of many
+ * structures in one go: the block items of all structures are merged per
+ * row count and each merged list runs as one persistent launch whose
+ * workgroups pull items from a work queue; the whole sequence can be
+ * captured in a HIP graph and replayed.  For callers that assemble many
+ * small or uneven cells (structure screening, many geometries).
+ *
+ * lcpao_batch_create takes n contexts on one device, each with tables set
+ * and the whole pair list of its structure (one rank); it builds their block
+ * plans.  The contexts must outlive the batch and must not be used from
+ * another thread while it runs.
+ * lcpao_batch_assemble: per structure s, d_V[s] (NN_s), d_DM[s] (flat pair
+ * blocks, Total_Hsize_s), outputs d_H[s] (scaled by GridVol) and d_rho[s]
+ * (overwritten); all device pointers.  use_graph != 0 replays a graph
+ * captured at the first call with these pointers (re-captured when they
+ * change).  The first call also times the two schedules (0 persistent work
+ * queue, 1 one workgroup per item) and keeps the faster (lcpao_batch_tune
+ * fixes one, -1 re-tunes).  Returns after the stream has been synchronised. */
+typedef struct lcpao_batch lcpao_batch;
+int lcpao_batch_create(lcpao_batch **out, int32_t n, lcpao_ctx *const *ctxs);
+int lcpao_batch_assemble(lcpao_batch *b, const double *const *d_V, const double *const *d_DM, double *const *d_H,
+                         double *const *d_rho, int32_t use_graph);
+int lcpao_batch_tune(lcpao_batch *b, int32_t sched);
+int lcpao_batch_tuned(const lcpao_batch *b, int32_t *sched, float *candidate_ms);
+int lcpao_batch_sizes(const lcpao_batch *b, int32_t *nstruct, int32_t *nclasses, int64_t *nitems);
+const char *lcpao_batch_last_error(const lcpao_batch *b);
+void lcpao_batch_destroy(lcpao_batch *b);
+
 /* latvecs: 3x3 row-major (rows are lattice vectors); gxyz natom x 3;
  * cut[natom] = Atoms_Cut1; no[natom] = Total_NumOrbs (may be NULL);
  * nprocs/rank select the rank-local pair range. */

@@ -1,0 +1,265 @@
+// batch.hip -- batched form: the split-BF16x3 grid Hamiltonian (1 channel)
+// and density (1 spin) of many structures at once.  The block items of all
+// structures are merged per row count; each merged list runs as one
+// persistent launch whose workgroups pull items from a work-queue counter
+// (longest items first), so small structures no longer leave CUs idle and
+// the per-structure launch and synchronisation gaps disappear.  The whole
+// sequence (fold, fills, merged H launches, per-structure reductions, merged
+// density launches) can be captured once in a HIP graph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ctx.hpp"
+#include "split.hpp"
+
+using lcpao::BlockArgs;
+using lcpao::HItem;
+
+struct lcpao_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = true;
+    std::string err;
+    std::vector<lcpao_ctx *> ctx;
+    struct Cls {
+        int32_t rows, first, count;
+    };
+    std::vector<Cls> classes;
+    HItem *d_items = nullptr;
+    BlockArgs *d_args_h = nullptr, *d_args_r = nullptr;
+    int32_t *d_queue = nullptr;   // one counter per launch: 2 per class
+    int max_slots = 0;
+    // schedule: 0 persistent work queue, 1 one workgroup per item (-1: tune)
+    int32_t sched = -1;
+    float sched_ms[2] = {0.f, 0.f};
+    // arguments of the last call and the graph captured with them
+    std::vector<const void *> bound;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+int berr(lcpao_batch *b, int code, const std::string &m) {
+    b->err = m;
+    return code;
+}
+
+#define BATCH_TRY(b, call)                                                                      \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) return berr(b, LCPAO_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+void drop_graph(lcpao_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// Binds the per-structure device arguments of the merged launches.
+int bind(lcpao_batch *b, const double *const *V, const double *const *DM, double *const *H, double *const *rho) {
+    const size_t n = b->ctx.size();
+    std::vector<const void *> now;
+    for (size_t s = 0; s < n; s++) {
+        if (!V[s] || !DM[s] || !H[s] || !rho[s]) return berr(b, LCPAO_EINVAL, "null structure pointer");
+        now.insert(now.end(), {V[s], DM[s], H[s], rho[s]});
+    }
+    if (now == b->bound) return LCPAO_OK;
+    std::vector<BlockArgs> ah(n), ar(n);
+    for (size_t s = 0; s < n; s++) {
+        ah[s] = lcpao::block_args(b->ctx[s], V[s], nullptr);
+        ar[s] = lcpao::block_args(b->ctx[s], nullptr, rho[s]);
+    }
+    BATCH_TRY(b, hipMemcpy(b->d_args_h, ah.data(), n * sizeof(BlockArgs), hipMemcpyHostToDevice));
+    BATCH_TRY(b, hipMemcpy(b->d_args_r, ar.data(), n * sizeof(BlockArgs), hipMemcpyHostToDevice));
+    b->bound = now;
+    drop_graph(b);  // the graph's small launches carry the old pointers
+    return LCPAO_OK;
+}
+
+// Enqueues one batched H + density on b->stream (no synchronisation, no
+// allocation, no memset: safe to capture).
+int enqueue(lcpao_batch *b, const double *const *DM, double *const *H, double *const *rho, int sched) {
+    const size_t n = b->ctx.size();
+    for (size_t s = 0; s < n; s++) {  // folded pair blocks (on-site blocks symmetrised) and zeroed densities
+        lcpao_ctx *c = b->ctx[s];
+        hipStream_t keep = c->stream;
+        c->stream = b->stream;
+        if (c->ncanon > 0) lcpao::launch_deff_f64(c, DM[s], 1);
+        c->stream = keep;
+        lcpao::launch_fill_f64(b->stream, c->nn, rho[s], 0.0);
+    }
+    const bool queue = sched == 0;
+    if (queue) lcpao::launch_fill_i32(b->stream, 2 * (int64_t)b->classes.size(), b->d_queue, 0);
+    const BlockArgs none{};
+    for (size_t k = 0; k < b->classes.size(); k++) {
+        const auto &cl = b->classes[k];
+        BATCH_TRY(b, lcpao::launch_block_kernel(false, b->stream, cl.rows, 0, b->d_items + cl.first, cl.count,
+                                                b->d_args_h, none, queue ? b->d_queue + 2 * k : nullptr,
+                                                queue ? 0u : (unsigned)cl.count));
+    }
+    for (size_t s = 0; s < n; s++) lcpao::launch_hblk_reduce(b->ctx[s], b->stream, H[s]);
+    for (size_t k = 0; k < b->classes.size(); k++) {
+        const auto &cl = b->classes[k];
+        BATCH_TRY(b, lcpao::launch_block_kernel(true, b->stream, cl.rows, b->max_slots, b->d_items + cl.first,
+                                                cl.count, b->d_args_r, none, queue ? b->d_queue + 2 * k + 1 : nullptr,
+                                                queue ? 0u : (unsigned)cl.count));
+    }
+    BATCH_TRY(b, hipGetLastError());
+    return LCPAO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lcpao_batch_create(lcpao_batch **out, int32_t n, lcpao_ctx *const *ctxs) {
+    if (!out) return LCPAO_EINVAL;
+    *out = nullptr;
+    if (n <= 0 || !ctxs) return LCPAO_EINVAL;
+    auto *b = new lcpao_batch();
+    int dev = -1;
+    for (int s = 0; s < n; s++) {
+        if (!ctxs[s]) { delete b; return LCPAO_EINVAL; }
+        if (s == 0) dev = ctxs[s]->device;
+        if (ctxs[s]->device != dev) { delete b; return LCPAO_EINVAL; }
+        b->ctx.push_back(ctxs[s]);
+    }
+    b->device = dev;
+    auto fail = [&](int rc) {
+        lcpao_batch_destroy(b);
+        return rc;
+    };
+    if (hipSetDevice(dev) != hipSuccess || hipStreamCreate(&b->stream) != hipSuccess) return fail(LCPAO_EDEVICE);
+    // every structure's block plan and bf16 tiles
+    std::vector<std::vector<HItem>> per_rows;
+    std::vector<int32_t> rows_of;
+    for (int s = 0; s < n; s++) {
+        lcpao_ctx *c = b->ctx[s];
+        if (!c->tables) return fail(LCPAO_ESTATE);
+        int rc = lcpao::split_tables(c, LCPAO_BF16X3);
+        if (rc) return fail(rc);
+        if (!c->d_deff) return fail(LCPAO_ESTATE);
+        const lcpao::BlockPlan *P = c->plan;
+        b->max_slots = std::max(b->max_slots, P->max_slots);
+        for (const auto &ic : P->classes) {
+            size_t k = std::find(rows_of.begin(), rows_of.end(), ic.rows) - rows_of.begin();
+            if (k == rows_of.size()) { rows_of.push_back(ic.rows); per_rows.emplace_back(); }
+            for (int i = 0; i < ic.count; i++) {
+                HItem it = P->items[ic.first + i];
+                it.pad = s;
+                per_rows[k].push_back(it);
+            }
+        }
+    }
+    // merged lists, longest items (most super-tiles) first
+    std::vector<HItem> items;
+    for (size_t k = 0; k < rows_of.size(); k++) {
+        auto &v = per_rows[k];
+        std::stable_sort(v.begin(), v.end(), [](const HItem &x, const HItem &y) { return x.nst > y.nst; });
+        b->classes.push_back({rows_of[k], (int32_t)items.size(), (int32_t)v.size()});
+        items.insert(items.end(), v.begin(), v.end());
+    }
+    const size_t ni = std::max<size_t>(1, items.size());
+    if (hipMalloc(&b->d_items, ni * sizeof(HItem)) != hipSuccess ||
+        hipMalloc(&b->d_args_h, n * sizeof(BlockArgs)) != hipSuccess ||
+        hipMalloc(&b->d_args_r, n * sizeof(BlockArgs)) != hipSuccess ||
+        hipMalloc(&b->d_queue, std::max<size_t>(2, 2 * b->classes.size()) * sizeof(int32_t)) != hipSuccess)
+        return fail(LCPAO_EDEVICE);
+    if (!items.empty() &&
+        hipMemcpy(b->d_items, items.data(), items.size() * sizeof(HItem), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(LCPAO_EDEVICE);
+    *out = b;
+    return LCPAO_OK;
+}
+
+int lcpao_batch_assemble(lcpao_batch *b, const double *const *d_V, const double *const *d_DM, double *const *d_H,
+                         double *const *d_rho, int32_t use_graph) {
+    if (!b) return LCPAO_EINVAL;
+    if (!d_V || !d_DM || !d_H || !d_rho) return berr(b, LCPAO_EINVAL, "null pointer array");
+    BATCH_TRY(b, hipSetDevice(b->device));
+    int rc = bind(b, d_V, d_DM, d_H, d_rho);
+    if (rc) return rc;
+    if (b->sched < 0) {  // tune the schedule: each candidate once untimed, once timed
+        hipEvent_t e0, e1;
+        BATCH_TRY(b, hipEventCreate(&e0));
+        BATCH_TRY(b, hipEventCreate(&e1));
+        int best = 0;
+        for (int v = 0; v < 2; v++) {
+            if ((rc = enqueue(b, d_DM, d_H, d_rho, v))) return rc;
+            BATCH_TRY(b, hipEventRecord(e0, b->stream));
+            if ((rc = enqueue(b, d_DM, d_H, d_rho, v))) return rc;
+            BATCH_TRY(b, hipEventRecord(e1, b->stream));
+            BATCH_TRY(b, hipEventSynchronize(e1));
+            BATCH_TRY(b, hipEventElapsedTime(&b->sched_ms[v], e0, e1));
+            if (b->sched_ms[v] < b->sched_ms[best]) best = v;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        b->sched = best;
+    }
+    if (!use_graph) {
+        if ((rc = enqueue(b, d_DM, d_H, d_rho, b->sched))) return rc;
+    } else {
+        if (!b->exec) {
+            BATCH_TRY(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            rc = enqueue(b, d_DM, d_H, d_rho, b->sched);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(b->stream, &g);
+            if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+            if (e != hipSuccess) return berr(b, LCPAO_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            b->graph = g;
+            BATCH_TRY(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        }
+        BATCH_TRY(b, hipGraphLaunch(b->exec, b->stream));
+    }
+    BATCH_TRY(b, hipStreamSynchronize(b->stream));
+    return LCPAO_OK;
+}
+
+int lcpao_batch_tune(lcpao_batch *b, int32_t sched) {
+    if (!b || sched < -1 || sched > 1) return LCPAO_EINVAL;
+    if (sched != b->sched) drop_graph(b);
+    b->sched = sched;
+    return LCPAO_OK;
+}
+
+int lcpao_batch_tuned(const lcpao_batch *b, int32_t *sched, float *candidate_ms) {
+    if (!b || !sched) return LCPAO_EINVAL;
+    *sched = b->sched;
+    if (candidate_ms) { candidate_ms[0] = b->sched_ms[0]; candidate_ms[1] = b->sched_ms[1]; }
+    return LCPAO_OK;
+}
+
+int lcpao_batch_sizes(const lcpao_batch *b, int32_t *nstruct, int32_t *nclasses, int64_t *nitems) {
+    if (!b) return LCPAO_EINVAL;
+    if (nstruct) *nstruct = (int32_t)b->ctx.size();
+    if (nclasses) *nclasses = (int32_t)b->classes.size();
+    if (nitems) {
+        int64_t t = 0;
+        for (const auto &c : b->classes) t += c.count;
+        *nitems = t;
+    }
+    return LCPAO_OK;
+}
+
+const char *lcpao_batch_last_error(const lcpao_batch *b) { return b ? b->err.c_str() : ""; }
+
+void lcpao_batch_destroy(lcpao_batch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    drop_graph(b);
+    void *ptrs[] = {b->d_items, b->d_args_h, b->d_args_r, b->d_queue};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+}  // extern "C"

@@ -25,7 +25,7 @@ typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
         }                                                                           \
     } while (0)
 
-constexpr int MAX_ROWS = 640;    // R limit (bf16x3 H: four R x 32 bf16 planes in 160 KB of LDS)
+constexpr int MAX_ROWS = 576;    // R limit (bf16x3 H: four R x 32 bf16 planes + V in 160 KB of LDS)
 constexpr int LROW32 = 36;       // LDS row stride of fp32 tiles (32 + 4 pad)
 
 __device__ __forceinline__ int st_index(int si, int sj, int S) { return si * S - (si * (si - 1)) / 2 + (sj - si); }
@@ -136,6 +136,39 @@ __device__ __forceinline__ Touch touch_chunk(const uint16_t *__restrict__ hi, co
 }
 __device__ __forceinline__ void release(const Touch &t) { asm volatile("" ::"v"(t.a), "v"(t.b)); }
 
+// Work distribution of the block kernels.  BATCH == false: one structure
+// (`one`), one workgroup per item (grid = item count).  BATCH == true: item.pad
+// selects the structure's entry of `args`; with queue == nullptr one
+// workgroup per item, otherwise the grid is persistent and each workgroup
+// takes the next item index from the queue counter (zeroed before the
+// launch) until the list is exhausted, so every workgroup reaches the exit.
+template <bool BATCH, class F>
+__device__ __forceinline__ void run_items(const HItem *__restrict__ items, int32_t nitems,
+                                          const BlockArgs *__restrict__ args, const BlockArgs &one,
+                                          int32_t *__restrict__ queue, F body) {
+    if constexpr (!BATCH) {
+        if ((int)blockIdx.x < nitems) body(items[blockIdx.x], one);
+        return;
+    }
+    if (!queue) {
+        if ((int)blockIdx.x < nitems) {
+            const HItem it = items[blockIdx.x];
+            body(it, args[it.pad]);
+        }
+        return;
+    }
+    __shared__ int32_t s_next;
+    for (;;) {
+        __syncthreads();  // the previous item is done with the LDS and s_next
+        if (threadIdx.x == 0) s_next = atomicAdd(queue, 1);
+        __syncthreads();
+        const int32_t i = s_next;
+        if (i >= nitems) return;
+        const HItem it = items[i];
+        body(it, args[it.pad]);
+    }
+}
+
 inline int nfrag(int rows) { return (rows / 16 + HB_WAVES - 1) / HB_WAVES; }
 
 __device__ __forceinline__ void tile_bf16x3(const u16x8 *__restrict__ PH, const u16x8 *__restrict__ PL,
@@ -163,15 +196,13 @@ __device__ __forceinline__ void tile_bf16x3(const u16x8 *__restrict__ PH, const 
 }
 
 template <int NF>
-__global__ __launch_bounds__(64 * HB_WAVES) void k_hblk_bf(const HItem *__restrict__ items,
-                                                          const BlockDev *__restrict__ blocks,
-                                                          const uint16_t *__restrict__ hi,
-                                                          const uint16_t *__restrict__ lo,
-                                                          const double *__restrict__ V, int n1, int n2, int n3,
-                                                          float *__restrict__ part) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    const HItem it = items[blockIdx.x];
-    const BlockDev B = blocks[it.block];
+__device__ __forceinline__ void hblk_item(const HItem it, const BlockArgs &A, unsigned char *lds_raw) {
+    const BlockDev B = A.blocks[it.block];
+    const uint16_t *__restrict__ hi = A.hi;
+    const uint16_t *__restrict__ lo = A.lo;
+    const double *__restrict__ V = A.V;
+    float *__restrict__ part = A.part;
+    const int n1 = A.n1, n2 = A.n2, n3 = A.n3;
     const int R = B.rows, S = R / ST, G = R / 16;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, q = lane >> 4;
     u16x8 *PH = reinterpret_cast<u16x8 *>(lds_raw);
@@ -265,6 +296,15 @@ __global__ __launch_bounds__(64 * HB_WAVES) void k_hblk_bf(const HItem *__restri
             for (int v = 0; v < 4; v++) out[(rt * 16 + 4 * q + v) * ST + ct * 16 + l16] = acc[t][v];
         }
     }
+}
+
+template <int NF, bool BATCH>
+__global__ __launch_bounds__(64 * HB_WAVES) void k_hblk_bf(const HItem *__restrict__ items, int32_t nitems,
+                                                          const BlockArgs *__restrict__ args, BlockArgs one,
+                                                          int32_t *__restrict__ queue) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    run_items<BATCH>(items, nitems, args, one, queue,
+              [&](const HItem &it, const BlockArgs &A) { hblk_item<NF>(it, A, lds_raw); });
 }
 
 // ---- fp32 form --------------------------------------------------------------
@@ -695,20 +735,9 @@ int split_hamiltonian(lcpao_ctx *ctx, int nch, const double *const *d_V, double 
             KTimer kt(ctx, KID_HAM);
             for (const ItemClass &ic : P->classes) {
                 if (bf) {
-                    const size_t lds = (size_t)4 * ic.rows * CHUNK * sizeof(uint16_t) + BLK_PTS * sizeof(float);
-                    auto kern = k_hblk_bf<MAXF>;
-                    switch (nfrag(ic.rows)) {
-                        case 1: kern = k_hblk_bf<1>; break;
-                        case 2: kern = k_hblk_bf<2>; break;
-                        case 3: kern = k_hblk_bf<3>; break;
-                        case 4: kern = k_hblk_bf<4>; break;
-                        default: break;
-                    }
-                    SPLIT_TRY(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                    hipLaunchKernelGGL(kern, dim3((unsigned)ic.count), dim3(64 * HB_WAVES), lds, ctx->stream,
-                                       P->d_items + ic.first, P->d_blocks, P->d_hi, P->d_lo, d_V[c], ctx->ngrid[0],
-                                       ctx->ngrid[1], ctx->ngrid[2], P->d_part);
+                    const BlockArgs one = block_args(ctx, d_V[c], nullptr);
+                    SPLIT_TRY(ctx, launch_block_kernel(false, ctx->stream, ic.rows, 0, P->d_items + ic.first, ic.count,
+                                                       nullptr, one, nullptr, (unsigned)ic.count));
                 } else {
                     const size_t lds = (size_t)ic.rows * LROW32 * sizeof(float) + CHUNK * sizeof(float);
                     SPLIT_TRY(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(k_hblk_f32),
@@ -763,23 +792,41 @@ __global__ void k_fill_f64(int64_t n, double *__restrict__ x, double v) {
     if (i < n) x[i] = v;
 }
 
+__global__ void k_fill_i32(int64_t n, int32_t *__restrict__ x, int32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+// host wrappers for the batched form (fills instead of memsets: they may be
+// captured in a graph)
+void launch_fill_f64(hipStream_t stream, int64_t n, double *x, double v) {
+    if (n > 0) hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, n, x, v);
+}
+void launch_fill_i32(hipStream_t stream, int64_t n, int32_t *x, int32_t v) {
+    if (n > 0) hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, n, x, v);
+}
+void launch_hblk_reduce(const lcpao_ctx *ctx, hipStream_t stream, double *H) {
+    const BlockPlan *P = ctx->plan;
+    if (ctx->ncanon > 0)
+        hipLaunchKernelGGL(k_hblk_reduce, dim3((unsigned)ctx->ncanon), dim3(256), 0, stream, ctx->d_hcanon, ctx->ncanon,
+                           P->d_contrib_off, P->d_contrib, P->d_blocks, ctx->d_pairs, P->d_part, ctx->gridvol, H);
+}
+
 constexpr int RB_BYTES_PER_ROW = 2 * CHUNK * 2;   // LDS bytes per row: hi + lo point fragments
 constexpr int RB_IMAGE_BYTES = HB_WAVES * 8 * 2 * 512 * 2;  // per-wave W images (16 KB each)
 constexpr int RB_SUMS_OFFSET = MAX_ROWS * RB_BYTES_PER_ROW;  // per-wave point sums, after the chunk planes
 static_assert(RB_SUMS_OFFSET + HB_WAVES * BLK_PTS * 4 <= RB_IMAGE_BYTES, "point sums exceed the image area");
 
 template <int NF>
-__global__ __launch_bounds__(64 * HB_WAVES) void k_rblk_bf(const HItem *__restrict__ items,
-                                                          const BlockDev *__restrict__ blocks,
-                                                          const uint16_t *__restrict__ thi,
-                                                          const uint16_t *__restrict__ tlo,
-                                                          const int16_t *__restrict__ rowinfo,
-                                                          const SlotPair *__restrict__ spair,
-                                                          const double *__restrict__ Deff, int n1, int n2, int n3,
-                                                          double *__restrict__ rho) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    const HItem it = items[blockIdx.x];
-    const BlockDev B = blocks[it.block];
+__device__ __forceinline__ void rblk_item(const HItem it, const BlockArgs &A, unsigned char *lds_raw) {
+    const BlockDev B = A.blocks[it.block];
+    const uint16_t *__restrict__ thi = A.thi;
+    const uint16_t *__restrict__ tlo = A.tlo;
+    const int16_t *__restrict__ rowinfo = A.rowinfo;
+    const SlotPair *__restrict__ spair = A.spair;
+    const double *__restrict__ Deff = A.Deff;
+    double *__restrict__ rho = A.rho;
+    const int n1 = A.n1, n2 = A.n2, n3 = A.n3;
     const int R = B.rows, S = R / ST, G = R / 16, G32 = R / 32;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, q = lane >> 4;
     u16x8 *TH = reinterpret_cast<u16x8 *>(lds_raw);
@@ -955,6 +1002,67 @@ __global__ __launch_bounds__(64 * HB_WAVES) void k_rblk_bf(const HItem *__restri
     }
 }
 
+template <int NF, bool BATCH>
+__global__ __launch_bounds__(64 * HB_WAVES) void k_rblk_bf(const HItem *__restrict__ items, int32_t nitems,
+                                                          const BlockArgs *__restrict__ args, BlockArgs one,
+                                                          int32_t *__restrict__ queue) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    run_items<BATCH>(items, nitems, args, one, queue,
+              [&](const HItem &it, const BlockArgs &A) { rblk_item<NF>(it, A, lds_raw); });
+}
+
+BlockArgs block_args(const lcpao_ctx *ctx, const double *V, double *rho) {
+    const BlockPlan *P = ctx->plan;
+    BlockArgs a{};
+    a.blocks = P->d_blocks;
+    a.hi = P->d_hi; a.lo = P->d_lo; a.thi = P->d_thi; a.tlo = P->d_tlo;
+    a.rowinfo = P->d_rowinfo; a.spair = P->d_spair;
+    a.V = V; a.Deff = ctx->d_deff; a.part = P->d_part; a.rho = rho;
+    a.n1 = ctx->ngrid[0]; a.n2 = ctx->ngrid[1]; a.n3 = ctx->ngrid[2];
+    return a;
+}
+
+size_t block_kernel_lds(bool density, int rows, int max_slots) {
+    // density: W images, then chunk planes and point sums, share the first
+    // RB_IMAGE_BYTES; the block's row descriptors and slot pairs follow
+    static_assert(MAX_ROWS * RB_BYTES_PER_ROW <= RB_IMAGE_BYTES, "chunk planes exceed the image area");
+    if (density)
+        return RB_IMAGE_BYTES + MAX_ROWS * sizeof(int16_t) + (size_t)max_slots * max_slots * sizeof(SlotPair);
+    return (size_t)4 * rows * CHUNK * sizeof(uint16_t) + BLK_PTS * sizeof(float);
+}
+
+hipError_t launch_block_kernel(bool density, hipStream_t stream, int rows, int max_slots, const HItem *items,
+                               int32_t nitems, const BlockArgs *args, const BlockArgs &one, int32_t *queue,
+                               unsigned grid) {
+    const size_t lds = block_kernel_lds(density, rows, max_slots);
+    void (*kern)(const HItem *, int32_t, const BlockArgs *, BlockArgs, int32_t *) = nullptr;
+    const bool b = args != nullptr;  // structures indexed by item (batched launch)
+#define PICK(NF) kern = density ? (b ? k_rblk_bf<NF, true> : k_rblk_bf<NF, false>) : (b ? k_hblk_bf<NF, true> : k_hblk_bf<NF, false>)
+    switch (nfrag(rows)) {
+        case 1: PICK(1); break;
+        case 2: PICK(2); break;
+        case 3: PICK(3); break;
+        case 4: PICK(4); break;
+        default: PICK(MAXF); break;
+    }
+#undef PICK
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) return e;
+    if (nitems <= 0) return hipSuccess;
+    if (grid == 0) {  // persistent: as many workgroups as can be resident at once
+        int dev = 0, ncu = 0, per_cu = 0;
+        if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+        if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+        if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(kern),
+                                                              64 * HB_WAVES, lds)) != hipSuccess)
+            return e;
+        grid = (unsigned)std::min<int64_t>(nitems, (int64_t)ncu * std::max(1, per_cu));
+    }
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * HB_WAVES), lds, stream, items, nitems, args, one, queue);
+    return hipGetLastError();
+}
+
 int split_density(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_offset, double *const *d_rho) {
     int rc = split_tables(ctx, LCPAO_BF16X3);
     if (rc) return rc;
@@ -972,24 +1080,9 @@ int split_density(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t 
         if (!P->items.empty()) {
             KTimer kt(ctx, KID_RHO);
             for (const ItemClass &ic : P->classes) {
-                // W images, then chunk planes and point sums, share the first
-                // RB_IMAGE_BYTES; the block's row descriptors and slot pairs follow
-                static_assert(MAX_ROWS * RB_BYTES_PER_ROW <= RB_IMAGE_BYTES, "chunk planes exceed the image area");
-                const size_t lds = RB_IMAGE_BYTES + MAX_ROWS * sizeof(int16_t) +
-                                   (size_t)P->max_slots * P->max_slots * sizeof(SlotPair);
-                auto kern = k_rblk_bf<MAXF>;
-                switch (nfrag(ic.rows)) {
-                    case 1: kern = k_rblk_bf<1>; break;
-                    case 2: kern = k_rblk_bf<2>; break;
-                    case 3: kern = k_rblk_bf<3>; break;
-                    case 4: kern = k_rblk_bf<4>; break;
-                    default: break;
-                }
-                SPLIT_TRY(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                hipLaunchKernelGGL(kern, dim3((unsigned)ic.count), dim3(64 * HB_WAVES), lds, ctx->stream,
-                                   P->d_items + ic.first, P->d_blocks, P->d_thi, P->d_tlo, P->d_rowinfo, P->d_spair,
-                                   ctx->d_deff, ctx->ngrid[0], ctx->ngrid[1], ctx->ngrid[2], d_rho[s]);
+                const BlockArgs one = block_args(ctx, nullptr, d_rho[s]);
+                SPLIT_TRY(ctx, launch_block_kernel(true, ctx->stream, ic.rows, P->max_slots, P->d_items + ic.first,
+                                                   ic.count, nullptr, one, nullptr, (unsigned)ic.count));
             }
         }
         SPLIT_TRY(ctx, hipGetLastError());

@@ -50,7 +50,22 @@ struct SlotPair {
 };
 
 struct HItem {
-    int32_t block, st0, nst, pad;
+    int32_t block, st0, nst, pad;   // pad: structure index in a batched launch
+};
+
+// Per-structure arguments of the bf16x3 block kernels (by value for one
+// structure, as a device array indexed by HItem::pad for a batch).
+struct BlockArgs {
+    const BlockDev *blocks;
+    const uint16_t *hi, *lo;       // row fragments (H)
+    const uint16_t *thi, *tlo;     // point fragments (density)
+    const int16_t *rowinfo;
+    const SlotPair *spair;
+    const double *V;               // H: potential of the channel
+    const double *Deff;            // density: folded pair blocks
+    float *part;                   // H: super-tile partials
+    double *rho;                   // density: output
+    int32_t n1, n2, n3, pad;
 };
 
 // items of one row count R, launched together (LDS sized for R)
@@ -96,6 +111,17 @@ struct BlockPlan {
 };
 
 int plan_build(lcpao_ctx *ctx);
+// bf16x3 block kernels, shared with the batched form
+BlockArgs block_args(const lcpao_ctx *ctx, const double *V, double *rho);
+size_t block_kernel_lds(bool density, int rows, int max_slots);
+void launch_fill_f64(hipStream_t stream, int64_t n, double *x, double v);
+void launch_fill_i32(hipStream_t stream, int64_t n, int32_t *x, int32_t v);
+void launch_hblk_reduce(const lcpao_ctx *ctx, hipStream_t stream, double *H);
+// args != nullptr: batched launch (item.pad indexes args, `one` unused);
+// grid == 0 with a queue: persistent grid (resident workgroups)
+hipError_t launch_block_kernel(bool density, hipStream_t stream, int rows, int max_slots, const HItem *items,
+                               int32_t nitems, const BlockArgs *args, const BlockArgs &one, int32_t *queue,
+                               unsigned grid);
 void plan_free(BlockPlan *p);
 int split_tables(lcpao_ctx *ctx, int precision);
 int split_hamiltonian(lcpao_ctx *ctx, int nch, const double *const *d_V, double *const *d_H, int precision);

@@ -42,6 +42,8 @@ EXPORTS = [
     "lcpao_grid_profile", "lcpao_grid_kernel_stats", "lcpao_grid_tune", "lcpao_grid_tuned", "lcpao_grid_sizes",
     "lcpao_grid_last_error", "lcpao_grid_destroy", "lcpao_ucell_build", "lcpao_ucell_scalars",
     "lcpao_ucell_copy", "lcpao_ucell_last_error", "lcpao_ucell_destroy",
+    "lcpao_batch_create", "lcpao_batch_assemble", "lcpao_batch_tune", "lcpao_batch_tuned", "lcpao_batch_sizes",
+    "lcpao_batch_last_error", "lcpao_batch_destroy",
 ]
 
 _lib = None
@@ -68,6 +70,15 @@ def load():
     L.lcpao_grid_set_dtables.argtypes = [vp, dpp, dpp, dpp]
     L.lcpao_grid_build_tables.argtypes = [vp, C.c_int32, C.POINTER(Pao), C.c_int32]
     L.lcpao_grid_set_positions.argtypes = [vp, dp]
+    L.lcpao_batch_create.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(vp)]
+    L.lcpao_batch_assemble.argtypes = [vp, dpp, dpp, dpp, dpp, C.c_int32]
+    L.lcpao_batch_tune.argtypes = [vp, C.c_int32]
+    L.lcpao_batch_tuned.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_float)]
+    L.lcpao_batch_sizes.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), lp]
+    L.lcpao_batch_last_error.argtypes = [vp]
+    L.lcpao_batch_last_error.restype = C.c_char_p
+    L.lcpao_batch_destroy.argtypes = [vp]
+    L.lcpao_batch_destroy.restype = None
     L.lcpao_grid_get_tables.argtypes = [vp, C.c_int32, dpp]
     L.lcpao_grid_hamiltonian.argtypes = [vp, C.c_int32, dpp, dpp, C.c_int32]
     L.lcpao_grid_density.argtypes = [vp, C.c_int32, dpp, C.c_int64, dpp, C.c_int32]

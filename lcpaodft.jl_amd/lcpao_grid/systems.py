@@ -148,6 +148,59 @@ def nacl(n, ngrid, disp=0.05, seed=3, a=10.66):
     return Structure("nacl%d" % len(frac), np.eye(3) * a * n, frac, sp, (ngrid,) * 3, a, disp, seed)
 
 
+def _supercell3(basis_frac, reps):
+    reps = np.array(reps, dtype=np.float64)
+    cells = np.array([[i, j, k] for i in range(int(reps[0])) for j in range(int(reps[1])) for k in range(int(reps[2]))],
+                     dtype=np.float64)
+    return ((cells[:, None, :] + basis_frac[None, :, :]) / reps).reshape(-1, 3) % 1.0
+
+
+def si_cells(reps, ngrid, disp=0.05, seed=0, a=10.26):
+    """Diamond Si, reps[0] x reps[1] x reps[2] conventional cells."""
+    frac = _supercell3(_DIAMOND, reps)
+    lat = np.diag(np.array(reps, dtype=np.float64) * a)
+    return Structure("si%d" % len(frac), lat, frac, ["Si"] * len(frac), ngrid, a, disp, seed)
+
+
+def nacl_cells(reps, ngrid, disp=0.05, seed=0, a=10.66):
+    """Rocksalt NaCl, reps[0] x reps[1] x reps[2] conventional cells."""
+    na, cl = _supercell3(_ROCKSALT_NA, reps), _supercell3(_ROCKSALT_CL, reps)
+    frac = np.concatenate([na, cl])
+    sp = ["Na"] * len(na) + ["Cl"] * len(cl)
+    lat = np.diag(np.array(reps, dtype=np.float64) * a)
+    return Structure("nacl%d" % len(frac), lat, frac, sp, ngrid, a, disp, seed)
+
+
+# The uneven batch: Si diamond cells of 8 to 216 atoms and NaCl rocksalt
+# cells of 8 to 128 atoms, grids at ~0.21 bohr (48 ... 208 points per edge).
+BATCH_MENU = [
+    ("si8", lambda s: si_cells((1, 1, 1), (48, 48, 48), seed=s)),
+    ("si16", lambda s: si_cells((2, 1, 1), (96, 48, 48), seed=s)),
+    ("si64", lambda s: si_cells((2, 2, 2), (96, 96, 96), seed=s)),
+    ("si216", lambda s: si_cells((3, 3, 3), (144, 144, 144), seed=s)),
+    ("nacl8", lambda s: nacl_cells((1, 1, 1), (52, 52, 52), seed=s)),
+    ("nacl16", lambda s: nacl_cells((2, 1, 1), (104, 52, 52), seed=s)),
+    ("nacl32", lambda s: nacl_cells((2, 2, 1), (104, 104, 52), seed=s)),
+    ("nacl64", lambda s: nacl_cells((2, 2, 2), (104, 104, 104), seed=s)),
+    ("nacl128", lambda s: nacl_cells((4, 2, 2), (208, 104, 104), seed=s)),
+]
+
+
+def batch_structures(count=32, seed=3, menu=None):
+    """`count` structures drawn uniformly from the menu with SplitMix64(seed);
+    structure k gets displacement seed 100 + k."""
+    menu = menu or BATCH_MENU
+    rng = SplitMix64(seed)
+    pick = (rng.uniform(count) * len(menu)).astype(np.int64)
+    return [menu[int(i)][1](100 + k) for k, i in enumerate(pick)]
+
+
+def batch_inputs(struct, ucell, k):
+    """V from 40 plane waves and a PSD density matrix with 2 occupied states
+    per atom, seed 300 + k."""
+    return inputs(struct, ucell, nwaves=40, nocc=2 * struct.Natom, seed=300 + k)
+
+
 CONFIGS = {"si2": si2, "si64": si64, "gaas216": gaas216, "si511": si511}
 
 

@@ -1,0 +1,81 @@
+"""Batched split-BF16x3 form (lcpao_batch_*): the merged persistent
+launches and their graph replays reproduce the per-structure form.
+
+Each structure's H comes from the same per-item partials and the same
+reduction order, so it must match the per-structure call bit for bit; the
+density adds one block sum per item into each grid point, identical too
+unless a block has three or more items (then the atomic order may differ in
+the last bit), so it is held to 1e-13 of the largest entry."""
+import numpy as np
+import pytest
+import torch
+
+from lcpao_grid import systems as S
+from lcpao_grid.batch import Batch
+from lcpao_grid.grid import GridContext
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small_batch():
+    structs = [S.si_cells((1, 1, 1), (24, 24, 24), seed=101), S.nacl_cells((1, 1, 1), (28, 28, 28), seed=102),
+               S.si2(24), S.si_cells((2, 1, 1), (40, 20, 20), seed=103), S.nacl_cells((1, 1, 1), (24, 24, 24), seed=104)]
+    dev = torch.device("cuda:0")
+    out = []
+    for k, st in enumerate(structs):
+        u = st.ucell()
+        V, DM = S.batch_inputs(st, u, k)
+        ctx = GridContext(u)
+        ctx.build_tables(st.paos())
+        out.append(dict(st=st, u=u, ctx=ctx, V=torch.from_numpy(V).to(dev), DM=torch.from_numpy(DM).to(dev),
+                        H=torch.empty(u.myHsize, dtype=torch.float64, device=dev),
+                        rho=torch.empty(u.NN, dtype=torch.float64, device=dev)))
+    ref = []
+    for s in out:  # per-structure form
+        s["ctx"].hamiltonian_dev([s["V"]], [s["H"]], "bf16x3")
+        s["ctx"].density_dev([s["DM"]], [s["rho"]], 0, "bf16x3")
+        ref.append((s["H"].cpu().numpy().copy(), s["rho"].cpu().numpy().copy()))
+    return out, ref
+
+
+def _check(out, ref):
+    for s, (H0, R0) in zip(out, ref):
+        H, R = s["H"].cpu().numpy(), s["rho"].cpu().numpy()
+        assert np.array_equal(H, H0), s["st"].name
+        assert np.abs(R - R0).max() <= 1e-13 * np.abs(R0).max(), s["st"].name
+
+
+def _run(b, out, use_graph):
+    for s in out:
+        s["H"].fill_(np.nan)
+        s["rho"].fill_(np.nan)
+    b.assemble([s["V"] for s in out], [s["DM"] for s in out], [s["H"] for s in out], [s["rho"] for s in out],
+               use_graph=use_graph)
+
+
+@pytest.mark.parametrize("sched", [0, 1])
+def test_batch_matches_per_structure(small_batch, sched):
+    out, ref = small_batch
+    b = Batch([s["ctx"] for s in out])
+    b.tune(sched)
+    _run(b, out, use_graph=False)
+    _check(out, ref)
+    n, ncls, nitems = b.sizes()
+    assert n == len(out) and ncls >= 1 and nitems > 0
+
+
+def test_batch_graph_replays(small_batch):
+    """Three replays of the captured graph, with eager per-structure work and
+    synchronisations in between; every replay is checked."""
+    out, ref = small_batch
+    b = Batch([s["ctx"] for s in out])
+    _run(b, out, use_graph=False)  # first call: tunes the schedule
+    assert b.tuned()[0] in (0, 1)
+    for rep in range(3):
+        _run(b, out, use_graph=True)
+        _check(out, ref)
+        scratch = torch.empty_like(out[0]["H"])
+        out[0]["ctx"].hamiltonian_dev([out[0]["V"]], [scratch], "bf16x3")
+        torch.cuda.synchronize()
+        assert np.array_equal(scratch.cpu().numpy(), ref[0][0])
