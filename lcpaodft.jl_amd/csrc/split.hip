@@ -169,6 +169,72 @@ __device__ __forceinline__ void run_items(const HItem *__restrict__ items, int32
     }
 }
 
+// Writes the super-tile accumulators of a wave into the compact pair
+// partials.  The block's row descriptors, slot-pair partial bases and slot
+// orbital counts are staged in LDS first (the chunk buffers are free by
+// then).  Element (r, c) of C_b, r <= c, with slots a <= b and orbitals i, j
+// goes to part[base(a, b) + i * NO_b + j] (row-major slot: the 16 lanes of a
+// row write consecutive floats); an on-site block (a == b) also gets the
+// mirror element, so every element of every slot is written exactly once.
+struct EpiTables {
+    const int16_t *rinfo;
+    const int64_t *cb;
+    const int16_t *no;
+    int ns;
+};
+__device__ __forceinline__ EpiTables stage_epilogue(const BlockArgs &A, const BlockDev &B, unsigned char *lds) {
+    int64_t *cb = reinterpret_cast<int64_t *>(lds);
+    int16_t *rinfo = reinterpret_cast<int16_t *>(cb + B.nslot * B.nslot);
+    int16_t *no = rinfo + B.rows;
+    __syncthreads();  // every wave is done with the chunk buffers
+    for (int x = threadIdx.x; x < B.nslot * B.nslot; x += blockDim.x) cb[x] = A.cbase[B.sp_off + x];
+    for (int x = threadIdx.x; x < B.rows; x += blockDim.x) rinfo[x] = A.rowinfo[B.row_off + x];
+    for (int x = threadIdx.x; x < B.nslot; x += blockDim.x) no[x] = A.slotno[B.slot_off + x];
+    __syncthreads();
+    return EpiTables{rinfo, cb, no, B.nslot};
+}
+__device__ __forceinline__ void store_partials(const f32x4 *acc, int si, int sj, int lane, const EpiTables &T,
+                                               float *__restrict__ part) {
+    const int l16 = lane & 15, q = lane >> 4;
+    // the lane's four columns (one per 16-column tile) and their slots' sizes
+    int ci[4], nb[4];
+#pragma unroll
+    for (int ct = 0; ct < 4; ct++) {
+        ci[ct] = T.rinfo[sj * ST + ct * 16 + l16];
+        nb[ct] = ci[ct] >= 0 ? T.no[ci[ct] >> 5] : 0;
+    }
+#pragma unroll
+    for (int rt = 0; rt < 4; rt++) {
+        const int r0 = si * ST + rt * 16 + 4 * q;
+        int ri[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) ri[v] = T.rinfo[r0 + v];
+        // partial bases for the slot of row r0 (the four rows mostly share it)
+        const int a0 = ri[0] >> 5;
+        int64_t base0[4];
+#pragma unroll
+        for (int ct = 0; ct < 4; ct++)
+            base0[ct] = (ri[0] >= 0 && ci[ct] >= 0) ? T.cb[a0 * T.ns + (ci[ct] >> 5)] : -1;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int r = r0 + v;
+            if (ri[v] < 0) continue;
+            const int a = ri[v] >> 5, i = ri[v] & 31;
+#pragma unroll
+            for (int ct = 0; ct < 4; ct++) {
+                const int c = sj * ST + ct * 16 + l16;
+                if (r > c || ci[ct] < 0) continue;  // r > c only in diagonal super-tiles: mirror of (c, r)
+                const int b = ci[ct] >> 5, j = ci[ct] & 31;
+                const int64_t base = a == a0 ? base0[ct] : T.cb[a * T.ns + b];
+                if (base < 0) continue;
+                const float x = acc[rt * 4 + ct][v];
+                part[base + i * nb[ct] + j] = x;
+                if (a == b && i != j) part[base + j * nb[ct] + i] = x;
+            }
+        }
+    }
+}
+
 inline int nfrag(int rows) { return (rows / 16 + HB_WAVES - 1) / HB_WAVES; }
 
 __device__ __forceinline__ void tile_bf16x3(const u16x8 *__restrict__ PH, const u16x8 *__restrict__ PL,
@@ -282,20 +348,9 @@ __device__ __forceinline__ void hblk_item(const HItem it, const BlockArgs &A, un
         step(c + 1, t1);
     }
     // lane holds D[4q + v][l16] of each 16x16 tile
-    const int l16 = lane & 15;
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-        if (h == 0 ? !has0 : !has1) continue;
-        const int st = it.st0 + wave + h * HB_WAVES;
-        float *out = part + (B.part_off + st) * (int64_t)(ST * ST);
-        const f32x4 *acc = h == 0 ? acc0 : acc1;
-#pragma unroll
-        for (int t = 0; t < 16; t++) {
-            const int rt = t >> 2, ct = t & 3;
-#pragma unroll
-            for (int v = 0; v < 4; v++) out[(rt * 16 + 4 * q + v) * ST + ct * 16 + l16] = acc[t][v];
-        }
-    }
+    const EpiTables T = stage_epilogue(A, B, lds_raw);
+    if (has0) store_partials(acc0, si0, sj0, lane, T, part);
+    if (has1) store_partials(acc1, si1, sj1, lane, T, part);
 }
 
 template <int NF, bool BATCH>
@@ -333,12 +388,13 @@ __device__ __forceinline__ void tile_f32(const float *__restrict__ A32, const fl
 }
 
 __global__ __launch_bounds__(64 * HB_WAVES) void k_hblk_f32(const HItem *__restrict__ items,
-                                                           const BlockDev *__restrict__ blocks,
-                                                           const float *__restrict__ t32, const double *__restrict__ V,
-                                                           int n1, int n2, int n3, float *__restrict__ part) {
+                                                           const float *__restrict__ t32, BlockArgs A) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const HItem it = items[blockIdx.x];
-    const BlockDev B = blocks[it.block];
+    const BlockDev B = A.blocks[it.block];
+    const double *__restrict__ V = A.V;
+    float *__restrict__ part = A.part;
+    const int n1 = A.n1, n2 = A.n2, n3 = A.n3;
     const int R = B.rows, S = R / ST;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, q = lane >> 4;
     float *A32 = reinterpret_cast<float *>(lds_raw);
@@ -370,52 +426,37 @@ __global__ __launch_bounds__(64 * HB_WAVES) void k_hblk_f32(const HItem *__restr
         if (has0) tile_f32(A32, Vs, si0, sj0, l16, q, acc0);
         if (has1) tile_f32(A32, Vs, si1, sj1, l16, q, acc1);
     }
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-        if (h == 0 ? !has0 : !has1) continue;
-        const int st = it.st0 + wave + h * HB_WAVES;
-        float *out = part + (B.part_off + st) * (int64_t)(ST * ST);
-        const f32x4 *acc = h == 0 ? acc0 : acc1;
-#pragma unroll
-        for (int t = 0; t < 16; t++) {
-            const int rt = t >> 2, ct = t & 3;
-#pragma unroll
-            for (int v = 0; v < 4; v++) out[(rt * 16 + 4 * q + v) * ST + ct * 16 + l16] = acc[t][v];
-        }
-    }
+    const EpiTables T = stage_epilogue(A, B, lds_raw);
+    if (has0) store_partials(acc0, si0, sj0, lane, T, part);
+    if (has1) store_partials(acc1, si1, sj1, lane, T, part);
 }
 
-// per computed pair: FP64 sum of its C_b sub-blocks over the blocks.  The
+// per computed pair: FP64 sum of its compact partials over the blocks, in
+// block order (deterministic), x GridVol; writes H[P] and H[P']^T.  The
 // pair's contribution records are staged in LDS so that every thread issues
-// independent loads of the partial buffer.
+// independent loads of contiguous partial slots.
 constexpr int RED_BATCH = 256;
 __global__ __launch_bounds__(256) void k_hblk_reduce(const int32_t *__restrict__ canon, int64_t ncanon,
                                                      const int32_t *__restrict__ coff,
                                                      const Contrib *__restrict__ contrib,
-                                                     const BlockDev *__restrict__ blocks,
                                                      const PairDev *__restrict__ pairs, const float *__restrict__ part,
                                                      double gridvol, double *__restrict__ H) {
     __shared__ int64_t s_base[RED_BATCH];
-    __shared__ int32_t s_r0[RED_BATCH], s_c0[RED_BATCH], s_tr[RED_BATCH], s_S[RED_BATCH];
+    __shared__ int32_t s_tr[RED_BATCH];
     const int64_t ci = blockIdx.x;
     if (ci >= ncanon) return;
     const int p = canon[ci];
     const PairDev P = pairs[p];
     const int ne = P.noa * P.nob;
     const int k0 = coff[ci], k1 = coff[ci + 1];
-    double acc[2] = {0.0, 0.0};   // elements threadIdx.x and threadIdx.x + 256 (ne <= 32*32 -> 4 per thread max)
-    double acc2[2] = {0.0, 0.0};
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};  // elements threadIdx.x + 256 h (ne <= 32*32)
     for (int kb = k0; kb < k1; kb += RED_BATCH) {
         const int nb = min(RED_BATCH, k1 - kb);
         __syncthreads();
         if ((int)threadIdx.x < nb) {
             const Contrib C = contrib[kb + threadIdx.x];
-            const BlockDev B = blocks[C.block];
-            s_base[threadIdx.x] = B.part_off * (int64_t)(ST * ST);
-            s_r0[threadIdx.x] = C.r0;
-            s_c0[threadIdx.x] = C.c0;
+            s_base[threadIdx.x] = C.base;
             s_tr[threadIdx.x] = C.trans;
-            s_S[threadIdx.x] = B.rows / ST;
         }
         __syncthreads();
 #pragma unroll
@@ -423,16 +464,11 @@ __global__ __launch_bounds__(256) void k_hblk_reduce(const int32_t *__restrict__
             const int e = threadIdx.x + h * 256;
             if (e >= ne) break;
             const int i = e / P.nob, j = e - i * P.nob;
+            const int et = j * P.noa + i;  // the element in a slot oriented (B, A)
             double s = 0.0;
 #pragma unroll 4
-            for (int k = 0; k < nb; k++) {
-                int r = s_tr[k] ? s_r0[k] + j : s_r0[k] + i;
-                int cc = s_tr[k] ? s_c0[k] + i : s_c0[k] + j;
-                if (r > cc) { const int t = r; r = cc; cc = t; }
-                const int st = st_index(r / ST, cc / ST, s_S[k]);
-                s += (double)part[s_base[k] + (int64_t)st * (ST * ST) + (r % ST) * ST + (cc % ST)];
-            }
-            if (h < 2) acc[h] += s; else acc2[h - 2] += s;
+            for (int k = 0; k < nb; k++) s += (double)part[s_base[k] + (s_tr[k] ? et : e)];
+            acc[h] += s;
         }
     }
 #pragma unroll
@@ -440,7 +476,7 @@ __global__ __launch_bounds__(256) void k_hblk_reduce(const int32_t *__restrict__
         const int e = threadIdx.x + h * 256;
         if (e >= ne) break;
         const int i = e / P.nob, j = e - i * P.nob;
-        const double s = (h < 2 ? acc[h] : acc2[h - 2]) * gridvol;
+        const double s = acc[h] * gridvol;
         H[P.hoff + e] = s;
         if (P.partner >= 0 && P.partner != p) H[pairs[P.partner].hoff + (int64_t)j * P.noa + i] = s;
     }
@@ -475,7 +511,7 @@ void plan_free(BlockPlan *p) {
     if (!p) return;
     void *ptrs[] = {p->d_blocks, p->d_items, p->d_sph_block, p->d_sph_atom, p->d_sph_row0, p->d_sph_map,
                     p->d_contrib, p->d_contrib_off, p->d_hi, p->d_lo, p->d_t32, p->d_part, p->d_thi, p->d_tlo,
-                    p->d_rowinfo, p->d_spair, p->d_contrib_pair};
+                    p->d_rowinfo, p->d_spair, p->d_contrib_pair, p->d_cbase, p->d_slotno};
     for (void *x : ptrs)
         if (x) (void)hipFree(x);
     delete p;
@@ -538,7 +574,7 @@ int plan_build(lcpao_ctx *ctx) {
     }
     P->blocks.resize(NB);
     std::vector<int64_t> sph_first(NB + 1, 0);
-    int64_t toff = 0, poff = 0, spoff = 0, roff = 0;
+    int64_t toff = 0, poff = 0, spoff = 0, roff = 0, soff = 0;
     for (int64_t b = 0; b < NB; b++) {
         int r = 0;
         rows0[b].resize(sph[b].size());
@@ -554,8 +590,9 @@ int plan_build(lcpao_ctx *ctx) {
         const int b1 = (int)(b / ((int64_t)P->nb[1] * P->nb[2])), b2 = (int)((b / P->nb[2]) % P->nb[1]), b3 = (int)(b % P->nb[2]);
         const int ns = (int)sph[b].size();
         if (roff + R > INT32_MAX) { ctx->err = "too many grid blocks for the split-precision plan"; delete P; return LCPAO_ERANGE; }
-        P->blocks[b] = {toff, poff, spoff, R, b1 * BLK, b2 * BLK, b3 * BLK, ns, (int32_t)roff};
+        P->blocks[b] = {toff, spoff, R, b1 * BLK, b2 * BLK, b3 * BLK, ns, (int32_t)roff, (int32_t)soff, 0};
         P->max_slots = std::max(P->max_slots, ns);
+        soff += ns;
         toff += (int64_t)R * BLK_PTS;
         poff += nst;
         spoff += (int64_t)ns * ns;
@@ -582,6 +619,10 @@ int plan_build(lcpao_ctx *ctx) {
             for (int i = 0; i < ctx->no[sph[b][x].first]; i++)
                 rowinfo[P->blocks[b].row_off + rows0[b][x] + i] = (int16_t)(x * 32 + i);
     if (ctx->my_hsize > INT32_MAX) { ctx->err = "pair blocks exceed the split-precision index range"; delete P; return LCPAO_ERANGE; }
+    std::vector<int16_t> slotno(std::max<int64_t>(1, soff), 0);
+    for (int64_t b = 0; b < NB; b++)
+        for (size_t x = 0; x < sph[b].size(); x++) slotno[P->blocks[b].slot_off + x] = (int16_t)ctx->no[sph[b][x].first];
+    std::vector<int64_t> cbase(std::max<int64_t>(1, spoff), -1);
     std::vector<SlotPair> spairs(std::max<int64_t>(1, spoff), SlotPair{-1, 0});
     // sphere tables and point maps
     std::vector<int32_t> sb(P->nsph), sa(P->nsph), sr(P->nsph), smap((size_t)P->nsph * BLK_PTS, -1);
@@ -613,7 +654,7 @@ int plan_build(lcpao_ctx *ctx) {
     }
     std::vector<int32_t> canon_pos(ctx->npair, -1);
     for (size_t c = 0; c < ctx->canon.size(); c++) canon_pos[ctx->canon[c]] = (int32_t)c;
-    struct CRec { int32_t cpos; Contrib c; int32_t x, y; };
+    struct CRec { int32_t cpos; int32_t block, trans; int32_t x, y; };
     std::vector<std::vector<CRec>> recs(nthreads);
     std::vector<std::vector<int64_t>> olg_count(nthreads, std::vector<int64_t>(ctx->npair, 0));
     {
@@ -643,7 +684,7 @@ int plan_build(lcpao_ctx *ctx) {
                             if (pxy >= 0 && canon_pos[pxy] >= 0) pc = pxy;
                             else if (pyx >= 0 && canon_pos[pyx] >= 0) { pc = pyx; trans = 1; }
                             if (pc < 0) continue;
-                            recs[w].push_back({canon_pos[pc], {(int32_t)b, rows0[b][x], rows0[b][y], trans}, (int32_t)x, (int32_t)y});
+                            recs[w].push_back({canon_pos[pc], (int32_t)b, trans, (int32_t)x, (int32_t)y});
                         }
                 }
             });
@@ -661,19 +702,26 @@ int plan_build(lcpao_ctx *ctx) {
     std::vector<CRec> all;
     for (auto &v : recs) all.insert(all.end(), v.begin(), v.end());
     std::stable_sort(all.begin(), all.end(), [](const CRec &x, const CRec &y) {
-        return x.cpos != y.cpos ? x.cpos < y.cpos : x.c.block < y.c.block;
+        return x.cpos != y.cpos ? x.cpos < y.cpos : x.block < y.block;
     });
     std::vector<Contrib> contrib(all.size());
     std::vector<int32_t> cpair(all.size());
     std::vector<int32_t> coff(ctx->canon.size() + 1, 0);
+    // compact partial slots: contribution k holds NO_x * NO_y floats, in
+    // canonical-pair order so that a pair's slots are contiguous
+    int64_t pel = 0;
     for (size_t i = 0; i < all.size(); i++) {
-        contrib[i] = all[i].c;
+        const BlockDev &B = P->blocks[all[i].block];
         cpair[i] = ctx->canon[all[i].cpos];
+        contrib[i] = Contrib{pel, all[i].block, all[i].trans};
         coff[all[i].cpos + 1]++;
-        const BlockDev &B = P->blocks[all[i].c.block];
         const PairDev &Q = ctx->pairs[cpair[i]];
-        spairs[B.sp_off + (int64_t)all[i].x * B.nslot + all[i].y] = SlotPair{(int32_t)Q.hoff, Q.nob | (all[i].c.trans << 6)};
+        const int64_t sp = B.sp_off + (int64_t)all[i].x * B.nslot + all[i].y;
+        spairs[sp] = SlotPair{(int32_t)Q.hoff, Q.nob | (all[i].trans << 6)};
+        cbase[sp] = pel;
+        pel += (int64_t)slotno[B.slot_off + all[i].x] * slotno[B.slot_off + all[i].y];
     }
+    P->part_elems = pel;
     P->ncontrib = (int64_t)all.size();
     for (size_t c = 0; c < ctx->canon.size(); c++) coff[c + 1] += coff[c];
     int rc;
@@ -682,7 +730,8 @@ int plan_build(lcpao_ctx *ctx) {
         (rc = up(ctx, &P->d_sph_block, sb)) || (rc = up(ctx, &P->d_sph_atom, sa)) || (rc = up(ctx, &P->d_sph_row0, sr)) ||
         (rc = up(ctx, &P->d_sph_map, smap)) || (rc = up(ctx, &P->d_contrib, contrib)) || (rc = up(ctx, &P->d_contrib_off, coff)) ||
         (rc = up(ctx, &P->d_contrib_pair, cpair)) || (rc = up(ctx, &P->d_rowinfo, rowinfo)) ||
-        (rc = up(ctx, &P->d_spair, spairs))) {
+        (rc = up(ctx, &P->d_spair, spairs)) || (rc = up(ctx, &P->d_cbase, cbase)) ||
+        (rc = up(ctx, &P->d_slotno, slotno))) {
         ctx->plan = nullptr;
         plan_free(P);
         return rc;
@@ -718,7 +767,7 @@ int split_tables(lcpao_ctx *ctx, int precision) {
                            P->d_sph_atom, P->d_sph_row0, P->d_sph_map, P->d_blocks, ctx->d_tab_off, ctx->d_no, ctx->d_tab,
                            precision, P->d_hi, P->d_lo, P->d_t32, P->d_thi, P->d_tlo);
     SPLIT_TRY(ctx, hipGetLastError());
-    if (!P->d_part) SPLIT_TRY(ctx, hipMalloc(&P->d_part, std::max<int64_t>(1, P->part_tiles) * ST * ST * sizeof(float)));
+    if (!P->d_part) SPLIT_TRY(ctx, hipMalloc(&P->d_part, std::max<int64_t>(1, P->part_elems) * sizeof(float)));
     SPLIT_TRY(ctx, hipStreamSynchronize(ctx->stream));
     (bf ? P->bf16_ready : P->f32_ready) = true;
     return LCPAO_OK;
@@ -743,16 +792,15 @@ int split_hamiltonian(lcpao_ctx *ctx, int nch, const double *const *d_V, double 
                     SPLIT_TRY(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(k_hblk_f32),
                                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
                     hipLaunchKernelGGL(k_hblk_f32, dim3((unsigned)ic.count), dim3(64 * HB_WAVES), lds, ctx->stream,
-                                       P->d_items + ic.first, P->d_blocks, P->d_t32, d_V[c], ctx->ngrid[0],
-                                       ctx->ngrid[1], ctx->ngrid[2], P->d_part);
+                                       P->d_items + ic.first, P->d_t32, block_args(ctx, d_V[c], nullptr));
                 }
             }
         }
         if (ctx->ncanon > 0) {
             KTimer kt(ctx, KID_HRED);
             hipLaunchKernelGGL(k_hblk_reduce, dim3((unsigned)ctx->ncanon), dim3(256), 0, ctx->stream, ctx->d_hcanon,
-                               ctx->ncanon, P->d_contrib_off, P->d_contrib, P->d_blocks, ctx->d_pairs, P->d_part,
-                               ctx->gridvol, d_H[c]);
+                               ctx->ncanon, P->d_contrib_off, P->d_contrib, ctx->d_pairs, P->d_part, ctx->gridvol,
+                               d_H[c]);
         }
         SPLIT_TRY(ctx, hipGetLastError());
     }
@@ -809,7 +857,7 @@ void launch_hblk_reduce(const lcpao_ctx *ctx, hipStream_t stream, double *H) {
     const BlockPlan *P = ctx->plan;
     if (ctx->ncanon > 0)
         hipLaunchKernelGGL(k_hblk_reduce, dim3((unsigned)ctx->ncanon), dim3(256), 0, stream, ctx->d_hcanon, ctx->ncanon,
-                           P->d_contrib_off, P->d_contrib, P->d_blocks, ctx->d_pairs, P->d_part, ctx->gridvol, H);
+                           P->d_contrib_off, P->d_contrib, ctx->d_pairs, P->d_part, ctx->gridvol, H);
 }
 
 constexpr int RB_BYTES_PER_ROW = 2 * CHUNK * 2;   // LDS bytes per row: hi + lo point fragments
@@ -1018,6 +1066,7 @@ BlockArgs block_args(const lcpao_ctx *ctx, const double *V, double *rho) {
     a.hi = P->d_hi; a.lo = P->d_lo; a.thi = P->d_thi; a.tlo = P->d_tlo;
     a.rowinfo = P->d_rowinfo; a.spair = P->d_spair;
     a.V = V; a.Deff = ctx->d_deff; a.part = P->d_part; a.rho = rho;
+    a.cbase = P->d_cbase; a.slotno = P->d_slotno;
     a.n1 = ctx->ngrid[0]; a.n2 = ctx->ngrid[1]; a.n3 = ctx->ngrid[2];
     return a;
 }

@@ -32,12 +32,13 @@ constexpr int ST_PER_WAVE = 2;
 
 struct BlockDev {
     int64_t tile_off;   // element offset of the block's tiles
-    int64_t part_off;   // super-tile offset of the block's partial C
     int64_t sp_off;     // offset of the block's slot-pair table (nslot x nslot)
     int32_t rows;       // R (multiple of 64)
     int32_t g1, g2, g3; // first grid point of the block
     int32_t nslot;      // sphere images covering the block
     int32_t row_off;    // offset of the block's row descriptors (R entries)
+    int32_t slot_off;   // offset of the block's slot orbital counts (nslot entries)
+    int32_t pad;
 };
 
 // Folded density block of the pair formed by two slots a <= b of a block:
@@ -65,6 +66,8 @@ struct BlockArgs {
     const double *Deff;            // density: folded pair blocks
     float *part;                   // H: super-tile partials
     double *rho;                   // density: output
+    const int64_t *cbase;          // H: compact partial slot of each slot pair (-1: none)
+    const int16_t *slotno;         // H: orbital count of each slot
     int32_t n1, n2, n3, pad;
 };
 
@@ -73,10 +76,13 @@ struct ItemClass {
     int32_t rows, first, count;
 };
 
-// one (pair, block) sub-block of C_b: element (i, j) of H[pair] is
-// C_b[r0 + i][c0 + j] (or C_b[r0 + j][c0 + i] when trans != 0)
+// one (pair, block) sub-block of C_b, kept as a compact partial of
+// NO_x * NO_y floats at `base` in the block's slot orientation (x <= y):
+// element (i, j) of H[pair] is partial[i * NO_B + j], or partial[j * NO_A + i]
+// when trans != 0 (the pair's atom A is slot y)
 struct Contrib {
-    int32_t block, r0, c0, trans;
+    int64_t base;
+    int32_t block, trans;
 };
 
 struct BlockPlan {
@@ -86,7 +92,7 @@ struct BlockPlan {
     std::vector<HItem> items;        // sorted by rows
     std::vector<ItemClass> classes;
     int64_t tile_elems = 0;    // total tile elements (per hi/lo plane)
-    int64_t part_tiles = 0;    // total super-tiles
+    int64_t part_tiles = 0;    // total upper super-tiles
     int32_t max_rows = 0;
     int32_t max_slots = 0;
     // sphere entries flattened: per (block, sphere)
@@ -107,7 +113,10 @@ struct BlockPlan {
     int64_t ncontrib = 0;
     float *d_t32 = nullptr;                     // fp32 tiles (permuted point order)
     bool bf16_ready = false, f32_ready = false;
-    float *d_part = nullptr;
+    float *d_part = nullptr;       // compact partials (part_elems floats)
+    int64_t part_elems = 0;
+    int64_t *d_cbase = nullptr;    // per block slot pair
+    int16_t *d_slotno = nullptr;   // per block slot
 };
 
 int plan_build(lcpao_ctx *ctx);
