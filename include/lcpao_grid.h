@@ -204,10 +204,10 @@ typedef struct lcpao_ucell lcpao_ucell;
 
 /* ---- batched form (no counterpart in the reference) ------------------------
  * The split-BF16x3 grid H (1 channel) and density (1 spin) of many
- * structures in one go: the block items of all structures are merged per
- * row count and each merged list runs as one persistent launch whose
- * workgroups pull items from a work queue; the whole sequence can be
- * captured in a HIP graph and replayed.  For callers that assemble many
+ * structures in one go: the block items of all structures are merged into
+ * one list per operation, run as one launch (persistent, its workgroups
+ * pulling items from a work queue, or one workgroup per item); the whole
+ * sequence can be captured in a HIP graph and replayed.  For callers that assemble many
  * small or uneven cells (structure screening, many geometries).
  *
  * lcpao_batch_create takes n contexts on one device, each with tables set

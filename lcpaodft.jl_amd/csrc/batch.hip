@@ -1,8 +1,8 @@
 // batch.hip -- batched form: the split-BF16x3 grid Hamiltonian (1 channel)
 // and density (1 spin) of many structures at once.  The block items of all
-// structures are merged per row count; each merged list runs as one
-// persistent launch whose workgroups pull items from a work-queue counter
-// (longest items first), so small structures no longer leave CUs idle and
+// structures are merged into one list that runs as one launch per
+// operation, persistent with a work-queue counter or one workgroup per item
+// (largest items first), so small structures no longer leave CUs idle and
 // the per-structure launch and synchronisation gaps disappear.  The whole
 // sequence (fold, fills, merged H launches, per-structure reductions, merged
 // density launches) can be captured once in a HIP graph and replayed.
@@ -137,9 +137,12 @@ int lcpao_batch_create(lcpao_batch **out, int32_t n, lcpao_ctx *const *ctxs) {
         return rc;
     };
     if (hipSetDevice(dev) != hipSuccess || hipStreamCreate(&b->stream) != hipSuccess) return fail(LCPAO_EDEVICE);
-    // every structure's block plan and bf16 tiles
-    std::vector<std::vector<HItem>> per_rows;
-    std::vector<int32_t> rows_of;
+    // every structure's block plan and bf16 tiles; all items in one list,
+    // largest (rows, then super-tiles) first, run as one launch per operation
+    // (the kernels hold one workgroup per CU whatever R is)
+    std::vector<HItem> items;
+    std::vector<int32_t> item_rows;
+    int32_t max_rows = 0;
     for (int s = 0; s < n; s++) {
         lcpao_ctx *c = b->ctx[s];
         if (!c->tables) return fail(LCPAO_ESTATE);
@@ -148,24 +151,24 @@ int lcpao_batch_create(lcpao_batch **out, int32_t n, lcpao_ctx *const *ctxs) {
         if (!c->d_deff) return fail(LCPAO_ESTATE);
         const lcpao::BlockPlan *P = c->plan;
         b->max_slots = std::max(b->max_slots, P->max_slots);
-        for (const auto &ic : P->classes) {
-            size_t k = std::find(rows_of.begin(), rows_of.end(), ic.rows) - rows_of.begin();
-            if (k == rows_of.size()) { rows_of.push_back(ic.rows); per_rows.emplace_back(); }
-            for (int i = 0; i < ic.count; i++) {
-                HItem it = P->items[ic.first + i];
-                it.pad = s;
-                per_rows[k].push_back(it);
-            }
+        max_rows = std::max(max_rows, P->max_rows);
+        for (HItem it : P->items) {
+            it.pad = s;
+            items.push_back(it);
+            item_rows.push_back(P->blocks[it.block].rows);
         }
     }
-    // merged lists, longest items (most super-tiles) first
-    std::vector<HItem> items;
-    for (size_t k = 0; k < rows_of.size(); k++) {
-        auto &v = per_rows[k];
-        std::stable_sort(v.begin(), v.end(), [](const HItem &x, const HItem &y) { return x.nst > y.nst; });
-        b->classes.push_back({rows_of[k], (int32_t)items.size(), (int32_t)v.size()});
-        items.insert(items.end(), v.begin(), v.end());
+    std::vector<size_t> order(items.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+        return item_rows[x] != item_rows[y] ? item_rows[x] > item_rows[y] : items[x].nst > items[y].nst;
+    });
+    {
+        std::vector<HItem> sorted(items.size());
+        for (size_t i = 0; i < order.size(); i++) sorted[i] = items[order[i]];
+        items.swap(sorted);
     }
+    b->classes.push_back({max_rows, 0, (int32_t)items.size()});
     const size_t ni = std::max<size_t>(1, items.size());
     if (hipMalloc(&b->d_items, ni * sizeof(HItem)) != hipSuccess ||
         hipMalloc(&b->d_args_h, n * sizeof(BlockArgs)) != hipSuccess ||

@@ -604,9 +604,12 @@ int plan_build(lcpao_ctx *ctx) {
     P->tile_elems = toff;
     P->part_tiles = poff;
     P->nsph = sph_first[NB];
-    // items grouped by row count (one launch per group, LDS sized for its R)
-    std::stable_sort(P->items.begin(), P->items.end(),
-                     [&](const HItem &x, const HItem &y) { return P->blocks[x.block].rows < P->blocks[y.block].rows; });
+    // items by decreasing work (rows, then super-tiles) so that the largest
+    // start first; the row classes are kept for the plan statistics
+    std::stable_sort(P->items.begin(), P->items.end(), [&](const HItem &x, const HItem &y) {
+        const int rx = P->blocks[x.block].rows, ry = P->blocks[y.block].rows;
+        return rx != ry ? rx > ry : x.nst > y.nst;
+    });
     for (size_t i = 0; i < P->items.size(); i++) {
         const int R = P->blocks[P->items[i].block].rows;
         if (P->classes.empty() || P->classes.back().rows != R) P->classes.push_back({R, (int32_t)i, 0});
@@ -782,18 +785,20 @@ int split_hamiltonian(lcpao_ctx *ctx, int nch, const double *const *d_V, double 
         if (!d_V[c] || !d_H[c]) { ctx->err = "null channel pointer"; return LCPAO_EINVAL; }
         if (!P->items.empty()) {
             KTimer kt(ctx, KID_HAM);
-            for (const ItemClass &ic : P->classes) {
-                if (bf) {
-                    const BlockArgs one = block_args(ctx, d_V[c], nullptr);
-                    SPLIT_TRY(ctx, launch_block_kernel(false, ctx->stream, ic.rows, 0, P->d_items + ic.first, ic.count,
-                                                       nullptr, one, nullptr, (unsigned)ic.count));
-                } else {
-                    const size_t lds = (size_t)ic.rows * LROW32 * sizeof(float) + CHUNK * sizeof(float);
-                    SPLIT_TRY(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(k_hblk_f32),
-                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                    hipLaunchKernelGGL(k_hblk_f32, dim3((unsigned)ic.count), dim3(64 * HB_WAVES), lds, ctx->stream,
-                                       P->d_items + ic.first, P->d_t32, block_args(ctx, d_V[c], nullptr));
-                }
+            // one launch for all items: the kernels run one workgroup per CU
+            // (register-bound) whatever R is, so LDS and prefetch depth are
+            // sized for the largest block at no cost in occupancy
+            const unsigned ni = (unsigned)P->items.size();
+            if (bf) {
+                const BlockArgs one = block_args(ctx, d_V[c], nullptr);
+                SPLIT_TRY(ctx, launch_block_kernel(false, ctx->stream, P->max_rows, 0, P->d_items, (int32_t)ni, nullptr,
+                                                   one, nullptr, ni));
+            } else {
+                const size_t lds = (size_t)P->max_rows * LROW32 * sizeof(float) + CHUNK * sizeof(float);
+                SPLIT_TRY(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(k_hblk_f32),
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                hipLaunchKernelGGL(k_hblk_f32, dim3(ni), dim3(64 * HB_WAVES), lds, ctx->stream, P->d_items, P->d_t32,
+                                   block_args(ctx, d_V[c], nullptr));
             }
         }
         if (ctx->ncanon > 0) {
@@ -1128,11 +1133,10 @@ int split_density(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t 
                                d_rho[s], 0.0);
         if (!P->items.empty()) {
             KTimer kt(ctx, KID_RHO);
-            for (const ItemClass &ic : P->classes) {
-                const BlockArgs one = block_args(ctx, nullptr, d_rho[s]);
-                SPLIT_TRY(ctx, launch_block_kernel(true, ctx->stream, ic.rows, P->max_slots, P->d_items + ic.first,
-                                                   ic.count, nullptr, one, nullptr, (unsigned)ic.count));
-            }
+            const unsigned ni = (unsigned)P->items.size();
+            const BlockArgs one = block_args(ctx, nullptr, d_rho[s]);
+            SPLIT_TRY(ctx, launch_block_kernel(true, ctx->stream, P->max_rows, P->max_slots, P->d_items, (int32_t)ni,
+                                               nullptr, one, nullptr, ni));
         }
         SPLIT_TRY(ctx, hipGetLastError());
     }
