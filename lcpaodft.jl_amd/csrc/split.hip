@@ -178,20 +178,60 @@ __device__ __forceinline__ void run_items(const HItem *__restrict__ items, int32
 // mirror element, so every element of every slot is written exactly once.
 struct EpiTables {
     const int16_t *rinfo;
-    const int64_t *cb;
+    const int32_t *cb;
     const int16_t *no;
     int ns;
 };
-__device__ __forceinline__ EpiTables stage_epilogue(const BlockArgs &A, const BlockDev &B, unsigned char *lds) {
-    int64_t *cb = reinterpret_cast<int64_t *>(lds);
+__device__ __forceinline__ EpiTables epi_layout(const BlockDev &B, unsigned char *lds) {
+    int32_t *cb = reinterpret_cast<int32_t *>(lds);
     int16_t *rinfo = reinterpret_cast<int16_t *>(cb + B.nslot * B.nslot);
-    int16_t *no = rinfo + B.rows;
+    return EpiTables{rinfo, cb, rinfo + B.rows, B.nslot};
+}
+__device__ __forceinline__ EpiTables stage_epilogue(const BlockArgs &A, const BlockDev &B, unsigned char *lds) {
+    const EpiTables T = epi_layout(B, lds);
     __syncthreads();  // every wave is done with the chunk buffers
-    for (int x = threadIdx.x; x < B.nslot * B.nslot; x += blockDim.x) cb[x] = A.cbase[B.sp_off + x];
-    for (int x = threadIdx.x; x < B.rows; x += blockDim.x) rinfo[x] = A.rowinfo[B.row_off + x];
-    for (int x = threadIdx.x; x < B.nslot; x += blockDim.x) no[x] = A.slotno[B.slot_off + x];
+    for (int x = threadIdx.x; x < B.nslot * B.nslot; x += blockDim.x) const_cast<int32_t *>(T.cb)[x] = A.cbase[B.sp_off + x];
+    for (int x = threadIdx.x; x < B.rows; x += blockDim.x) const_cast<int16_t *>(T.rinfo)[x] = A.rowinfo[B.row_off + x];
+    for (int x = threadIdx.x; x < B.nslot; x += blockDim.x) const_cast<int16_t *>(T.no)[x] = A.slotno[B.slot_off + x];
     __syncthreads();
-    return EpiTables{rinfo, cb, no, B.nslot};
+    return T;
+}
+// The same tables fetched into registers while the last chunk is multiplied
+// (512 threads: up to 4 slot-pair entries, 2 rows and 1 slot each), written
+// to LDS after it.
+constexpr int EPI_SP = 4, EPI_ROWS = 2;
+struct EpiRegs {
+    int32_t cb[EPI_SP];
+    int32_t ri[EPI_ROWS];
+    int32_t no;
+};
+__device__ __forceinline__ void epi_fetch(EpiRegs &E, const BlockArgs &A, const BlockDev &B, int tid) {
+    const int nsp = B.nslot * B.nslot;
+#pragma unroll
+    for (int k = 0; k < EPI_SP; k++) {
+        const int x = tid + k * 64 * HB_WAVES;
+        E.cb[k] = x < nsp ? A.cbase[B.sp_off + x] : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < EPI_ROWS; k++) {
+        const int x = tid + k * 64 * HB_WAVES;
+        E.ri[k] = x < B.rows ? A.rowinfo[B.row_off + x] : -1;
+    }
+    E.no = tid < B.nslot ? A.slotno[B.slot_off + tid] : 0;
+}
+__device__ __forceinline__ EpiTables epi_store(const EpiRegs &E, const BlockDev &B, unsigned char *lds, int tid) {
+    const EpiTables T = epi_layout(B, lds);
+    __syncthreads();  // every wave is done with the chunk buffers
+    const int nsp = B.nslot * B.nslot;
+#pragma unroll
+    for (int k = 0; k < EPI_SP; k++)
+        if (tid + k * 64 * HB_WAVES < nsp) const_cast<int32_t *>(T.cb)[tid + k * 64 * HB_WAVES] = E.cb[k];
+#pragma unroll
+    for (int k = 0; k < EPI_ROWS; k++)
+        if (tid + k * 64 * HB_WAVES < B.rows) const_cast<int16_t *>(T.rinfo)[tid + k * 64 * HB_WAVES] = (int16_t)E.ri[k];
+    if (tid < B.nslot) const_cast<int16_t *>(T.no)[tid] = (int16_t)E.no;
+    __syncthreads();
+    return T;
 }
 __device__ __forceinline__ void store_partials(const f32x4 *acc, int si, int sj, int lane, const EpiTables &T,
                                                float *__restrict__ part) {
@@ -211,7 +251,7 @@ __device__ __forceinline__ void store_partials(const f32x4 *acc, int si, int sj,
         for (int v = 0; v < 4; v++) ri[v] = T.rinfo[r0 + v];
         // partial bases for the slot of row r0 (the four rows mostly share it)
         const int a0 = ri[0] >> 5;
-        int64_t base0[4];
+        int32_t base0[4];
 #pragma unroll
         for (int ct = 0; ct < 4; ct++)
             base0[ct] = (ri[0] >= 0 && ci[ct] >= 0) ? T.cb[a0 * T.ns + (ci[ct] >> 5)] : -1;
@@ -225,7 +265,7 @@ __device__ __forceinline__ void store_partials(const f32x4 *acc, int si, int sj,
                 const int c = sj * ST + ct * 16 + l16;
                 if (r > c || ci[ct] < 0) continue;  // r > c only in diagonal super-tiles: mirror of (c, r)
                 const int b = ci[ct] >> 5, j = ci[ct] & 31;
-                const int64_t base = a == a0 ? base0[ct] : T.cb[a * T.ns + b];
+                const int32_t base = a == a0 ? base0[ct] : T.cb[a * T.ns + b];
                 if (base < 0) continue;
                 const float x = acc[rt * 4 + ct][v];
                 part[base + i * nb[ct] + j] = x;
@@ -291,6 +331,7 @@ __device__ __forceinline__ void hblk_item(const HItem it, const BlockArgs &A, un
         acc0[t] = f32x4{0.f, 0.f, 0.f, 0.f};
         acc1[t] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
+    EpiRegs epi;
     u16x8 fh[NF], fl[NF];
     // chunk c: fragments g = wave + 8 f of both planes
     auto prefetch = [&](int c) {
@@ -331,10 +372,10 @@ __device__ __forceinline__ void hblk_item(const HItem it, const BlockArgs &A, un
         }
         release(slot);
         __syncthreads();
-        // past the last chunk these reload chunk 15 (cache hits), keeping the
-        // number of loads in flight the same in every step
-        prefetch(min(c + 1, NCHUNK - 1));
-        slot = touch_chunk(hi, lo, B.tile_off + (int64_t)min(c + 2, NCHUNK - 1) * G * FRAG, R, tid);
+        // c is a constant in the unrolled loop: these branches fold away
+        if (c + 1 < NCHUNK) prefetch(c + 1);
+        else epi_fetch(epi, A, B, tid);  // the epilogue's tables, behind the last chunk's MFMAs
+        if (c + 2 < NCHUNK) slot = touch_chunk(hi, lo, B.tile_off + (int64_t)(c + 2) * G * FRAG, R, tid);
         if (has0) tile_bf16x3(PH, PL, VH, VL, si0, sj0, lane, acc0);
         if (has1) tile_bf16x3(PH, PL, VH, VL, si1, sj1, lane, acc1);
     };
@@ -348,7 +389,7 @@ __device__ __forceinline__ void hblk_item(const HItem it, const BlockArgs &A, un
         step(c + 1, t1);
     }
     // lane holds D[4q + v][l16] of each 16x16 tile
-    const EpiTables T = stage_epilogue(A, B, lds_raw);
+    const EpiTables T = epi_store(epi, B, lds_raw, tid);
     if (has0) store_partials(acc0, si0, sj0, lane, T, part);
     if (has1) store_partials(acc1, si1, sj1, lane, T, part);
 }
@@ -592,6 +633,11 @@ int plan_build(lcpao_ctx *ctx) {
         if (roff + R > INT32_MAX) { ctx->err = "too many grid blocks for the split-precision plan"; delete P; return LCPAO_ERANGE; }
         P->blocks[b] = {toff, spoff, R, b1 * BLK, b2 * BLK, b3 * BLK, ns, (int32_t)roff, (int32_t)soff, 0};
         P->max_slots = std::max(P->max_slots, ns);
+        if (ns * ns > EPI_SP * 64 * HB_WAVES) {
+            ctx->err = "a grid block is covered by more sphere images than the split-precision kernels hold";
+            delete P;
+            return LCPAO_ERANGE;
+        }
         soff += ns;
         toff += (int64_t)R * BLK_PTS;
         poff += nst;
@@ -625,7 +671,7 @@ int plan_build(lcpao_ctx *ctx) {
     std::vector<int16_t> slotno(std::max<int64_t>(1, soff), 0);
     for (int64_t b = 0; b < NB; b++)
         for (size_t x = 0; x < sph[b].size(); x++) slotno[P->blocks[b].slot_off + x] = (int16_t)ctx->no[sph[b][x].first];
-    std::vector<int64_t> cbase(std::max<int64_t>(1, spoff), -1);
+    std::vector<int32_t> cbase(std::max<int64_t>(1, spoff), -1);
     std::vector<SlotPair> spairs(std::max<int64_t>(1, spoff), SlotPair{-1, 0});
     // sphere tables and point maps
     std::vector<int32_t> sb(P->nsph), sa(P->nsph), sr(P->nsph), smap((size_t)P->nsph * BLK_PTS, -1);
@@ -721,10 +767,11 @@ int plan_build(lcpao_ctx *ctx) {
         const PairDev &Q = ctx->pairs[cpair[i]];
         const int64_t sp = B.sp_off + (int64_t)all[i].x * B.nslot + all[i].y;
         spairs[sp] = SlotPair{(int32_t)Q.hoff, Q.nob | (all[i].trans << 6)};
-        cbase[sp] = pel;
+        cbase[sp] = (int32_t)pel;
         pel += (int64_t)slotno[B.slot_off + all[i].x] * slotno[B.slot_off + all[i].y];
     }
     P->part_elems = pel;
+    if (pel > INT32_MAX) { ctx->err = "H partials exceed the split-precision index range"; delete P; return LCPAO_ERANGE; }
     P->ncontrib = (int64_t)all.size();
     for (size_t c = 0; c < ctx->canon.size(); c++) coff[c + 1] += coff[c];
     int rc;

@@ -66,7 +66,7 @@ struct BlockArgs {
     const double *Deff;            // density: folded pair blocks
     float *part;                   // H: super-tile partials
     double *rho;                   // density: output
-    const int64_t *cbase;          // H: compact partial slot of each slot pair (-1: none)
+    const int32_t *cbase;          // H: compact partial slot of each slot pair (-1: none)
     const int16_t *slotno;         // H: orbital count of each slot
     int32_t n1, n2, n3, pad;
 };
@@ -115,7 +115,7 @@ struct BlockPlan {
     bool bf16_ready = false, f32_ready = false;
     float *d_part = nullptr;       // compact partials (part_elems floats)
     int64_t part_elems = 0;
-    int64_t *d_cbase = nullptr;    // per block slot pair
+    int32_t *d_cbase = nullptr;    // per block slot pair
     int16_t *d_slotno = nullptr;   // per block slot
 };
 
