@@ -35,9 +35,10 @@ def parse():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--config", default=None, choices=["si2", "si64", "gaas216", "si511"],
                    help="workload (default gaas216 for --op assembly, si511 for --op force3)")
-    p.add_argument("--op", default="assembly", choices=["assembly", "force3", "batch"],
+    p.add_argument("--op", default="assembly", choices=["assembly", "force3", "batch", "scf"],
                    help="assembly: grid H + rho (the headline metric); force3: the force contraction; "
-                        "batch: H + rho of 32 uneven structures, batched vs a per-structure loop")
+                        "batch: H + rho of 32 uneven structures, batched vs a per-structure loop; "
+                        "scf: total energy of a minimal Gamma-point SCF in each precision form")
     p.add_argument("--batch-count", type=int, default=32)
     p.add_argument("--precision", default="bf16x3", choices=["fp64", "fp32", "bf16x3"],
                    help="headline form (all three forms are timed)")
@@ -338,6 +339,43 @@ def bench_batch(args):
     return 0
 
 
+def bench_scf(args):
+    import torch
+    from lcpao_grid import systems as S
+    from lcpao_grid.grid import GridContext
+    from lcpao_grid.scf import GammaSCF
+    cfg = args.config or "gaas216"
+    zval = {"Si": 4.0, "Ga": 3.0, "As": 5.0, "Na": 1.0, "Cl": 7.0}
+    t0 = time.time()
+    st = S.CONFIGS[cfg]()
+    u = st.ucell()
+    ctx = GridContext(u, device=0)
+    ctx.build_tables(st.paos(), with_derivatives=True)
+    scf = GammaSCF(st, u, ctx, zval)
+    setup_s = time.time() - t0
+    res = {}
+    for prec, tol in (("fp64", 1e-9), ("fp32", 1e-7), ("bf16x3", 1e-7)):
+        torch.cuda.synchronize()
+        t1 = time.time()
+        e, it = scf.run(prec, tol=tol)
+        torch.cuda.synchronize()
+        res[prec] = {"E_Ha": e, "iterations": it, "s": round(time.time() - t1, 2)}
+    e64 = res["fp64"]["E_Ha"]
+    for prec in ("fp32", "bf16x3"):
+        res[prec]["dE_uHa_per_atom"] = 1e6 * abs(res[prec]["E_Ha"] - e64) / st.Natom
+    out = {
+        "metric": "SCF total-energy error per atom of the grid-assembly precision forms (minimal Gamma-point SCF)",
+        "value": res["bf16x3"]["dE_uHa_per_atom"], "unit": "uHa/atom", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": None, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16x3 vs f64", "data": "synthetic (model SCF: grid kinetic, Gaussian ions, Hartree, LDA exchange)",
+        "config": {"workload": "%s: %d atoms, %d orbitals, %g electrons, grid %s" % (cfg, st.Natom, scf.norb, scf.nel,
+                                                                                 "x".join(map(str, st.Ngrid)))},
+        "forms": res, "setup_s": round(setup_s, 2),
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -353,6 +391,8 @@ def main():
         return bench_force3(args)
     if args.op == "batch":
         return bench_batch(args)
+    if args.op == "scf":
+        return bench_scf(args)
     args.config = args.config or "gaas216"
 
     t_setup = time.time()
