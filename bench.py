@@ -189,7 +189,7 @@ def bench_force3(args):
     dV = torch.from_numpy(V).to(dev)
     dDM = torch.from_numpy(DM).to(dev)
     stream = torch.cuda.current_stream(dev)
-    ctx.L.lcpao_grid_set_stream(ctx.h, stream.cuda_stream)
+    ctx.set_stream(stream)
     forms = ["fp32", "fp64"] if args.precision != "fp64" else ["fp64", "fp32"]
     res, timing = {}, {}
     for f in forms:
@@ -266,7 +266,7 @@ def bench_batch(args):
                           rho=torch.empty(u.NN, dtype=torch.float64, device=dev), Vh=V, DMh=DM))
     stream = torch.cuda.current_stream(dev)
     for it in items:
-        it["ctx"].L.lcpao_grid_set_stream(it["ctx"].h, stream.cuda_stream)
+        it["ctx"].set_stream(stream)
 
     def loop(prec):
         for it in items:
@@ -407,7 +407,7 @@ def main():
     dH = torch.empty(u.myHsize, dtype=torch.float64, device=dev)
     drho = torch.empty(u.NN, dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream(dev)
-    ctx.L.lcpao_grid_set_stream(ctx.h, stream.cuda_stream)
+    ctx.set_stream(stream)
     # one untimed call per form builds its per-geometry data (block plan, tiles)
     forms = [args.precision] + [f for f in ("fp64", "fp32", "bf16x3") if f != args.precision]
     timed = [args.precision] if args.only_headline else forms

@@ -147,7 +147,10 @@ int lcpao_grid_force3(lcpao_ctx *ctx, int32_t nch, const double *const *V, const
 /* Device-resident variants of the three contractions: same semantics, all
  * array pointers are device pointers; d_force stays natom x 3 column-major
  * and is accumulated into.  These are the entry points timed by the
- * benchmark (inputs already in HBM). */
+ * benchmark (inputs already in HBM).  They launch on the context's stream
+ * (its own non-blocking stream, or the one bound by lcpao_grid_set_stream)
+ * and return when it has finished; inputs written by other streams must be
+ * complete before the call (synchronise their producer, or bind it). */
 int lcpao_grid_hamiltonian_dev(lcpao_ctx *ctx, int32_t nch, const double *const *d_V, double *const *d_H,
                                int32_t precision);
 int lcpao_grid_density_dev(lcpao_ctx *ctx, int32_t nspin, const double *const *d_DM, int64_t dm_offset,

@@ -34,6 +34,7 @@ class Batch:
         n = len(self.contexts)
         if not (len(dV) == len(dDM) == len(dH) == len(drho) == n):
             raise ValueError("one V, DM, H and rho per structure")
+        self.contexts[0]._order()  # the batch stream does not follow torch's stream
         self._check(self.L.lcpao_batch_assemble(self.h, self._ptrs(dV), self._ptrs(dDM), self._ptrs(dH),
                                                 self._ptrs(drho), int(bool(use_graph))))
 

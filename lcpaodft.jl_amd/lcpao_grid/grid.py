@@ -94,6 +94,8 @@ class GridContext:
             raise GridError(rc, L.lcpao_grid_last_error(None).decode())
         self._keep = []  # the library copied everything
         self.L = L
+        self.device = int(device)
+        self._stream = None  # caller stream bound by set_stream, else None
         hs, nn, to, tg = (C.c_int64() for _ in range(4))
         L.lcpao_grid_sizes(self.h, C.byref(hs), C.byref(nn), C.byref(to), C.byref(tg))
         self.my_hsize, self.nn, self.total_olg, self.total_gridn = hs.value, nn.value, to.value, tg.value
@@ -198,15 +200,35 @@ class GridContext:
         self._check(self.L.lcpao_grid_tuned(self.h, self.TUNE_OPS[op], C.byref(v), ms, 4))
         return v.value, [round(float(x), 4) for x in ms[:2]]
 
+    def set_stream(self, stream):
+        """Launch on `stream` (a torch.cuda.Stream, or None for the context's
+        own stream).  The default stream of torch is the HIP null stream, which
+        the library takes as "own stream": work on it is then ordered by a
+        wait before each device call."""
+        handle = 0 if stream is None else int(stream.cuda_stream)
+        self._check(self.L.lcpao_grid_set_stream(self.h, C.c_void_p(handle or None)))
+        self._stream = stream if handle else None
+
+    def _order(self):
+        """The device calls read their inputs on the context's stream, which
+        does not follow the torch stream that produced them: wait for that
+        stream unless the context launches on it."""
+        if self._stream is None:
+            import torch
+            torch.cuda.current_stream(self.device).synchronize()
+
     def hamiltonian_dev(self, dV, dH, precision="fp64"):
+        self._order()
         self._check(self.L.lcpao_grid_hamiltonian_dev(self.h, len(dV), self._dptrs(dV), self._dptrs(dH), _prec(precision)))
 
     def density_dev(self, dDM, drho, dm_offset=0, precision="fp64"):
+        self._order()
         self._check(self.L.lcpao_grid_density_dev(self.h, len(dDM), self._dptrs(dDM), int(dm_offset), self._dptrs(drho),
                                                   _prec(precision)))
 
     def force3_dev(self, dV, dDM, dforce, weights=None, dm_offset=0, precision="fp64"):
         w = np.ascontiguousarray(weights if weights is not None else WEIGHTS_FORCE3[len(dV)], np.float64)
+        self._order()
         self._check(self.L.lcpao_grid_force3_dev(self.h, len(dV), self._dptrs(dV), self._dptrs(dDM), int(dm_offset),
                                                  w.ctypes.data_as(_lib.dp), C.cast(dforce.data_ptr(), _lib.dp),
                                                  _prec(precision)))

@@ -142,19 +142,24 @@ class GammaSCF:
         return 0.5 * T
 
     # ---- the loop -----------------------------------------------------------
-    def run(self, precision, max_iter=100, tol=1e-9, ndiis=8, beta=0.5, verbose=False):
+    def run(self, precision, max_iter=100, tol=1e-9, ndiis=8, beta=0.5, sthresh=1e-5, verbose=False):
+        """SCF with the grid path in `precision`; combinations of basis
+        functions whose overlap eigenvalue is below sthresh * max (near
+        linear dependence of the basis on the grid) are dropped."""
         ctx, dev = self.ctx, self.dev
         ev, U = torch.linalg.eigh(self.S)
-        keep = ev > 1e-8 * ev.max()
+        keep = ev > sthresh * ev.max()
         X = U[:, keep] / torch.sqrt(ev[keep])  # canonical orthogonalisation
         dV = torch.empty(self.u.NN, dtype=torch.float64, device=dev)
         dH = torch.empty(self.u.myHsize, dtype=torch.float64, device=dev)
         drho = torch.empty(self.u.NN, dtype=torch.float64, device=dev)
-        n_in = None
+        # start from the uniform density (the ions' G = 0 partner); every step
+        # is mixed, so the bare-ion first density never enters unmixed
+        n_in = torch.full((self.u.NN,), self.nel / self.omega, dtype=torch.float64, device=dev)
         hist_n, hist_r = [], []
         E_prev, E = None, None
         for it in range(max_iter):
-            veff = self.vion if n_in is None else self.vion + self.hartree(n_in) + self.exchange(n_in)[0]
+            veff = self.vion + self.hartree(n_in) + self.exchange(n_in)[0]
             dV.copy_(veff)
             ctx.hamiltonian_dev([dV], [dH], precision)
             H = self.T + self.gamma(dH)
@@ -164,19 +169,15 @@ class GammaSCF:
             D = (C * (0.5 * f)) @ C.T  # per spin
             ctx.density_dev([self.flat_dm(D)], [drho], 0, precision)
             n_out = 2.0 * drho
-            if n_in is not None:
-                vh = self.hartree(n_in)
-                vx, ex = self.exchange(n_in)
-                E = float((f * eps).sum() - self.gv * torch.dot(n_in, vh + vx)
-                          + 0.5 * self.gv * torch.dot(n_in, vh) + self.gv * ex.sum()) + ts
-                if verbose:
-                    print("  it %2d  E %.10f  |dn| %.3e" % (it, E, float((n_out - n_in).abs().max())), flush=True)
-                if E_prev is not None and abs(E - E_prev) < tol:
-                    return E, it
-                E_prev = E
-            if n_in is None:
-                n_in = n_out
-                continue
+            vh = self.hartree(n_in)
+            vx, ex = self.exchange(n_in)
+            E = float((f * eps).sum() - self.gv * torch.dot(n_in, vh + vx)
+                      + 0.5 * self.gv * torch.dot(n_in, vh) + self.gv * ex.sum()) + ts
+            if verbose:
+                print("  it %2d  E %.10f  |dn| %.3e" % (it, E, float((n_out - n_in).abs().max())), flush=True)
+            if E_prev is not None and abs(E - E_prev) < tol:
+                return E, it
+            E_prev = E
             # Pulay mixing of the density
             hist_n.append(n_in.clone())
             hist_r.append(n_out - n_in)
