@@ -187,6 +187,56 @@ enum { LCPAO_TUNE_RHO64 = 0, LCPAO_TUNE_F364 = 1 };
 int lcpao_grid_tune(lcpao_ctx *ctx, int32_t op, int32_t variant);
 int lcpao_grid_tuned(const lcpao_ctx *ctx, int32_t op, int32_t *variant, float *candidate_ms, int32_t ncand);
 
+/* ---- grid fields next to the contractions --------------------------------
+ * Radial data of one species for the atomic densities: the PAO log mesh and
+ * valence density (Spe_PAO_XV/RV, Spe_Atomic_Den of Read_PAO.jl:183-303, the
+ * density already renormalised to the species' core charge) and the VPS log
+ * mesh and partial-core density (Spe_VPS_XV/RV, Spe_Atomic_PCC of
+ * Read_VPS.jl:196-272).  The two density arrays hold nmesh + 4 values, mesh
+ * point k at index k + 1, with the reference's linear end points. */
+typedef struct {
+    int32_t nmesh_pao;
+    const double *pao_xv, *pao_rv; /* [nmesh_pao] */
+    const double *atomic_den;      /* [nmesh_pao + 4] */
+    int32_t nmesh_vps;
+    const double *vps_xv, *vps_rv; /* [nmesh_vps] */
+    const double *atomic_pcc;      /* [nmesh_vps + 4] */
+} lcpao_atomic_radial;
+
+/* Set_AdenPCC_Grid (Set_AdenPCC_Grid.jl:1-142): atomic density ADensity,
+ * partial-core density PCCDensity (NN each) and the initial spin densities
+ * rho[nspin] (NN each; nspin 1 "off", 2 "on", 4 "nc", the nc rotation of
+ * :125-138 applied) from the atoms' grid lists.  init_nspin: natom x 2 (up,
+ * down electrons, Init_Atoms_Nspin); init_angle: natom x 2 (theta, phi,
+ * Init_Atoms_Angle[1:2]; only read for nspin 4, may be NULL otherwise).
+ * Needs the lattice, positions, species and cell lists of lcpao_geom.  Every
+ * atom of the context contributes, so the result is the reference's after
+ * its MPI.Allreduce! (:118-122).  Each grid point's sum runs in the
+ * reference's atom order.  The _dev variant takes device output pointers. */
+int lcpao_grid_aden_pcc(lcpao_ctx *ctx, int32_t nspin, int32_t nspecies, const lcpao_atomic_radial *spe,
+                        const double *init_nspin, const double *init_angle, double *aden, double *pcc,
+                        double *const *rho);
+int lcpao_grid_aden_pcc_dev(lcpao_ctx *ctx, int32_t nspin, int32_t nspecies, const lcpao_atomic_radial *spe,
+                            const double *init_nspin, const double *init_angle, double *d_aden, double *d_pcc,
+                            double *const *d_rho);
+
+/* Solve_Poisson! (Poisson.jl:1-108): dVHart (NN) of the charge
+ * Density_xyz = 2 rho[0] - 2 aden (nspin 1) or rho[0] + rho[1] - 2 aden
+ * (nspin 2 or 4), as Set_Density_xyz! forms it; nspin 0 solves for rho[0]
+ * itself (the inner Solve_Poisson!(Ngrid, Recvecs, ...), aden unused).
+ * V(G) = 4 pi n(G) / |G|^2 with the reference's signed indices (index i of
+ * an axis of N points counts as i - N from N / 2 on, integer division, so
+ * for odd N the middle index is negative), V(0) = 0, real part of the
+ * normalised inverse transform.  recvecs: Recvecs = 2 pi inv(Latvecs') in
+ * Julia's column-major order (recvecs[3 j + i] = Recvecs[i+1, j+1]).  FFTs
+ * by the library's own LDS-resident mixed-radix passes (axes of at most 1024
+ * points, prime factors up to 13).  The _dev variant takes device
+ * pointers for rho, aden and vhart. */
+int lcpao_grid_poisson(lcpao_ctx *ctx, int32_t nspin, const double *const *rho, const double *aden,
+                       const double *recvecs, double *vhart);
+int lcpao_grid_poisson_dev(lcpao_ctx *ctx, int32_t nspin, const double *const *d_rho, const double *d_aden,
+                           const double *recvecs, double *d_vhart);
+
 /* Sizes of the context (rank-local H size, grid size, sums of the lists). */
 int lcpao_grid_sizes(const lcpao_ctx *ctx, int64_t *my_hsize, int64_t *nn, int64_t *total_olg,
                      int64_t *total_gridn);

@@ -34,6 +34,11 @@ struct SpeciesDev {
 // Block-dense structure of the split-precision forms (built on first use).
 struct BlockPlan;
 
+// FFT plans and work buffers of the grid-field routines (Poisson solve),
+// created on first use; released with the context.
+struct FieldPlan;
+void field_free(FieldPlan *f);
+
 }  // namespace lcpao
 
 using namespace lcpao;
@@ -90,6 +95,8 @@ struct lcpao_ctx {
     int32_t *d_flag = nullptr;
     // split-precision forms
     lcpao::BlockPlan *plan = nullptr;
+    // grid-field routines (atomic densities, Poisson)
+    lcpao::FieldPlan *fields = nullptr;
     float *d_tab32 = nullptr, *d_dtab32[3] = {nullptr, nullptr, nullptr};
     float *d_deff32 = nullptr;
     std::vector<int32_t> h_glist, h_clist;  // host copies for the block plan
@@ -168,3 +175,9 @@ inline void collect_timings(lcpao_ctx *ctx) {
         }                                                                                      \
     } while (0)
 
+namespace lcpao {
+// Shared host helpers of the translation units (lcpao_grid.hip).
+int ctx_enter(lcpao_ctx *ctx);                          // null check + hipSetDevice
+int ctx_error(lcpao_ctx *ctx, int code, const std::string &msg);
+std::vector<double> cell_vectors(const lcpao_ctx *ctx);  // atv[cell][3], Generation_ATV order
+}  // namespace lcpao

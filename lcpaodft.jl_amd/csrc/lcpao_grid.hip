@@ -1307,12 +1307,30 @@ void free_all(lcpao_ctx *ctx) {
     if (ctx->d_deff32) (void)hipFree(ctx->d_deff32);
     plan_free(ctx->plan);
     ctx->plan = nullptr;
+    field_free(ctx->fields);
+    ctx->fields = nullptr;
     for (auto &p : ctx->pending) { (void)hipEventDestroy(p.beg); (void)hipEventDestroy(p.end); }
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     if (ctx->stream && ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
 }
 
 }  // namespace
+
+namespace lcpao {
+int ctx_enter(lcpao_ctx *ctx) { return check_ctx(ctx); }
+int ctx_error(lcpao_ctx *ctx, int code, const std::string &msg) { return set_err(ctx, code, msg); }
+std::vector<double> cell_vectors(const lcpao_ctx *ctx) {
+    const int C = ctx->cpy, W = 2 * C + 1, ncell = W * W * W;
+    std::vector<double> atv((size_t)ncell * 3, 0.0);
+    for (int c = 1; c < ncell; c++) {
+        int ijk[3];
+        cell_ijk(C, c, ijk);
+        const double di = ijk[0], dj = ijk[1], dk = ijk[2];
+        for (int x = 0; x < 3; x++) atv[3 * (size_t)c + x] = di * ctx->lat[x] + dj * ctx->lat[3 + x] + dk * ctx->lat[6 + x];
+    }
+    return atv;
+}
+}  // namespace lcpao
 
 extern "C" {
 
@@ -1589,14 +1607,7 @@ int lcpao_grid_build_tables(lcpao_ctx *ctx, int32_t nspecies, const lcpao_pao *p
     SpeciesDev *d_sd = nullptr;
     double *d_gxyz = nullptr, *d_atv = nullptr;
     int32_t *d_a2s = nullptr;
-    const int C = ctx->cpy, W = 2 * C + 1, ncell = W * W * W;
-    std::vector<double> atv((size_t)ncell * 3, 0.0);
-    for (int c = 1; c < ncell; c++) {
-        int ijk[3];
-        cell_ijk(C, c, ijk);
-        const double di = ijk[0], dj = ijk[1], dk = ijk[2];
-        for (int x = 0; x < 3; x++) atv[3 * (size_t)c + x] = di * ctx->lat[x] + dj * ctx->lat[3 + x] + dk * ctx->lat[6 + x];
-    }
+    const std::vector<double> atv = cell_vectors(ctx);
     bool ok = hipMalloc(&d_sd, sizeof(SpeciesDev) * nspecies) == hipSuccess &&
               hipMalloc(&d_gxyz, sizeof(double) * 3 * ctx->natom) == hipSuccess &&
               hipMalloc(&d_atv, sizeof(double) * atv.size()) == hipSuccess &&

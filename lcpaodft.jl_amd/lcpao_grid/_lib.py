@@ -34,6 +34,11 @@ class Pao(C.Structure):
     _fields_ = [("maxl", C.c_int32), ("num_basis", C.c_int32 * 4), ("nmesh", C.c_int32), ("rv", dp), ("rwf", dpp)]
 
 
+class AtomicRadial(C.Structure):
+    _fields_ = [("nmesh_pao", C.c_int32), ("pao_xv", dp), ("pao_rv", dp), ("atomic_den", dp),
+                ("nmesh_vps", C.c_int32), ("vps_xv", dp), ("vps_rv", dp), ("atomic_pcc", dp)]
+
+
 # every extern "C" symbol declared in include/lcpao_grid.h
 EXPORTS = [
     "lcpao_grid_create", "lcpao_grid_set_tables", "lcpao_grid_set_dtables", "lcpao_grid_build_tables",
@@ -44,6 +49,7 @@ EXPORTS = [
     "lcpao_ucell_copy", "lcpao_ucell_last_error", "lcpao_ucell_destroy",
     "lcpao_batch_create", "lcpao_batch_assemble", "lcpao_batch_tune", "lcpao_batch_tuned", "lcpao_batch_sizes",
     "lcpao_batch_last_error", "lcpao_batch_destroy",
+    "lcpao_grid_aden_pcc", "lcpao_grid_aden_pcc_dev", "lcpao_grid_poisson", "lcpao_grid_poisson_dev",
 ]
 
 _lib = None
@@ -63,6 +69,14 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError("liblcpao_grid.so is not built (run __graft_entry__.build() or make -C lcpaodft.jl_amd/csrc)")
+    # torch ships its own HIP and HSA runtimes under the same sonames.  Loaded
+    # first, they are the ones this library binds to; loaded after it, torch
+    # brings a second pair into the process and its runtime then finds no
+    # GPU.  One runtime per process: torch (when installed) goes first.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     vp = C.c_void_p
     L.lcpao_grid_create.argtypes = [C.POINTER(vp), C.POINTER(Geom), C.c_int]
@@ -87,6 +101,11 @@ def load():
     L.lcpao_grid_density_dev.argtypes = [vp, C.c_int32, dpp, C.c_int64, dpp, C.c_int32]
     L.lcpao_grid_force3_dev.argtypes = [vp, C.c_int32, dpp, dpp, C.c_int64, dp, dp, C.c_int32]
     L.lcpao_grid_sizes.argtypes = [vp, lp, lp, lp, lp]
+    ar = C.POINTER(AtomicRadial)
+    L.lcpao_grid_aden_pcc.argtypes = [vp, C.c_int32, C.c_int32, ar, dp, dp, dp, dp, dpp]
+    L.lcpao_grid_aden_pcc_dev.argtypes = [vp, C.c_int32, C.c_int32, ar, dp, dp, dp, dp, dpp]
+    L.lcpao_grid_poisson.argtypes = [vp, C.c_int32, dpp, dp, dp, dp]
+    L.lcpao_grid_poisson_dev.argtypes = [vp, C.c_int32, dpp, dp, dp, dp]
     L.lcpao_grid_set_stream.argtypes = [vp, vp]
     L.lcpao_grid_profile.argtypes = [vp, C.c_int32]
     L.lcpao_grid_kernel_stats.argtypes = [vp, C.c_int32, lp, dp]

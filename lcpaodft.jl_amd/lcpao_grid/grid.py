@@ -149,6 +149,78 @@ class GridContext:
             arr[s].rwf = C.cast(rw, _lib.dpp)
         self._check(self.L.lcpao_grid_build_tables(self.h, len(paos), arr, int(bool(with_derivatives))))
 
+    # grid fields next to the contractions ------------------------------------
+    @staticmethod
+    def _radial(paos, vpss):
+        """lcpao_atomic_radial per species (PAO with Spe_Atomic_Den, VPS)."""
+        arr = (_lib.AtomicRadial * len(paos))()
+        keep = []
+        for s, (p, v) in enumerate(zip(paos, vpss)):
+            fields = [np.ascontiguousarray(x, np.float64) for x in
+                      (p.Spe_PAO_XV, p.Spe_PAO_RV, p.Spe_Atomic_Den, v.Spe_VPS_XV, v.Spe_VPS_RV, v.Spe_Atomic_PCC)]
+            if fields[2].size != fields[0].size + 4 or fields[5].size != fields[3].size + 4:
+                raise ValueError("atomic densities need nmesh + 4 values")
+            keep += fields
+            arr[s].nmesh_pao, arr[s].nmesh_vps = fields[0].size, fields[3].size
+            arr[s].pao_xv, arr[s].pao_rv, arr[s].atomic_den = (f.ctypes.data_as(_lib.dp) for f in fields[:3])
+            arr[s].vps_xv, arr[s].vps_rv, arr[s].atomic_pcc = (f.ctypes.data_as(_lib.dp) for f in fields[3:])
+        return arr, keep
+
+    @staticmethod
+    def _spins(natom, init_nspin, init_angle):
+        ns = np.ascontiguousarray(np.asarray(init_nspin, np.float64)[:, :2]).reshape(natom, 2)
+        an = None if init_angle is None else np.ascontiguousarray(np.asarray(init_angle, np.float64)[:, :2]).reshape(natom, 2)
+        return ns, an
+
+    def aden_pcc(self, nspin, paos, vpss, init_nspin, init_angle=None):
+        """(ADensity, PCCDensity, [Density per spin]) on the grid, as
+        Set_AdenPCC_Grid returns them; paos carry Spe_Atomic_Den (read with
+        the core charge), vpss the partial-core density."""
+        arr, keep = self._radial(paos, vpss)
+        ns, an = self._spins(self.natom, init_nspin, init_angle)
+        A, P = np.zeros(self.nn), np.zeros(self.nn)
+        R = [np.zeros(self.nn) for _ in range(nspin)]
+        self._check(self.L.lcpao_grid_aden_pcc(self.h, nspin, len(paos), arr, ns.ctypes.data_as(_lib.dp),
+                                               None if an is None else an.ctypes.data_as(_lib.dp),
+                                               A.ctypes.data_as(_lib.dp), P.ctypes.data_as(_lib.dp), _ptrs(R)))
+        return A, P, R
+
+    def aden_pcc_dev(self, nspin, paos, vpss, init_nspin, dA, dP, dR, init_angle=None):
+        arr, keep = self._radial(paos, vpss)
+        ns, an = self._spins(self.natom, init_nspin, init_angle)
+        self._order()
+        self._check(self.L.lcpao_grid_aden_pcc_dev(self.h, nspin, len(paos), arr, ns.ctypes.data_as(_lib.dp),
+                                                   None if an is None else an.ctypes.data_as(_lib.dp),
+                                                   C.cast(dA.data_ptr(), _lib.dp), C.cast(dP.data_ptr(), _lib.dp),
+                                                   self._dptrs(dR)))
+
+    @staticmethod
+    def _recvecs(recvecs):
+        """Recvecs (rows = reciprocal vectors) in Julia's column-major order."""
+        r = np.asarray(recvecs, np.float64).reshape(3, 3)
+        return np.ascontiguousarray(r.T).reshape(-1)
+
+    def poisson(self, rho, aden, recvecs, nspin=None):
+        """dVHart of Solve_Poisson! from the spin densities rho (list) and
+        the atomic density; nspin 0 (rho = [Density_xyz], aden None) runs the
+        inner solver on rho[0] as given."""
+        nspin = len(rho) if nspin is None else nspin
+        R = [np.ascontiguousarray(r, np.float64) for r in rho]
+        a = None if aden is None else np.ascontiguousarray(aden, np.float64)
+        rec = self._recvecs(recvecs)
+        V = np.zeros(self.nn)
+        self._check(self.L.lcpao_grid_poisson(self.h, nspin, _ptrs(R), None if a is None else a.ctypes.data_as(_lib.dp),
+                                              rec.ctypes.data_as(_lib.dp), V.ctypes.data_as(_lib.dp)))
+        return V
+
+    def poisson_dev(self, drho, daden, recvecs, dV, nspin=None):
+        nspin = len(drho) if nspin is None else nspin
+        rec = self._recvecs(recvecs)
+        self._order()
+        self._check(self.L.lcpao_grid_poisson_dev(self.h, nspin, self._dptrs(drho),
+                                                  None if daden is None else C.cast(daden.data_ptr(), _lib.dp),
+                                                  rec.ctypes.data_as(_lib.dp), C.cast(dV.data_ptr(), _lib.dp)))
+
     def set_positions(self, gxyz):
         """Atom positions (natom x 3, bohr) for the next build_tables; lists kept."""
         g = np.ascontiguousarray(gxyz, np.float64).reshape(-1, 3)
@@ -307,3 +379,17 @@ def Force3_spin(pao, Orbs_Grid, Vpot_Grid, DM, ucell, precision="fp64"):
 def Force3_nc(pao, Orbs_Grid, dVHart_Grid, Vxc_Grid, DM, ucell, precision="fp64"):
     V = [dVHart_Grid + Vxc_Grid[0], dVHart_Grid + Vxc_Grid[1], Vxc_Grid[2], Vxc_Grid[3]]
     return Orbs_Grid.force3(V, list(DM[:4]), WEIGHTS_FORCE3[4], ucell.MPHks_rank, precision)
+
+
+def Set_AdenPCC_Grid(SpinPol, Init_Atoms_Nspin, Init_Atoms_Angle, pao, pspot, ucell, Orbs_Grid):
+    """Set_AdenPCC_Grid (Set_AdenPCC_Grid.jl:1-142) on the device of the
+    context Orbs_Grid: returns (ADensity_Grid, PCCDensity_Grid, Density_Grid)."""
+    nspin = {"off": 1, "on": 2, "nc": 4}[SpinPol]
+    angle = None if Init_Atoms_Angle is None else [list(x)[:2] for x in Init_Atoms_Angle]
+    return Orbs_Grid.aden_pcc(nspin, pao, pspot, [list(x)[:2] for x in Init_Atoms_Nspin], angle)
+
+
+def Solve_Poisson(SpinPol, Recvecs, Density_Grid, ADensity_Grid, dVHart_Grid, Orbs_Grid):
+    """Solve_Poisson! (Poisson.jl:1-7): dVHart_Grid is overwritten."""
+    nspin = {"off": 1, "on": 2, "nc": 4}[SpinPol]
+    dVHart_Grid[:] = Orbs_Grid.poisson(list(Density_Grid[:min(nspin, 2)]), ADensity_Grid, Recvecs, nspin)

@@ -29,7 +29,7 @@ def get_ialpha_index(orbitals):
 
 
 class PAO:
-    def __init__(self, symbol, cutoff, orbitals, pao_dir=DATA_DIR, extra=""):
+    def __init__(self, symbol, cutoff, orbitals, pao_dir=DATA_DIR, extra="", core_charge=None):
         self.Atom_symbol = symbol
         self.Atom_extra = extra
         self.Atom_orbital = orbitals
@@ -60,13 +60,21 @@ class PAO:
         self.Spe_Num_Mesh_PAO, self.Spe_Atom_Cut1, self.Spe_PAO_Lmax, self.Spe_PAO_Mul = mesh, cut, lmax, mul
         self.Spe_PAO_XV = np.zeros(mesh)
         self.Spe_PAO_RV = np.zeros(mesh)
+        # atomic density at index k + 1 for mesh point k; two extrapolated
+        # end points and two zeros of padding (Read_PAO.jl:183-203, 288-303)
+        self.Spe_Atomic_Den = np.zeros(mesh + 4)
         i = self._find(lines, "<valence.charge.density")
         for k in range(mesh):
             tok = lines[i + 1 + k].split()
             self.Spe_PAO_XV[k] = float(tok[0])
             self.Spe_PAO_RV[k] = float(tok[1])
+            self.Spe_Atomic_Den[k + 1] = float(tok[2])
         if "valence.charge.density>" not in lines[i + 1 + mesh]:
             raise ValueError("please check valence_charge_density")
+        if self.Spe_Atomic_Den[1] < 1e-10:
+            raise ValueError("please check Spe_Atomic_Den")
+        if core_charge is not None:
+            self._renormalise_density(float(core_charge))
         self.Spe_PAO_RWF = []
         for L in range(lmax + 1):
             i = self._find(lines, "<pseudo.atomic.orbitals.L=%d" % L)
@@ -74,6 +82,21 @@ class PAO:
             if "pseudo.atomic.orbitals.L=%d>" % L not in lines[i + 1 + mesh]:
                 raise ValueError("please check pseudo_atomic_orbitals")
             self.Spe_PAO_RWF.append([np.ascontiguousarray(block[:, m]) for m in range(mul)])
+
+    def _renormalise_density(self, core_charge):
+        """Scale the atomic density to hold core_charge electrons
+        (4 pi dx sum rho r^3 on the log mesh), then extrapolate the two end
+        points linearly (Read_PAO.jl:288-303)."""
+        den, xv, n = self.Spe_Atomic_Den, self.Spe_PAO_XV, self.Spe_Num_Mesh_PAO
+        total = 0.0
+        dx = xv[1] - xv[0]
+        for k in range(n):
+            total += den[k + 1] * np.exp(3 * xv[k])
+        total *= 4 * np.pi * dx
+        for k in range(n):
+            den[k + 1] = den[k + 1] * core_charge / total
+        den[0] = 2 * den[1] - den[2]
+        den[n + 1] = 2 * den[n] - den[n - 1]
 
     @staticmethod
     def _find(lines, mark):
@@ -85,5 +108,5 @@ class PAO:
 
 def Read_PAO(Spe_Core_Charge, Atom_symbol, Atom_cutoff, Atom_orb, Atom_extra="", pao_dir=DATA_DIR):
     """Same argument order as Read_PAO (Read_PAO.jl:248-254); the core charge
-    only renormalises the atomic density, which the grid path does not use."""
-    return PAO(Atom_symbol, float(Atom_cutoff), Atom_orb, pao_dir, Atom_extra)
+    renormalises the atomic density (Spe_Atomic_Den)."""
+    return PAO(Atom_symbol, float(Atom_cutoff), Atom_orb, pao_dir, Atom_extra, core_charge=Spe_Core_Charge)

@@ -88,6 +88,15 @@ class Structure:
             self._paos = [PAO(*BASES[s]) for s in self.species_names]
         return self._paos
 
+    def atomic(self, xc="LDA"):
+        """Per species (PAO with Spe_Atomic_Den renormalised to the core
+        charge of the VPS, VPS) for the atomic densities on the grid."""
+        from .pao import Read_PAO
+        from .vps import Read_VPS
+        vps = [Read_VPS(s, "", xc) for s in self.species_names]
+        pao = [Read_PAO(v.Spe_Core_Charge, *BASES[s]) for s, v in zip(self.species_names, vps)]
+        return pao, vps
+
     def cutoffs(self):
         return np.array([BASES[s][1] for s in self.species], dtype=np.float64)
 

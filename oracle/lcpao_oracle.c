@@ -1497,3 +1497,234 @@ int orc_cpu_baseline(const orc_geom *G, const double *const *tab, const double *
     free(cum); free(J); free(th);
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* Atomic densities on the grid: Set_AdenPCC_Grid.jl:1-142            */
+/* ------------------------------------------------------------------ */
+
+/* KumoF, utils/KumoF.jl:1-62.  xv, rv: the log mesh (N points, 0-based);
+ * yv: N + 4 values with mesh point k at yv[k + 1] (Julia's yv[k + 2]). */
+double orc_kumof(int N, double x, const double *xv, const double *rv, const double *yv) {
+    if (x < xv[0]) {
+        /* KumoF.jl:3-40, m = 5 (1-based): rv[m] -> rv[4], yv[m] -> yv[4] */
+        double r = exp(x);
+        double rm = rv[4];
+        double h1 = rv[3] - rv[2];
+        double h2 = rv[4] - rv[3];
+        double h3 = rv[5] - rv[4];
+        double f1 = yv[2], f2 = yv[3], f3 = yv[4], f4 = yv[5];
+        double g1 = ((f3 - f2) * h1 / h2 + (f2 - f1) * h2 / h1) / (h1 + h2);
+        double g2 = ((f4 - f3) * h2 / h3 + (f3 - f2) * h3 / h2) / (h2 + h3);
+        double x1 = rm - rv[3];
+        double x2 = rm - rv[4];
+        double y1 = x1 / h2;
+        double y2 = x2 / h2;
+        double y12 = y1 * y1;
+        double y22 = y2 * y2;
+        double A = 2 * f2 + h2 * g1;
+        double B = 2 * f3 - h2 * g2;
+        double C = 3 * f2 + h2 * g1;
+        double D = 3 * f3 - h2 * g2;
+        double E = C + A * y2;
+        double F = D - B * y1;
+        double f = y22 * E + y12 * F;
+        double df = (2 * y2 / h2) * E + y22 * A / h2 + (2 * y1 / h2) * F - y12 * B / h2;
+        double a = 0.5 * df / rm;
+        double b = f - a * rm * rm;
+        return a * r * r + b;
+    }
+    /* KumoF.jl:41-60: 1-based i = floor(tmp) + 1 reads yv[i .. i+3] */
+    double xmin = xv[0], xmax = xv[N - 1];
+    if (x > xmax) x = xmax;
+    if (x < xmin) x = xmin;
+    double tmp = (double)(N - 1) * (x - xmin) / (xmax - xmin);
+    int i = (int)floor(tmp);
+    double dt = tmp - i;
+    double d0 = yv[i + 1] - yv[i];
+    double d1 = yv[i + 2] - yv[i + 1];
+    double d2 = yv[i + 3] - yv[i + 2];
+    double a = d0 - 2 * d1 + d2;
+    double b = -2 * d0 + 3 * d1 - d2;
+    double c = d0 + d1;
+    return yv[i + 1] + 0.5 * (((a * dt + b) * dt + c) * dt);
+}
+
+/* Set_AdenPCC_Grid (Set_AdenPCC_Grid.jl:1-142) with one rank: the atom loop
+ * (:56-115) over every atom's grid list, then the nc rotation (:125-138).
+ * Per species s: npao[s], pxv/prv[s] (npao), pden[s] (npao + 4); nvps[s],
+ * vxv/vrv[s] (nvps), vpcc[s] (nvps + 4).  init_nspin: natom x 2;
+ * init_angle: natom x 2 (theta, phi) or NULL.  Outputs NN each. */
+int orc_aden_pcc(const orc_geom *G, int nspin, const int *atom2spe, const int *npao, const double *const *pxv,
+                 const double *const *prv, const double *const *pden, const int *nvps, const double *const *vxv,
+                 const double *const *vrv, const double *const *vpcc, const double *init_nspin,
+                 const double *init_angle, double *aden, double *pcc, double *const *rho) {
+    if (nspin != 1 && nspin != 2 && nspin != 4) return fail("nspin must be 1, 2 or 4");
+    const int N1 = G->ngrid[0], N2 = G->ngrid[1], N3 = G->ngrid[2];
+    const int64_t NN = (int64_t)N1 * N2 * N3;
+    double glat[9];
+    for (int j = 0; j < 3; j++) { glat[j] = G->lat[j] / N1; glat[3 + j] = G->lat[3 + j] / N2; glat[6 + j] = G->lat[6 + j] / N3; }
+    memset(aden, 0, sizeof(double) * NN);
+    memset(pcc, 0, sizeof(double) * NN);
+    for (int s = 0; s < nspin; s++) memset(rho[s], 0, sizeof(double) * NN);
+    for (int atom = 0; atom < G->natom; atom++) {
+        const int spe = atom2spe[atom];
+        const double Nu = init_nspin[2 * atom], Nd = init_nspin[2 * atom + 1];
+        const double Nele = Nu + Nd;
+        const double ocupcy_u = Nele > 1e-15 ? Nu / Nele : 0.0;
+        const double ocupcy_p = Nele > 1e-15 ? Nd / Nele : 0.0;
+        const double theta = init_angle ? init_angle[2 * atom] : 0.0;
+        const double phi = init_angle ? init_angle[2 * atom + 1] : 0.0;
+        for (int xyz = 0; xyz < G->gridn[atom]; xyz++) {
+            int GNc = G->glist[G->gl_off[atom] + xyz];
+            int GRc = G->clist[G->gl_off[atom] + xyz];
+            int n1 = GNc / (N2 * N3);
+            int n2 = (GNc - n1 * N2 * N3) / N3;
+            int n3 = GNc - n1 * N2 * N3 - n2 * N3;
+            double Cx = n1 * glat[0] + n2 * glat[3] + n3 * glat[6] + G->origin[0];
+            double Cy = n1 * glat[1] + n2 * glat[4] + n3 * glat[7] + G->origin[1];
+            double Cz = n1 * glat[2] + n2 * glat[5] + n3 * glat[8] + G->origin[2];
+            double dx = Cx + G->atv[3 * GRc] - G->gxyz[3 * atom];
+            double dy = Cy + G->atv[3 * GRc + 1] - G->gxyz[3 * atom + 1];
+            double dz = Cz + G->atv[3 * GRc + 2] - G->gxyz[3 * atom + 2];
+            double x = 0.5 * log(dx * dx + dy * dy + dz * dz);
+            double ADen = orc_kumof(npao[spe], x, pxv[spe], prv[spe], pden[spe]);
+            double PCCDen = orc_kumof(nvps[spe], x, vxv[spe], vrv[spe], vpcc[spe]);
+            aden[GNc] += 0.5 * ADen;
+            pcc[GNc] += 0.5 * PCCDen;
+            if (nspin == 1) {
+                rho[0][GNc] += 0.5 * ADen;
+            } else if (nspin == 2) {
+                rho[0][GNc] += ocupcy_u * ADen;
+                rho[1][GNc] += ocupcy_p * ADen;
+            } else {
+                double mag = (ocupcy_u - ocupcy_p) * ADen;
+                rho[0][GNc] += ADen;
+                rho[1][GNc] += mag * sin(theta) * cos(phi);
+                rho[2][GNc] += mag * sin(theta) * sin(phi);
+                rho[3][GNc] += mag * cos(theta);
+            }
+        }
+    }
+    if (nspin == 4)
+        for (int64_t i = 0; i < NN; i++) {
+            double r = rho[0][i], mx = rho[1][i], my = rho[2][i], mz = rho[3][i];
+            rho[0][i] = 0.5 * (r + mz);
+            rho[1][i] = 0.5 * (r - mz);
+            rho[2][i] = 0.5 * mx;
+            rho[3][i] = -0.5 * my;
+        }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Poisson: Poisson.jl:1-108                                          */
+/* ------------------------------------------------------------------ */
+
+/* 1-D DFT of n complex values in place, sign -1 forward (FFTW's fft),
+ * +1 backward scaled by 1/n (FFTW's normalised ifft): recursive mixed-radix
+ * Cooley-Tukey on the smallest prime factor, direct sums for a prime n. */
+static void dft_rec(int n, int sign, const double *in, int stride, double *out, double *tmp) {
+    if (n == 1) { out[0] = in[0]; out[1] = in[1]; return; }
+    int p = 2;
+    while (p * p <= n && n % p) p++;
+    if (n % p) p = n;
+    const int m = n / p;
+    if (p == n) {
+        for (int k = 0; k < n; k++) {
+            double sr = 0.0, si = 0.0;
+            for (int j = 0; j < n; j++) {
+                double ang = sign * 2.0 * ORC_PI * (double)((int64_t)j * k % n) / n;
+                double c = cos(ang), s = sin(ang);
+                double xr = in[2 * (int64_t)j * stride], xi = in[2 * (int64_t)j * stride + 1];
+                sr += xr * c - xi * s;
+                si += xr * s + xi * c;
+            }
+            out[2 * k] = sr;
+            out[2 * k + 1] = si;
+        }
+        return;
+    }
+    /* p sub-transforms of length m over the residues r: x[r + p j] */
+    for (int r = 0; r < p; r++) dft_rec(m, sign, in + 2 * (int64_t)r * stride, stride * p, tmp + 2 * (int64_t)r * m, out);
+    for (int k = 0; k < m; k++)
+        for (int q = 0; q < p; q++) {
+            const int kk = k + m * q;
+            double sr = 0.0, si = 0.0;
+            for (int r = 0; r < p; r++) {
+                double ang = sign * 2.0 * ORC_PI * (double)((int64_t)r * kk % n) / n;
+                double c = cos(ang), s = sin(ang);
+                double yr = tmp[2 * ((int64_t)r * m + k)], yi = tmp[2 * ((int64_t)r * m + k) + 1];
+                sr += yr * c - yi * s;
+                si += yr * s + yi * c;
+            }
+            out[2 * kk] = sr;
+            out[2 * kk + 1] = si;
+        }
+}
+
+static void dft_line(int n, int sign, double *line, double *work) {
+    /* work: 6 n doubles (result, scratch of the recursion, copy of the input) */
+    double *res = work, *scr = work + 2 * n, *cp = work + 4 * n;
+    memcpy(cp, line, sizeof(double) * 2 * n);
+    dft_rec(n, sign, cp, 1, res, scr);
+    for (int k = 0; k < 2 * n; k++) line[k] = sign > 0 ? res[k] / n : res[k];
+}
+
+/* transform along one axis of the complex N1 x N2 x N3 array (n3 fastest) */
+static void dft_axis(double *z, const int *N, int axis, int sign) {
+    const int n = N[axis];
+    double *line = malloc(sizeof(double) * 2 * n), *work = malloc(sizeof(double) * 6 * n);
+    const int64_t s3 = 1, s2 = N[2], s1 = (int64_t)N[1] * N[2];
+    const int64_t st = axis == 0 ? s1 : (axis == 1 ? s2 : s3);
+    const int o1 = axis == 0 ? 1 : N[0], o2 = axis == 1 ? 1 : N[1], o3 = axis == 2 ? 1 : N[2];
+    for (int a = 0; a < o1; a++)
+        for (int b = 0; b < o2; b++)
+            for (int c = 0; c < o3; c++) {
+                const int64_t base = a * s1 + b * s2 + c * s3;
+                for (int j = 0; j < n; j++) { line[2 * j] = z[2 * (base + j * st)]; line[2 * j + 1] = z[2 * (base + j * st) + 1]; }
+                dft_line(n, sign, line, work);
+                for (int j = 0; j < n; j++) { z[2 * (base + j * st)] = line[2 * j]; z[2 * (base + j * st) + 1] = line[2 * j + 1]; }
+            }
+    free(line);
+    free(work);
+}
+
+/* Solve_Poisson!(Ngrid, Recvecs, Density_xyz, dVHart_Grid), Poisson.jl:29-108:
+ * forward 1-D transforms along axes 3, 2, 1; times 4 pi / |G|^2 with the
+ * signed indices of :65-80; G = 0 set to zero; normalised inverse transforms
+ * along axes 1, 2, 3; the real part.  rec: Recvecs column-major
+ * (rec[3 j + i] = Recvecs[i+1, j+1]). */
+int orc_poisson(const int *ngrid, const double *rec, const double *density_xyz, double *vhart) {
+    const int N1 = ngrid[0], N2 = ngrid[1], N3 = ngrid[2];
+    const int64_t NN = (int64_t)N1 * N2 * N3;
+    double *z = malloc(sizeof(double) * 2 * NN);
+    if (!z) return fail("oom");
+    for (int64_t g = 0; g < NN; g++) { z[2 * g] = density_xyz[g]; z[2 * g + 1] = 0.0; }
+    dft_axis(z, ngrid, 2, -1);
+    dft_axis(z, ngrid, 1, -1);
+    dft_axis(z, ngrid, 0, -1);
+    const double temp = 4 * ORC_PI;
+    const int Ngrid11 = N1 / 2, Ngrid22 = N2 / 2, Ngrid33 = N3 / 2;
+    for (int i = 0; i < N1; i++)
+        for (int j = 0; j < N2; j++)
+            for (int k = 0; k < N3; k++) {
+                int sk1 = i < Ngrid11 ? i : i - N1;
+                int sk2 = j < Ngrid22 ? j : j - N2;
+                int sk3 = k < Ngrid33 ? k : k - N3;
+                double Gx = sk1 * rec[0] + sk2 * rec[1] + sk3 * rec[2];
+                double Gy = sk1 * rec[3] + sk2 * rec[4] + sk3 * rec[5];
+                double Gz = sk1 * rec[6] + sk2 * rec[7] + sk3 * rec[8];
+                double g2 = Gx * Gx + Gy * Gy + Gz * Gz;
+                int64_t g = ((int64_t)i * N2 + j) * N3 + k;
+                z[2 * g] = temp * z[2 * g] / g2;
+                z[2 * g + 1] = temp * z[2 * g + 1] / g2;
+            }
+    z[0] = 0.0;
+    z[1] = 0.0;
+    dft_axis(z, ngrid, 0, 1);
+    dft_axis(z, ngrid, 1, 1);
+    dft_axis(z, ngrid, 2, 1);
+    for (int64_t g = 0; g < NN; g++) vhart[g] = z[2 * g];
+    free(z);
+    return 0;
+}

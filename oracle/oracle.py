@@ -263,3 +263,45 @@ def cpu_baseline(geom, tabs, V, DM, nworkers=1, use_blas=True, pair_limit=0):
                                   H.ctypes.data_as(_dp), rho.ctypes.data_as(_dp), nworkers, int(bool(use_blas)),
                                   C.c_int64(pair_limit), C.byref(secs), C.byref(done)))
     return secs.value, done.value, H, rho
+
+
+def kumof(N, x, xv, rv, yv):
+    """KumoF (utils/KumoF.jl:1-62); yv holds mesh point k at index k + 1."""
+    L = lib()
+    L.orc_kumof.restype = C.c_double
+    L.orc_kumof.argtypes = [C.c_int, C.c_double, _dp, _dp, _dp]
+    xv, rv, yv = (np.ascontiguousarray(a, dtype=np.float64) for a in (xv, rv, yv))
+    return L.orc_kumof(int(N), float(x), xv.ctypes.data_as(_dp), rv.ctypes.data_as(_dp), yv.ctypes.data_as(_dp))
+
+
+def aden_pcc(geom, nspin, atom2spe, radial, init_nspin, init_angle=None):
+    """Set_AdenPCC_Grid (Set_AdenPCC_Grid.jl:1-142), one rank.  radial: per
+    species (pao_xv, pao_rv, atomic_den, vps_xv, vps_rv, atomic_pcc).
+    Returns (ADensity, PCCDensity, [rho per spin])."""
+    ns = len(radial)
+    arrs = [[np.ascontiguousarray(r[k], dtype=np.float64) for r in radial] for k in range(6)]
+    npao = np.array([a.size for a in arrs[0]], dtype=np.int32)
+    nvps = np.array([a.size for a in arrs[3]], dtype=np.int32)
+    a2s = np.ascontiguousarray(atom2spe, dtype=np.int32)
+    ins = np.ascontiguousarray(np.asarray(init_nspin, dtype=np.float64)[:, :2])
+    ang = None if init_angle is None else np.ascontiguousarray(np.asarray(init_angle, dtype=np.float64)[:, :2])
+    A, P = np.zeros(geom.nn), np.zeros(geom.nn)
+    R = [np.zeros(geom.nn) for _ in range(nspin)]
+    _check(lib().orc_aden_pcc(geom.h, nspin, a2s.ctypes.data_as(_ip), npao.ctypes.data_as(_ip), _dptrs(arrs[0]),
+                              _dptrs(arrs[1]), _dptrs(arrs[2]), nvps.ctypes.data_as(_ip), _dptrs(arrs[3]),
+                              _dptrs(arrs[4]), _dptrs(arrs[5]), ins.ctypes.data_as(_dp),
+                              None if ang is None else ang.ctypes.data_as(_dp), A.ctypes.data_as(_dp),
+                              P.ctypes.data_as(_dp), _dptrs(R)))
+    return A, P, R
+
+
+def poisson(ngrid, recvecs, density_xyz):
+    """Solve_Poisson!(Ngrid, Recvecs, Density_xyz, dVHart_Grid) (Poisson.jl:29-108);
+    recvecs rows are the reciprocal vectors (Recvecs[i, :])."""
+    ng = np.ascontiguousarray(ngrid, dtype=np.int32)
+    rec = np.ascontiguousarray(np.asarray(recvecs, dtype=np.float64).reshape(3, 3).T).reshape(-1)
+    d = np.ascontiguousarray(density_xyz, dtype=np.float64)
+    V = np.zeros(d.size)
+    _check(lib().orc_poisson(ng.ctypes.data_as(_ip), rec.ctypes.data_as(_dp), d.ctypes.data_as(_dp),
+                             V.ctypes.data_as(_dp)))
+    return V

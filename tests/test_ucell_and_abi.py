@@ -103,3 +103,16 @@ def test_tune_rejects_null_context():
     assert L.lcpao_grid_tune(None, 0, 0) == _lib.LCPAO_EINVAL
     v = ctypes.c_int32()
     assert L.lcpao_grid_tuned(None, 0, ctypes.byref(v), None, 0) == _lib.LCPAO_EINVAL
+
+
+def test_loader_keeps_one_hip_runtime_in_the_process():
+    """Loading the library maps exactly one HIP and one HSA runtime (torch's,
+    when torch is installed), whatever is imported afterwards."""
+    import subprocess
+    import sys
+    code = ("import re, sys; sys.path.insert(0, %r); from lcpao_grid import _lib; _lib.load(); import torch; "
+            "m = {l.split()[-1] for l in open('/proc/self/maps') if re.search('libamdhip64|libhsa-runtime', l)}; "
+            "print(len(m))") % os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lcpaodft.jl_amd")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.strip().splitlines()[-1] == "2"
