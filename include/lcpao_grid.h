@@ -170,7 +170,9 @@ int lcpao_grid_set_stream(lcpao_ctx *ctx, void *hip_stream);
  * enable != 0 starts recording (and clears the counters); kernel_stats
  * returns the launch count and total milliseconds of one kernel id:
  * 0 H units, 1 H reduce, 2 density fold, 3 density units, 4 density gather,
- * 5 Force3 units, 6 Force3 reduce. */
+ * 5 Force3 units, 6 Force3 reduce, 7 atomic densities, 8 FFT passes,
+ * 9 Poisson grid kernels (charge, 4 pi / G^2 factor, potential),
+ * 10 Bloch sums, 11 density matrix from states. */
 int lcpao_grid_profile(lcpao_ctx *ctx, int32_t enable);
 int lcpao_grid_kernel_stats(const lcpao_ctx *ctx, int32_t kernel_id, int64_t *launches, double *total_ms);
 
@@ -236,6 +238,38 @@ int lcpao_grid_poisson(lcpao_ctx *ctx, int32_t nspin, const double *const *rho, 
                        const double *recvecs, double *vhart);
 int lcpao_grid_poisson_dev(lcpao_ctx *ctx, int32_t nspin, const double *const *d_rho, const double *d_aden,
                            const double *recvecs, double *d_vhart);
+
+/* ---- either side of the k-point diagonalisation -------------------------
+ * Dense matrices are fsize x fsize complex (fsize = sum of Total_NumOrbs),
+ * column-major with interleaved real / imaginary parts, as Julia's
+ * Matrix{ComplexF64}; row MP[A] + i is orbital i of atom A.  Both routines
+ * cover the pairs held by the context (all pairs when it was built for one
+ * rank); flat arrays are the context's my_hsize pair blocks.
+ *
+ * HS_matrix! with a k-point (HS_matrix.jl:88-110, 143-166):
+ * dense[m][A_i, B_j] = sum over the pairs (A, B, cell) of
+ * flat[m][pair][i, j] * cispi(2 k . atv_ijk[cell]); kpt = (ka, kb, kc) in
+ * the reciprocal basis; nmat 1..4 matrices (e.g. S and H) in one pass.
+ * Each element's images are summed in the pair-list order. */
+int lcpao_grid_bloch_sum(lcpao_ctx *ctx, const double *kpt, int32_t nmat, const double *const *flat,
+                         double *const *dense);
+int lcpao_grid_bloch_sum_dev(lcpao_ctx *ctx, const double *kpt, int32_t nmat, const double *const *d_flat,
+                             double *const *d_dense);
+/* fsize and the number of distinct (A, B) atom pairs of the context. */
+int lcpao_grid_bloch_sizes(lcpao_ctx *ctx, int64_t *fsize, int64_t *ngroups);
+
+/* Calc_DM_Crystal_Collinear_nopol! (Crystal_DFT.jl:219-270) from the states:
+ * DM[pair][i, j] = sum over the nk k-points (in order) of
+ * Re(DM_k[B_j, A_i] * cispi(2 k . atv_ijk[cell]) * scale), with
+ * DM_k = C[k] C[k]^H formed only for the (A, B) blocks the pairs need
+ * (no fsize x fsize matrix).  C[k]: fsize x nstates complex, column-major,
+ * columns scaled by sqrt(occupation * k weight) as Calc_fnkCnk! leaves
+ * them; kpts: nk x 3; scale: 1 / AllNkpt.  DM (my_hsize) is overwritten;
+ * the reference's MPI.Allreduce! over k-point ranks stays the caller's. */
+int lcpao_grid_dm_states(lcpao_ctx *ctx, int32_t nk, const double *kpts, int64_t nstates, const double *const *C,
+                         double scale, double *DM);
+int lcpao_grid_dm_states_dev(lcpao_ctx *ctx, int32_t nk, const double *kpts, int64_t nstates,
+                             const double *const *d_C, double scale, double *d_DM);
 
 /* Sizes of the context (rank-local H size, grid size, sums of the lists). */
 int lcpao_grid_sizes(const lcpao_ctx *ctx, int64_t *my_hsize, int64_t *nn, int64_t *total_olg,

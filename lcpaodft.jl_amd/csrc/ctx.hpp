@@ -39,9 +39,19 @@ struct BlockPlan;
 struct FieldPlan;
 void field_free(FieldPlan *f);
 
+// Atom-pair groups of the Bloch sums (bloch.hip), built on first use.
+struct BlochPlan;
+void bloch_free(BlochPlan *b);
+
 }  // namespace lcpao
 
 using namespace lcpao;
+
+// kernel ids of the per-kernel event timing (lcpao_grid_profile)
+enum {
+    KID_HAM = 0, KID_HRED = 1, KID_DEFF = 2, KID_RHO = 3, KID_RGATHER = 4, KID_F3 = 5, KID_F3RED = 6,
+    KID_ADEN = 7, KID_FFT = 8, KID_PGRID = 9, KID_BLOCH = 10, KID_DMK = 11, KID_COUNT = 12
+};
 
 struct lcpao_ctx {
     int device = 0;
@@ -97,6 +107,7 @@ struct lcpao_ctx {
     lcpao::BlockPlan *plan = nullptr;
     // grid-field routines (atomic densities, Poisson)
     lcpao::FieldPlan *fields = nullptr;
+    lcpao::BlochPlan *bloch = nullptr;
     float *d_tab32 = nullptr, *d_dtab32[3] = {nullptr, nullptr, nullptr};
     float *d_deff32 = nullptr;
     std::vector<int32_t> h_glist, h_clist;  // host copies for the block plan
@@ -111,15 +122,15 @@ struct lcpao_ctx {
     // first call with this geometry), see lcpao_grid_tune
     int32_t variant[4] = {-1, -1, -1, -1};
     float variant_ms[4][4] = {};
-    int64_t prof_launches[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    double prof_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t prof_launches[KID_COUNT] = {};
+    double prof_ms[KID_COUNT] = {};
 };
 
 // tunable operations and their candidate counts
 enum { TUNE_RHO64 = 0, TUNE_F364 = 1, TUNE_NOPS = 2 };
 constexpr int TUNE_NCAND[TUNE_NOPS] = {2, 2};
 
-enum { KID_HAM = 0, KID_HRED = 1, KID_DEFF = 2, KID_RHO = 3, KID_RGATHER = 4, KID_F3 = 5, KID_F3RED = 6 };
+
 
 inline hipEvent_t take_event(lcpao_ctx *ctx) {
     if (!ctx->event_pool.empty()) {
@@ -180,4 +191,5 @@ namespace lcpao {
 int ctx_enter(lcpao_ctx *ctx);                          // null check + hipSetDevice
 int ctx_error(lcpao_ctx *ctx, int code, const std::string &msg);
 std::vector<double> cell_vectors(const lcpao_ctx *ctx);  // atv[cell][3], Generation_ATV order
+void cell_triplet(const lcpao_ctx *ctx, int cell, int *ijk);  // atv_ijk[cell]
 }  // namespace lcpao

@@ -71,8 +71,23 @@ __device__ double kumo(int N, double x, const double *__restrict__ xv, const dou
 // grid-point index of the atom lists (entries ascending = atoms ascending,
 // then list order), so each sum is formed in the reference's order; the
 // non-collinear rotation (:125-138) is applied at the end.
-__global__ void k_aden_pcc(int64_t nn, const int64_t *__restrict__ ent_off, const int64_t *__restrict__ ent,
-                           const int64_t *__restrict__ gl_off, const int32_t *__restrict__ clist,
+// (atom, cell) of every entry of the grid-point index, in its order: built
+// once per context so that the atomic-density kernel reads one 8-byte record
+// per list point instead of searching the owning atom.
+__global__ void k_entry_info(int64_t nent, const int64_t *__restrict__ ent, const int64_t *__restrict__ gl_off,
+                             int natom, const int32_t *__restrict__ clist, int2 *__restrict__ info) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nent) return;
+    const int64_t e = ent[k];
+    int lo = 0, hi = natom;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (gl_off[mid] <= e) lo = mid; else hi = mid;
+    }
+    info[k] = make_int2(lo, clist[e]);
+}
+
+__global__ void k_aden_pcc(int64_t nn, const int64_t *__restrict__ ent_off, const int2 *__restrict__ info,
                            const double *__restrict__ gxyz, const int32_t *__restrict__ a2s,
                            const double *__restrict__ atv, FieldGeom fg, const RadialDev *__restrict__ spe,
                            const AtomSpinDev *__restrict__ spin, double *__restrict__ aden, double *__restrict__ pcc,
@@ -88,14 +103,8 @@ __global__ void k_aden_pcc(int64_t nn, const int64_t *__restrict__ ent_off, cons
     const double Cz = n1 * fg.glat[2] + n2 * fg.glat[5] + n3 * fg.glat[8] + fg.origin[2];
     double sa = 0.0, sp = 0.0, s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     for (int64_t k = ent_off[g]; k < ent_off[g + 1]; k++) {
-        const int64_t e = ent[k];
-        int lo = 0, hi = fg.natom;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (gl_off[mid] <= e) lo = mid; else hi = mid;
-        }
-        const int a = lo;
-        const int c = clist[e];
+        const int2 ac = info[k];
+        const int a = ac.x, c = ac.y;
         const double dx = Cx + atv[3 * c] - gxyz[3 * a];
         const double dy = Cy + atv[3 * c + 1] - gxyz[3 * a + 1];
         const double dz = Cz + atv[3 * c + 2] - gxyz[3 * a + 2];
@@ -145,7 +154,7 @@ __global__ void k_aden_pcc(int64_t nn, const int64_t *__restrict__ ent_off, cons
 // the lines back.  sign -1: FFTW's forward transform; +1: the unnormalised
 // inverse.
 constexpr int FFT_MAXST = 24;
-constexpr int FFT_LDS_ELEMS = 2048;  // two buffers of B * n complex values: 64 KiB
+constexpr int FFT_LDS_ELEMS = 4096;  // at most two buffers of 2048 complex values: 64 KiB
 constexpr int FFT_THREADS = 256;
 
 struct FftPass {
@@ -258,12 +267,14 @@ struct FieldPlan {
     double2 *buf = nullptr;   // NN complex values
     double *stage = nullptr;  // host-pointer entry points: inputs and outputs
     int64_t stage_n = 0;
+    int2 *entinfo = nullptr;  // (atom, cell) per grid-point index entry
 };
 
 void field_free(FieldPlan *f) {
     if (!f) return;
     if (f->buf) (void)hipFree(f->buf);
     if (f->stage) (void)hipFree(f->stage);
+    if (f->entinfo) (void)hipFree(f->entinfo);
     delete f;
 }
 
@@ -342,14 +353,18 @@ int field_plan(lcpao_ctx *ctx, FieldPlan **out) {
     const int64_t N1 = ctx->ngrid[0], N2 = ctx->ngrid[1], N3 = ctx->ngrid[2];
     for (int x = 0; x < 3; x++) {
         f->n[x] = ctx->ngrid[x];
-        if (f->n[x] > FFT_LDS_ELEMS / 2) return ctx_error(ctx, LCPAO_ERANGE, "grid axis longer than 1024 points");
+        if (f->n[x] > 1024) return ctx_error(ctx, LCPAO_ERANGE, "grid axis longer than 1024 points");
         FftPass &P = f->pass[x];
         P = FftPass{};
         P.n = f->n[x];
         P.nst = fft_radices(P.n, P.rad);
         if (P.nst < 0) return ctx_error(ctx, LCPAO_ERANGE, "grid axis with a prime factor above 13");
         for (int k = 0; k < P.nst; k++) f->wide[x] = f->wide[x] || P.rad[k] > 5;
-        P.B = std::max(1, FFT_LDS_ELEMS / 2 / P.n);
+        // contiguous lines: B * n <= 1024 (32 KiB of LDS); strided axes: B a
+        // multiple of 8 so that each element index is read as whole 128-byte
+        // lines (B * n <= 2048)
+        if (x == 2) P.B = std::max(1, 1024 / P.n);
+        else P.B = P.n <= 256 ? (FFT_LDS_ELEMS / 2 / P.n) & ~7 : std::max(1, FFT_LDS_ELEMS / 2 / P.n);
         int64_t outer;
         if (x == 2) {         // contiguous lines, one after the other
             P.es = 1; P.tstride = N3; P.T = N1 * N2; P.ostride = 0; outer = 1;
@@ -372,6 +387,7 @@ hipError_t fft_axis(lcpao_ctx *ctx, FieldPlan *f, int axis, int sign) {
     FftPass P = f->pass[axis];
     P.sign = sign;
     const size_t lds = sizeof(double2) * 2 * (size_t)P.B * P.n;
+    KTimer kt(ctx, KID_FFT);
     if (f->wide[axis])
         hipLaunchKernelGGL(k_fft_pass<true>, dim3(f->grid[axis]), dim3(FFT_THREADS), lds, ctx->stream, f->buf, P);
     else
@@ -402,17 +418,27 @@ int poisson_run(lcpao_ctx *ctx, int nspin, const double *const *d_rho, const dou
     RecGeom rg;
     for (int x = 0; x < 9; x++) rg.rec[x] = rec[x];
     rg.n1 = ctx->ngrid[0]; rg.n2 = ctx->ngrid[1]; rg.n3 = ctx->ngrid[2]; rg.pad = 0;
-    hipLaunchKernelGGL(k_poisson_load, dim3(nb), dim3(256), 0, ctx->stream, nn, nspin, d_rho[0],
-                       nspin >= 2 ? d_rho[1] : nullptr, d_aden, f->buf);
+    {
+        KTimer kt(ctx, KID_PGRID);
+        hipLaunchKernelGGL(k_poisson_load, dim3(nb), dim3(256), 0, ctx->stream, nn, nspin, d_rho[0],
+                           nspin >= 2 ? d_rho[1] : nullptr, d_aden, f->buf);
+    }
     HIP_TRY(ctx, hipGetLastError());
     // forward along n3, n2, n1 and back along n1, n2, n3, as Poisson.jl:37-104
     for (int axis = 2; axis >= 0; axis--) HIP_TRY(ctx, fft_axis(ctx, f, axis, -1));
-    hipLaunchKernelGGL(k_poisson_kernel, dim3(nb), dim3(256), 0, ctx->stream, nn, rg, f->buf);
+    {
+        KTimer kt(ctx, KID_PGRID);
+        hipLaunchKernelGGL(k_poisson_kernel, dim3(nb), dim3(256), 0, ctx->stream, nn, rg, f->buf);
+    }
     HIP_TRY(ctx, hipGetLastError());
     for (int axis = 0; axis < 3; axis++) HIP_TRY(ctx, fft_axis(ctx, f, axis, 1));
-    hipLaunchKernelGGL(k_poisson_store, dim3(nb), dim3(256), 0, ctx->stream, nn, f->buf, d_v);
+    {
+        KTimer kt(ctx, KID_PGRID);
+        hipLaunchKernelGGL(k_poisson_store, dim3(nb), dim3(256), 0, ctx->stream, nn, f->buf, d_v);
+    }
     HIP_TRY(ctx, hipGetLastError());
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    collect_timings(ctx);
     return LCPAO_OK;
 }
 
@@ -492,13 +518,30 @@ int aden_run(lcpao_ctx *ctx, int32_t nspin, int32_t nspecies, const lcpao_atomic
         }
         fg.n2 = ctx->ngrid[1]; fg.n3 = ctx->ngrid[2]; fg.natom = ctx->natom; fg.nspin = nspin;
         const unsigned nb = (unsigned)((ctx->nn + 127) / 128);
-        hipLaunchKernelGGL(k_aden_pcc, dim3(nb), dim3(128), 0, ctx->stream, ctx->nn, ctx->d_ent_off, ctx->d_ent,
-                           ctx->d_gl_off, ctx->d_clist, d_blob + o_gxyz, d_a2s, d_blob + o_atv, fg, d_rd, d_sp, d_aden,
-                           d_pcc, d_rho[0], nspin >= 2 ? d_rho[1] : nullptr, nspin == 4 ? d_rho[2] : nullptr,
-                           nspin == 4 ? d_rho[3] : nullptr);
-        hipError_t e = hipGetLastError();
+        FieldPlan *f = fields_of(ctx);
+        hipError_t e = hipSuccess;
+        if (!f->entinfo) {
+            const int64_t nent = std::max<int64_t>(1, ctx->total_gridn);
+            e = hipMalloc(&f->entinfo, sizeof(int2) * (size_t)nent);
+            if (e == hipSuccess && ctx->total_gridn > 0) {
+                hipLaunchKernelGGL(k_entry_info, dim3((unsigned)((ctx->total_gridn + 255) / 256)), dim3(256), 0,
+                                   ctx->stream, ctx->total_gridn, ctx->d_ent, ctx->d_gl_off, ctx->natom, ctx->d_clist,
+                                   f->entinfo);
+                e = hipGetLastError();
+            }
+            if (e != hipSuccess && f->entinfo) { (void)hipFree(f->entinfo); f->entinfo = nullptr; }
+        }
+        if (e == hipSuccess) {
+            KTimer kt(ctx, KID_ADEN);
+            hipLaunchKernelGGL(k_aden_pcc, dim3(nb), dim3(128), 0, ctx->stream, ctx->nn, ctx->d_ent_off, f->entinfo,
+                               d_blob + o_gxyz, d_a2s, d_blob + o_atv, fg, d_rd, d_sp, d_aden, d_pcc, d_rho[0],
+                               nspin >= 2 ? d_rho[1] : nullptr, nspin == 4 ? d_rho[2] : nullptr,
+                               nspin == 4 ? d_rho[3] : nullptr);
+            e = hipGetLastError();
+        }
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) result = ctx_error(ctx, LCPAO_EDEVICE, std::string("atomic densities: ") + hipGetErrorString(e));
+        else collect_timings(ctx);
     }
     (void)hipFree(d_blob); (void)hipFree(d_rd); (void)hipFree(d_sp); (void)hipFree(d_a2s);
     return result;

@@ -1309,6 +1309,8 @@ void free_all(lcpao_ctx *ctx) {
     ctx->plan = nullptr;
     field_free(ctx->fields);
     ctx->fields = nullptr;
+    bloch_free(ctx->bloch);
+    ctx->bloch = nullptr;
     for (auto &p : ctx->pending) { (void)hipEventDestroy(p.beg); (void)hipEventDestroy(p.end); }
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     if (ctx->stream && ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
@@ -1319,6 +1321,7 @@ void free_all(lcpao_ctx *ctx) {
 namespace lcpao {
 int ctx_enter(lcpao_ctx *ctx) { return check_ctx(ctx); }
 int ctx_error(lcpao_ctx *ctx, int code, const std::string &msg) { return set_err(ctx, code, msg); }
+void cell_triplet(const lcpao_ctx *ctx, int cell, int *ijk) { cell_ijk(ctx->cpy, cell, ijk); }
 std::vector<double> cell_vectors(const lcpao_ctx *ctx) {
     const int C = ctx->cpy, W = 2 * C + 1, ncell = W * W * W;
     std::vector<double> atv((size_t)ncell * 3, 0.0);
@@ -1480,7 +1483,7 @@ int lcpao_grid_set_stream(lcpao_ctx *ctx, void *hip_stream) {
 int lcpao_grid_profile(lcpao_ctx *ctx, int32_t enable) {
     if (!ctx) return LCPAO_EINVAL;
     ctx->profiling = enable != 0;
-    for (int k = 0; k < 8; k++) { ctx->prof_launches[k] = 0; ctx->prof_ms[k] = 0.0; }
+    for (int k = 0; k < KID_COUNT; k++) { ctx->prof_launches[k] = 0; ctx->prof_ms[k] = 0.0; }
     return LCPAO_OK;
 }
 
@@ -1501,7 +1504,7 @@ int lcpao_grid_tuned(const lcpao_ctx *ctx, int32_t op, int32_t *variant, float *
 }
 
 int lcpao_grid_kernel_stats(const lcpao_ctx *ctx, int32_t kernel_id, int64_t *launches, double *total_ms) {
-    if (!ctx || kernel_id < 0 || kernel_id >= 8) return LCPAO_EINVAL;
+    if (!ctx || kernel_id < 0 || kernel_id >= KID_COUNT) return LCPAO_EINVAL;
     if (launches) *launches = ctx->prof_launches[kernel_id];
     if (total_ms) *total_ms = ctx->prof_ms[kernel_id];
     return LCPAO_OK;

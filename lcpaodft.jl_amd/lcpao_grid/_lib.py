@@ -50,6 +50,8 @@ EXPORTS = [
     "lcpao_batch_create", "lcpao_batch_assemble", "lcpao_batch_tune", "lcpao_batch_tuned", "lcpao_batch_sizes",
     "lcpao_batch_last_error", "lcpao_batch_destroy",
     "lcpao_grid_aden_pcc", "lcpao_grid_aden_pcc_dev", "lcpao_grid_poisson", "lcpao_grid_poisson_dev",
+    "lcpao_grid_bloch_sum", "lcpao_grid_bloch_sum_dev", "lcpao_grid_bloch_sizes", "lcpao_grid_dm_states",
+    "lcpao_grid_dm_states_dev",
 ]
 
 _lib = None
@@ -106,6 +108,11 @@ def load():
     L.lcpao_grid_aden_pcc_dev.argtypes = [vp, C.c_int32, C.c_int32, ar, dp, dp, dp, dp, dpp]
     L.lcpao_grid_poisson.argtypes = [vp, C.c_int32, dpp, dp, dp, dp]
     L.lcpao_grid_poisson_dev.argtypes = [vp, C.c_int32, dpp, dp, dp, dp]
+    L.lcpao_grid_bloch_sum.argtypes = [vp, dp, C.c_int32, dpp, dpp]
+    L.lcpao_grid_bloch_sum_dev.argtypes = [vp, dp, C.c_int32, dpp, dpp]
+    L.lcpao_grid_bloch_sizes.argtypes = [vp, lp, lp]
+    L.lcpao_grid_dm_states.argtypes = [vp, C.c_int32, dp, C.c_int64, dpp, C.c_double, dp]
+    L.lcpao_grid_dm_states_dev.argtypes = [vp, C.c_int32, dp, C.c_int64, dpp, C.c_double, dp]
     L.lcpao_grid_set_stream.argtypes = [vp, vp]
     L.lcpao_grid_profile.argtypes = [vp, C.c_int32]
     L.lcpao_grid_kernel_stats.argtypes = [vp, C.c_int32, lp, dp]

@@ -221,6 +221,51 @@ class GridContext:
                                                   None if daden is None else C.cast(daden.data_ptr(), _lib.dp),
                                                   rec.ctypes.data_as(_lib.dp), C.cast(dV.data_ptr(), _lib.dp)))
 
+    # either side of the k-point diagonalisation --------------------------------
+    def bloch_sizes(self):
+        fs, ng = C.c_int64(), C.c_int64()
+        self._check(self.L.lcpao_grid_bloch_sizes(self.h, C.byref(fs), C.byref(ng)))
+        return fs.value, ng.value
+
+    def bloch_sum(self, kpt, flats):
+        """HS_matrix! with a k-point: flat pair arrays (my_hsize) -> dense
+        complex (fsize, fsize) matrices."""
+        fsize, _ = self.bloch_sizes()
+        F = [np.ascontiguousarray(f, np.float64) for f in flats]
+        D = [np.zeros((fsize, fsize), np.complex128, order="F") for _ in F]
+        k = np.ascontiguousarray(kpt, np.float64)
+        self._check(self.L.lcpao_grid_bloch_sum(self.h, k.ctypes.data_as(_lib.dp), len(F), _ptrs(F),
+                                                _ptrs([d.reshape(-1, order="F").view(np.float64) for d in D])))
+        return D
+
+    def bloch_sum_dev(self, kpt, dflats, ddense):
+        """Device form: ddense are complex128 torch tensors (fsize, fsize)
+        stored column-major (a transposed view of a contiguous tensor, or any
+        tensor whose memory is in Julia's order)."""
+        k = np.ascontiguousarray(kpt, np.float64)
+        self._order()
+        self._check(self.L.lcpao_grid_bloch_sum_dev(self.h, k.ctypes.data_as(_lib.dp), len(dflats),
+                                                    self._dptrs(dflats), self._dptrs(ddense)))
+
+    def dm_states(self, kpts, states, scale):
+        """Calc_DM_Crystal_Collinear_nopol!: states per k-point (complex
+        (fsize, nstates), columns scaled) -> flat DM (my_hsize)."""
+        kp = np.ascontiguousarray(kpts, np.float64).reshape(-1, 3)
+        cs = [np.asfortranarray(c, np.complex128) for c in states]
+        nst = cs[0].shape[1]
+        flat = [np.ascontiguousarray(c.reshape(-1, order="F")).view(np.float64) for c in cs]
+        DM = np.zeros(self.my_hsize)
+        self._check(self.L.lcpao_grid_dm_states(self.h, len(cs), kp.ctypes.data_as(_lib.dp), nst, _ptrs(flat),
+                                                float(scale), DM.ctypes.data_as(_lib.dp)))
+        return DM
+
+    def dm_states_dev(self, kpts, dstates, nstates, scale, dDM):
+        kp = np.ascontiguousarray(kpts, np.float64).reshape(-1, 3)
+        self._order()
+        self._check(self.L.lcpao_grid_dm_states_dev(self.h, kp.shape[0], kp.ctypes.data_as(_lib.dp), int(nstates),
+                                                    self._dptrs(dstates), float(scale),
+                                                    C.cast(dDM.data_ptr(), _lib.dp)))
+
     def set_positions(self, gxyz):
         """Atom positions (natom x 3, bohr) for the next build_tables; lists kept."""
         g = np.ascontiguousarray(gxyz, np.float64).reshape(-1, 3)
@@ -393,3 +438,20 @@ def Solve_Poisson(SpinPol, Recvecs, Density_Grid, ADensity_Grid, dVHart_Grid, Or
     """Solve_Poisson! (Poisson.jl:1-7): dVHart_Grid is overwritten."""
     nspin = {"off": 1, "on": 2, "nc": 4}[SpinPol]
     dVHart_Grid[:] = Orbs_Grid.poisson(list(Density_Grid[:min(nspin, 2)]), ADensity_Grid, Recvecs, nspin)
+
+
+def HS_matrix(S, H, OLP, Hks, kpts, Orbs_Grid):
+    """HS_matrix!(S, H, OLP::Vector{Float64}, Hks::Vector{Float64}, ..., kpts)
+    (HS_matrix.jl:143-166) on the device of the context Orbs_Grid; S and H
+    (complex fsize x fsize) are overwritten."""
+    Sk, Hk = Orbs_Grid.bloch_sum(kpts, [OLP, Hks])
+    S[:, :] = Sk
+    H[:, :] = Hk
+
+
+def Calc_DM_Crystal_Collinear_nopol(Cnk, MPI_kpts, AllNkpt, DM, Orbs_Grid):
+    """The accumulation of Calc_DM_Crystal_Collinear_nopol! (Crystal_DFT.jl:219-270)
+    for this rank's k-points (states scaled as Calc_fnkCnk! leaves them);
+    DM[0] is overwritten with this rank's part (the reference then sums the
+    ranks with MPI.Allreduce!)."""
+    DM[0][:] = Orbs_Grid.dm_states(MPI_kpts, Cnk, 1.0 / AllNkpt)

@@ -1728,3 +1728,100 @@ int orc_poisson(const int *ngrid, const double *rec, const double *density_xyz, 
     free(z);
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* Either side of the diagonalisation                                 */
+/* ------------------------------------------------------------------ */
+
+/* cispi(x) = (cos(pi x), sin(pi x)) with the argument reduced exactly as
+ * Julia's sincospi does for the phase arguments 2 k.R (reduced mod 2 first) */
+static void orc_cispi(double x, double *c, double *s) {
+    double r = fmod(x, 2.0);
+    if (r < 0) r += 2.0;
+    *c = cos(ORC_PI * r);
+    *s = sin(ORC_PI * r);
+}
+
+static int64_t *orbital_offsets(const orc_geom *G) {
+    int64_t *mp = malloc(sizeof(int64_t) * (G->natom + 1));
+    mp[0] = 0;
+    for (int a = 0; a < G->natom; a++) mp[a + 1] = mp[a] + G->no[a];
+    return mp;
+}
+
+/* HS_matrix! with k (HS_matrix.jl:88-110, 143-166) over the geometry's local
+ * pairs (all pairs for one rank): dense[m] (fsize x fsize, column-major,
+ * interleaved complex) = sum of flat[m][pair][ist, jst] * cispi(2 kRn). */
+int orc_hs_matrix(const orc_geom *G, const double *kpt, int nmat, const double *const *flat, double *const *dense) {
+    int64_t *mp = orbital_offsets(G);
+    const int64_t fsize = mp[G->natom];
+    for (int m = 0; m < nmat; m++) memset(dense[m], 0, sizeof(double) * 2 * fsize * fsize);
+    const double ka = kpt[0], kb = kpt[1], kc = kpt[2];
+    for (int64_t p = 0; p < G->npair; p++) {
+        const int atom = G->p_atom[p], jatom = G->p_natn[p], cell = G->p_ncn[p];
+        const int NO0 = G->no[atom], NO1 = G->no[jatom];
+        const int64_t Anum = mp[atom], Bnum = mp[jatom];
+        const double kRn = ka * G->atv_ijk[3 * cell] + kb * G->atv_ijk[3 * cell + 1] + kc * G->atv_ijk[3 * cell + 2];
+        double c, s;
+        orc_cispi(2 * kRn, &c, &s);
+        int64_t hst = G->p_hoff[p];
+        for (int ist = 0; ist < NO0; ist++)
+            for (int jst = 0; jst < NO1; jst++, hst++) {
+                const int64_t o = (Anum + ist) + (Bnum + jst) * fsize;
+                for (int m = 0; m < nmat; m++) {
+                    dense[m][2 * o] += flat[m][hst] * c;
+                    dense[m][2 * o + 1] += flat[m][hst] * s;
+                }
+            }
+    }
+    free(mp);
+    return 0;
+}
+
+/* Calc_DM_Crystal_Collinear_nopol! (Crystal_DFT.jl:219-270): per k-point the
+ * dense DM_dense = C C^H (mul!, a plain triple loop here), then for each pair
+ * DM[DMst] += real(DM_dense[Bnum+jst, Anum+ist] * cispi(2 kRn) * scale).
+ * C[k]: fsize x nstates interleaved complex, column-major. */
+int orc_dm_collinear(const orc_geom *G, int nk, const double *kpts, int64_t nstates, const double *const *C,
+                     double scale, double *DM) {
+    int64_t *mp = orbital_offsets(G);
+    const int64_t fsize = mp[G->natom];
+    double *D = malloc(sizeof(double) * 2 * fsize * fsize);
+    if (!D) { free(mp); return fail("oom"); }
+    memset(DM, 0, sizeof(double) * G->my_hsize);
+    for (int k = 0; k < nk; k++) {
+        const double *c = C[k];
+        for (int64_t col = 0; col < fsize; col++)
+            for (int64_t row = 0; row < fsize; row++) {
+                double re = 0.0, im = 0.0;
+                for (int64_t mu = 0; mu < nstates; mu++) {
+                    const double ar = c[2 * (row + mu * fsize)], ai = c[2 * (row + mu * fsize) + 1];
+                    const double br = c[2 * (col + mu * fsize)], bi = c[2 * (col + mu * fsize) + 1];
+                    re += ar * br + ai * bi;  /* a * conj(b) */
+                    im += ai * br - ar * bi;
+                }
+                D[2 * (row + col * fsize)] = re;
+                D[2 * (row + col * fsize) + 1] = im;
+            }
+        const double ka = kpts[3 * k], kb = kpts[3 * k + 1], kc = kpts[3 * k + 2];
+        for (int64_t p = 0; p < G->npair; p++) {
+            const int atom = G->p_atom[p], jatom = G->p_natn[p], cell = G->p_ncn[p];
+            const int NO0 = G->no[atom], NO1 = G->no[jatom];
+            const int64_t Anum = mp[atom], Bnum = mp[jatom];
+            const double kRn = ka * G->atv_ijk[3 * cell] + kb * G->atv_ijk[3 * cell + 1] + kc * G->atv_ijk[3 * cell + 2];
+            double er, ei;
+            orc_cispi(2 * kRn, &er, &ei);
+            er = er * scale;
+            ei = ei * scale;
+            int64_t DMst = G->p_hoff[p];
+            for (int ist = 0; ist < NO0; ist++)
+                for (int jst = 0; jst < NO1; jst++, DMst++) {
+                    const int64_t o = (Bnum + jst) + (Anum + ist) * fsize;
+                    DM[DMst] += D[2 * o] * er - D[2 * o + 1] * ei;
+                }
+        }
+    }
+    free(D);
+    free(mp);
+    return 0;
+}

@@ -305,3 +305,28 @@ def poisson(ngrid, recvecs, density_xyz):
     _check(lib().orc_poisson(ng.ctypes.data_as(_ip), rec.ctypes.data_as(_dp), d.ctypes.data_as(_dp),
                              V.ctypes.data_as(_dp)))
     return V
+
+
+def hs_matrix(geom, kpt, flats):
+    """HS_matrix! with k (HS_matrix.jl:88-110): list of flat pair arrays ->
+    list of dense complex (fsize, fsize) matrices (A_i row, B_j column)."""
+    fsize = int(np.asarray(geom.no, dtype=np.int64).sum())
+    F = [np.ascontiguousarray(f, dtype=np.float64) for f in flats]
+    D = [np.zeros(2 * fsize * fsize) for _ in F]
+    k = np.ascontiguousarray(kpt, dtype=np.float64)
+    _check(lib().orc_hs_matrix(geom.h, k.ctypes.data_as(_dp), len(F), _dptrs(F), _dptrs(D)))
+    return [d.view(np.complex128).reshape(fsize, fsize, order="F") for d in D]
+
+
+def dm_collinear(geom, kpts, states, scale):
+    """Calc_DM_Crystal_Collinear_nopol! (Crystal_DFT.jl:219-270): states is a
+    list (one per k-point) of complex (fsize, nstates) matrices, columns
+    scaled by sqrt(occupation * weight); returns the flat DM (my_hsize)."""
+    kp = np.ascontiguousarray(kpts, dtype=np.float64).reshape(-1, 3)
+    cs = [np.asfortranarray(c, dtype=np.complex128) for c in states]
+    nst = cs[0].shape[1]
+    flat = [np.ascontiguousarray(c.reshape(-1, order="F")).view(np.float64) for c in cs]
+    DM = np.zeros(geom.my_hsize)
+    _check(lib().orc_dm_collinear(geom.h, len(cs), kp.ctypes.data_as(_dp), C.c_int64(nst), _dptrs(flat),
+                                  C.c_double(scale), DM.ctypes.data_as(_dp)))
+    return DM
