@@ -601,8 +601,9 @@ def bench_bloch(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: the multi-GPU split of the pair list is described in DESIGN.md "
-              "and not measured here (--gpus %d requested)" % args.gpus, file=sys.stderr)
+        print("bench.py runs on one GPU only: across GPUs the pair list would be split with the reference's "
+              "weighted contiguous ranges, which is not measured here (--gpus %d requested)" % args.gpus,
+              file=sys.stderr)
         return 2
     import torch
     from lcpao_grid import systems as S

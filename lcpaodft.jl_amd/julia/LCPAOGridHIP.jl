@@ -10,8 +10,12 @@
 #   Calc_MatrixElements_dVH_Vxc_{off,on,nc}!   Hamiltonian.jl:235,280,327
 #   Set_Density_Grid_{nonpol,pol,nc}!          Set_Density_Grid.jl:60,112,172
 #   Force3_{nospin,spin}, Force3_nc            Force.jl:407,454,506
+#   Set_AdenPCC_Grid                           Set_AdenPCC_Grid.jl:1
+#   Solve_Poisson!(SpinPol, dft_mixing, ...)   Poisson.jl:1
 # Force3 receives the nested DM_Vec in the reference; the wrapper flattens it
 # back to the global flat layout (Convert_Matrix.jl:1-24 order).
+# The k-point routines (HS_matrix!, Calc_DM_Crystal_Collinear_nopol!) are in
+# LCPAOGridHIP_kpoint.jl: they need a context that holds every pair.
 
 const LIBGRID = get(ENV, "LCPAO_GRID_LIB", joinpath(@__DIR__, "..", "lcpao_grid", "liblcpao_grid.so"))
 const GRID_PRECISION = Ref{Int32}(0)   # 0 FP64, 1 FP32, 2 split-BF16x3
@@ -46,6 +50,7 @@ mutable struct GridHandle
 end
 
 const GRID_CTX = IdDict{Any,GridHandle}()
+const LAST_GRID = Ref{Any}(nothing)   # (ucell, handle) of the latest context: Solve_Poisson! has no ucell
 
 function _grid_check(rc, ctx)
     rc == 0 && return nothing
@@ -78,6 +83,7 @@ function _grid_context(ucell::UCell)
     h = GridHandle(ctxref[], nothing, nothing)
     finalizer(x -> ccall((:lcpao_grid_destroy, LIBGRID), Cvoid, (Ptr{Cvoid},), x.ptr), h)
     GRID_CTX[ucell] = h
+    LAST_GRID[] = (ucell, h)
     return h
 end
 
@@ -179,3 +185,62 @@ Force3_spin(pao::Vector{PAO}, Orbs_Grid, Vpot_Grid, DM, ucell::UCell) =
 Force3_nc(pao::Vector{PAO}, Orbs_Grid, dVHart_Grid, Vxc_Grid, DM, ucell::UCell) =
     _grid_force3(pao, Orbs_Grid, [dVHart_Grid + Vxc_Grid[1], dVHart_Grid + Vxc_Grid[2], Vxc_Grid[3], Vxc_Grid[4]],
                  DM, ucell, (2.0, 2.0, 4.0, -4.0))
+
+# ---- Set_AdenPCC_Grid (Set_AdenPCC_Grid.jl:1-142) ---------------------------
+# Every atom contributes on every rank, so the result equals the reference's
+# after its MPI.Allreduce! calls; no reduction follows here.
+struct LcpaoAtomicRadial
+    nmesh_pao::Int32
+    pao_xv::Ptr{Float64}
+    pao_rv::Ptr{Float64}
+    atomic_den::Ptr{Float64}
+    nmesh_vps::Int32
+    vps_xv::Ptr{Float64}
+    vps_rv::Ptr{Float64}
+    atomic_pcc::Ptr{Float64}
+end
+
+function Set_AdenPCC_Grid(SpinPol::AbstractString, Init_Atoms_Nspin, Init_Atoms_Angle, pao::Vector{PAO},
+                          pspot::Vector{Pspot}, ucell::UCell)
+    h = _grid_context(ucell)
+    Nspin = SpinPol == "off" ? 1 : SpinPol == "on" ? 2 : SpinPol == "nc" ? 4 : error("please check SpinPol")
+    natom = ucell.system_grid.Natom
+    NN = prod(ucell.system_grid.Ngrid)
+    rad = [LcpaoAtomicRadial(p.Spe_Num_Mesh_PAO, pointer(p.Spe_PAO_XV), pointer(p.Spe_PAO_RV), pointer(p.Spe_Atomic_Den),
+                             v.Spe_Num_Mesh_VPS, pointer(v.Spe_VPS_XV), pointer(v.Spe_VPS_RV), pointer(v.Spe_Atomic_PCC))
+           for (p, v) in zip(pao, pspot)]
+    ins = Float64[x for a in 1:natom for x in (Init_Atoms_Nspin[a][1], Init_Atoms_Nspin[a][2])]
+    ang = Float64[x for a in 1:natom for x in (Init_Atoms_Angle[a][1], Init_Atoms_Angle[a][2])]
+    ADensity_Grid = zeros(Float64, NN)
+    PCCDensity_Grid = zeros(Float64, NN)
+    Density_Grid = [zeros(Float64, NN) for _ in 1:Nspin]
+    rp = [pointer(d) for d in Density_Grid]
+    GC.@preserve pao pspot rad ins ang ADensity_Grid PCCDensity_Grid Density_Grid rp begin
+        rc = ccall((:lcpao_grid_aden_pcc, LIBGRID), Cint,
+                   (Ptr{Cvoid}, Int32, Int32, Ptr{LcpaoAtomicRadial}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                    Ptr{Float64}, Ptr{Ptr{Float64}}),
+                   h.ptr, Nspin, length(rad), rad, ins, ang, ADensity_Grid, PCCDensity_Grid, rp)
+        _grid_check(rc, h.ptr)
+    end
+    return ADensity_Grid, PCCDensity_Grid, Density_Grid
+end
+
+# ---- Solve_Poisson! (Poisson.jl:1-108) -------------------------------------
+# Runs on the context of the latest UCell (the method receives no UCell); its
+# grid must be dft_mixing.Ngrid.
+function Solve_Poisson!(SpinPol::String, dft_mixing::Rhor_Mixing, Density_Grid, ADensity_Grid, dVHart_Grid)
+    last = LAST_GRID[]
+    last === nothing && error("Solve_Poisson!: no grid context yet (build one through a grid routine first)")
+    ucell, h = last
+    Tuple(ucell.system_grid.Ngrid) == Tuple(dft_mixing.Ngrid) || error("Solve_Poisson!: grid differs from the context's")
+    nspin = SpinPol == "off" ? 1 : 2
+    rp = [pointer(Density_Grid[s]) for s in 1:nspin]
+    rec = vec(Matrix{Float64}(dft_mixing.Recvecs))      # column-major, as the library expects
+    GC.@preserve Density_Grid ADensity_Grid dVHart_Grid rp rec begin
+        rc = ccall((:lcpao_grid_poisson, LIBGRID), Cint,
+                   (Ptr{Cvoid}, Int32, Ptr{Ptr{Float64}}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                   h.ptr, nspin, rp, ADensity_Grid, rec, dVHart_Grid)
+        _grid_check(rc, h.ptr)
+    end
+    return nothing
+end
