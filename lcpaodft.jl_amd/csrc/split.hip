@@ -913,12 +913,44 @@ void launch_hblk_reduce(const lcpao_ctx *ctx, hipStream_t stream, double *H) {
 }
 
 constexpr int RB_BYTES_PER_ROW = 2 * CHUNK * 2;   // LDS bytes per row: hi + lo point fragments
-constexpr int RB_IMAGE_BYTES = HB_WAVES * 8 * 2 * 512 * 2;  // per-wave W images (16 KB each)
+constexpr int RB_WAVES = 16;                       // waves of the density workgroup, one super-tile each
 constexpr int RB_SUMS_OFFSET = MAX_ROWS * RB_BYTES_PER_ROW;  // per-wave point sums, after the chunk planes
-static_assert(RB_SUMS_OFFSET + HB_WAVES * BLK_PTS * 4 <= RB_IMAGE_BYTES, "point sums exceed the image area");
+constexpr int RB_AREA_BYTES = RB_SUMS_OFFSET + RB_WAVES * BLK_PTS * 4;
+constexpr int RB_IMAGE_BYTES = HB_WAVES * 8 * 2 * 512 * 2;    // W images of the 8-wave (batched) form
+static_assert(RB_AREA_BYTES <= RB_IMAGE_BYTES, "the 8-wave form's LDS covers the planes and sums");
+static_assert(RB_WAVES * (16 / RB_WAVES) == HB_WAVES * 2, "an item's super-tiles fill the density waves");
 
-template <int NF>
+// L2 warm-up of a chunk by NT threads: one dword per 128-byte line, K loads
+// per thread (lines past the chunk repeat line 0), as touch_chunk.
+template <int NT>
+struct TouchN {
+    static constexpr int K = (MAX_ROWS + NT - 1) / NT;
+    uint32_t v[K];
+};
+template <int NT>
+__device__ __forceinline__ TouchN<NT> touch_chunk_n(const uint16_t *__restrict__ hi, const uint16_t *__restrict__ lo,
+                                                    int64_t off, int R, int tid) {
+    TouchN<NT> t;
+    const int half = R / 2;
+#pragma unroll
+    for (int k = 0; k < TouchN<NT>::K; k++) {
+        const int x = tid + k * NT < R ? tid + k * NT : 0;
+        t.v[k] = *reinterpret_cast<const uint32_t *>((x < half ? hi : lo) + off + (int64_t)(x < half ? x : x - half) * 64);
+    }
+    return t;
+}
+template <int NT>
+__device__ __forceinline__ void release_n(const TouchN<NT> &t) {
+#pragma unroll
+    for (int k = 0; k < TouchN<NT>::K; k++) asm volatile("" ::"v"(t.v[k]));
+}
+
+// Density of one item: W waves, each owning PER = 16 / W of the item's
+// super-tiles.  With W = 16 a wave holds the W fragments of one super-tile
+// (64 VGPRs), so four waves fit per SIMD.
+template <int NF, int W>
 __device__ __forceinline__ void rblk_item(const HItem it, const BlockArgs &A, unsigned char *lds_raw) {
+    constexpr int PER = 16 / W;
     const BlockDev B = A.blocks[it.block];
     const uint16_t *__restrict__ thi = A.thi;
     const uint16_t *__restrict__ tlo = A.tlo;
@@ -931,90 +963,145 @@ __device__ __forceinline__ void rblk_item(const HItem it, const BlockArgs &A, un
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, q = lane >> 4;
     u16x8 *TH = reinterpret_cast<u16x8 *>(lds_raw);
     u16x8 *TL = TH + G * 64;
-    // LDS: [0, RB_IMAGE_BYTES) the W images, later the chunk planes and the
-    // per-wave point sums; then the block's row descriptors and slot pairs
+    // LDS: the chunk planes, the per-wave point sums, then the block's row
+    // descriptors and slot pairs
     float *rcw = reinterpret_cast<float *>(lds_raw + RB_SUMS_OFFSET);  // [wave][512 points]
-    int16_t *s_rinfo = reinterpret_cast<int16_t *>(lds_raw + RB_IMAGE_BYTES);
+    int16_t *s_rinfo = reinterpret_cast<int16_t *>(lds_raw + (PER == 1 ? RB_AREA_BYTES : RB_IMAGE_BYTES));
     SlotPair *s_sp = reinterpret_cast<SlotPair *>(s_rinfo + MAX_ROWS);
-    for (int x = tid; x < R; x += 64 * HB_WAVES) s_rinfo[x] = rowinfo[B.row_off + x];
-    for (int x = tid; x < B.nslot * B.nslot; x += 64 * HB_WAVES) s_sp[x] = spair[B.sp_off + x];
+    for (int x = tid; x < R; x += 64 * W) s_rinfo[x] = rowinfo[B.row_off + x];
+    for (int x = tid; x < B.nslot * B.nslot; x += 64 * W) s_sp[x] = spair[B.sp_off + x];
     // W fragments of this wave's super-tiles (A operand: W[16 rt + l16][32 ks + 8 q + e])
-    bf16x8 wh[2][4][2], wl[2][4][2];
-    int si[2] = {0, 0}, sj[2] = {0, 0};
-    bool has[2];
+    bf16x8 wh[PER][4][2], wl[PER][4][2];
+    int si[PER], sj[PER];
+    bool has[PER];
     // the first two chunks are fetched into L2 while W is gathered
-    Touch t0 = touch_chunk(thi, tlo, B.tile_off, R, tid);
-    Touch t1 = touch_chunk(thi, tlo, B.tile_off + (int64_t)G * FRAG, R, tid);
+    TouchN<64 * W> t0 = touch_chunk_n<64 * W>(thi, tlo, B.tile_off, R, tid);
+    TouchN<64 * W> t1 = touch_chunk_n<64 * W>(thi, tlo, B.tile_off + (int64_t)G * FRAG, R, tid);
     __syncthreads();
-    // W of each super-tile is gathered into a per-wave LDS image laid out as
-    // the A-operand fragments [rt * 2 + ks][hi, lo][lane][8], then read back
-    // into registers.  lane = (column group cg = lane / 8, row rr = lane % 8):
-    // the lane fills W[8 t8 + rr][8 cg .. 8 cg + 7], one 16-byte piece of a
-    // fragment per plane.  W element = mask * Deff[one index]: on-site blocks
-    // hold D + D^T, so every element is a single load; 32 loads per batch.
-    uint16_t *img = reinterpret_cast<uint16_t *>(lds_raw) + wave * (8 * 2 * FRAG);
-    const int rr = lane & 7, cg = lane >> 3;
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-        has[h] = wave + h * HB_WAVES < it.nst;
-        if (has[h]) {
-            st_decode(it.st0 + wave + h * HB_WAVES, S, si[h], sj[h]);
-            const float wdiag = si[h] == sj[h] ? 0.5f : 1.f;
-            int ci[8];
+    for (int h = 0; h < PER; h++) {
+        has[h] = wave + h * W < it.nst;
+        si[h] = sj[h] = 0;
+        if (has[h]) st_decode(it.st0 + wave + h * W, S, si[h], sj[h]);
+    }
+    if constexpr (PER == 1) {
+        // W gathered straight into the A-operand fragments: fragment (rt, ks)
+        // of lane l holds W[16 rt + l % 16][32 ks + 8 (l / 16) + e], e < 8.
+        // W element = mask * Deff[one index] (on-site blocks hold D + D^T, so
+        // every element is a single load); 16 loads in flight per batch, the
+        // batches kept apart so that the compiler does not hoist all 64 loads.
+        const float wdiag = si[0] == sj[0] ? 0.5f : 1.f;
+        const float keep = has[0] ? 1.f : 0.f;
 #pragma unroll
-            for (int e = 0; e < 8; e++) ci[e] = s_rinfo[sj[h] * ST + cg * 8 + e];
-#pragma unroll 1
-            for (int tb = 0; tb < 8; tb += 4) {
-                double dv[4][8];
-                uint32_t okbits = 0;
+        for (int b2 = 0; b2 < 8; b2 += 2) {
+            double dv[2][8];
+            uint32_t okbits = 0;
 #pragma unroll
-                for (int r4 = 0; r4 < 4; r4++) {
-                    const int rinf = s_rinfo[si[h] * ST + (tb + r4) * 8 + rr];
-                    const int ra = rinf >> 5, ri = rinf & 31;
+            for (int r2 = 0; r2 < 2; r2++) {
+                const int rt = (b2 + r2) >> 1, ks = (b2 + r2) & 1;
+                const int rinf = s_rinfo[si[0] * ST + 16 * rt + l16];
+                const int ra = rinf >> 5, ri = rinf & 31;
 #pragma unroll
-                    for (int e = 0; e < 8; e++) {
-                        const int cb = ci[e] >> 5, cj = ci[e] & 31;
-                        const bool valid = rinf >= 0 && ci[e] >= 0;
-                        const SlotPair P = s_sp[valid ? min(ra, cb) * B.nslot + max(ra, cb) : 0];
-                        const bool sw = ra > cb;  // lower triangle of a diagonal super-tile: W is symmetric
-                        const int i = sw ? cj : ri, j = sw ? ri : cj;
-                        const bool ok = valid && P.hoff >= 0;
-                        const int ld = P.meta & 63;
-                        const int x1 = (P.meta >> 6) ? j * ld + i : i * ld + j;
-                        dv[r4][e] = Deff[ok ? P.hoff + x1 : 0];  // unconditional: loads issue back to back
-                        okbits |= (ok ? 1u : 0u) << (r4 * 8 + e);
-                    }
-                }
-#pragma unroll
-                for (int r4 = 0; r4 < 4; r4++) {
-                    u16x8 wh8, wl8;
-#pragma unroll
-                    for (int e = 0; e < 8; e++) {
-                        // D is finite, so masking by a product keeps the load unconditional
-                        const float w = (float)dv[r4][e] * (((okbits >> (r4 * 8 + e)) & 1u) ? wdiag : 0.f);
-                        const __bf16 hb = (__bf16)w;
-                        wh8[e] = __builtin_bit_cast(uint16_t, hb);
-                        wl8[e] = __builtin_bit_cast(uint16_t, (__bf16)(w - (float)hb));
-                    }
-                    const int row = (tb + r4) * 8 + rr;
-                    const int frag = (row / 16) * 2 + cg / 4, at = ((cg % 4) * 16 + row % 16) * 8;
-                    *reinterpret_cast<u16x8 *>(&img[(frag * 2 + 0) * FRAG + at]) = wh8;
-                    *reinterpret_cast<u16x8 *>(&img[(frag * 2 + 1) * FRAG + at]) = wl8;
+                for (int e = 0; e < 8; e++) {
+                    const int cinf = s_rinfo[sj[0] * ST + 32 * ks + 8 * q + e];
+                    const int cb = cinf >> 5, cj = cinf & 31;
+                    const bool valid = rinf >= 0 && cinf >= 0;
+                    const SlotPair P = s_sp[valid ? min(ra, cb) * B.nslot + max(ra, cb) : 0];
+                    const bool sw = ra > cb;  // lower triangle of a diagonal super-tile: W is symmetric
+                    const int i = sw ? cj : ri, j = sw ? ri : cj;
+                    const bool ok = valid && P.hoff >= 0;
+                    const int ld = P.meta & 63;
+                    const int x1 = (P.meta >> 6) ? j * ld + i : i * ld + j;
+                    dv[r2][e] = Deff[ok ? P.hoff + x1 : 0];  // unconditional: loads issue back to back
+                    okbits |= (ok ? 1u : 0u) << (r2 * 8 + e);
                 }
             }
-        }
-        __syncthreads();
-        if (has[h]) {
-            const u16x8 *im = reinterpret_cast<const u16x8 *>(img);
 #pragma unroll
-            for (int rt = 0; rt < 4; rt++)
+            for (int r2 = 0; r2 < 2; r2++) {
+                const int rt = (b2 + r2) >> 1, ks = (b2 + r2) & 1;
+                u16x8 wh8, wl8;
 #pragma unroll
-                for (int ks = 0; ks < 2; ks++) {
-                    wh[h][rt][ks] = __builtin_bit_cast(bf16x8, im[((rt * 2 + ks) * 2 + 0) * 64 + lane]);
-                    wl[h][rt][ks] = __builtin_bit_cast(bf16x8, im[((rt * 2 + ks) * 2 + 1) * 64 + lane]);
+                for (int e = 0; e < 8; e++) {
+                    // D is finite, so masking by a product keeps the load unconditional
+                    const float w = (float)dv[r2][e] * (((okbits >> (r2 * 8 + e)) & 1u) ? wdiag * keep : 0.f);
+                    const __bf16 hb = (__bf16)w;
+                    wh8[e] = __builtin_bit_cast(uint16_t, hb);
+                    wl8[e] = __builtin_bit_cast(uint16_t, (__bf16)(w - (float)hb));
                 }
+                wh[0][rt][ks] = __builtin_bit_cast(bf16x8, wh8);
+                wl[0][rt][ks] = __builtin_bit_cast(bf16x8, wl8);
+            }
+            __builtin_amdgcn_sched_barrier(0);
         }
-        __syncthreads();
+    } else {
+        // two super-tiles per wave: W of each is gathered into a per-wave LDS
+        // image laid out as the A-operand fragments [rt * 2 + ks][hi, lo][lane][8]
+        // and read back into registers, one super-tile at a time (the image of
+        // the second one is built while the first one's fragments are live).
+        // lane = (column group cg = lane / 8, row rr = lane % 8): the lane
+        // fills W[8 t8 + rr][8 cg .. 8 cg + 7]; 32 loads per batch.
+        static_assert(W * 8 * 2 * FRAG * 2 <= RB_IMAGE_BYTES, "W images exceed their LDS area");
+        uint16_t *img = reinterpret_cast<uint16_t *>(lds_raw) + wave * (8 * 2 * FRAG);
+        const int rr = lane & 7, cg = lane >> 3;
+#pragma unroll
+        for (int h = 0; h < PER; h++) {
+            if (has[h]) {
+                const float wdiag = si[h] == sj[h] ? 0.5f : 1.f;
+                int ci[8];
+#pragma unroll
+                for (int e = 0; e < 8; e++) ci[e] = s_rinfo[sj[h] * ST + cg * 8 + e];
+#pragma unroll 1
+                for (int tb = 0; tb < 8; tb += 4) {
+                    double dv[4][8];
+                    uint32_t okbits = 0;
+#pragma unroll
+                    for (int r4 = 0; r4 < 4; r4++) {
+                        const int rinf = s_rinfo[si[h] * ST + (tb + r4) * 8 + rr];
+                        const int ra = rinf >> 5, ri = rinf & 31;
+#pragma unroll
+                        for (int e = 0; e < 8; e++) {
+                            const int cb = ci[e] >> 5, cj = ci[e] & 31;
+                            const bool valid = rinf >= 0 && ci[e] >= 0;
+                            const SlotPair P = s_sp[valid ? min(ra, cb) * B.nslot + max(ra, cb) : 0];
+                            const bool sw = ra > cb;
+                            const int i = sw ? cj : ri, j = sw ? ri : cj;
+                            const bool ok = valid && P.hoff >= 0;
+                            const int ld = P.meta & 63;
+                            const int x1 = (P.meta >> 6) ? j * ld + i : i * ld + j;
+                            dv[r4][e] = Deff[ok ? P.hoff + x1 : 0];
+                            okbits |= (ok ? 1u : 0u) << (r4 * 8 + e);
+                        }
+                    }
+#pragma unroll
+                    for (int r4 = 0; r4 < 4; r4++) {
+                        u16x8 wh8, wl8;
+#pragma unroll
+                        for (int e = 0; e < 8; e++) {
+                            const float w = (float)dv[r4][e] * (((okbits >> (r4 * 8 + e)) & 1u) ? wdiag : 0.f);
+                            const __bf16 hb = (__bf16)w;
+                            wh8[e] = __builtin_bit_cast(uint16_t, hb);
+                            wl8[e] = __builtin_bit_cast(uint16_t, (__bf16)(w - (float)hb));
+                        }
+                        const int row = (tb + r4) * 8 + rr;
+                        const int frag = (row / 16) * 2 + cg / 4, at = ((cg % 4) * 16 + row % 16) * 8;
+                        *reinterpret_cast<u16x8 *>(&img[(frag * 2 + 0) * FRAG + at]) = wh8;
+                        *reinterpret_cast<u16x8 *>(&img[(frag * 2 + 1) * FRAG + at]) = wl8;
+                    }
+                }
+            }
+            __syncthreads();
+            if (has[h]) {
+                const u16x8 *im = reinterpret_cast<const u16x8 *>(img);
+#pragma unroll
+                for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+                    for (int ks = 0; ks < 2; ks++) {
+                        wh[h][rt][ks] = __builtin_bit_cast(bf16x8, im[((rt * 2 + ks) * 2 + 0) * 64 + lane]);
+                        wl[h][rt][ks] = __builtin_bit_cast(bf16x8, im[((rt * 2 + ks) * 2 + 1) * 64 + lane]);
+                    }
+            }
+            __syncthreads();
+        }
     }
     u16x8 fh[NF], fl[NF];
     auto prefetch = [&](int c) {
@@ -1022,30 +1109,30 @@ __device__ __forceinline__ void rblk_item(const HItem it, const BlockArgs &A, un
         const uint16_t *sl = tlo + B.tile_off + (int64_t)c * G * FRAG;
 #pragma unroll
         for (int f = 0; f < NF; f++) {  // unconditional (clamped) loads: see touch_chunk
-            const int g = min(wave + f * HB_WAVES, G - 1);
+            const int g = min(wave + f * W, G - 1);
             fh[f] = *reinterpret_cast<const u16x8 *>(sh + (g * 64 + lane) * 8);
             fl[f] = *reinterpret_cast<const u16x8 *>(sl + (g * 64 + lane) * 8);
         }
     };
-    auto step = [&](int c, Touch &slot) {
+    auto step = [&](int c, TouchN<64 * W> &slot) {
         __syncthreads();
 #pragma unroll
         for (int f = 0; f < NF; f++) {  // branch-free, as in k_hblk_bf
-            const int g = min(wave + f * HB_WAVES, G - 1);
+            const int g = min(wave + f * W, G - 1);
             TH[g * 64 + lane] = fh[f];
             TL[g * 64 + lane] = fl[f];
         }
-        release(slot);
+        release_n(slot);
         __syncthreads();
         prefetch(min(c + 1, NCHUNK - 1));
-        slot = touch_chunk(thi, tlo, B.tile_off + (int64_t)min(c + 2, NCHUNK - 1) * G * FRAG, R, tid);
-        // per 16-point group: a lane's partial sum over both super-tiles; the
+        slot = touch_chunk_n<64 * W>(thi, tlo, B.tile_off + (int64_t)min(c + 2, NCHUNK - 1) * G * FRAG, R, tid);
+        // per 16-point group: a lane's partial sum over its super-tiles; the
         // four lanes of a point then add into its LDS sum (no shuffles)
 #pragma unroll 1
         for (int pt = 0; pt < 2; pt++) {
             float part = 0.f;
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
+            for (int h = 0; h < PER; h++) {
                 if (!has[h]) continue;
                 bf16x8 bh[2], bl[2];
 #pragma unroll
@@ -1092,23 +1179,32 @@ __device__ __forceinline__ void rblk_item(const HItem it, const BlockArgs &A, un
         step(c + 1, t1);
     }
     __syncthreads();
-    {
+    if (tid < BLK_PTS) {
         const int k = tid;
         float sum = 0.f;
 #pragma unroll
-        for (int w = 0; w < HB_WAVES; w++) sum += rcw[w * BLK_PTS + k];
+        for (int w = 0; w < W; w++) sum += rcw[w * BLK_PTS + k];
         const int x1 = B.g1 + k / (BLK * BLK), x2 = B.g2 + (k / BLK) % BLK, x3 = B.g3 + k % BLK;
         if (x1 < n1 && x2 < n2 && x3 < n3) atomicAdd(&rho[((int64_t)x1 * n2 + x2) * n3 + x3], (double)sum);
     }
 }
 
+// fragments per wave and plane of the density kernel for R rows
+inline int nfrag_rho(int rows) { return (rows / 16 + RB_WAVES - 1) / RB_WAVES; }
+constexpr int MAXF_RHO = (MAX_ROWS / 16 + RB_WAVES - 1) / RB_WAVES;
+
+// waves of the density workgroup: 16 for one structure; the batched
+// (persistent) form keeps 8, whose loop state would not fit 128 VGPRs
+template <bool BATCH>
+constexpr int rb_waves() { return BATCH ? HB_WAVES : RB_WAVES; }
+
 template <int NF, bool BATCH>
-__global__ __launch_bounds__(64 * HB_WAVES) void k_rblk_bf(const HItem *__restrict__ items, int32_t nitems,
-                                                          const BlockArgs *__restrict__ args, BlockArgs one,
-                                                          int32_t *__restrict__ queue) {
+__global__ __launch_bounds__(64 * rb_waves<BATCH>()) void k_rblk_bf(const HItem *__restrict__ items, int32_t nitems,
+                                                                   const BlockArgs *__restrict__ args, BlockArgs one,
+                                                                   int32_t *__restrict__ queue) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     run_items<BATCH>(items, nitems, args, one, queue,
-              [&](const HItem &it, const BlockArgs &A) { rblk_item<NF>(it, A, lds_raw); });
+              [&](const HItem &it, const BlockArgs &A) { rblk_item<NF, rb_waves<BATCH>()>(it, A, lds_raw); });
 }
 
 BlockArgs block_args(const lcpao_ctx *ctx, const double *V, double *rho) {
@@ -1123,30 +1219,48 @@ BlockArgs block_args(const lcpao_ctx *ctx, const double *V, double *rho) {
     return a;
 }
 
-size_t block_kernel_lds(bool density, int rows, int max_slots) {
-    // density: W images, then chunk planes and point sums, share the first
-    // RB_IMAGE_BYTES; the block's row descriptors and slot pairs follow
-    static_assert(MAX_ROWS * RB_BYTES_PER_ROW <= RB_IMAGE_BYTES, "chunk planes exceed the image area");
+size_t block_kernel_lds(bool density, int rows, int max_slots, bool batched) {
+    // density: chunk planes and point sums (the 8-wave form: W images first),
+    // then the block's row descriptors and slot pairs
     if (density)
-        return RB_IMAGE_BYTES + MAX_ROWS * sizeof(int16_t) + (size_t)max_slots * max_slots * sizeof(SlotPair);
+        return (batched ? RB_IMAGE_BYTES : RB_AREA_BYTES) + MAX_ROWS * sizeof(int16_t) +
+               (size_t)max_slots * max_slots * sizeof(SlotPair);
     return (size_t)4 * rows * CHUNK * sizeof(uint16_t) + BLK_PTS * sizeof(float);
 }
 
 hipError_t launch_block_kernel(bool density, hipStream_t stream, int rows, int max_slots, const HItem *items,
                                int32_t nitems, const BlockArgs *args, const BlockArgs &one, int32_t *queue,
                                unsigned grid) {
-    const size_t lds = block_kernel_lds(density, rows, max_slots);
+    const size_t lds = block_kernel_lds(density, rows, max_slots, args != nullptr);
     void (*kern)(const HItem *, int32_t, const BlockArgs *, BlockArgs, int32_t *) = nullptr;
     const bool b = args != nullptr;  // structures indexed by item (batched launch)
-#define PICK(NF) kern = density ? (b ? k_rblk_bf<NF, true> : k_rblk_bf<NF, false>) : (b ? k_hblk_bf<NF, true> : k_hblk_bf<NF, false>)
-    switch (nfrag(rows)) {
-        case 1: PICK(1); break;
-        case 2: PICK(2); break;
-        case 3: PICK(3); break;
-        case 4: PICK(4); break;
-        default: PICK(MAXF); break;
-    }
+    if (density && !b) {
+        static_assert(MAXF_RHO <= 3, "density prefetch depths instantiated: 1 .. 3");
+        switch (nfrag_rho(rows)) {
+            case 1: kern = k_rblk_bf<1, false>; break;
+            case 2: kern = k_rblk_bf<2, false>; break;
+            default: kern = k_rblk_bf<3, false>; break;
+        }
+    } else if (density) {
+        switch (nfrag(rows)) {
+            case 1: kern = k_rblk_bf<1, true>; break;
+            case 2: kern = k_rblk_bf<2, true>; break;
+            case 3: kern = k_rblk_bf<3, true>; break;
+            case 4: kern = k_rblk_bf<4, true>; break;
+            default: kern = k_rblk_bf<MAXF, true>; break;
+        }
+    } else {
+#define PICK(NF) kern = b ? k_hblk_bf<NF, true> : k_hblk_bf<NF, false>
+        switch (nfrag(rows)) {
+            case 1: PICK(1); break;
+            case 2: PICK(2); break;
+            case 3: PICK(3); break;
+            case 4: PICK(4); break;
+            default: PICK(MAXF); break;
+        }
 #undef PICK
+    }
+    const int threads = 64 * (density && !b ? RB_WAVES : HB_WAVES);
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lds);
     if (e != hipSuccess) return e;
@@ -1156,11 +1270,11 @@ hipError_t launch_block_kernel(bool density, hipStream_t stream, int rows, int m
         if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
         if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
         if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(kern),
-                                                              64 * HB_WAVES, lds)) != hipSuccess)
+                                                              threads, lds)) != hipSuccess)
             return e;
         grid = (unsigned)std::min<int64_t>(nitems, (int64_t)ncu * std::max(1, per_cu));
     }
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * HB_WAVES), lds, stream, items, nitems, args, one, queue);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), lds, stream, items, nitems, args, one, queue);
     return hipGetLastError();
 }
 

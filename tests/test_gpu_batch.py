@@ -2,10 +2,13 @@
 launches and their graph replays reproduce the per-structure form.
 
 Each structure's H comes from the same per-item partials and the same
-reduction order, so it must match the per-structure call bit for bit; the
-density adds one block sum per item into each grid point, identical too
-unless a block has three or more items (then the atomic order may differ in
-the last bit), so it is held to 1e-13 of the largest entry."""
+reduction order, so it must match the per-structure call bit for bit.  The
+density kernels differ: the batched form keeps two super-tiles per wave (8
+waves), the per-structure form one (16 waves), so each point's FP32 partial
+sums associate differently; the batch is held to 2e-6 of the largest entry
+against the per-structure form (the form's error against FP64 is ~1e-5) and,
+between graph replays of the same kernels, to 1e-13 (only the order of the
+per-item atomic adds can change)."""
 import numpy as np
 import pytest
 import torch
@@ -39,11 +42,15 @@ def small_batch():
     return out, ref
 
 
-def _check(out, ref):
+def _check(out, ref, rho_bound=2e-6):
     for s, (H0, R0) in zip(out, ref):
         H, R = s["H"].cpu().numpy(), s["rho"].cpu().numpy()
         assert np.array_equal(H, H0), s["st"].name
-        assert np.abs(R - R0).max() <= 1e-13 * np.abs(R0).max(), s["st"].name
+        assert np.abs(R - R0).max() <= rho_bound * np.abs(R0).max(), s["st"].name
+
+
+def _snapshot(out):
+    return [(s["H"].cpu().numpy().copy(), s["rho"].cpu().numpy().copy()) for s in out]
 
 
 def _run(b, out, use_graph):
@@ -72,9 +79,11 @@ def test_batch_graph_replays(small_batch):
     b = Batch([s["ctx"] for s in out])
     _run(b, out, use_graph=False)  # first call: tunes the schedule
     assert b.tuned()[0] in (0, 1)
+    _check(out, ref)
+    first = _snapshot(out)
     for rep in range(3):
         _run(b, out, use_graph=True)
-        _check(out, ref)
+        _check(out, first, rho_bound=1e-13)
         scratch = torch.empty_like(out[0]["H"])
         out[0]["ctx"].hamiltonian_dev([out[0]["V"]], [scratch], "bf16x3")
         torch.cuda.synchronize()
