@@ -91,7 +91,7 @@ int enqueue(lcpao_batch *b, const double *const *DM, double *const *H, double *c
         lcpao_ctx *c = b->ctx[s];
         hipStream_t keep = c->stream;
         c->stream = b->stream;
-        if (c->ncanon > 0) lcpao::launch_deff_f64(c, DM[s], 1);
+        if (c->ncanon > 0) lcpao::launch_deff_f32(c, DM[s], 1);
         c->stream = keep;
         lcpao::launch_fill_f64(b->stream, c->nn, rho[s], 0.0);
     }

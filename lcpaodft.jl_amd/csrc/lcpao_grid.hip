@@ -1154,6 +1154,10 @@ void lcpao::launch_deff_f64(lcpao_ctx *ctx, const double *D, int sym_self) {
     hipLaunchKernelGGL(k_deff<double>, dim3((unsigned)ctx->ncanon), dim3(128), 0, ctx->stream, ctx->d_hcanon, ctx->ncanon,
                        ctx->d_pairs, D, ctx->d_deff, sym_self);
 }
+void lcpao::launch_deff_f32(lcpao_ctx *ctx, const double *D, int sym_self) {
+    hipLaunchKernelGGL(k_deff<float>, dim3((unsigned)ctx->ncanon), dim3(128), 0, ctx->stream, ctx->d_hcanon, ctx->ncanon,
+                       ctx->d_pairs, D, ctx->d_deff32, sym_self);
+}
 
 namespace {
 

@@ -795,6 +795,7 @@ int split_tables(lcpao_ctx *ctx, int precision) {
     BlockPlan *P = ctx->plan;
     const bool bf = precision == LCPAO_BF16X3;
     if (bf ? P->bf16_ready : P->f32_ready) return LCPAO_OK;
+    if (!ctx->d_deff32) SPLIT_TRY(ctx, hipMalloc(&ctx->d_deff32, std::max<int64_t>(1, ctx->my_hsize) * sizeof(float)));
     if (bf) {
         if (!P->d_hi) {
             const size_t bytes = std::max<int64_t>(1, P->tile_elems) * sizeof(uint16_t);
@@ -956,7 +957,7 @@ __device__ __forceinline__ void rblk_item(const HItem it, const BlockArgs &A, un
     const uint16_t *__restrict__ tlo = A.tlo;
     const int16_t *__restrict__ rowinfo = A.rowinfo;
     const SlotPair *__restrict__ spair = A.spair;
-    const double *__restrict__ Deff = A.Deff;
+    const float *__restrict__ Deff = A.Deff;
     double *__restrict__ rho = A.rho;
     const int n1 = A.n1, n2 = A.n2, n3 = A.n3;
     const int R = B.rows, S = R / ST, G = R / 16, G32 = R / 32;
@@ -988,16 +989,17 @@ __device__ __forceinline__ void rblk_item(const HItem it, const BlockArgs &A, un
         // W gathered straight into the A-operand fragments: fragment (rt, ks)
         // of lane l holds W[16 rt + l % 16][32 ks + 8 (l / 16) + e], e < 8.
         // W element = mask * Deff[one index] (on-site blocks hold D + D^T, so
-        // every element is a single load); 16 loads in flight per batch, the
+        // every element is a single load); 32 loads in flight per batch, the
         // batches kept apart so that the compiler does not hoist all 64 loads.
+        constexpr int GB = 4;
         const float wdiag = si[0] == sj[0] ? 0.5f : 1.f;
         const float keep = has[0] ? 1.f : 0.f;
 #pragma unroll
-        for (int b2 = 0; b2 < 8; b2 += 2) {
-            double dv[2][8];
+        for (int b2 = 0; b2 < 8; b2 += GB) {
+            float dv[GB][8];
             uint32_t okbits = 0;
 #pragma unroll
-            for (int r2 = 0; r2 < 2; r2++) {
+            for (int r2 = 0; r2 < GB; r2++) {
                 const int rt = (b2 + r2) >> 1, ks = (b2 + r2) & 1;
                 const int rinf = s_rinfo[si[0] * ST + 16 * rt + l16];
                 const int ra = rinf >> 5, ri = rinf & 31;
@@ -1017,13 +1019,13 @@ __device__ __forceinline__ void rblk_item(const HItem it, const BlockArgs &A, un
                 }
             }
 #pragma unroll
-            for (int r2 = 0; r2 < 2; r2++) {
+            for (int r2 = 0; r2 < GB; r2++) {
                 const int rt = (b2 + r2) >> 1, ks = (b2 + r2) & 1;
                 u16x8 wh8, wl8;
 #pragma unroll
                 for (int e = 0; e < 8; e++) {
                     // D is finite, so masking by a product keeps the load unconditional
-                    const float w = (float)dv[r2][e] * (((okbits >> (r2 * 8 + e)) & 1u) ? wdiag * keep : 0.f);
+                    const float w = dv[r2][e] * (((okbits >> (r2 * 8 + e)) & 1u) ? wdiag * keep : 0.f);
                     const __bf16 hb = (__bf16)w;
                     wh8[e] = __builtin_bit_cast(uint16_t, hb);
                     wl8[e] = __builtin_bit_cast(uint16_t, (__bf16)(w - (float)hb));
@@ -1052,7 +1054,7 @@ __device__ __forceinline__ void rblk_item(const HItem it, const BlockArgs &A, un
                 for (int e = 0; e < 8; e++) ci[e] = s_rinfo[sj[h] * ST + cg * 8 + e];
 #pragma unroll 1
                 for (int tb = 0; tb < 8; tb += 4) {
-                    double dv[4][8];
+                    float dv[4][8];
                     uint32_t okbits = 0;
 #pragma unroll
                     for (int r4 = 0; r4 < 4; r4++) {
@@ -1077,7 +1079,7 @@ __device__ __forceinline__ void rblk_item(const HItem it, const BlockArgs &A, un
                         u16x8 wh8, wl8;
 #pragma unroll
                         for (int e = 0; e < 8; e++) {
-                            const float w = (float)dv[r4][e] * (((okbits >> (r4 * 8 + e)) & 1u) ? wdiag : 0.f);
+                            const float w = dv[r4][e] * (((okbits >> (r4 * 8 + e)) & 1u) ? wdiag : 0.f);
                             const __bf16 hb = (__bf16)w;
                             wh8[e] = __builtin_bit_cast(uint16_t, hb);
                             wl8[e] = __builtin_bit_cast(uint16_t, (__bf16)(w - (float)hb));
@@ -1104,12 +1106,19 @@ __device__ __forceinline__ void rblk_item(const HItem it, const BlockArgs &A, un
         }
     }
     u16x8 fh[NF], fl[NF];
+    // fragment f of this wave; past the last one a wave reloads (and restages)
+    // its first fragment, which its L1 already holds, rather than all such
+    // waves fetching the last fragment together
+    auto frag_of = [&](int f) {
+        const int g = wave + f * W;
+        return g < G ? g : min(wave, G - 1);
+    };
     auto prefetch = [&](int c) {
         const uint16_t *sh = thi + B.tile_off + (int64_t)c * G * FRAG;
         const uint16_t *sl = tlo + B.tile_off + (int64_t)c * G * FRAG;
 #pragma unroll
-        for (int f = 0; f < NF; f++) {  // unconditional (clamped) loads: see touch_chunk
-            const int g = min(wave + f * W, G - 1);
+        for (int f = 0; f < NF; f++) {  // unconditional loads: see touch_chunk
+            const int g = frag_of(f);
             fh[f] = *reinterpret_cast<const u16x8 *>(sh + (g * 64 + lane) * 8);
             fl[f] = *reinterpret_cast<const u16x8 *>(sl + (g * 64 + lane) * 8);
         }
@@ -1118,7 +1127,7 @@ __device__ __forceinline__ void rblk_item(const HItem it, const BlockArgs &A, un
         __syncthreads();
 #pragma unroll
         for (int f = 0; f < NF; f++) {  // branch-free, as in k_hblk_bf
-            const int g = min(wave + f * W, G - 1);
+            const int g = frag_of(f);
             TH[g * 64 + lane] = fh[f];
             TL[g * 64 + lane] = fl[f];
         }
@@ -1213,7 +1222,7 @@ BlockArgs block_args(const lcpao_ctx *ctx, const double *V, double *rho) {
     a.blocks = P->d_blocks;
     a.hi = P->d_hi; a.lo = P->d_lo; a.thi = P->d_thi; a.tlo = P->d_tlo;
     a.rowinfo = P->d_rowinfo; a.spair = P->d_spair;
-    a.V = V; a.Deff = ctx->d_deff; a.part = P->d_part; a.rho = rho;
+    a.V = V; a.Deff = ctx->d_deff32; a.part = P->d_part; a.rho = rho;
     a.cbase = P->d_cbase; a.slotno = P->d_slotno;
     a.n1 = ctx->ngrid[0]; a.n2 = ctx->ngrid[1]; a.n3 = ctx->ngrid[2];
     return a;
@@ -1287,7 +1296,7 @@ int split_density(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t 
         const double *D = d_DM[s] + dm_offset;
         if (ctx->ncanon > 0) {
             KTimer kt(ctx, KID_DEFF);
-            launch_deff_f64(ctx, D, 1);
+            launch_deff_f32(ctx, D, 1);
         }
         if (ctx->nn > 0)
             hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((ctx->nn + 255) / 256)), dim3(256), 0, ctx->stream, ctx->nn,

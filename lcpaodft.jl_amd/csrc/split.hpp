@@ -63,7 +63,7 @@ struct BlockArgs {
     const int16_t *rowinfo;
     const SlotPair *spair;
     const double *V;               // H: potential of the channel
-    const double *Deff;            // density: folded pair blocks
+    const float *Deff;             // density: folded pair blocks (FP32)
     float *part;                   // H: super-tile partials
     double *rho;                   // density: output
     const int32_t *cbase;          // H: compact partial slot of each slot pair (-1: none)
@@ -140,5 +140,6 @@ void invalidate_split_tables(lcpao_ctx *ctx);
 void launch_to_f32(hipStream_t stream, int64_t n, const double *x, float *y);
 // folded pair blocks into ctx->d_deff (sym_self: on-site blocks as D + D^T)
 void launch_deff_f64(lcpao_ctx *ctx, const double *D, int sym_self);
+void launch_deff_f32(lcpao_ctx *ctx, const double *D, int sym_self);  // into d_deff32
 
 }  // namespace lcpao
