@@ -661,6 +661,29 @@ int plan_build(lcpao_ctx *ctx) {
         if (P->classes.empty() || P->classes.back().rows != R) P->classes.push_back({R, (int32_t)i, 0});
         P->classes.back().count++;
     }
+    // XCD-aware order for the one-workgroup-per-item launches: workgroup i
+    // runs on XCD i % 8 (round-robin dispatch), so the items are dealt from 8
+    // slabs of blocks along n1 -- slab x fills positions x, x + 8, ... -- and
+    // within a slab they follow the block order.  The items that run together
+    // on one XCD are then neighbouring blocks whose density gathers read the
+    // same folded pair blocks through that XCD's L2.
+    {
+        constexpr int NXCD = 8;
+        std::vector<std::vector<HItem>> slab(NXCD);
+        std::vector<HItem> sorted = P->items;
+        std::stable_sort(sorted.begin(), sorted.end(), [](const HItem &x, const HItem &y) {
+            return x.block != y.block ? x.block < y.block : x.st0 < y.st0;
+        });
+        for (const HItem &it : sorted) {
+            const int64_t g1 = P->blocks[it.block].g1;
+            slab[std::min<int64_t>(NXCD - 1, g1 * NXCD / std::max(1, ctx->ngrid[0]))].push_back(it);
+        }
+        size_t k = 0;
+        std::vector<size_t> pos(NXCD, 0);
+        while (k < P->items.size())
+            for (int x = 0; x < NXCD; x++)
+                if (pos[x] < slab[x].size()) P->items[k++] = slab[x][pos[x]++];
+    }
     // row descriptors: slot * 32 + orbital (-1 for padding rows)
     std::vector<int16_t> rowinfo(std::max<int64_t>(1, roff), -1);
     for (int64_t b = 0; b < NB; b++)
