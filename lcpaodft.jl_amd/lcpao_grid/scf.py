@@ -9,7 +9,8 @@ between runs only through the precision of the grid assembly (H and rho):
   computed once in FP64;
 * overlap S_ij = GridVol sum_p phi_i phi_j (the contraction with V = 1), FP64;
 * ions: Gaussian charges Z_s (width RC) on the atoms, their potential solved
-  by FFT on the grid; Hartree potential of the electron density by FFT;
+  by FFT on the grid; Hartree potential of the electron density by the
+  library's device Poisson solve (Solve_Poisson!'s inner solver);
   LDA (Slater) exchange;
 * Gamma point only, spin-unpaired, Fermi-Dirac occupations at KT (which
   keeps degenerate levels at the Fermi energy from oscillating), Pulay
@@ -88,6 +89,7 @@ class GammaSCF:
         self.invg2 = torch.from_numpy(inv).to(self.dev)
         self.shape = (N1, N2, N3)
         self.omega = abs(np.linalg.det(lat))
+        self.recvecs = rec
 
     def _ions(self, zval):
         """Potential of Gaussian ion charges: V(G) = -4 pi rho_ion(G) / G^2."""
@@ -107,8 +109,10 @@ class GammaSCF:
         return torch.fft.ifftn(rg * self.g2 / (self.g2 + Q0 * Q0)).real.reshape(-1)
 
     def hartree(self, n):
-        ng = torch.fft.fftn(n.reshape(self.shape))
-        return torch.fft.ifftn(4 * np.pi * ng * self.invg2).real.reshape(-1)
+        """4 pi n(G) / |G|^2 by lcpao_grid_poisson_dev (nspin 0: n as given)."""
+        v = torch.empty_like(n)
+        self.ctx.poisson_dev([n.contiguous()], None, self.recvecs, v, nspin=0)
+        return v
 
     @staticmethod
     def exchange(n):
