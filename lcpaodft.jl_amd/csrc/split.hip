@@ -101,6 +101,22 @@ __device__ __forceinline__ void st_decode(int st, int S, int &si, int &sj) {
     sj = si + st;
 }
 
+// Pointers that a kernel reads from device memory (the batched launches'
+// per-structure arguments) are generic to the compiler, which then addresses
+// them with 64-bit flat VGPR addresses; reading through global-address-space
+// pointers lets it use SGPR bases and global loads, as for kernel arguments.
+// (The host pass parses these device functions too, and its targets have no
+// such address space: there the qualifier is empty.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GLOBAL_AS __attribute__((address_space(1)))
+#else
+#define GLOBAL_AS
+#endif
+template <class T>
+__device__ __forceinline__ GLOBAL_AS T *as_global(T *p) {
+    return (GLOBAL_AS T *)p;
+}
+
 __device__ __forceinline__ float bf_bits_to_f32(uint16_t u) { return __uint_as_float((uint32_t)u << 16); }
 
 // ---- bf16x3 form ------------------------------------------------------------
@@ -122,51 +138,55 @@ constexpr int MAXF = MAX_ROWS / 16 / HB_WAVES;     // fragments per wave and pla
 struct Touch {
     uint32_t a, b;
 };
-__device__ __forceinline__ Touch touch_chunk(const uint16_t *__restrict__ hi, const uint16_t *__restrict__ lo,
-                                             int64_t off, int R, int tid) {
+__device__ __forceinline__ Touch touch_chunk(const GLOBAL_AS uint16_t *__restrict__ hi,
+                                             const GLOBAL_AS uint16_t *__restrict__ lo, int64_t off, int R, int tid) {
     // always exactly two loads per thread (indices past the chunk repeat line
     // 0), so that the compiler can count them when it places its waits
     Touch t;
     const int half = R / 2;
     int x = tid < R ? tid : 0;
-    t.a = *reinterpret_cast<const uint32_t *>((x < half ? hi : lo) + off + (int64_t)(x < half ? x : x - half) * 64);
+    t.a = *reinterpret_cast<const GLOBAL_AS uint32_t *>((x < half ? hi : lo) + off + (int64_t)(x < half ? x : x - half) * 64);
     x = tid + 64 * HB_WAVES < R ? tid + 64 * HB_WAVES : 0;
-    t.b = *reinterpret_cast<const uint32_t *>((x < half ? hi : lo) + off + (int64_t)(x < half ? x : x - half) * 64);
+    t.b = *reinterpret_cast<const GLOBAL_AS uint32_t *>((x < half ? hi : lo) + off + (int64_t)(x < half ? x : x - half) * 64);
     return t;
 }
 __device__ __forceinline__ void release(const Touch &t) { asm volatile("" ::"v"(t.a), "v"(t.b)); }
 
-// Work distribution of the block kernels.  BATCH == false: one structure
-// (`one`), one workgroup per item (grid = item count).  BATCH == true: item.pad
-// selects the structure's entry of `args`; with queue == nullptr one
-// workgroup per item, otherwise the grid is persistent and each workgroup
-// takes the next item index from the queue counter (zeroed before the
-// launch) until the list is exhausted, so every workgroup reaches the exit.
-template <bool BATCH, class F>
+// Work distribution of the block kernels (MODE, a template argument, so that
+// each schedule compiles to a kernel of its own and the one-pass forms carry
+// no loop).  LAUNCH_ONE: one structure (`one`), one workgroup per item (grid =
+// item count).  LAUNCH_BATCH: item.pad selects the structure's entry of
+// `args`, one workgroup per item.  LAUNCH_QUEUE: as LAUNCH_BATCH, but the grid
+// is persistent and each workgroup takes the next item index from the queue
+// counter (zeroed before the launch) until the list is exhausted, so every
+// workgroup reaches the exit.
+enum { LAUNCH_ONE = 0, LAUNCH_BATCH = 1, LAUNCH_QUEUE = 2 };
+template <int MODE, class F>
 __device__ __forceinline__ void run_items(const HItem *__restrict__ items, int32_t nitems,
                                           const BlockArgs *__restrict__ args, const BlockArgs &one,
                                           int32_t *__restrict__ queue, F body) {
-    if constexpr (!BATCH) {
+    if constexpr (MODE == LAUNCH_ONE) {
         if ((int)blockIdx.x < nitems) body(items[blockIdx.x], one);
-        return;
-    }
-    if (!queue) {
+    } else if constexpr (MODE == LAUNCH_BATCH) {
         if ((int)blockIdx.x < nitems) {
             const HItem it = items[blockIdx.x];
             body(it, args[it.pad]);
         }
-        return;
+    } else {
+        __shared__ int32_t s_next;
+        for (;;) {
+            __syncthreads();  // the previous item is done with the LDS and s_next
+            if (threadIdx.x == 0) s_next = atomicAdd(queue, 1);
+            __syncthreads();
+            const int32_t i = s_next;
+            if (i >= nitems) return;
+            const HItem it = items[i];
+            body(it, args[it.pad]);
+        }
     }
-    __shared__ int32_t s_next;
-    for (;;) {
-        __syncthreads();  // the previous item is done with the LDS and s_next
-        if (threadIdx.x == 0) s_next = atomicAdd(queue, 1);
-        __syncthreads();
-        const int32_t i = s_next;
-        if (i >= nitems) return;
-        const HItem it = items[i];
-        body(it, args[it.pad]);
-    }
+}
+inline int launch_mode(const BlockArgs *args, const int32_t *queue) {
+    return args == nullptr ? LAUNCH_ONE : queue == nullptr ? LAUNCH_BATCH : LAUNCH_QUEUE;
 }
 
 // Writes the super-tile accumulators of a wave into the compact pair
@@ -190,9 +210,11 @@ __device__ __forceinline__ EpiTables epi_layout(const BlockDev &B, unsigned char
 __device__ __forceinline__ EpiTables stage_epilogue(const BlockArgs &A, const BlockDev &B, unsigned char *lds) {
     const EpiTables T = epi_layout(B, lds);
     __syncthreads();  // every wave is done with the chunk buffers
-    for (int x = threadIdx.x; x < B.nslot * B.nslot; x += blockDim.x) const_cast<int32_t *>(T.cb)[x] = A.cbase[B.sp_off + x];
-    for (int x = threadIdx.x; x < B.rows; x += blockDim.x) const_cast<int16_t *>(T.rinfo)[x] = A.rowinfo[B.row_off + x];
-    for (int x = threadIdx.x; x < B.nslot; x += blockDim.x) const_cast<int16_t *>(T.no)[x] = A.slotno[B.slot_off + x];
+    const GLOBAL_AS int32_t *cbase = as_global(A.cbase);
+    const GLOBAL_AS int16_t *rowinfo = as_global(A.rowinfo), *slotno = as_global(A.slotno);
+    for (int x = threadIdx.x; x < B.nslot * B.nslot; x += blockDim.x) const_cast<int32_t *>(T.cb)[x] = cbase[B.sp_off + x];
+    for (int x = threadIdx.x; x < B.rows; x += blockDim.x) const_cast<int16_t *>(T.rinfo)[x] = rowinfo[B.row_off + x];
+    for (int x = threadIdx.x; x < B.nslot; x += blockDim.x) const_cast<int16_t *>(T.no)[x] = slotno[B.slot_off + x];
     __syncthreads();
     return T;
 }
@@ -207,17 +229,19 @@ struct EpiRegs {
 };
 __device__ __forceinline__ void epi_fetch(EpiRegs &E, const BlockArgs &A, const BlockDev &B, int tid) {
     const int nsp = B.nslot * B.nslot;
+    const GLOBAL_AS int32_t *cbase = as_global(A.cbase);
+    const GLOBAL_AS int16_t *rowinfo = as_global(A.rowinfo), *slotno = as_global(A.slotno);
 #pragma unroll
     for (int k = 0; k < EPI_SP; k++) {
         const int x = tid + k * 64 * HB_WAVES;
-        E.cb[k] = x < nsp ? A.cbase[B.sp_off + x] : -1;
+        E.cb[k] = x < nsp ? cbase[B.sp_off + x] : -1;
     }
 #pragma unroll
     for (int k = 0; k < EPI_ROWS; k++) {
         const int x = tid + k * 64 * HB_WAVES;
-        E.ri[k] = x < B.rows ? A.rowinfo[B.row_off + x] : -1;
+        E.ri[k] = x < B.rows ? rowinfo[B.row_off + x] : -1;
     }
-    E.no = tid < B.nslot ? A.slotno[B.slot_off + tid] : 0;
+    E.no = tid < B.nslot ? slotno[B.slot_off + tid] : 0;
 }
 __device__ __forceinline__ EpiTables epi_store(const EpiRegs &E, const BlockDev &B, unsigned char *lds, int tid) {
     const EpiTables T = epi_layout(B, lds);
@@ -234,7 +258,7 @@ __device__ __forceinline__ EpiTables epi_store(const EpiRegs &E, const BlockDev 
     return T;
 }
 __device__ __forceinline__ void store_partials(const f32x4 *acc, int si, int sj, int lane, const EpiTables &T,
-                                               float *__restrict__ part) {
+                                               GLOBAL_AS float *__restrict__ part) {
     const int l16 = lane & 15, q = lane >> 4;
     // the lane's four columns (one per 16-column tile) and their slots' sizes
     int ci[4], nb[4];
@@ -303,11 +327,11 @@ __device__ __forceinline__ void tile_bf16x3(const u16x8 *__restrict__ PH, const 
 
 template <int NF>
 __device__ __forceinline__ void hblk_item(const HItem it, const BlockArgs &A, unsigned char *lds_raw) {
-    const BlockDev B = A.blocks[it.block];
-    const uint16_t *__restrict__ hi = A.hi;
-    const uint16_t *__restrict__ lo = A.lo;
-    const double *__restrict__ V = A.V;
-    float *__restrict__ part = A.part;
+    const BlockDev B = as_global(A.blocks)[it.block];
+    const GLOBAL_AS uint16_t *__restrict__ hi = as_global(A.hi);
+    const GLOBAL_AS uint16_t *__restrict__ lo = as_global(A.lo);
+    const GLOBAL_AS double *__restrict__ V = as_global(A.V);
+    GLOBAL_AS float *__restrict__ part = as_global(A.part);
     const int n1 = A.n1, n2 = A.n2, n3 = A.n3;
     const int R = B.rows, S = R / ST, G = R / 16;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, q = lane >> 4;
@@ -335,13 +359,13 @@ __device__ __forceinline__ void hblk_item(const HItem it, const BlockArgs &A, un
     u16x8 fh[NF], fl[NF];
     // chunk c: fragments g = wave + 8 f of both planes
     auto prefetch = [&](int c) {
-        const uint16_t *sh = hi + B.tile_off + (int64_t)c * G * FRAG;
-        const uint16_t *sl = lo + B.tile_off + (int64_t)c * G * FRAG;
+        const GLOBAL_AS uint16_t *sh = hi + B.tile_off + (int64_t)c * G * FRAG;
+        const GLOBAL_AS uint16_t *sl = lo + B.tile_off + (int64_t)c * G * FRAG;
 #pragma unroll
         for (int f = 0; f < NF; f++) {  // unconditional (clamped) loads: see touch_chunk
             const int g = min(wave + f * HB_WAVES, G - 1);
-            fh[f] = *reinterpret_cast<const u16x8 *>(sh + (g * 64 + lane) * 8);
-            fl[f] = *reinterpret_cast<const u16x8 *>(sl + (g * 64 + lane) * 8);
+            fh[f] = *reinterpret_cast<const GLOBAL_AS u16x8 *>(sh + (g * 64 + lane) * 8);
+            fl[f] = *reinterpret_cast<const GLOBAL_AS u16x8 *>(sl + (g * 64 + lane) * 8);
         }
     };
     // chunk c is touched two iterations before it is staged (c - 2), loaded
@@ -394,13 +418,13 @@ __device__ __forceinline__ void hblk_item(const HItem it, const BlockArgs &A, un
     if (has1) store_partials(acc1, si1, sj1, lane, T, part);
 }
 
-template <int NF, bool BATCH>
+template <int NF, int MODE>
 __global__ __launch_bounds__(64 * HB_WAVES) void k_hblk_bf(const HItem *__restrict__ items, int32_t nitems,
                                                           const BlockArgs *__restrict__ args, BlockArgs one,
                                                           int32_t *__restrict__ queue) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    run_items<BATCH>(items, nitems, args, one, queue,
-              [&](const HItem &it, const BlockArgs &A) { hblk_item<NF>(it, A, lds_raw); });
+    run_items<MODE>(items, nitems, args, one, queue,
+                    [&](const HItem &it, const BlockArgs &A) { hblk_item<NF>(it, A, lds_raw); });
 }
 
 // ---- fp32 form --------------------------------------------------------------
@@ -434,7 +458,7 @@ __global__ __launch_bounds__(64 * HB_WAVES) void k_hblk_f32(const HItem *__restr
     const HItem it = items[blockIdx.x];
     const BlockDev B = A.blocks[it.block];
     const double *__restrict__ V = A.V;
-    float *__restrict__ part = A.part;
+    GLOBAL_AS float *__restrict__ part = as_global(A.part);
     const int n1 = A.n1, n2 = A.n2, n3 = A.n3;
     const int R = B.rows, S = R / ST;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, q = lane >> 4;
@@ -952,14 +976,16 @@ struct TouchN {
     uint32_t v[K];
 };
 template <int NT>
-__device__ __forceinline__ TouchN<NT> touch_chunk_n(const uint16_t *__restrict__ hi, const uint16_t *__restrict__ lo,
-                                                    int64_t off, int R, int tid) {
+__device__ __forceinline__ TouchN<NT> touch_chunk_n(const GLOBAL_AS uint16_t *__restrict__ hi,
+                                                    const GLOBAL_AS uint16_t *__restrict__ lo, int64_t off, int R,
+                                                    int tid) {
     TouchN<NT> t;
     const int half = R / 2;
 #pragma unroll
     for (int k = 0; k < TouchN<NT>::K; k++) {
         const int x = tid + k * NT < R ? tid + k * NT : 0;
-        t.v[k] = *reinterpret_cast<const uint32_t *>((x < half ? hi : lo) + off + (int64_t)(x < half ? x : x - half) * 64);
+        t.v[k] = *reinterpret_cast<const GLOBAL_AS uint32_t *>((x < half ? hi : lo) + off +
+                                                               (int64_t)(x < half ? x : x - half) * 64);
     }
     return t;
 }
@@ -975,12 +1001,12 @@ __device__ __forceinline__ void release_n(const TouchN<NT> &t) {
 template <int NF, int W>
 __device__ __forceinline__ void rblk_item(const HItem it, const BlockArgs &A, unsigned char *lds_raw) {
     constexpr int PER = 16 / W;
-    const BlockDev B = A.blocks[it.block];
-    const uint16_t *__restrict__ thi = A.thi;
-    const uint16_t *__restrict__ tlo = A.tlo;
-    const int16_t *__restrict__ rowinfo = A.rowinfo;
-    const SlotPair *__restrict__ spair = A.spair;
-    const float *__restrict__ Deff = A.Deff;
+    const BlockDev B = as_global(A.blocks)[it.block];
+    const GLOBAL_AS uint16_t *__restrict__ thi = as_global(A.thi);
+    const GLOBAL_AS uint16_t *__restrict__ tlo = as_global(A.tlo);
+    const GLOBAL_AS int16_t *__restrict__ rowinfo = as_global(A.rowinfo);
+    const GLOBAL_AS SlotPair *__restrict__ spair = as_global(A.spair);
+    const GLOBAL_AS float *__restrict__ Deff = as_global(A.Deff);
     double *__restrict__ rho = A.rho;
     const int n1 = A.n1, n2 = A.n2, n3 = A.n3;
     const int R = B.rows, S = R / ST, G = R / 16, G32 = R / 32;
@@ -1137,13 +1163,13 @@ __device__ __forceinline__ void rblk_item(const HItem it, const BlockArgs &A, un
         return g < G ? g : min(wave, G - 1);
     };
     auto prefetch = [&](int c) {
-        const uint16_t *sh = thi + B.tile_off + (int64_t)c * G * FRAG;
-        const uint16_t *sl = tlo + B.tile_off + (int64_t)c * G * FRAG;
+        const GLOBAL_AS uint16_t *sh = thi + B.tile_off + (int64_t)c * G * FRAG;
+        const GLOBAL_AS uint16_t *sl = tlo + B.tile_off + (int64_t)c * G * FRAG;
 #pragma unroll
         for (int f = 0; f < NF; f++) {  // unconditional loads: see touch_chunk
             const int g = frag_of(f);
-            fh[f] = *reinterpret_cast<const u16x8 *>(sh + (g * 64 + lane) * 8);
-            fl[f] = *reinterpret_cast<const u16x8 *>(sl + (g * 64 + lane) * 8);
+            fh[f] = *reinterpret_cast<const GLOBAL_AS u16x8 *>(sh + (g * 64 + lane) * 8);
+            fl[f] = *reinterpret_cast<const GLOBAL_AS u16x8 *>(sl + (g * 64 + lane) * 8);
         }
     };
     auto step = [&](int c, TouchN<64 * W> &slot) {
@@ -1227,16 +1253,16 @@ constexpr int MAXF_RHO = (MAX_ROWS / 16 + RB_WAVES - 1) / RB_WAVES;
 
 // waves of the density workgroup: 16 for one structure; the batched
 // (persistent) form keeps 8, whose loop state would not fit 128 VGPRs
-template <bool BATCH>
-constexpr int rb_waves() { return BATCH ? HB_WAVES : RB_WAVES; }
+template <int MODE>
+constexpr int rb_waves() { return MODE == LAUNCH_QUEUE ? HB_WAVES : RB_WAVES; }
 
-template <int NF, bool BATCH>
-__global__ __launch_bounds__(64 * rb_waves<BATCH>()) void k_rblk_bf(const HItem *__restrict__ items, int32_t nitems,
-                                                                   const BlockArgs *__restrict__ args, BlockArgs one,
-                                                                   int32_t *__restrict__ queue) {
+template <int NF, int MODE>
+__global__ __launch_bounds__(64 * rb_waves<MODE>()) void k_rblk_bf(const HItem *__restrict__ items, int32_t nitems,
+                                                                  const BlockArgs *__restrict__ args, BlockArgs one,
+                                                                  int32_t *__restrict__ queue) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    run_items<BATCH>(items, nitems, args, one, queue,
-              [&](const HItem &it, const BlockArgs &A) { rblk_item<NF, rb_waves<BATCH>()>(it, A, lds_raw); });
+    run_items<MODE>(items, nitems, args, one, queue,
+                    [&](const HItem &it, const BlockArgs &A) { rblk_item<NF, rb_waves<MODE>()>(it, A, lds_raw); });
 }
 
 BlockArgs block_args(const lcpao_ctx *ctx, const double *V, double *rho) {
@@ -1251,11 +1277,11 @@ BlockArgs block_args(const lcpao_ctx *ctx, const double *V, double *rho) {
     return a;
 }
 
-size_t block_kernel_lds(bool density, int rows, int max_slots, bool batched) {
+size_t block_kernel_lds(bool density, int rows, int max_slots, bool persistent) {
     // density: chunk planes and point sums (the 8-wave form: W images first),
     // then the block's row descriptors and slot pairs
     if (density)
-        return (batched ? RB_IMAGE_BYTES : RB_AREA_BYTES) + MAX_ROWS * sizeof(int16_t) +
+        return (persistent ? RB_IMAGE_BYTES : RB_AREA_BYTES) + MAX_ROWS * sizeof(int16_t) +
                (size_t)max_slots * max_slots * sizeof(SlotPair);
     return (size_t)4 * rows * CHUNK * sizeof(uint16_t) + BLK_PTS * sizeof(float);
 }
@@ -1263,36 +1289,39 @@ size_t block_kernel_lds(bool density, int rows, int max_slots, bool batched) {
 hipError_t launch_block_kernel(bool density, hipStream_t stream, int rows, int max_slots, const HItem *items,
                                int32_t nitems, const BlockArgs *args, const BlockArgs &one, int32_t *queue,
                                unsigned grid) {
-    const size_t lds = block_kernel_lds(density, rows, max_slots, args != nullptr);
+    const int mode = launch_mode(args, queue);
+    const bool persistent = mode == LAUNCH_QUEUE;
+    const size_t lds = block_kernel_lds(density, rows, max_slots, persistent);
     void (*kern)(const HItem *, int32_t, const BlockArgs *, BlockArgs, int32_t *) = nullptr;
-    const bool b = args != nullptr;  // structures indexed by item (batched launch)
-    if (density && !b) {
+#define BY_MODE(K, NF)                                                                             \
+    kern = mode == LAUNCH_ONE ? K<NF, LAUNCH_ONE> : mode == LAUNCH_BATCH ? K<NF, LAUNCH_BATCH> \
+                                                                         : K<NF, LAUNCH_QUEUE>
+    if (density && !persistent) {
         static_assert(MAXF_RHO <= 3, "density prefetch depths instantiated: 1 .. 3");
         switch (nfrag_rho(rows)) {
-            case 1: kern = k_rblk_bf<1, false>; break;
-            case 2: kern = k_rblk_bf<2, false>; break;
-            default: kern = k_rblk_bf<3, false>; break;
+            case 1: BY_MODE(k_rblk_bf, 1); break;
+            case 2: BY_MODE(k_rblk_bf, 2); break;
+            default: BY_MODE(k_rblk_bf, 3); break;
         }
     } else if (density) {
         switch (nfrag(rows)) {
-            case 1: kern = k_rblk_bf<1, true>; break;
-            case 2: kern = k_rblk_bf<2, true>; break;
-            case 3: kern = k_rblk_bf<3, true>; break;
-            case 4: kern = k_rblk_bf<4, true>; break;
-            default: kern = k_rblk_bf<MAXF, true>; break;
+            case 1: kern = k_rblk_bf<1, LAUNCH_QUEUE>; break;
+            case 2: kern = k_rblk_bf<2, LAUNCH_QUEUE>; break;
+            case 3: kern = k_rblk_bf<3, LAUNCH_QUEUE>; break;
+            case 4: kern = k_rblk_bf<4, LAUNCH_QUEUE>; break;
+            default: kern = k_rblk_bf<MAXF, LAUNCH_QUEUE>; break;
         }
     } else {
-#define PICK(NF) kern = b ? k_hblk_bf<NF, true> : k_hblk_bf<NF, false>
         switch (nfrag(rows)) {
-            case 1: PICK(1); break;
-            case 2: PICK(2); break;
-            case 3: PICK(3); break;
-            case 4: PICK(4); break;
-            default: PICK(MAXF); break;
+            case 1: BY_MODE(k_hblk_bf, 1); break;
+            case 2: BY_MODE(k_hblk_bf, 2); break;
+            case 3: BY_MODE(k_hblk_bf, 3); break;
+            case 4: BY_MODE(k_hblk_bf, 4); break;
+            default: BY_MODE(k_hblk_bf, MAXF); break;
         }
-#undef PICK
     }
-    const int threads = 64 * (density && !b ? RB_WAVES : HB_WAVES);
+#undef BY_MODE
+    const int threads = 64 * (density && !persistent ? RB_WAVES : HB_WAVES);
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lds);
     if (e != hipSuccess) return e;

@@ -122,7 +122,7 @@ struct BlockPlan {
 int plan_build(lcpao_ctx *ctx);
 // bf16x3 block kernels, shared with the batched form
 BlockArgs block_args(const lcpao_ctx *ctx, const double *V, double *rho);
-size_t block_kernel_lds(bool density, int rows, int max_slots, bool batched);
+size_t block_kernel_lds(bool density, int rows, int max_slots, bool persistent);
 void launch_fill_f64(hipStream_t stream, int64_t n, double *x, double v);
 void launch_fill_i32(hipStream_t stream, int64_t n, int32_t *x, int32_t v);
 void launch_hblk_reduce(const lcpao_ctx *ctx, hipStream_t stream, double *H);
