@@ -193,7 +193,8 @@ def bench_force3(args):
     dDM = torch.from_numpy(DM).to(dev)
     stream = torch.cuda.current_stream(dev)
     ctx.set_stream(stream)
-    forms = ["fp32", "fp64"] if args.precision != "fp64" else ["fp64", "fp32"]
+    forms = {"bf16x3": ["bf16x3", "fp32", "fp64"], "fp32": ["fp32", "bf16x3", "fp64"],
+             "fp64": ["fp64", "bf16x3", "fp32"]}[args.precision]
     res, timing = {}, {}
     for f in forms:
         dF = torch.zeros((3, u.Natom), dtype=torch.float64, device=dev)
@@ -229,18 +230,20 @@ def bench_force3(args):
         "metric": "ms per Force3 contraction (dV/dphi force, nospin)", "value": timing[head]["ms"], "unit": "ms",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": timing[head]["ms"],
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32 products, f64 pair sums"}[head],
+        "dtype": {"fp64": "f64", "fp32": "f32 products, f64 pair sums",
+                  "bf16x3": "bf16x3 MFMA, f32 acc, f64 slot sums"}[head],
         "data": "synthetic (seeded plane-wave V_eff, random PSD density matrix)",
         "config": {"workload": "%s: Force3 (1 channel), %d atoms, grid %s, %d pairs, %d overlap points"
                                % (cfg, struct.Natom, "x".join(map(str, struct.Ngrid)), u.MPI_size, u.olg1.size),
                    "precision": head},
         "forms_ms": {f: timing[f]["ms"] for f in forms},
         "kernels": {f: timing[f]["kernels"] for f in forms},
-        "max_abs_dF_vs_fp64_Ha_per_bohr": float(np.abs(res[head] - res["fp64"]).max()),
+        "max_abs_dF_vs_fp64_Ha_per_bohr": {f: float(np.abs(res[f] - res["fp64"]).max()) for f in forms},
         "max_abs_F_Ha_per_bohr": float(np.abs(res["fp64"]).max()),
         "autotune": {"f364": {"variant": ctx.tuned("f364")[0], "candidate_ms": ctx.tuned("f364")[1]}},
         "setup_s": round(setup_s, 2),
-        "roofline": {"bound": "mfma" if head != "fp32" else "valu", "kernel": "k_f3<float>" if head == "fp32" else "k_f3",
+        "roofline": {"bound": "mfma" if head != "fp32" else "valu",
+                     "kernel": {"fp32": "k_f3<float>", "fp64": "k_f3", "bf16x3": "k_fblk_bf"}[head],
                      "achieved": round(flops / kt / 1e12, 3), "peak": round(peak / 1e12, 1), "unit": "TFLOP/s",
                      "frac": round(flops / kt / peak, 4), "traffic": None, "flops_per_launch": flops},
         "cpu_baseline": {"value": None, "reason": "not timed for Force3: the CPU restatement would need the "

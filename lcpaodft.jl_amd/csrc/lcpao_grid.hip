@@ -1253,7 +1253,8 @@ int f3_dev(lcpao_ctx *ctx, int nch, const double *const *d_V, const double *cons
     if (!weight || !d_force) return set_err(ctx, LCPAO_EINVAL, "null weight or force pointer");
     int rc = check_precision(ctx, precision);
     if (rc) return rc;
-    const bool f32 = precision != LCPAO_FP64;
+    const bool bf = precision == LCPAO_BF16X3;
+    const bool f32 = precision == LCPAO_FP32;
     if (f32 && (rc = fp32_tables(ctx, true))) return rc;
     const int nu = (int)ctx->funits.size();
     if (ctx->natom > 0)
@@ -1262,6 +1263,10 @@ int f3_dev(lcpao_ctx *ctx, int nch, const double *const *d_V, const double *cons
     for (int c = 0; c < nch; c++) {
         if (!d_V[c] || !d_DM[c]) return set_err(ctx, LCPAO_EINVAL, "null channel pointer");
         const double *D = d_DM[c] + dm_offset;
+        if (bf) {  // block form on the matrix cores
+            if ((rc = split_force3(ctx, d_V[c], D, weight[c], ctx->d_fsum))) return rc;
+            continue;
+        }
         if (f32) launch_to_f32(ctx->stream, ctx->my_hsize, D, ctx->d_deff32);
         // FP64 candidates: 0 = matrix cores (k_f3_m64), 1 = vector FMA (k_f3<double>)
         auto f364 = [&](int v) {

@@ -27,6 +27,7 @@ typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
 
 constexpr int MAX_ROWS = 576;    // R limit (bf16x3 H: four R x 32 bf16 planes + V in 160 KB of LDS)
 constexpr int LROW32 = 36;       // LDS row stride of fp32 tiles (32 + 4 pad)
+constexpr int FB_WAVES = 8;      // waves of the Force3 workgroup, two super-tiles of one row each
 
 __device__ __forceinline__ int st_index(int si, int sj, int S) { return si * S - (si * (si - 1)) / 2 + (sj - si); }
 
@@ -570,20 +571,45 @@ void cell_ijk3(int cpy, int cell, int *ijk) {
     ijk[2] = lin % W - cpy;
 }
 
+// XCD-aware order for the one-workgroup-per-item launches: workgroup i runs
+// on XCD i % 8 (round-robin dispatch), so the items are dealt from 8 slabs of
+// blocks along n1 -- slab x fills positions x, x + 8, ... -- and within a slab
+// they follow the block order.  The items that run together on one XCD are
+// then neighbouring blocks whose density-matrix gathers read the same pair
+// blocks through that XCD's L2.
+void deal_xcd(const BlockPlan *P, std::vector<HItem> &items, int n1) {
+    constexpr int NXCD = 8;
+    std::vector<std::vector<HItem>> slab(NXCD);
+    std::vector<HItem> sorted = items;
+    std::stable_sort(sorted.begin(), sorted.end(), [](const HItem &x, const HItem &y) {
+        return x.block != y.block ? x.block < y.block : x.st0 < y.st0;
+    });
+    for (const HItem &it : sorted) {
+        const int64_t g1 = P->blocks[it.block].g1;
+        slab[std::min<int64_t>(NXCD - 1, g1 * NXCD / std::max(1, n1))].push_back(it);
+    }
+    size_t k = 0;
+    std::vector<size_t> pos(NXCD, 0);
+    while (k < items.size())
+        for (int x = 0; x < NXCD; x++)
+            if (pos[x] < slab[x].size()) items[k++] = slab[x][pos[x]++];
+}
+
 }  // namespace
 
 void plan_free(BlockPlan *p) {
     if (!p) return;
     void *ptrs[] = {p->d_blocks, p->d_items, p->d_sph_block, p->d_sph_atom, p->d_sph_row0, p->d_sph_map,
                     p->d_contrib, p->d_contrib_off, p->d_hi, p->d_lo, p->d_t32, p->d_part, p->d_thi, p->d_tlo,
-                    p->d_rowinfo, p->d_spair, p->d_contrib_pair, p->d_cbase, p->d_slotno};
+                    p->d_rowinfo, p->d_spair, p->d_contrib_pair, p->d_cbase, p->d_slotno, p->d_fpair,
+                    p->d_fitems, p->d_fbase, p->d_fout, p->d_fatom_off, p->d_fatom_idx, p->d_dtile};
     for (void *x : ptrs)
         if (x) (void)hipFree(x);
     delete p;
 }
 
 void invalidate_split_tables(lcpao_ctx *ctx) {
-    if (ctx->plan) { ctx->plan->bf16_ready = false; ctx->plan->f32_ready = false; }
+    if (ctx->plan) { ctx->plan->bf16_ready = false; ctx->plan->f32_ready = false; ctx->plan->dtile_ready = false; }
     if (ctx->d_tab32) { (void)hipFree(ctx->d_tab32); ctx->d_tab32 = nullptr; }
     for (int x = 0; x < 3; x++)
         if (ctx->d_dtab32[x]) { (void)hipFree(ctx->d_dtab32[x]); ctx->d_dtab32[x] = nullptr; }
@@ -685,29 +711,7 @@ int plan_build(lcpao_ctx *ctx) {
         if (P->classes.empty() || P->classes.back().rows != R) P->classes.push_back({R, (int32_t)i, 0});
         P->classes.back().count++;
     }
-    // XCD-aware order for the one-workgroup-per-item launches: workgroup i
-    // runs on XCD i % 8 (round-robin dispatch), so the items are dealt from 8
-    // slabs of blocks along n1 -- slab x fills positions x, x + 8, ... -- and
-    // within a slab they follow the block order.  The items that run together
-    // on one XCD are then neighbouring blocks whose density gathers read the
-    // same folded pair blocks through that XCD's L2.
-    {
-        constexpr int NXCD = 8;
-        std::vector<std::vector<HItem>> slab(NXCD);
-        std::vector<HItem> sorted = P->items;
-        std::stable_sort(sorted.begin(), sorted.end(), [](const HItem &x, const HItem &y) {
-            return x.block != y.block ? x.block < y.block : x.st0 < y.st0;
-        });
-        for (const HItem &it : sorted) {
-            const int64_t g1 = P->blocks[it.block].g1;
-            slab[std::min<int64_t>(NXCD - 1, g1 * NXCD / std::max(1, ctx->ngrid[0]))].push_back(it);
-        }
-        size_t k = 0;
-        std::vector<size_t> pos(NXCD, 0);
-        while (k < P->items.size())
-            for (int x = 0; x < NXCD; x++)
-                if (pos[x] < slab[x].size()) P->items[k++] = slab[x][pos[x]++];
-    }
+    deal_xcd(P, P->items, ctx->ngrid[0]);
     // row descriptors: slot * 32 + orbital (-1 for padding rows)
     std::vector<int16_t> rowinfo(std::max<int64_t>(1, roff), -1);
     for (int64_t b = 0; b < NB; b++)
@@ -720,6 +724,7 @@ int plan_build(lcpao_ctx *ctx) {
         for (size_t x = 0; x < sph[b].size(); x++) slotno[P->blocks[b].slot_off + x] = (int16_t)ctx->no[sph[b][x].first];
     std::vector<int32_t> cbase(std::max<int64_t>(1, spoff), -1);
     std::vector<SlotPair> spairs(std::max<int64_t>(1, spoff), SlotPair{-1, 0});
+    std::vector<SlotPair> fpairs(std::max<int64_t>(1, spoff), SlotPair{-1, 0});  // blocks own disjoint ranges
     // sphere tables and point maps
     std::vector<int32_t> sb(P->nsph), sa(P->nsph), sr(P->nsph), smap((size_t)P->nsph * BLK_PTS, -1);
     std::vector<uint64_t> mask((size_t)P->nsph * (BLK_PTS / 64), 0);
@@ -776,6 +781,11 @@ int plan_build(lcpao_ctx *ctx) {
                             const int32_t pyx = fy == pair_of.end() ? -1 : fy->second;
                             if (pxy >= 0) olg_count[w][pxy] += ov;
                             if (pyx >= 0 && x != y) olg_count[w][pyx] += ov;
+                            const int64_t sp0 = P->blocks[b].sp_off;
+                            if (pxy >= 0)
+                                fpairs[sp0 + x * ns + y] = SlotPair{(int32_t)ctx->pairs[pxy].hoff, ctx->pairs[pxy].nob};
+                            if (pyx >= 0)
+                                fpairs[sp0 + y * ns + x] = SlotPair{(int32_t)ctx->pairs[pyx].hoff, ctx->pairs[pyx].nob};
                             int32_t pc = -1, trans = 0;
                             if (pxy >= 0 && canon_pos[pxy] >= 0) pc = pxy;
                             else if (pyx >= 0 && canon_pos[pyx] >= 0) { pc = pyx; trans = 1; }
@@ -821,6 +831,30 @@ int plan_build(lcpao_ctx *ctx) {
     if (pel > INT32_MAX) { ctx->err = "H partials exceed the split-precision index range"; delete P; return LCPAO_ERANGE; }
     P->ncontrib = (int64_t)all.size();
     for (size_t c = 0; c < ctx->canon.size(); c++) coff[c + 1] += coff[c];
+    // Force3 items: the S x S super-tiles of a block in units of two of one
+    // row ((si, 2 u), (si, 2 u + 1); U = ceil(S / 2) units per row), FB_WAVES
+    // units per item
+    for (int64_t b = 0; b < NB; b++) {
+        const int S = P->blocks[b].rows / ST, U = (S + 1) / 2;
+        for (int u0 = 0; u0 < S * U; u0 += FB_WAVES)
+            P->fitems.push_back({(int32_t)b, u0, std::min(FB_WAVES, S * U - u0), 0});
+    }
+    deal_xcd(P, P->fitems, ctx->ngrid[0]);
+    std::vector<int32_t> fbase(std::max<size_t>(1, P->fitems.size()));
+    std::vector<std::vector<int32_t>> fatom(ctx->natom);
+    for (size_t i = 0; i < P->fitems.size(); i++) {
+        const int64_t b = P->fitems[i].block;
+        fbase[i] = (int32_t)P->fout;
+        for (size_t x = 0; x < sph[b].size(); x++) fatom[sph[b][x].first].push_back((int32_t)(P->fout + (int64_t)x));
+        P->fout += (int64_t)sph[b].size();
+    }
+    if (P->fout > INT32_MAX) { ctx->err = "Force3 sums exceed the split-precision index range"; delete P; return LCPAO_ERANGE; }
+    std::vector<int32_t> fatom_off(ctx->natom + 1, 0), fatom_idx;
+    fatom_idx.reserve(std::max<int64_t>(1, P->fout));
+    for (int a = 0; a < ctx->natom; a++) {
+        fatom_idx.insert(fatom_idx.end(), fatom[a].begin(), fatom[a].end());
+        fatom_off[a + 1] = (int32_t)fatom_idx.size();
+    }
     int rc;
     ctx->plan = P;
     if ((rc = up(ctx, &P->d_blocks, P->blocks)) || (rc = up(ctx, &P->d_items, P->items)) ||
@@ -828,7 +862,10 @@ int plan_build(lcpao_ctx *ctx) {
         (rc = up(ctx, &P->d_sph_map, smap)) || (rc = up(ctx, &P->d_contrib, contrib)) || (rc = up(ctx, &P->d_contrib_off, coff)) ||
         (rc = up(ctx, &P->d_contrib_pair, cpair)) || (rc = up(ctx, &P->d_rowinfo, rowinfo)) ||
         (rc = up(ctx, &P->d_spair, spairs)) || (rc = up(ctx, &P->d_cbase, cbase)) ||
-        (rc = up(ctx, &P->d_slotno, slotno))) {
+        (rc = up(ctx, &P->d_slotno, slotno)) || (rc = up(ctx, &P->d_fpair, fpairs)) ||
+        (rc = up(ctx, &P->d_fitems, P->fitems)) ||
+        (rc = up(ctx, &P->d_fbase, fbase)) || (rc = up(ctx, &P->d_fatom_off, fatom_off)) ||
+        (rc = up(ctx, &P->d_fatom_idx, fatom_idx))) {
         ctx->plan = nullptr;
         plan_free(P);
         return rc;
@@ -1364,6 +1401,368 @@ int split_density(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t 
     }
     SPLIT_TRY(ctx, hipStreamSynchronize(ctx->stream));
     collect_timings(ctx);
+    return LCPAO_OK;
+}
+
+// ----------------------------------------------------------------------------
+// block Force3 (bf16x3): per block, T = D' Phi on the matrix cores with D' the
+// ordered-pair density blocks (row orbital i of slot x, column orbital j of
+// slot y: D(pair x -> y)[i][j]), then per row i and direction d
+//   g[i][d] = sum_p V(p) dphi_d[i](p) T[i](p)
+// on the vector units, and S_A[d] = sum of g over the rows of A's slots.
+// ----------------------------------------------------------------------------
+
+// gradient tiles: element (d, row, point k = 32 c + 16 pt + k16) of a block
+// at ((((c * 2 + pt) * 3 + d) * (R / 16) + row / 16) * 64 + lane) * 4 + k16 % 4
+// with lane = 16 (k16 / 4) + row % 16: lane l of a wave reads 16 contiguous
+// bytes, orbital 16 g + l % 16 at the four points 4 (l / 16) .. + 3 of a
+// 16-point group, which are the points of its matrix-core outputs
+__host__ __device__ __forceinline__ int64_t gradfrag_index(int c, int pt, int d, int row, int k16, int R) {
+    return ((((int64_t)(c * 2 + pt) * 3 + d) * (R / 16) + row / 16) * 64 + (k16 / 4) * 16 + row % 16) * 4 + k16 % 4;
+}
+
+__global__ void k_dtiles(int64_t nsph, const int32_t *__restrict__ sph_block, const int32_t *__restrict__ sph_atom,
+                         const int32_t *__restrict__ sph_row0, const int32_t *__restrict__ sph_map,
+                         const BlockDev *__restrict__ blocks, const int64_t *__restrict__ tab_off,
+                         const int32_t *__restrict__ no, const double *__restrict__ dx, const double *__restrict__ dy,
+                         const double *__restrict__ dz, float *__restrict__ dtile) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= nsph * BLK_PTS) return;
+    const int64_t s = gid / BLK_PTS;
+    const int k = (int)(gid - s * BLK_PTS);
+    const int t = sph_map[gid];
+    if (t < 0) return;  // the tiles are zeroed first
+    const int a = sph_atom[s];
+    const int NO = no[a];
+    const BlockDev B = blocks[sph_block[s]];
+    const int c = k / CHUNK, pt = (k % CHUNK) / 16, k16 = k % 16;
+    const int64_t col = tab_off[a] + (int64_t)t * NO;
+    float *out = dtile + 3 * B.tile_off;
+    for (int i = 0; i < NO; i++) {
+        const int row = sph_row0[s] + i;
+        out[gradfrag_index(c, pt, 0, row, k16, B.rows)] = (float)dx[col + i];
+        out[gradfrag_index(c, pt, 1, row, k16, B.rows)] = (float)dy[col + i];
+        out[gradfrag_index(c, pt, 2, row, k16, B.rows)] = (float)dz[col + i];
+    }
+}
+
+struct ForceArgs {
+    const BlockDev *blocks;
+    const uint16_t *thi, *tlo;     // point fragments of Phi
+    const float *dtile;            // gradient tiles
+    const int16_t *rowinfo;
+    const SlotPair *fpair;         // ordered slot pairs
+    const int16_t *slotno;
+    const float *D;                // the channel's density-matrix slice (FP32)
+    const double *V;
+    const int32_t *base;           // per item: its first (item, slot) output
+    double *out;                   // per (item, slot): the three sums
+    int32_t n1, n2, n3, pad;
+};
+
+// LDS of the Force3 workgroup: the chunk planes (as the density kernel),
+// V at the block's points, per-wave row sums, per-row FP64 sums, row
+// descriptors, slot orbital counts, then the ordered slot pairs
+constexpr int FB_VB_OFFSET = MAX_ROWS * RB_BYTES_PER_ROW;
+constexpr int FB_SUM_OFFSET = FB_VB_OFFSET + BLK_PTS * 4;
+constexpr int FB_TOT_OFFSET = FB_SUM_OFFSET + FB_WAVES * ST * 3 * 4;
+constexpr int FB_RINFO_OFFSET = FB_TOT_OFFSET + MAX_ROWS * 3 * 8;
+constexpr int FB_MAX_SLOTS = 64;
+constexpr int FB_NO_OFFSET = FB_RINFO_OFFSET + MAX_ROWS * 2;
+constexpr int FB_FP_OFFSET = FB_NO_OFFSET + FB_MAX_SLOTS * 2;
+static_assert(FB_TOT_OFFSET % 8 == 0 && FB_FP_OFFSET % 8 == 0, "Force3 LDS areas are aligned");
+
+size_t force_kernel_lds(int max_slots) { return FB_FP_OFFSET + (size_t)max_slots * max_slots * sizeof(SlotPair); }
+
+// L2 warm-up of a chunk's gradient tiles (3 R lines of 128 bytes) by the
+// workgroup's threads, as touch_chunk_n
+struct GradTouch {
+    uint32_t v[(3 * MAX_ROWS + 64 * FB_WAVES - 1) / (64 * FB_WAVES)];
+};
+__device__ __forceinline__ GradTouch touch_grad(const GLOBAL_AS float *__restrict__ g, int64_t off, int R, int tid) {
+    GradTouch t;
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(t.v) / sizeof(t.v[0])); k++) {
+        const int x = tid + k * 64 * FB_WAVES < 3 * R ? tid + k * 64 * FB_WAVES : 0;
+        t.v[k] = *reinterpret_cast<const GLOBAL_AS uint32_t *>(g + off + (int64_t)x * 32);
+    }
+    return t;
+}
+__device__ __forceinline__ void release_grad(const GradTouch &t) {
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(t.v) / sizeof(t.v[0])); k++) asm volatile("" ::"v"(t.v[k]));
+}
+
+// One 512-thread workgroup per item (a block and up to 8 units of its S x S
+// super-tiles), one unit per wave: super-tiles (si, sj) and (si, sj + 1).
+// The wave holds D'[si][sj] and D'[si][sj + 1] as B-operand fragments (lane
+// l: D'[64 si + 16 ct + l % 16][64 sj + 32 ks + 8 (l / 16) + e]) and
+// multiplies the point fragments of Phi (rows sj, sj + 1) as the A operand,
+// so that output element v of lane l is T[orbital 64 si + 16 ct + l % 16]
+// [point 16 pt + 4 (l / 16) + v], summed over both super-tiles in one
+// accumulator.  Each lane accumulates g for its four orbitals over its
+// points; the lanes of an orbital are added at the end, the waves of a
+// super-tile row in wave order (FP64), and the rows of a slot in row order:
+// deterministic.
+template <int NF>
+__global__ __launch_bounds__(64 * FB_WAVES) void k_fblk_bf(const HItem *__restrict__ items, int32_t nitems,
+                                                          ForceArgs A) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    if ((int)blockIdx.x >= nitems) return;
+    const HItem it = items[blockIdx.x];
+    const BlockDev B = as_global(A.blocks)[it.block];
+    const GLOBAL_AS uint16_t *__restrict__ thi = as_global(A.thi);
+    const GLOBAL_AS uint16_t *__restrict__ tlo = as_global(A.tlo);
+    const GLOBAL_AS float *__restrict__ dtile = as_global(A.dtile) + 3 * B.tile_off;
+    const GLOBAL_AS float *__restrict__ Dm = as_global(A.D);
+    const int R = B.rows, S = R / ST, U = (S + 1) / 2, G = R / 16, G32 = R / 32;
+    const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63, l16 = lane & 15,
+              q = lane >> 4;
+    u16x8 *TH = reinterpret_cast<u16x8 *>(lds_raw);
+    u16x8 *TL = TH + G * 64;
+    float *Vb = reinterpret_cast<float *>(lds_raw + FB_VB_OFFSET);
+    float *rsum = reinterpret_cast<float *>(lds_raw + FB_SUM_OFFSET);     // [wave][64 rows][3]
+    double *rtot = reinterpret_cast<double *>(lds_raw + FB_TOT_OFFSET);   // [row][3]
+    int16_t *s_rinfo = reinterpret_cast<int16_t *>(lds_raw + FB_RINFO_OFFSET);
+    int16_t *s_no = reinterpret_cast<int16_t *>(lds_raw + FB_NO_OFFSET);
+    SlotPair *s_fp = reinterpret_cast<SlotPair *>(lds_raw + FB_FP_OFFSET);
+    constexpr int NT = 64 * FB_WAVES;
+    for (int x = tid; x < R; x += NT) s_rinfo[x] = as_global(A.rowinfo)[B.row_off + x];
+    for (int x = tid; x < B.nslot; x += NT) s_no[x] = as_global(A.slotno)[B.slot_off + x];
+    for (int x = tid; x < B.nslot * B.nslot; x += NT) s_fp[x] = as_global(A.fpair)[B.sp_off + x];
+    for (int k = tid; k < BLK_PTS; k += NT) {
+        const int x1 = B.g1 + k / (BLK * BLK), x2 = B.g2 + (k / BLK) % BLK, x3 = B.g3 + k % BLK;
+        Vb[k] = (x1 < A.n1 && x2 < A.n2 && x3 < A.n3) ? (float)as_global(A.V)[((int64_t)x1 * A.n2 + x2) * A.n3 + x3]
+                                                      : 0.f;
+    }
+    // the first two chunks are fetched into L2 while D' is gathered
+    const int64_t gstride = (int64_t)6 * G * 64 * 4;  // floats of one chunk's gradient tiles
+    __syncthreads();
+    const bool has = wave < it.nst;
+    int si = 0, sj = 0;
+    if (has) {
+        si = (it.st0 + wave) / U;
+        sj = 2 * ((it.st0 + wave) % U);
+    }
+    const bool has2 = has && sj + 1 < S;  // the row's last unit of an odd S holds one super-tile
+    // D' gathered straight into the B-operand fragments, 32 loads per batch
+    bf16x8 wh[2][4][2], wl[2][4][2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        constexpr int GB = 4;
+        const float keep = (h == 0 ? has : has2) ? 1.f : 0.f;
+        const int cj0 = (sj + h < S ? sj + h : sj) * ST;
+#pragma unroll
+        for (int b2 = 0; b2 < 8; b2 += GB) {
+            float dv[GB][8];
+            uint32_t okbits = 0;
+#pragma unroll
+            for (int r2 = 0; r2 < GB; r2++) {
+                const int ct = (b2 + r2) >> 1, ks = (b2 + r2) & 1;
+                const int rinf = s_rinfo[si * ST + 16 * ct + l16];
+                const int ra = rinf >> 5, ri = rinf & 31;
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const int cinf = s_rinfo[cj0 + 32 * ks + 8 * q + e];
+                    const int cb = cinf >> 5, cj = cinf & 31;
+                    const bool valid = rinf >= 0 && cinf >= 0;
+                    const SlotPair P = s_fp[valid ? ra * B.nslot + cb : 0];
+                    const bool ok = valid && P.hoff >= 0;
+                    dv[r2][e] = Dm[ok ? P.hoff + ri * P.meta + cj : 0];  // unconditional: loads issue back to back
+                    okbits |= (ok ? 1u : 0u) << (r2 * 8 + e);
+                }
+            }
+#pragma unroll
+            for (int r2 = 0; r2 < GB; r2++) {
+                const int ct = (b2 + r2) >> 1, ks = (b2 + r2) & 1;
+                u16x8 wh8, wl8;
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const float w = dv[r2][e] * (((okbits >> (r2 * 8 + e)) & 1u) ? keep : 0.f);
+                    const __bf16 hb = (__bf16)w;
+                    wh8[e] = __builtin_bit_cast(uint16_t, hb);
+                    wl8[e] = __builtin_bit_cast(uint16_t, (__bf16)(w - (float)hb));
+                }
+                wh[h][ct][ks] = __builtin_bit_cast(bf16x8, wh8);
+                wl[h][ct][ks] = __builtin_bit_cast(bf16x8, wl8);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    TouchN<NT> t0 = touch_chunk_n<NT>(thi, tlo, B.tile_off, R, tid);
+    TouchN<NT> t1 = touch_chunk_n<NT>(thi, tlo, B.tile_off + (int64_t)G * FRAG, R, tid);
+    GradTouch u0 = touch_grad(dtile, 0, R, tid);
+    GradTouch u1 = touch_grad(dtile, gstride, R, tid);
+    float g[4][3];
+#pragma unroll
+    for (int ct = 0; ct < 4; ct++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) g[ct][d] = 0.f;
+    u16x8 fh[NF], fl[NF];
+    auto frag_of = [&](int f) {
+        const int gg = wave + f * FB_WAVES;
+        return gg < G ? gg : min(wave, G - 1);
+    };
+    auto prefetch = [&](int c) {
+        const GLOBAL_AS uint16_t *sh = thi + B.tile_off + (int64_t)c * G * FRAG;
+        const GLOBAL_AS uint16_t *sl = tlo + B.tile_off + (int64_t)c * G * FRAG;
+#pragma unroll
+        for (int f = 0; f < NF; f++) {  // unconditional loads: see touch_chunk
+            const int gg = frag_of(f);
+            fh[f] = *reinterpret_cast<const GLOBAL_AS u16x8 *>(sh + (gg * 64 + lane) * 8);
+            fl[f] = *reinterpret_cast<const GLOBAL_AS u16x8 *>(sl + (gg * 64 + lane) * 8);
+        }
+    };
+    const int sj2 = sj + 1 < S ? sj + 1 : sj;  // in bounds; its D' is zero when absent
+    auto step = [&](int c, TouchN<NT> &slot, GradTouch &gslot) {
+        __syncthreads();
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            const int gg = frag_of(f);
+            TH[gg * 64 + lane] = fh[f];
+            TL[gg * 64 + lane] = fl[f];
+        }
+        release_n(slot);
+        release_grad(gslot);
+        __syncthreads();
+        prefetch(min(c + 1, NCHUNK - 1));
+        const int c2 = min(c + 2, NCHUNK - 1);
+        slot = touch_chunk_n<NT>(thi, tlo, B.tile_off + (int64_t)c2 * G * FRAG, R, tid);
+        gslot = touch_grad(dtile, c2 * gstride, R, tid);
+        const GLOBAL_AS f32x4 *dg = reinterpret_cast<const GLOBAL_AS f32x4 *>(dtile + c * gstride);
+#pragma unroll 1
+        for (int pt = 0; pt < 2; pt++) {
+            bf16x8 bh[2][2], bl[2][2];
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int ks = 0; ks < 2; ks++) {
+                    const int fi = pt * G32 + (h == 0 ? sj : sj2) * 2 + ks;
+                    bh[h][ks] = __builtin_bit_cast(bf16x8, TH[fi * 64 + lane]);
+                    bl[h][ks] = __builtin_bit_cast(bf16x8, TL[fi * 64 + lane]);
+                }
+            const f32x4 vv = *reinterpret_cast<const f32x4 *>(&Vb[c * CHUNK + pt * 16 + 4 * q]);
+#pragma unroll
+            for (int ct = 0; ct < 4; ct++) {
+                __builtin_amdgcn_sched_barrier(0);
+                f32x4 dd[3];
+#pragma unroll
+                for (int d = 0; d < 3; d++) dd[d] = dg[((pt * 3 + d) * G + si * 4 + ct) * 64 + lane];
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int h = 0; h < 2; h++)
+#pragma unroll
+                    for (int ks = 0; ks < 2; ks++) {
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[h][ks], wh[h][ct][ks], acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[h][ks], wl[h][ct][ks], acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[h][ks], wh[h][ct][ks], acc, 0, 0, 0);
+                    }
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const float tv = acc[v] * vv[v];
+#pragma unroll
+                    for (int d = 0; d < 3; d++) g[ct][d] = fmaf(tv, dd[d][v], g[ct][d]);
+                }
+            }
+        }
+    };
+    prefetch(0);
+#pragma unroll
+    for (int c = 0; c < NCHUNK; c += 2) {
+        step(c, t0, u0);
+        step(c + 1, t1, u1);
+    }
+    // sum over the four lanes of each orbital (lanes 16 apart) with the lane swaps
+#pragma unroll
+    for (int ct = 0; ct < 4; ct++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            float x = g[ct][d];
+            const auto s32 = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+            x = __uint_as_float(s32[0]) + __uint_as_float(s32[1]);
+            const auto s16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+            g[ct][d] = __uint_as_float(s16[0]) + __uint_as_float(s16[1]);
+        }
+    if (q == 0 && has)
+#pragma unroll
+        for (int ct = 0; ct < 4; ct++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) rsum[(wave * ST + 16 * ct + l16) * 3 + d] = g[ct][d];
+    __syncthreads();
+    for (int x = tid; x < R * 3; x += NT) {
+        const int r = x / 3, d = x % 3;
+        double t = 0.0;
+        for (int w = 0; w < it.nst; w++)
+            if ((it.st0 + w) / U == r / ST) t += (double)rsum[(w * ST + r % ST) * 3 + d];
+        rtot[x] = t;
+    }
+    __syncthreads();
+    for (int x = tid; x < B.nslot * 3; x += NT) {
+        const int xs = x / 3, d = x % 3;
+        int r0 = 0;
+        for (int y = 0; y < xs; y++) r0 += s_no[y];
+        double t = 0.0;
+        for (int i = 0; i < s_no[xs]; i++) t += rtot[(r0 + i) * 3 + d];
+        A.out[(int64_t)(A.base[blockIdx.x] + xs) * 3 + d] = t;
+    }
+}
+
+// per atom: the sums of its (item, slot) entries in item order, x w, added
+// into fsum[a][d]
+__global__ void k_fblk_reduce(const int32_t *__restrict__ off, const int32_t *__restrict__ idx, int natom,
+                              const double *__restrict__ out, double w, double *__restrict__ fsum) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= natom) return;
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int u = off[a]; u < off[a + 1]; u++)
+        for (int d = 0; d < 3; d++) s[d] += out[3 * (int64_t)idx[u] + d];
+    for (int d = 0; d < 3; d++) fsum[3 * a + d] += w * s[d];
+}
+
+int split_force3(lcpao_ctx *ctx, const double *d_V, const double *d_DM, double w, double *d_fsum) {
+    int rc = split_tables(ctx, LCPAO_BF16X3);
+    if (rc) return rc;
+    BlockPlan *P = ctx->plan;
+    if (P->max_slots > FB_MAX_SLOTS) { ctx->err = "too many sphere images per block for the bf16x3 Force3"; return LCPAO_ERANGE; }
+    if (!P->dtile_ready) {
+        const size_t n = (size_t)std::max<int64_t>(1, P->tile_elems) * 3;
+        if (!P->d_dtile) SPLIT_TRY(ctx, hipMalloc(&P->d_dtile, n * sizeof(float)));
+        if (!P->d_fout) SPLIT_TRY(ctx, hipMalloc(&P->d_fout, std::max<int64_t>(1, P->fout) * 3 * sizeof(double)));
+        SPLIT_TRY(ctx, hipMemsetAsync(P->d_dtile, 0, n * sizeof(float), ctx->stream));
+        const int64_t nthr = P->nsph * BLK_PTS;
+        if (nthr > 0)
+            hipLaunchKernelGGL(k_dtiles, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, ctx->stream, P->nsph,
+                               P->d_sph_block, P->d_sph_atom, P->d_sph_row0, P->d_sph_map, P->d_blocks, ctx->d_tab_off,
+                               ctx->d_no, ctx->d_dtab[0], ctx->d_dtab[1], ctx->d_dtab[2], P->d_dtile);
+        SPLIT_TRY(ctx, hipGetLastError());
+        SPLIT_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        P->dtile_ready = true;
+    }
+    launch_to_f32(ctx->stream, ctx->my_hsize, d_DM, ctx->d_deff32);
+    if (!P->fitems.empty()) {
+        KTimer kt(ctx, KID_F3);
+        const ForceArgs A{P->d_blocks, P->d_thi, P->d_tlo, P->d_dtile, P->d_rowinfo, P->d_fpair, P->d_slotno,
+                          ctx->d_deff32, d_V, P->d_fbase, P->d_fout, ctx->ngrid[0], ctx->ngrid[1], ctx->ngrid[2], 0};
+        void (*kern)(const HItem *, int32_t, ForceArgs) = nullptr;
+        switch (nfrag(P->max_rows)) {
+            case 1: kern = k_fblk_bf<1>; break;
+            case 2: kern = k_fblk_bf<2>; break;
+            case 3: kern = k_fblk_bf<3>; break;
+            case 4: kern = k_fblk_bf<4>; break;
+            default: kern = k_fblk_bf<MAXF>; break;
+        }
+        const size_t lds = force_kernel_lds(P->max_slots);
+        SPLIT_TRY(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        const unsigned ni = (unsigned)P->fitems.size();
+        hipLaunchKernelGGL(kern, dim3(ni), dim3(64 * FB_WAVES), lds, ctx->stream, P->d_fitems, (int32_t)ni, A);
+    }
+    if (ctx->natom > 0) {
+        KTimer kt(ctx, KID_F3RED);
+        hipLaunchKernelGGL(k_fblk_reduce, dim3((ctx->natom + 127) / 128), dim3(128), 0, ctx->stream, P->d_fatom_off,
+                           P->d_fatom_idx, ctx->natom, P->d_fout, w, d_fsum);
+    }
+    SPLIT_TRY(ctx, hipGetLastError());
     return LCPAO_OK;
 }
 

@@ -117,6 +117,19 @@ struct BlockPlan {
     int64_t part_elems = 0;
     int32_t *d_cbase = nullptr;    // per block slot pair
     int16_t *d_slotno = nullptr;   // per block slot
+    // bf16x3 Force3: the ordered pair of every slot pair (hoff of D(pair),
+    // meta = orbitals of the second slot; hoff < 0: no pair), items over all
+    // S x S super-tiles, FP32 gradient tiles, per-(item, slot) sums and the
+    // per-atom lists of those sums (item order: deterministic)
+    SlotPair *d_fpair = nullptr;
+    std::vector<HItem> fitems;
+    HItem *d_fitems = nullptr;
+    int32_t *d_fbase = nullptr;    // per force item: its first (item, slot) output
+    int64_t fout = 0;
+    double *d_fout = nullptr;      // fout x 3
+    int32_t *d_fatom_off = nullptr, *d_fatom_idx = nullptr;
+    float *d_dtile = nullptr;      // 3 x tile_elems floats, gradient fragment order
+    bool dtile_ready = false;
 };
 
 int plan_build(lcpao_ctx *ctx);
@@ -135,6 +148,8 @@ void plan_free(BlockPlan *p);
 int split_tables(lcpao_ctx *ctx, int precision);
 int split_hamiltonian(lcpao_ctx *ctx, int nch, const double *const *d_V, double *const *d_H, int precision);
 int split_density(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_offset, double *const *d_rho);
+// bf16x3 Force3 of one channel: adds w * S_A into d_fsum (natom x 3, atom-major)
+int split_force3(lcpao_ctx *ctx, const double *d_V, const double *d_DM, double w, double *d_fsum);
 int fp32_tables(lcpao_ctx *ctx, bool deriv);
 void invalidate_split_tables(lcpao_ctx *ctx);
 void launch_to_f32(hipStream_t stream, int64_t n, const double *x, float *y);

@@ -188,6 +188,26 @@ def test_force3_split(name):
     assert r <= 1e-5
 
 
+@pytest.mark.parametrize("name", ["si2", "si8", "nacl8", "si2_r1of3", "nacl8_r0of2"])
+def test_force3_bf16x3_block_form(name):
+    """bf16x3 Force3 on the matrix cores (block-dense form: ordered-pair D
+    blocks, gradient tiles), one and two channels, against the oracle;
+    bound 1e-4 of the sum of absolute terms (as the bf16x3 density)."""
+    c = case(name)
+    dt = O.dorbitals(c.g, c.opaos, c.st.atom2spe)
+    c.ctx.set_tables(c.tabs)
+    c.ctx.set_dtables(*dt)
+    for Vs, Ds in (([c.V], [c.DM]), ([c.V, c.V2], [c.DM, c.DM2])):
+        F = c.ctx.force3(Vs, Ds, dm_offset=c.u.MPHks_rank, precision="bf16x3")
+        Fref = O.force3(c.g, c.tabs, dt, Vs, Ds)
+        r = worst_ratio(F, Fref, _force_bound(c, dt, Vs, Ds))
+        print(name, len(Vs), "F3 bf16x3 worst ratio", r, flush=True)
+        assert r <= 1e-4
+    # deterministic: a second call gives the same bits
+    F2 = c.ctx.force3([c.V, c.V2], [c.DM, c.DM2], dm_offset=c.u.MPHks_rank, precision="bf16x3")
+    assert np.array_equal(F, F2)
+
+
 def test_split_tables_follow_table_updates():
     """A second set_tables call invalidates the split-precision tiles."""
     c = case("si2")

@@ -2,8 +2,8 @@
 vacancy (511 atoms, 192^3 grid, +-0.1 bohr displacements, seed 4), V from
 60 plane waves and a PSD density matrix with 1022 occupied states.
 
-* the FP32 form against the FP64 form: |dF| <= 1e-5 Ha/bohr on every
-  component (the accuracy bar for forces);
+* the FP32 and bf16x3 forms against the FP64 form: |dF| <= 1e-5 Ha/bohr
+  on every component (the accuracy bar for forces);
 * the FP64 form against central differences (step 1e-4 bohr) of the grid
   energy E(R) = 2 GridVol sum_g V(g) rho_R(g), with the tables rebuilt for
   the displaced atom and the grid lists kept fixed (the sum Force3 runs
@@ -36,6 +36,16 @@ def test_force3_fp32_within_1e5_of_fp64(si511):
     F32 = ctx.force3([V], [DM], precision="fp32")
     err = np.abs(F32 - F64).max()
     print("max |F32 - F64| = %.3e Ha/bohr (max |F| = %.3e)" % (err, np.abs(F64).max()))
+    assert err <= 1e-5
+
+
+def test_force3_bf16x3_within_1e5_of_fp64(si511):
+    """The block form on the matrix cores (split-BF16x3 products, FP32
+    accumulation, FP64 per-slot and per-atom sums)."""
+    st, u, V, DM, ctx, F64 = si511
+    Fb = ctx.force3([V], [DM], precision="bf16x3")
+    err = np.abs(Fb - F64).max()
+    print("max |Fbf16x3 - F64| = %.3e Ha/bohr (max |F| = %.3e)" % (err, np.abs(F64).max()), flush=True)
     assert err <= 1e-5
 
 
