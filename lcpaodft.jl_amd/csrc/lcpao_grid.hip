@@ -660,15 +660,20 @@ __global__ __launch_bounds__(64) void k_f3(const F3Unit *__restrict__ units, con
 }
 
 // Per-atom sums of Force3 unit partials (pairs are atom-major, units
-// consecutive per pair): fsum[a][xyz] += w * sum.
-__global__ void k_f3_reduce(const int32_t *__restrict__ atom_ub, int natom, const double *__restrict__ fpart,
-                            double w, double *__restrict__ fsum) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= natom) return;
+// consecutive per pair): fsum[a][xyz] += w * sum.  One wave per atom: lane l
+// adds units l, l + 64, ... in order, then a fixed butterfly (deterministic).
+__global__ __launch_bounds__(256) void k_f3_reduce(const int32_t *__restrict__ atom_ub, int natom,
+                                                   const double *__restrict__ fpart, double w,
+                                                   double *__restrict__ fsum) {
+    const int a = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (a >= natom) return;  // whole waves
     double s[3] = {0.0, 0.0, 0.0};
-    for (int u = atom_ub[a]; u < atom_ub[a + 1]; u++)
+    for (int u = atom_ub[a] + lane; u < atom_ub[a + 1]; u += 64)
         for (int x = 0; x < 3; x++) s[x] += fpart[3 * (int64_t)u + x];
-    for (int x = 0; x < 3; x++) fsum[3 * a + x] += w * s[x];
+    for (int off = 32; off >= 1; off >>= 1)
+        for (int x = 0; x < 3; x++) s[x] += __shfl_xor(s[x], off, 64);
+    if (lane == 0)
+        for (int x = 0; x < 3; x++) fsum[3 * a + x] += w * s[x];
 }
 
 // force[a + natom*xyz] += fsum[a][xyz] * GridVol
@@ -1291,7 +1296,7 @@ int f3_dev(lcpao_ctx *ctx, int nch, const double *const *d_V, const double *cons
         }
         if (nu > 0) {
             KTimer kt(ctx, KID_F3RED);
-            hipLaunchKernelGGL(k_f3_reduce, dim3((ctx->natom + 127) / 128), dim3(128), 0, ctx->stream, ctx->d_atom_ub,
+            hipLaunchKernelGGL(k_f3_reduce, dim3((ctx->natom + 3) / 4), dim3(256), 0, ctx->stream, ctx->d_atom_ub,
                                ctx->natom, ctx->d_fpart, weight[c], ctx->d_fsum);
         }
         HIP_TRY(ctx, hipGetLastError());

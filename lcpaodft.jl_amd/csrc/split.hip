@@ -335,7 +335,7 @@ __device__ __forceinline__ void hblk_item(const HItem it, const BlockArgs &A, un
     GLOBAL_AS float *__restrict__ part = as_global(A.part);
     const int n1 = A.n1, n2 = A.n2, n3 = A.n3;
     const int R = B.rows, S = R / ST, G = R / 16;
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, q = lane >> 4;
+    const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63, q = lane >> 4;
     u16x8 *PH = reinterpret_cast<u16x8 *>(lds_raw);
     u16x8 *PL = PH + G * 64;
     u16x8 *VH = PL + G * 64;
@@ -1047,7 +1047,7 @@ __device__ __forceinline__ void rblk_item(const HItem it, const BlockArgs &A, un
     double *__restrict__ rho = A.rho;
     const int n1 = A.n1, n2 = A.n2, n3 = A.n3;
     const int R = B.rows, S = R / ST, G = R / 16, G32 = R / 32;
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, q = lane >> 4;
+    const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63, l16 = lane & 15, q = lane >> 4;
     u16x8 *TH = reinterpret_cast<u16x8 *>(lds_raw);
     u16x8 *TL = TH + G * 64;
     // LDS: the chunk planes, the per-wave point sums, then the block's row
@@ -1707,16 +1707,21 @@ __global__ __launch_bounds__(64 * FB_WAVES) void k_fblk_bf(const HItem *__restri
     }
 }
 
-// per atom: the sums of its (item, slot) entries in item order, x w, added
-// into fsum[a][d]
-__global__ void k_fblk_reduce(const int32_t *__restrict__ off, const int32_t *__restrict__ idx, int natom,
-                              const double *__restrict__ out, double w, double *__restrict__ fsum) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= natom) return;
+// per atom: the sums of its (item, slot) entries, x w, added into
+// fsum[a][d]; one wave per atom, lane l adds entries l, l + 64, ... in item
+// order, then a fixed butterfly (deterministic)
+__global__ __launch_bounds__(256) void k_fblk_reduce(const int32_t *__restrict__ off, const int32_t *__restrict__ idx,
+                                                     int natom, const double *__restrict__ out, double w,
+                                                     double *__restrict__ fsum) {
+    const int a = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (a >= natom) return;  // whole waves
     double s[3] = {0.0, 0.0, 0.0};
-    for (int u = off[a]; u < off[a + 1]; u++)
+    for (int u = off[a] + lane; u < off[a + 1]; u += 64)
         for (int d = 0; d < 3; d++) s[d] += out[3 * (int64_t)idx[u] + d];
-    for (int d = 0; d < 3; d++) fsum[3 * a + d] += w * s[d];
+    for (int o = 32; o >= 1; o >>= 1)
+        for (int d = 0; d < 3; d++) s[d] += __shfl_xor(s[d], o, 64);
+    if (lane == 0)
+        for (int d = 0; d < 3; d++) fsum[3 * a + d] += w * s[d];
 }
 
 int split_force3(lcpao_ctx *ctx, const double *d_V, const double *d_DM, double w, double *d_fsum) {
@@ -1759,7 +1764,7 @@ int split_force3(lcpao_ctx *ctx, const double *d_V, const double *d_DM, double w
     }
     if (ctx->natom > 0) {
         KTimer kt(ctx, KID_F3RED);
-        hipLaunchKernelGGL(k_fblk_reduce, dim3((ctx->natom + 127) / 128), dim3(128), 0, ctx->stream, P->d_fatom_off,
+        hipLaunchKernelGGL(k_fblk_reduce, dim3((ctx->natom + 3) / 4), dim3(256), 0, ctx->stream, P->d_fatom_off,
                            P->d_fatom_idx, ctx->natom, P->d_fout, w, d_fsum);
     }
     SPLIT_TRY(ctx, hipGetLastError());
