@@ -140,7 +140,9 @@ int lcpao_grid_density(lcpao_ctx *ctx, int32_t nspin, const double *const *DM, i
  * DM[c], weighted by weight[c] and GridVol, are ADDED into force
  * (natom x 3, column-major like the Julia Matrix VpotForce).  Reference
  * weights: nospin {4}; spin {2,2}; nc {2,2,4,-4}.  Needs the gradient
- * tables. */
+ * tables.  LCPAO_BF16X3 runs the block form on the matrix cores (first call:
+ * FP32 gradient tiles of 12 bytes per block row and point are built);
+ * LCPAO_FP32 the pair-list kernels on FP32 table copies. */
 int lcpao_grid_force3(lcpao_ctx *ctx, int32_t nch, const double *const *V, const double *const *DM,
                       int64_t dm_offset, const double *weight, double *force, int32_t precision);
 
