@@ -502,11 +502,13 @@ __global__ __launch_bounds__(64 * HB_WAVES) void k_hblk_f32(const HItem *__restr
 // pair's contribution records are staged in LDS so that every thread issues
 // independent loads of contiguous partial slots.
 constexpr int RED_BATCH = 256;
-__global__ __launch_bounds__(256) void k_hblk_reduce(const int32_t *__restrict__ canon, int64_t ncanon,
-                                                     const int32_t *__restrict__ coff,
-                                                     const Contrib *__restrict__ contrib,
-                                                     const PairDev *__restrict__ pairs, const float *__restrict__ part,
-                                                     double gridvol, double *__restrict__ H) {
+// One workgroup per canonical pair with one thread per element of its block
+// (the launch's width covers the largest NO_A x NO_B, at most 32 x 32).
+__global__ __launch_bounds__(1024) void k_hblk_reduce(const int32_t *__restrict__ canon, int64_t ncanon,
+                                                      const int32_t *__restrict__ coff,
+                                                      const Contrib *__restrict__ contrib,
+                                                      const PairDev *__restrict__ pairs, const float *__restrict__ part,
+                                                      double gridvol, double *__restrict__ H) {
     __shared__ int64_t s_base[RED_BATCH];
     __shared__ int32_t s_tr[RED_BATCH];
     const int64_t ci = blockIdx.x;
@@ -515,34 +517,28 @@ __global__ __launch_bounds__(256) void k_hblk_reduce(const int32_t *__restrict__
     const PairDev P = pairs[p];
     const int ne = P.noa * P.nob;
     const int k0 = coff[ci], k1 = coff[ci + 1];
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};  // elements threadIdx.x + 256 h (ne <= 32*32)
+    const int e = threadIdx.x;
+    const int i = e / P.nob, j = e - i * P.nob;
+    const int et = j * P.noa + i;  // the element in a slot oriented (B, A)
+    double acc = 0.0;
     for (int kb = k0; kb < k1; kb += RED_BATCH) {
         const int nb = min(RED_BATCH, k1 - kb);
         __syncthreads();
-        if ((int)threadIdx.x < nb) {
-            const Contrib C = contrib[kb + threadIdx.x];
-            s_base[threadIdx.x] = C.base;
-            s_tr[threadIdx.x] = C.trans;
+        for (int x = threadIdx.x; x < nb; x += blockDim.x) {  // the width may be below RED_BATCH
+            const Contrib C = contrib[kb + x];
+            s_base[x] = C.base;
+            s_tr[x] = C.trans;
         }
         __syncthreads();
-#pragma unroll
-        for (int h = 0; h < 4; h++) {
-            const int e = threadIdx.x + h * 256;
-            if (e >= ne) break;
-            const int i = e / P.nob, j = e - i * P.nob;
-            const int et = j * P.noa + i;  // the element in a slot oriented (B, A)
+        if (e < ne) {
             double s = 0.0;
-#pragma unroll 4
+#pragma unroll 8
             for (int k = 0; k < nb; k++) s += (double)part[s_base[k] + (s_tr[k] ? et : e)];
-            acc[h] += s;
+            acc += s;
         }
     }
-#pragma unroll
-    for (int h = 0; h < 4; h++) {
-        const int e = threadIdx.x + h * 256;
-        if (e >= ne) break;
-        const int i = e / P.nob, j = e - i * P.nob;
-        const double s = acc[h] * gridvol;
+    if (e < ne) {
+        const double s = acc * gridvol;
         H[P.hoff + e] = s;
         if (P.partner >= 0 && P.partner != p) H[pairs[P.partner].hoff + (int64_t)j * P.noa + i] = s;
     }
@@ -830,6 +826,8 @@ int plan_build(lcpao_ctx *ctx) {
     P->part_elems = pel;
     if (pel > INT32_MAX) { ctx->err = "H partials exceed the split-precision index range"; delete P; return LCPAO_ERANGE; }
     P->ncontrib = (int64_t)all.size();
+    for (int32_t p : ctx->canon)
+        P->red_threads = std::max(P->red_threads, (ctx->pairs[p].noa * ctx->pairs[p].nob + 63) / 64 * 64);
     for (size_t c = 0; c < ctx->canon.size(); c++) coff[c + 1] += coff[c];
     // Force3 items: the S x S super-tiles of a block in units of two of one
     // row ((si, 2 u), (si, 2 u + 1); U = ceil(S / 2) units per row), FB_WAVES
@@ -935,7 +933,7 @@ int split_hamiltonian(lcpao_ctx *ctx, int nch, const double *const *d_V, double 
         }
         if (ctx->ncanon > 0) {
             KTimer kt(ctx, KID_HRED);
-            hipLaunchKernelGGL(k_hblk_reduce, dim3((unsigned)ctx->ncanon), dim3(256), 0, ctx->stream, ctx->d_hcanon,
+            hipLaunchKernelGGL(k_hblk_reduce, dim3((unsigned)ctx->ncanon), dim3(P->red_threads), 0, ctx->stream, ctx->d_hcanon,
                                ctx->ncanon, P->d_contrib_off, P->d_contrib, ctx->d_pairs, P->d_part, ctx->gridvol,
                                d_H[c]);
         }
@@ -993,7 +991,7 @@ void launch_fill_i32(hipStream_t stream, int64_t n, int32_t *x, int32_t v) {
 void launch_hblk_reduce(const lcpao_ctx *ctx, hipStream_t stream, double *H) {
     const BlockPlan *P = ctx->plan;
     if (ctx->ncanon > 0)
-        hipLaunchKernelGGL(k_hblk_reduce, dim3((unsigned)ctx->ncanon), dim3(256), 0, stream, ctx->d_hcanon, ctx->ncanon,
+        hipLaunchKernelGGL(k_hblk_reduce, dim3((unsigned)ctx->ncanon), dim3(P->red_threads), 0, stream, ctx->d_hcanon, ctx->ncanon,
                            P->d_contrib_off, P->d_contrib, ctx->d_pairs, P->d_part, ctx->gridvol, H);
 }
 

@@ -111,6 +111,7 @@ struct BlockPlan {
     SlotPair *d_spair = nullptr;
     int32_t *d_contrib_pair = nullptr;
     int64_t ncontrib = 0;
+    int32_t red_threads = 64;      // reduce width: the largest NO_A x NO_B of a computed pair, rounded to 64
     float *d_t32 = nullptr;                     // fp32 tiles (permuted point order)
     bool bf16_ready = false, f32_ready = false;
     float *d_part = nullptr;       // compact partials (part_elems floats)
