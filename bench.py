@@ -132,6 +132,28 @@ def traffic_from_profile(config, kernel):
     return (sum(hits) if hits else None), os.path.relpath(path, ROOT)
 
 
+def mfma_from_profile(config, kernel):
+    """Matrix-core counters of `kernel` from the committed rocprofv3 --pmc
+    summary of this workload: executed MFMA FLOPs per dispatch (padding
+    included), their rate against the type's dense peak and the MFMA-busy
+    fraction (None when there is no summary)."""
+    path = os.path.join(ROOT, "profiles", "mfma_%s.md" % config)
+    if not os.path.exists(path):
+        return None
+    from lcpao_grid.profsum import read_mfma_table
+    tab = read_mfma_table(path)
+
+    def norm(n):
+        return n.split("lcpao::")[-1].replace(" ", "")
+
+    hits = [v for n, v in tab.items() if norm(n) == norm(kernel) or norm(n).split("<")[0] == kernel]
+    if not hits:
+        return None
+    typ, gf, tf, of_peak, busy = hits[0]
+    return {"type": typ, "executed_gflop_per_launch": gf, "executed_TFLOPs": tf, "of_type_peak": of_peak,
+            "mfma_busy": busy, "source": os.path.relpath(path, ROOT)}
+
+
 def copy_calibration(torch):
     a = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")  # 1 GiB
     b = torch.empty_like(a)
@@ -702,6 +724,9 @@ def main():
     if traffic is not None:
         roofline["traffic_source"] = traffic_src
         roofline["traffic_GBps"] = round(traffic / (stats[dom]["avg_ms"] * 1e-3) / 1e9, 1)
+    counters = mfma_from_profile(args.config, kern)
+    if counters is not None:
+        roofline["mfma_counters"] = counters
     out = {
         "metric": METRIC, "value": round(ms, 4), "unit": "ms", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": False, "scaling": "weak",

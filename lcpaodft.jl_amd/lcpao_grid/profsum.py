@@ -3,6 +3,8 @@
   python -m lcpao_grid.profsum stats  KERNEL_STATS_CSV [--title T] [--command C]
   python -m lcpao_grid.profsum traffic --fetch FETCH_CSV --write WRITE_CSV
          --copy-match SUBSTR --copy-bytes N [--title T] [--command C]
+  python -m lcpao_grid.profsum mfma COUNTER_CSV --agent AGENT_INFO_CSV
+         [--title T] [--command C]
 
 `stats` condenses a `--kernel-trace --stats` run (kernel_stats.csv) to a
 table of calls and average/total durations.  `traffic` turns two `--pmc`
@@ -11,6 +13,17 @@ per dispatch of every kernel, each counter scaled by what it reads for a
 plain copy kernel that moves a known number of bytes (N read and N written):
 factor = N / (counter KiB * 1024).  The table that `traffic` prints is read
 back by bench.py (column "traffic B/dispatch").
+
+`mfma` turns one `--pmc` run of SQ_INSTS_VALU_MFMA_MOPS_{BF16,F32,F64},
+SQ_VALU_MFMA_BUSY_CYCLES and GRBM_GUI_ACTIVE into the matrix-core use of
+every kernel: executed MFMA FLOPs per dispatch (512 per MOP, rocprofv3's own
+derived-counter rule), their rate over the dispatch's duration against the
+MI355X dense peak of that type, and the busy fraction
+SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE per XCC * SIMDs) (rocprofv3's
+MFMA-utilisation expression; the collected GRBM_GUI_ACTIVE is the sum over
+the XCCs, so it is divided by their number from the agent info).  Durations
+come from the counter run's own timestamps (serialised dispatches, slightly
+longer than in a plain run); bench.py reads the busy fraction back.
 """
 import argparse
 import csv
@@ -111,6 +124,77 @@ def read_traffic_table(path):
     return out
 
 
+MFMA_PEAK_TF = {"BF16": 2500.0, "F32": 157.3, "F64": 78.6}  # MI355X dense matrix peaks
+
+
+def read_mfma(path):
+    """{kernel: dict(n, ms, mops{type}, busy, gui)} averaged over dispatches."""
+    acc = defaultdict(lambda: defaultdict(list))
+    times = defaultdict(dict)
+    with open(path, newline="") as f:
+        for r in csv.DictReader(f):
+            k = short_name(r["Kernel_Name"])
+            acc[k][r["Counter_Name"]].append(float(r["Counter_Value"]))
+            times[k][r["Dispatch_Id"]] = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+    out = {}
+    for k, c in acc.items():
+        avg = {n: sum(v) / len(v) for n, v in c.items()}
+        ts = list(times[k].values())
+        out[k] = {"n": len(ts), "ms": sum(ts) / len(ts) * 1e-6, "busy": avg.get("SQ_VALU_MFMA_BUSY_CYCLES", 0.0),
+                  "gui": avg.get("GRBM_GUI_ACTIVE", 0.0),
+                  "mops": {t: avg.get("SQ_INSTS_VALU_MFMA_MOPS_" + t, 0.0) for t in MFMA_PEAK_TF}}
+    return out
+
+
+def read_agent(path):
+    with open(path, newline="") as f:
+        for r in csv.DictReader(f):
+            if r["Agent_Type"] == "GPU":
+                return int(r["Simd_Count"]), int(r["Num_Xcc"]), r["Gfx_Target_Version"]
+    sys.exit("no GPU agent in %s" % path)
+
+
+def cmd_mfma(a):
+    simds, nxcc, gfx = read_agent(a.agent)
+    rows = read_mfma(a.csv)
+    print("# %s" % (a.title or "Matrix-core use per dispatch (rocprofv3 --pmc)"))
+    print()
+    if a.command:
+        print("Command: `%s`" % a.command)
+        print()
+    print("Agent: %d SIMDs in %d XCCs (gfx target %s).  Executed MFMA FLOPs = 512 x SQ_INSTS_VALU_MFMA_MOPS_<type>; "
+          "rate over the counter run's dispatch duration; peak = MI355X dense matrix peak of the type (BF16 2500, "
+          "F32 157.3, F64 78.6 TFLOP/s); busy = SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE / %d x %d)."
+          % (simds, nxcc, gfx, nxcc, simds))
+    print()
+    print("| kernel | dispatches | ms | type | executed GFLOP/dispatch | MFMA TFLOP/s | of type peak | MFMA busy |")
+    print("|---|---:|---:|---|---:|---:|---:|---:|")
+    for k in sorted(rows, key=lambda x: -max(rows[x]["mops"].values())):
+        r = rows[k]
+        typ = max(r["mops"], key=lambda t: r["mops"][t])
+        if r["mops"][typ] <= 0:
+            continue
+        gf = 512.0 * r["mops"][typ] / 1e9
+        tf = gf / r["ms"] if r["ms"] > 0 else 0.0
+        busy = r["busy"] / (r["gui"] / nxcc * simds) if r["gui"] > 0 else 0.0
+        print("| `%s` | %d | %.4f | %s | %.1f | %.1f | %.3f | %.3f |"
+              % (k, r["n"], r["ms"], typ, gf, tf, tf / MFMA_PEAK_TF[typ], busy))
+
+
+def read_mfma_table(path):
+    """{kernel: (type, executed GFLOP per dispatch, TFLOP/s, of-peak, busy)} from an `mfma` summary."""
+    out = {}
+    with open(path) as f:
+        for line in f:
+            m = re.match(r"\|\s*`([^`]+)`\s*\|(.*)\|\s*$", line)
+            if not m:
+                continue
+            cols = [c.strip() for c in m.group(2).split("|")]
+            if len(cols) == 7:
+                out[m.group(1)] = (cols[2], float(cols[3]), float(cols[4]), float(cols[5]), float(cols[6]))
+    return out
+
+
 def main(argv=None):
     p = argparse.ArgumentParser(prog="lcpao_grid.profsum")
     sub = p.add_subparsers(dest="cmd", required=True)
@@ -125,8 +209,13 @@ def main(argv=None):
     t.add_argument("--copy-bytes", type=float, required=True)
     t.add_argument("--title", default="")
     t.add_argument("--command", default="")
+    m = sub.add_parser("mfma")
+    m.add_argument("csv")
+    m.add_argument("--agent", required=True)
+    m.add_argument("--title", default="")
+    m.add_argument("--command", default="")
     a = p.parse_args(argv)
-    {"stats": cmd_stats, "traffic": cmd_traffic}[a.cmd](a)
+    {"stats": cmd_stats, "traffic": cmd_traffic, "mfma": cmd_mfma}[a.cmd](a)
 
 
 if __name__ == "__main__":
