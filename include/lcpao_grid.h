@@ -272,6 +272,39 @@ int lcpao_grid_dm_states(lcpao_ctx *ctx, int32_t nk, const double *kpts, int64_t
                          double scale, double *DM);
 int lcpao_grid_dm_states_dev(lcpao_ctx *ctx, int32_t nk, const double *kpts, int64_t nstates,
                              const double *const *d_C, double scale, double *d_DM);
+/* Calc_DM_Crystal_Collinear_pol! (Crystal_DFT.jl:276-339) is two calls of
+ * lcpao_grid_dm_states, one per spin (Cnk[1] -> DM[1], Cnk[2] -> DM[2]). */
+
+/* HS_matrix_NC! with a k-point, flat Hks (HS_matrix.jl:211-257): H is one
+ * (2 fsize) x (2 fsize) complex matrix, column-major, overwritten:
+ *   H[A_i, B_j]                 = sum (hks[0] + im ihks[0]) * ex   (up-up)
+ *   H[fsize+A_i, fsize+B_j]     = sum (hks[1] + im ihks[1]) * ex   (down-down)
+ *   H[A_i, fsize+B_j]           = sum (hks[2] + im (hks[3] + ihks[2])) * ex
+ * with ex = cispi(2 k . atv_ijk[cell]); the down-up block stays zero as in
+ * the reference.  hks: the four flat Hks channels (my_hsize each); ihks: the
+ * three nested iHks arrays flattened in pair order (atom, Rn, ist, jst),
+ * which is the flat pair-block layout. */
+int lcpao_grid_bloch_sum_nc(lcpao_ctx *ctx, const double *kpt, const double *const *hks, const double *const *ihks,
+                            double *H);
+int lcpao_grid_bloch_sum_nc_dev(lcpao_ctx *ctx, const double *kpt, const double *const *d_hks,
+                                const double *const *d_ihks, double *d_H);
+
+/* Calc_DM_Crystal_NonCollinear! (Crystal_DFT.jl:342-410) from the spinor
+ * states: C[k] is (2 fsize) x nstates complex, column-major, scaled as
+ * Calc_fnkCnk! leaves it.  With P = C C^H formed per (A, B) block only:
+ *   DM[0] = sum_k Re(P[B_j, A_i] ex)              (up-up)
+ *   DM[1] = sum_k Re(P[fsize+B_j, fsize+A_i] ex)  (down-down)
+ *   DM[2] = sum_k Re(P[fsize+B_j, A_i] ex), DM[3] = sum_k Im(P[fsize+B_j, A_i] ex)
+ * with ex = cispi(2 k . atv_ijk[cell]) * scale.  iDM (two pointers, or NULL)
+ * receives the k-point sums of Calc_iDM_Crystal_NonCollinear!
+ * (Crystal_DFT.jl:413-486): iDM[0] = sum_k Im(P[B_j, A_i] ex), iDM[1] the
+ * same on the down-down block, in the flat pair order of its nested arrays.
+ * All outputs (my_hsize each) are overwritten; the MPI.Allreduce! stays the
+ * caller's. */
+int lcpao_grid_dm_states_nc(lcpao_ctx *ctx, int32_t nk, const double *kpts, int64_t nstates, const double *const *C,
+                            double scale, double *const *DM, double *const *iDM);
+int lcpao_grid_dm_states_nc_dev(lcpao_ctx *ctx, int32_t nk, const double *kpts, int64_t nstates,
+                                const double *const *d_C, double scale, double *const *d_DM, double *const *d_iDM);
 
 /* Sizes of the context (rank-local H size, grid size, sums of the lists). */
 int lcpao_grid_sizes(const lcpao_ctx *ctx, int64_t *my_hsize, int64_t *nn, int64_t *total_olg,

@@ -1,10 +1,12 @@
 // bloch.hip -- the steps on either side of the k-point diagonalisation:
-// the Bloch sums H(k), S(k) of the flat pair blocks (HS_matrix!) and the
-// flat pair density matrix from the occupied states of the k-points
-// (Calc_DM_Crystal_Collinear_nopol!).  Both work per atom pair (A, B): the
-// images (A, B, R) of a pair share one dense block, so a workgroup owns one
-// (A, B) and sums its images in the pair-list order, which is the order of
-// the reference's atom / neighbour loop.
+// the Bloch sums H(k), S(k) of the flat pair blocks (HS_matrix!, and the
+// 2x2 spin-block HS_matrix_NC!) and the flat pair density matrix from the
+// occupied states of the k-points (Calc_DM_Crystal_Collinear_nopol!, and
+// the four spin channels of Calc_DM_Crystal_NonCollinear! with the
+// imaginary parts of Calc_iDM_Crystal_NonCollinear!).  Both work per atom
+// pair (A, B): the images (A, B, R) of a pair share one dense block, so a
+// workgroup owns one (A, B) and sums its images in the pair-list order,
+// which is the order of the reference's atom / neighbour loop.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -45,15 +47,21 @@ constexpr int BL_MAXMAT = 4;
 
 struct BlochArgs {
     double k[3];
-    int64_t fsize;
+    int64_t ld;                       // leading dimension of the dense matrices
     int32_t nmat, pad;
-    const double *flat[BL_MAXMAT];
+    const double *flat[BL_MAXMAT];    // real parts of the flat pair arrays
+    const double *iflat[BL_MAXMAT][2];  // imaginary parts (summed, either may be null)
+    int64_t off_r[BL_MAXMAT], off_c[BL_MAXMAT];  // row / column offset of the block in dense
     double2 *dense[BL_MAXMAT];
 };
 
 // HS_matrix! with k (HS_matrix.jl:88-110, 143-166): dense[m][A_i, B_j] =
 // sum over the images (A, B, R) of flat[m][pair block][i, j] * cispi(2 k.R),
-// R = atv_ijk[cell]; dense is column-major (fsize x fsize) like Julia's matrix.
+// R = atv_ijk[cell]; dense is column-major (ld x ld) like Julia's matrix.
+// With imaginary parts (HS_matrix_NC!, HS_matrix.jl:211-257) the term is
+// (flat + im*(iflat0 + iflat1)) * cispi(2 k.R), a complex product in
+// Julia's order (re*c - im*s, re*s + im*c), at rows off_r + A_i and
+// columns off_c + B_j.
 __global__ __launch_bounds__(BL_THREADS) void k_bloch(const GroupDev *__restrict__ groups,
                                                        const GPair *__restrict__ gp, BlochArgs A) {
     const GroupDev G = groups[blockIdx.x];
@@ -73,14 +81,21 @@ __global__ __launch_bounds__(BL_THREADS) void k_bloch(const GroupDev *__restrict
             for (int m = 0; m < BL_MAXMAT; m++)
                 if (m < A.nmat) {
                     const double v = A.flat[m][x];
-                    acc[m].x += v * cs;
-                    acc[m].y += v * sn;
+                    if (A.iflat[m][0] || A.iflat[m][1]) {
+                        double w = A.iflat[m][0] ? A.iflat[m][0][x] : 0.0;
+                        if (A.iflat[m][1]) w = w + A.iflat[m][1][x];
+                        acc[m].x += v * cs - w * sn;
+                        acc[m].y += v * sn + w * cs;
+                    } else {
+                        acc[m].x += v * cs;
+                        acc[m].y += v * sn;
+                    }
                 }
         }
-        const int64_t o = (int64_t)(G.row_a + i) + (int64_t)(G.row_b + j) * A.fsize;
 #pragma unroll
         for (int m = 0; m < BL_MAXMAT; m++)
-            if (m < A.nmat) A.dense[m][o] = acc[m];
+            if (m < A.nmat)
+                A.dense[m][(A.off_r[m] + G.row_a + i) + (A.off_c[m] + G.row_b + j) * A.ld] = acc[m];
     }
 }
 
@@ -90,20 +105,26 @@ __global__ void k_zero_c(int64_t n, double2 *__restrict__ x) {
 }
 
 // Calc_DM_Crystal_Collinear_nopol! (Crystal_DFT.jl:219-270) without the dense
-// DM: per k-point, P[j][i] = sum_mu C[B_j, mu] conj(C[A_i, mu]) (the element
-// DM_dense[Bnum+jst, Anum+ist] of C C^H) for the (A, B) of the group only,
-// then DM[pair][i, j] += Re(P * cispi(2 k.R) / Nk) for each image, k-points in
-// order.  C[k]: fsize x nstates, column-major, columns already scaled by
-// sqrt(occupation * weight) (Calc_fnkCnk!).
+// DM: per k-point, P[j][i] = sum_mu C[rb + B_j, mu] conj(C[ra + A_i, mu]) (the
+// element DM_dense[rb+Bnum+jst, ra+Anum+ist] of C C^H) for the (A, B) of the
+// group only, then DM[pair][i, j] += Re(P * cispi(2 k.R) / Nk) and
+// iDM[pair][i, j] += Im(P * cispi(2 k.R) / Nk) for each image, k-points in
+// order.  C[k]: ld x nstates, column-major, columns already scaled by
+// sqrt(occupation * weight) (Calc_fnkCnk!).  Collinear: ld = fsize, ra = rb
+// = 0; the non-collinear spin blocks (Crystal_DFT.jl:342-410) take ld =
+// 2 fsize and (ra, rb) = (0, 0) for up-up, (fsize, fsize) for down-down and
+// (0, fsize) for the up-down block.
 constexpr int DM_CHUNK = 32;  // states per LDS tile
 
 struct DmArgs {
     int32_t nk, pad;
-    int64_t fsize, nstates;
+    int64_t ld, nstates;
+    int64_t ra, rb;           // row offsets of the A and B sides in C
     double scale;             // 1 / AllNkpt
     const double *kpts;       // nk x 3
     const double2 *const *C;  // nk pointers
-    double *DM;
+    double *DM;               // real part (may be null)
+    double *iDM;              // imaginary part (may be null)
 };
 
 __global__ __launch_bounds__(BL_THREADS) void k_dm_states(const GroupDev *__restrict__ groups,
@@ -125,8 +146,8 @@ __global__ __launch_bounds__(BL_THREADS) void k_dm_states(const GroupDev *__rest
                 const int r = x % (G.noa + G.nob), mu = x / (G.noa + G.nob);
                 double2 v = make_double2(0.0, 0.0);
                 if (mu < mc) {
-                    const int row = r < G.noa ? G.row_a + r : G.row_b + (r - G.noa);
-                    v = C[(int64_t)row + (m0 + mu) * A.fsize];
+                    const int64_t row = r < G.noa ? A.ra + G.row_a + r : A.rb + G.row_b + (r - G.noa);
+                    v = C[row + (m0 + mu) * A.ld];
                 }
                 if (r < G.noa) ta[r][mu] = v;
                 else tb[r - G.noa][mu] = v;
@@ -159,8 +180,14 @@ __global__ __launch_bounds__(BL_THREADS) void k_dm_states(const GroupDev *__rest
                 const int e = threadIdx.x + t * BL_THREADS;
                 if (e < nel) {
                     const int64_t x = P.hoff + e;  // e = i * nob + j
-                    const double v = acc[t].x * er - acc[t].y * ei;
-                    A.DM[x] = (kk == 0 ? 0.0 : A.DM[x]) + v;
+                    if (A.DM) {
+                        const double v = acc[t].x * er - acc[t].y * ei;
+                        A.DM[x] = (kk == 0 ? 0.0 : A.DM[x]) + v;
+                    }
+                    if (A.iDM) {
+                        const double v = acc[t].x * ei + acc[t].y * er;
+                        A.iDM[x] = (kk == 0 ? 0.0 : A.iDM[x]) + v;
+                    }
                 }
             }
         }
@@ -232,25 +259,15 @@ int bloch_stage(lcpao_ctx *ctx, BlochPlan *b, int64_t ndoubles) {
     return LCPAO_OK;
 }
 
-int bloch_run(lcpao_ctx *ctx, const double *kpt, int32_t nmat, const double *const *d_flat, double *const *d_dense) {
-    if (!kpt || !d_flat || !d_dense || nmat < 1 || nmat > BL_MAXMAT)
-        return ctx_error(ctx, LCPAO_EINVAL, "null argument or nmat not in 1..4");
-    for (int m = 0; m < nmat; m++)
-        if (!d_flat[m] || !d_dense[m]) return ctx_error(ctx, LCPAO_EINVAL, "null matrix pointer");
-    BlochPlan *b = nullptr;
-    int rc = bloch_plan(ctx, &b);
-    if (rc) return rc;
-    BlochArgs A{};
+// Zeroes the dense outputs (ld x ld complex each, a fill kernel) and runs
+// k_bloch once for the nmat blocks described by A (flat, iflat, offsets,
+// dense pointers filled in by the caller).
+int bloch_launch(lcpao_ctx *ctx, BlochPlan *b, const double *kpt, BlochArgs &A, int nzero, double2 *const *zero) {
     for (int x = 0; x < 3; x++) A.k[x] = kpt[x];
-    A.fsize = b->fsize;
-    A.nmat = nmat;
-    const int64_t nd = b->fsize * b->fsize;
-    for (int m = 0; m < nmat; m++) {
-        A.flat[m] = d_flat[m];
-        A.dense[m] = reinterpret_cast<double2 *>(d_dense[m]);
+    const int64_t nd = A.ld * A.ld;
+    for (int z = 0; z < nzero; z++)
         if (nd > 0)
-            hipLaunchKernelGGL(k_zero_c, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, ctx->stream, nd, A.dense[m]);
-    }
+            hipLaunchKernelGGL(k_zero_c, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, ctx->stream, nd, zero[z]);
     if (b->ngroups > 0) {
         KTimer kt(ctx, KID_BLOCH);
         hipLaunchKernelGGL(k_bloch, dim3((unsigned)b->ngroups), dim3(BL_THREADS), 0, ctx->stream, b->d_groups,
@@ -262,9 +279,63 @@ int bloch_run(lcpao_ctx *ctx, const double *kpt, int32_t nmat, const double *con
     return LCPAO_OK;
 }
 
-int dm_run(lcpao_ctx *ctx, int32_t nk, const double *kpts, int64_t nstates, const double *const *d_C, double scale,
-           double *d_DM) {
-    if (nk < 1 || !kpts || !d_C || !d_DM || nstates < 0) return ctx_error(ctx, LCPAO_EINVAL, "null argument or nk < 1");
+int bloch_run(lcpao_ctx *ctx, const double *kpt, int32_t nmat, const double *const *d_flat, double *const *d_dense) {
+    if (!kpt || !d_flat || !d_dense || nmat < 1 || nmat > BL_MAXMAT)
+        return ctx_error(ctx, LCPAO_EINVAL, "null argument or nmat not in 1..4");
+    for (int m = 0; m < nmat; m++)
+        if (!d_flat[m] || !d_dense[m]) return ctx_error(ctx, LCPAO_EINVAL, "null matrix pointer");
+    BlochPlan *b = nullptr;
+    int rc = bloch_plan(ctx, &b);
+    if (rc) return rc;
+    BlochArgs A{};
+    A.ld = b->fsize;
+    A.nmat = nmat;
+    double2 *zero[BL_MAXMAT];
+    for (int m = 0; m < nmat; m++) {
+        A.flat[m] = d_flat[m];
+        A.dense[m] = zero[m] = reinterpret_cast<double2 *>(d_dense[m]);
+    }
+    return bloch_launch(ctx, b, kpt, A, nmat, zero);
+}
+
+// HS_matrix_NC! with k (HS_matrix.jl:211-257): one (2 fsize)^2 matrix, the
+// up-up block from Hks[0] + im iHks[0], down-down from Hks[1] + im iHks[1],
+// up-down (rows A up, columns B down) from Hks[2] + im (Hks[3] + iHks[2]);
+// the down-up block stays zero, as in the reference.
+int bloch_run_nc(lcpao_ctx *ctx, const double *kpt, const double *const *d_hks, const double *const *d_ihks,
+                 double *d_H) {
+    if (!kpt || !d_hks || !d_ihks || !d_H) return ctx_error(ctx, LCPAO_EINVAL, "null argument");
+    for (int m = 0; m < 4; m++)
+        if (!d_hks[m] || (m < 3 && !d_ihks[m])) return ctx_error(ctx, LCPAO_EINVAL, "null matrix pointer");
+    BlochPlan *b = nullptr;
+    int rc = bloch_plan(ctx, &b);
+    if (rc) return rc;
+    BlochArgs A{};
+    A.ld = 2 * b->fsize;
+    A.nmat = 3;
+    double2 *H = reinterpret_cast<double2 *>(d_H);
+    const int64_t f = b->fsize;
+    const int64_t off_r[3] = {0, f, 0}, off_c[3] = {0, f, f};
+    for (int m = 0; m < 3; m++) {
+        A.flat[m] = d_hks[m];
+        A.iflat[m][0] = m < 2 ? d_ihks[m] : d_hks[3];
+        A.iflat[m][1] = m < 2 ? nullptr : d_ihks[2];
+        A.off_r[m] = off_r[m];
+        A.off_c[m] = off_c[m];
+        A.dense[m] = H;
+    }
+    return bloch_launch(ctx, b, kpt, A, 1, &H);
+}
+
+// One spin block of the density matrix from states (see k_dm_states).
+struct DmBlock {
+    int64_t ra, rb;
+    double *DM, *iDM;
+};
+
+int dm_run_blocks(lcpao_ctx *ctx, int32_t nk, const double *kpts, int64_t nstates, const double *const *d_C,
+                  double scale, int64_t ld_factor, int nblk, const DmBlock *blk) {
+    if (nk < 1 || !kpts || !d_C || nstates < 0) return ctx_error(ctx, LCPAO_EINVAL, "null argument or nk < 1");
     for (int k = 0; k < nk; k++)
         if (!d_C[k]) return ctx_error(ctx, LCPAO_EINVAL, "null state pointer");
     BlochPlan *b = nullptr;
@@ -278,15 +349,18 @@ int dm_run(lcpao_ctx *ctx, int32_t nk, const double *kpts, int64_t nstates, cons
     std::memcpy(h.data(), kpts, sizeof(double) * 3 * nk);
     std::memcpy(h.data() + sizeof(double) * 3 * nk, d_C, sizeof(void *) * nk);
     hipError_t e = hipMemcpy(d_blk, h.data(), bytes, hipMemcpyHostToDevice);
-    if (e == hipSuccess && b->ngroups > 0) {
+    for (int x = 0; x < nblk && e == hipSuccess && b->ngroups > 0; x++) {
         DmArgs A{};
         A.nk = nk;
-        A.fsize = b->fsize;
+        A.ld = ld_factor * b->fsize;
         A.nstates = nstates;
+        A.ra = blk[x].ra;
+        A.rb = blk[x].rb;
         A.scale = scale;
         A.kpts = reinterpret_cast<const double *>(d_blk);
         A.C = reinterpret_cast<const double2 *const *>(d_blk + sizeof(double) * 3 * nk);
-        A.DM = d_DM;
+        A.DM = blk[x].DM;
+        A.iDM = blk[x].iDM;
         {
             KTimer kt(ctx, KID_DMK);
             hipLaunchKernelGGL(k_dm_states, dim3((unsigned)b->ngroups), dim3(BL_THREADS), 0, ctx->stream, b->d_groups,
@@ -299,6 +373,32 @@ int dm_run(lcpao_ctx *ctx, int32_t nk, const double *kpts, int64_t nstates, cons
     if (e != hipSuccess) return ctx_error(ctx, LCPAO_EDEVICE, std::string("density matrix: ") + hipGetErrorString(e));
     collect_timings(ctx);
     return LCPAO_OK;
+}
+
+int dm_run(lcpao_ctx *ctx, int32_t nk, const double *kpts, int64_t nstates, const double *const *d_C, double scale,
+           double *d_DM) {
+    if (!d_DM) return ctx_error(ctx, LCPAO_EINVAL, "null density-matrix pointer");
+    const DmBlock blk{0, 0, d_DM, nullptr};
+    return dm_run_blocks(ctx, nk, kpts, nstates, d_C, scale, 1, 1, &blk);
+}
+
+// Calc_DM_Crystal_NonCollinear! (Crystal_DFT.jl:342-410): DM[0..3] = Re uu,
+// Re dd, Re ud, Im ud; with iDM (both or neither), the k-point sums of
+// Calc_iDM_Crystal_NonCollinear! (Crystal_DFT.jl:413-486): iDM[0..1] =
+// Im uu, Im dd, taken in the same passes.
+int dm_run_nc(lcpao_ctx *ctx, int32_t nk, const double *kpts, int64_t nstates, const double *const *d_C, double scale,
+              double *const *d_DM, double *const *d_iDM) {
+    if (!d_DM || !d_DM[0] || !d_DM[1] || !d_DM[2] || !d_DM[3])
+        return ctx_error(ctx, LCPAO_EINVAL, "null density-matrix pointer");
+    if (d_iDM && (!d_iDM[0] || !d_iDM[1])) return ctx_error(ctx, LCPAO_EINVAL, "null iDM pointer");
+    BlochPlan *b = nullptr;
+    int rc = bloch_plan(ctx, &b);
+    if (rc) return rc;
+    const int64_t f = b->fsize;
+    const DmBlock blk[3] = {{0, 0, d_DM[0], d_iDM ? d_iDM[0] : nullptr},
+                            {f, f, d_DM[1], d_iDM ? d_iDM[1] : nullptr},
+                            {0, f, d_DM[2], d_DM[3]}};
+    return dm_run_blocks(ctx, nk, kpts, nstates, d_C, scale, 2, 3, blk);
 }
 
 }  // namespace
@@ -372,6 +472,74 @@ int lcpao_grid_dm_states(lcpao_ctx *ctx, int32_t nk, const double *kpts, int64_t
     double *d_DM = b->stage + nk * per;
     if ((rc = lcpao::dm_run(ctx, nk, kpts, nstates, d_C.data(), scale, d_DM))) return rc;
     HIP_TRY(ctx, hipMemcpy(DM, d_DM, sizeof(double) * ctx->my_hsize, hipMemcpyDeviceToHost));
+    return LCPAO_OK;
+}
+
+int lcpao_grid_bloch_sum_nc_dev(lcpao_ctx *ctx, const double *kpt, const double *const *d_hks,
+                                const double *const *d_ihks, double *d_H) {
+    int rc = lcpao::ctx_enter(ctx);
+    if (rc) return rc;
+    return lcpao::bloch_run_nc(ctx, kpt, d_hks, d_ihks, d_H);
+}
+
+int lcpao_grid_bloch_sum_nc(lcpao_ctx *ctx, const double *kpt, const double *const *hks, const double *const *ihks,
+                            double *H) {
+    int rc = lcpao::ctx_enter(ctx);
+    if (rc) return rc;
+    if (!kpt || !hks || !ihks || !H) return lcpao::ctx_error(ctx, LCPAO_EINVAL, "null argument");
+    for (int m = 0; m < 4; m++)
+        if (!hks[m] || (m < 3 && !ihks[m])) return lcpao::ctx_error(ctx, LCPAO_EINVAL, "null matrix pointer");
+    lcpao::BlochPlan *b = nullptr;
+    if ((rc = lcpao::bloch_plan(ctx, &b))) return rc;
+    const int64_t hs = std::max<int64_t>(1, ctx->my_hsize), nd = 2 * std::max<int64_t>(1, 4 * b->fsize * b->fsize);
+    if ((rc = lcpao::bloch_stage(ctx, b, 7 * hs + nd))) return rc;
+    const double *d_hks[4], *d_ihks[3];
+    for (int m = 0; m < 7; m++) {
+        const double *src = m < 4 ? hks[m] : ihks[m - 4];
+        double *dst = b->stage + m * hs;
+        HIP_TRY(ctx, hipMemcpy(dst, src, sizeof(double) * ctx->my_hsize, hipMemcpyHostToDevice));
+        if (m < 4) d_hks[m] = dst;
+        else d_ihks[m - 4] = dst;
+    }
+    double *d_H = b->stage + 7 * hs;
+    if ((rc = lcpao::bloch_run_nc(ctx, kpt, d_hks, d_ihks, d_H))) return rc;
+    HIP_TRY(ctx, hipMemcpy(H, d_H, sizeof(double) * 8 * b->fsize * b->fsize, hipMemcpyDeviceToHost));
+    return LCPAO_OK;
+}
+
+int lcpao_grid_dm_states_nc_dev(lcpao_ctx *ctx, int32_t nk, const double *kpts, int64_t nstates,
+                                const double *const *d_C, double scale, double *const *d_DM, double *const *d_iDM) {
+    int rc = lcpao::ctx_enter(ctx);
+    if (rc) return rc;
+    return lcpao::dm_run_nc(ctx, nk, kpts, nstates, d_C, scale, d_DM, d_iDM);
+}
+
+int lcpao_grid_dm_states_nc(lcpao_ctx *ctx, int32_t nk, const double *kpts, int64_t nstates, const double *const *C,
+                            double scale, double *const *DM, double *const *iDM) {
+    int rc = lcpao::ctx_enter(ctx);
+    if (rc) return rc;
+    if (nk < 1 || !C || !DM || nstates < 0) return lcpao::ctx_error(ctx, LCPAO_EINVAL, "null argument or nk < 1");
+    for (int m = 0; m < 4; m++)
+        if (!DM[m]) return lcpao::ctx_error(ctx, LCPAO_EINVAL, "null density-matrix pointer");
+    if (iDM && (!iDM[0] || !iDM[1])) return lcpao::ctx_error(ctx, LCPAO_EINVAL, "null iDM pointer");
+    lcpao::BlochPlan *b = nullptr;
+    if ((rc = lcpao::bloch_plan(ctx, &b))) return rc;
+    const int64_t per = 4 * std::max<int64_t>(1, b->fsize * nstates), hs = std::max<int64_t>(1, ctx->my_hsize);
+    const int nout = iDM ? 6 : 4;
+    if ((rc = lcpao::bloch_stage(ctx, b, nk * per + nout * hs))) return rc;
+    std::vector<const double *> d_C(nk);
+    for (int k = 0; k < nk; k++) {
+        if (!C[k]) return lcpao::ctx_error(ctx, LCPAO_EINVAL, "null state pointer");
+        d_C[k] = b->stage + k * per;
+        HIP_TRY(ctx, hipMemcpy((void *)d_C[k], C[k], sizeof(double) * 4 * b->fsize * nstates, hipMemcpyHostToDevice));
+    }
+    double *d_out[6];
+    for (int m = 0; m < nout; m++) d_out[m] = b->stage + nk * per + m * hs;
+    if ((rc = lcpao::dm_run_nc(ctx, nk, kpts, nstates, d_C.data(), scale, d_out, iDM ? d_out + 4 : nullptr)))
+        return rc;
+    for (int m = 0; m < nout; m++)
+        HIP_TRY(ctx, hipMemcpy(m < 4 ? DM[m] : iDM[m - 4], d_out[m], sizeof(double) * ctx->my_hsize,
+                               hipMemcpyDeviceToHost));
     return LCPAO_OK;
 }
 

@@ -51,7 +51,8 @@ EXPORTS = [
     "lcpao_batch_last_error", "lcpao_batch_destroy",
     "lcpao_grid_aden_pcc", "lcpao_grid_aden_pcc_dev", "lcpao_grid_poisson", "lcpao_grid_poisson_dev",
     "lcpao_grid_bloch_sum", "lcpao_grid_bloch_sum_dev", "lcpao_grid_bloch_sizes", "lcpao_grid_dm_states",
-    "lcpao_grid_dm_states_dev",
+    "lcpao_grid_dm_states_dev", "lcpao_grid_bloch_sum_nc", "lcpao_grid_bloch_sum_nc_dev", "lcpao_grid_dm_states_nc",
+    "lcpao_grid_dm_states_nc_dev",
 ]
 
 _lib = None
@@ -113,6 +114,10 @@ def load():
     L.lcpao_grid_bloch_sizes.argtypes = [vp, lp, lp]
     L.lcpao_grid_dm_states.argtypes = [vp, C.c_int32, dp, C.c_int64, dpp, C.c_double, dp]
     L.lcpao_grid_dm_states_dev.argtypes = [vp, C.c_int32, dp, C.c_int64, dpp, C.c_double, dp]
+    L.lcpao_grid_bloch_sum_nc.argtypes = [vp, dp, dpp, dpp, dp]
+    L.lcpao_grid_bloch_sum_nc_dev.argtypes = [vp, dp, dpp, dpp, dp]
+    L.lcpao_grid_dm_states_nc.argtypes = [vp, C.c_int32, dp, C.c_int64, dpp, C.c_double, dpp, dpp]
+    L.lcpao_grid_dm_states_nc_dev.argtypes = [vp, C.c_int32, dp, C.c_int64, dpp, C.c_double, dpp, dpp]
     L.lcpao_grid_set_stream.argtypes = [vp, vp]
     L.lcpao_grid_profile.argtypes = [vp, C.c_int32]
     L.lcpao_grid_kernel_stats.argtypes = [vp, C.c_int32, lp, dp]

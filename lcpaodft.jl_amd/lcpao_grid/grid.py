@@ -266,6 +266,48 @@ class GridContext:
                                                     self._dptrs(dstates), float(scale),
                                                     C.cast(dDM.data_ptr(), _lib.dp)))
 
+    def bloch_sum_nc(self, kpt, hks, ihks):
+        """HS_matrix_NC! with a k-point: four flat Hks channels and the three
+        iHks arrays flattened in pair order -> dense complex (2 fsize, 2 fsize)."""
+        fsize, _ = self.bloch_sizes()
+        Hs = [np.ascontiguousarray(f, np.float64) for f in hks]
+        Is = [np.ascontiguousarray(f, np.float64) for f in ihks]
+        if len(Hs) != 4 or len(Is) != 3:
+            raise ValueError("HS_matrix_NC needs four Hks and three iHks arrays")
+        H = np.zeros((2 * fsize, 2 * fsize), np.complex128, order="F")
+        k = np.ascontiguousarray(kpt, np.float64)
+        self._check(self.L.lcpao_grid_bloch_sum_nc(self.h, k.ctypes.data_as(_lib.dp), _ptrs(Hs), _ptrs(Is),
+                                                   H.reshape(-1, order="F").view(np.float64).ctypes.data_as(_lib.dp)))
+        return H
+
+    def bloch_sum_nc_dev(self, kpt, dhks, dihks, dH):
+        """Device form; dH is a complex128 tensor (2 fsize, 2 fsize) in
+        column-major memory."""
+        k = np.ascontiguousarray(kpt, np.float64)
+        self._order()
+        self._check(self.L.lcpao_grid_bloch_sum_nc_dev(self.h, k.ctypes.data_as(_lib.dp), self._dptrs(dhks),
+                                                       self._dptrs(dihks), C.cast(dH.data_ptr(), _lib.dp)))
+
+    def dm_states_nc(self, kpts, states, scale, with_idm=False):
+        """Calc_DM_Crystal_NonCollinear!: spinor states per k-point (complex
+        (2 fsize, nstates), columns scaled) -> (DM[4], iDM[2] or None)."""
+        kp = np.ascontiguousarray(kpts, np.float64).reshape(-1, 3)
+        cs = [np.asfortranarray(c, np.complex128) for c in states]
+        nst = cs[0].shape[1]
+        flat = [np.ascontiguousarray(c.reshape(-1, order="F")).view(np.float64) for c in cs]
+        DM = [np.zeros(self.my_hsize) for _ in range(4)]
+        iDM = [np.zeros(self.my_hsize) for _ in range(2)] if with_idm else None
+        self._check(self.L.lcpao_grid_dm_states_nc(self.h, len(cs), kp.ctypes.data_as(_lib.dp), nst, _ptrs(flat),
+                                                   float(scale), _ptrs(DM), _ptrs(iDM) if with_idm else None))
+        return DM, iDM
+
+    def dm_states_nc_dev(self, kpts, dstates, nstates, scale, dDM, diDM=None):
+        kp = np.ascontiguousarray(kpts, np.float64).reshape(-1, 3)
+        self._order()
+        self._check(self.L.lcpao_grid_dm_states_nc_dev(self.h, kp.shape[0], kp.ctypes.data_as(_lib.dp), int(nstates),
+                                                       self._dptrs(dstates), float(scale), self._dptrs(dDM),
+                                                       None if diDM is None else self._dptrs(diDM)))
+
     def set_positions(self, gxyz):
         """Atom positions (natom x 3, bohr) for the next build_tables; lists kept."""
         g = np.ascontiguousarray(gxyz, np.float64).reshape(-1, 3)
@@ -455,3 +497,37 @@ def Calc_DM_Crystal_Collinear_nopol(Cnk, MPI_kpts, AllNkpt, DM, Orbs_Grid):
     DM[0] is overwritten with this rank's part (the reference then sums the
     ranks with MPI.Allreduce!)."""
     DM[0][:] = Orbs_Grid.dm_states(MPI_kpts, Cnk, 1.0 / AllNkpt)
+
+
+def Calc_DM_Crystal_Collinear_pol(Cnk, MPI_kpts, AllNkpt, DM, Orbs_Grid):
+    """The accumulation of Calc_DM_Crystal_Collinear_pol! (Crystal_DFT.jl:276-339):
+    Cnk[s] are the scaled states of spin s per k-point; DM[0] and DM[1] are
+    overwritten with this rank's part, one device pass per spin."""
+    for s in range(2):
+        DM[s][:] = Orbs_Grid.dm_states(MPI_kpts, Cnk[s], 1.0 / AllNkpt)
+
+
+def Calc_DM_Crystal_NonCollinear(Cnk, MPI_kpts, AllNkpt, DM, Orbs_Grid):
+    """The accumulation of Calc_DM_Crystal_NonCollinear! (Crystal_DFT.jl:342-410):
+    Cnk are the scaled spinor states (2 fsize, nstates) per k-point; DM[0..3]
+    (up-up, down-down, Re up-down, Im up-down) are overwritten."""
+    out, _ = Orbs_Grid.dm_states_nc(MPI_kpts, Cnk, 1.0 / AllNkpt)
+    for s in range(4):
+        DM[s][:] = out[s]
+
+
+def Calc_iDM_Crystal_NonCollinear(Cnk, MPI_kpts, AllNkpt, iDM, Orbs_Grid):
+    """The k-point sums of Calc_iDM_Crystal_NonCollinear! (Crystal_DFT.jl:413-486,
+    cal_mode != 2): iDM[0], iDM[1] (imaginary parts of the up-up and down-down
+    blocks, the nested arrays flattened in pair order) are overwritten."""
+    _, out = Orbs_Grid.dm_states_nc(MPI_kpts, Cnk, 1.0 / AllNkpt, with_idm=True)
+    iDM[0][:] = out[0]
+    iDM[1][:] = out[1]
+
+
+def HS_matrix_NC(H, Hks, iHks, kpts, Orbs_Grid):
+    """HS_matrix_NC!(H, Hks::Vector{Vector{Float64}}, iHks, ..., kpts)
+    (HS_matrix.jl:211-257): Hks are the four flat channels, iHks the three
+    nested arrays flattened in pair order; H (complex 2 fsize x 2 fsize) is
+    overwritten (its down-up block zero, as in the reference)."""
+    H[:, :] = Orbs_Grid.bloch_sum_nc(kpts, Hks, iHks)

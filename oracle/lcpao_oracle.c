@@ -27,6 +27,17 @@
  *   orc_hamiltonian/density : brute-force point-major sums, duality,
  *                             Hermiticity, linearity (tests)
  *   orc_force3              : central differences of the grid energy
+ *   orc_hs_matrix           : Hermiticity, k-periodicity, Gamma sum rule,
+ *                             duality with orc_dm_collinear (tests)
+ *   orc_dm_collinear        : duality with orc_hs_matrix, Gamma Gram blocks
+ *   orc_hs_matrix_nc        : linearity reduction of each spin block to
+ *                             orc_hs_matrix (real part + im * imaginary part
+ *                             at its block position), zero down-up block
+ *   orc_dm_nc               : spin-diagonal channels = orc_dm_collinear of
+ *                             each spin half; up-down channels and iDM from
+ *                             states with C_down = C_up and C_down = i C_up,
+ *                             which turn them into orc_dm_collinear outputs
+ *                             (tests)
  */
 #define _GNU_SOURCE
 #include <ctype.h>
@@ -1774,6 +1785,109 @@ int orc_hs_matrix(const orc_geom *G, const double *kpt, int nmat, const double *
                 }
             }
     }
+    free(mp);
+    return 0;
+}
+
+/* HS_matrix_NC! with k and flat Hks (HS_matrix.jl:211-257): H ((2 fsize)^2,
+ * column-major, interleaved complex) is zeroed, then for each pair and
+ * (ist, jst), with ex = cispi(2 kRn):
+ *   H[A+i, B+j]             += (Hks[0] + im*iHks[0]) * ex
+ *   H[fsize+A+i, fsize+B+j] += (Hks[1] + im*iHks[1]) * ex
+ *   H[A+i, fsize+B+j]       += (Hks[2] + im*(Hks[3] + iHks[2])) * ex
+ * iHks are the nested arrays flattened in (atom, Rn, ist, jst) order. */
+static void cmul_add(double *z, double ar, double ai, double br, double bi) {
+    z[0] += ar * br - ai * bi;
+    z[1] += ar * bi + ai * br;
+}
+
+int orc_hs_matrix_nc(const orc_geom *G, const double *kpt, const double *const *hks, const double *const *ihks,
+                     double *H) {
+    int64_t *mp = orbital_offsets(G);
+    const int64_t fsize = mp[G->natom], ld = 2 * fsize;
+    memset(H, 0, sizeof(double) * 2 * ld * ld);
+    const double ka = kpt[0], kb = kpt[1], kc = kpt[2];
+    for (int64_t p = 0; p < G->npair; p++) {
+        const int atom = G->p_atom[p], jatom = G->p_natn[p], cell = G->p_ncn[p];
+        const int NO0 = G->no[atom], NO1 = G->no[jatom];
+        const int64_t Anum = mp[atom], Bnum = mp[jatom];
+        const double kRn = ka * G->atv_ijk[3 * cell] + kb * G->atv_ijk[3 * cell + 1] + kc * G->atv_ijk[3 * cell + 2];
+        double c, s;
+        orc_cispi(2 * kRn, &c, &s);
+        int64_t hst = G->p_hoff[p];
+        for (int ist = 0; ist < NO0; ist++)
+            for (int jst = 0; jst < NO1; jst++, hst++) {
+                const int64_t uu = (Anum + ist) + (Bnum + jst) * ld;
+                const int64_t dd = (fsize + Anum + ist) + (fsize + Bnum + jst) * ld;
+                const int64_t ud = (Anum + ist) + (fsize + Bnum + jst) * ld;
+                cmul_add(H + 2 * uu, hks[0][hst], ihks[0][hst], c, s);
+                cmul_add(H + 2 * dd, hks[1][hst], ihks[1][hst], c, s);
+                cmul_add(H + 2 * ud, hks[2][hst], hks[3][hst] + ihks[2][hst], c, s);
+            }
+    }
+    free(mp);
+    return 0;
+}
+
+/* Calc_DM_Crystal_NonCollinear! (Crystal_DFT.jl:342-410) and the k-point sums
+ * of Calc_iDM_Crystal_NonCollinear! (:413-486): per k-point the dense
+ * DM_dense = C C^H ((2 fsize)^2, a plain triple loop), then for each pair
+ * with ex = cispi(2 kRn) * scale:
+ *   DM[0] += Re(D[B+j, A+i] ex), DM[1] += Re(D[f+B+j, f+A+i] ex),
+ *   DM[2] += Re(D[f+B+j, A+i] ex), DM[3] += Im(D[f+B+j, A+i] ex),
+ *   iDM[0] += Im(D[B+j, A+i] ex), iDM[1] += Im(D[f+B+j, f+A+i] ex)
+ * (iDM may be NULL).  C[k]: (2 fsize) x nstates interleaved complex. */
+int orc_dm_nc(const orc_geom *G, int nk, const double *kpts, int64_t nstates, const double *const *C, double scale,
+              double *const *DM, double *const *iDM) {
+    int64_t *mp = orbital_offsets(G);
+    const int64_t fsize = mp[G->natom], ld = 2 * fsize;
+    double *D = malloc(sizeof(double) * 2 * ld * ld);
+    if (!D) { free(mp); return fail("oom"); }
+    for (int m = 0; m < 4; m++) memset(DM[m], 0, sizeof(double) * G->my_hsize);
+    if (iDM)
+        for (int m = 0; m < 2; m++) memset(iDM[m], 0, sizeof(double) * G->my_hsize);
+    for (int k = 0; k < nk; k++) {
+        const double *c = C[k];
+        for (int64_t col = 0; col < ld; col++)
+            for (int64_t row = 0; row < ld; row++) {
+                double re = 0.0, im = 0.0;
+                for (int64_t mu = 0; mu < nstates; mu++) {
+                    const double ar = c[2 * (row + mu * ld)], ai = c[2 * (row + mu * ld) + 1];
+                    const double br = c[2 * (col + mu * ld)], bi = c[2 * (col + mu * ld) + 1];
+                    re += ar * br + ai * bi; /* a * conj(b) */
+                    im += ai * br - ar * bi;
+                }
+                D[2 * (row + col * ld)] = re;
+                D[2 * (row + col * ld) + 1] = im;
+            }
+        const double ka = kpts[3 * k], kb = kpts[3 * k + 1], kc = kpts[3 * k + 2];
+        for (int64_t p = 0; p < G->npair; p++) {
+            const int atom = G->p_atom[p], jatom = G->p_natn[p], cell = G->p_ncn[p];
+            const int NO0 = G->no[atom], NO1 = G->no[jatom];
+            const int64_t Anum = mp[atom], Bnum = mp[jatom];
+            const double kRn = ka * G->atv_ijk[3 * cell] + kb * G->atv_ijk[3 * cell + 1] + kc * G->atv_ijk[3 * cell + 2];
+            double er, ei;
+            orc_cispi(2 * kRn, &er, &ei);
+            er = er * scale;
+            ei = ei * scale;
+            int64_t DMst = G->p_hoff[p];
+            for (int ist = 0; ist < NO0; ist++)
+                for (int jst = 0; jst < NO1; jst++, DMst++) {
+                    const double *uu = D + 2 * ((Bnum + jst) + (Anum + ist) * ld);
+                    const double *dd = D + 2 * ((fsize + Bnum + jst) + (fsize + Anum + ist) * ld);
+                    const double *ud = D + 2 * ((fsize + Bnum + jst) + (Anum + ist) * ld);
+                    DM[0][DMst] += uu[0] * er - uu[1] * ei;
+                    DM[1][DMst] += dd[0] * er - dd[1] * ei;
+                    DM[2][DMst] += ud[0] * er - ud[1] * ei;
+                    DM[3][DMst] += ud[0] * ei + ud[1] * er;
+                    if (iDM) {
+                        iDM[0][DMst] += uu[0] * ei + uu[1] * er;
+                        iDM[1][DMst] += dd[0] * ei + dd[1] * er;
+                    }
+                }
+        }
+    }
+    free(D);
     free(mp);
     return 0;
 }

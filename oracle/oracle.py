@@ -330,3 +330,31 @@ def dm_collinear(geom, kpts, states, scale):
     _check(lib().orc_dm_collinear(geom.h, len(cs), kp.ctypes.data_as(_dp), C.c_int64(nst), _dptrs(flat),
                                   C.c_double(scale), DM.ctypes.data_as(_dp)))
     return DM
+
+
+def hs_matrix_nc(geom, kpt, hks, ihks):
+    """HS_matrix_NC! with k and flat Hks (HS_matrix.jl:211-257): hks = four
+    flat channels, ihks = the three nested iHks arrays flattened in pair
+    order -> dense complex (2 fsize, 2 fsize) matrix."""
+    fsize = int(np.asarray(geom.no, dtype=np.int64).sum())
+    Hs = [np.ascontiguousarray(f, dtype=np.float64) for f in hks]
+    Is = [np.ascontiguousarray(f, dtype=np.float64) for f in ihks]
+    out = np.zeros(2 * (2 * fsize) ** 2)
+    k = np.ascontiguousarray(kpt, dtype=np.float64)
+    _check(lib().orc_hs_matrix_nc(geom.h, k.ctypes.data_as(_dp), _dptrs(Hs), _dptrs(Is), out.ctypes.data_as(_dp)))
+    return out.view(np.complex128).reshape(2 * fsize, 2 * fsize, order="F")
+
+
+def dm_nc(geom, kpts, states, scale, with_idm=True):
+    """Calc_DM_Crystal_NonCollinear! (Crystal_DFT.jl:342-410) and the k-point
+    sums of Calc_iDM_Crystal_NonCollinear! (:413-486): states per k-point are
+    complex (2 fsize, nstates); returns (DM[4], iDM[2] or None)."""
+    kp = np.ascontiguousarray(kpts, dtype=np.float64).reshape(-1, 3)
+    cs = [np.asfortranarray(c, dtype=np.complex128) for c in states]
+    nst = cs[0].shape[1]
+    flat = [np.ascontiguousarray(c.reshape(-1, order="F")).view(np.float64) for c in cs]
+    DM = [np.zeros(geom.my_hsize) for _ in range(4)]
+    iDM = [np.zeros(geom.my_hsize) for _ in range(2)] if with_idm else None
+    _check(lib().orc_dm_nc(geom.h, len(cs), kp.ctypes.data_as(_dp), C.c_int64(nst), _dptrs(flat), C.c_double(scale),
+                           _dptrs(DM), _dptrs(iDM) if with_idm else None))
+    return DM, iDM

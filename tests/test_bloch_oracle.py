@@ -76,3 +76,64 @@ def test_dm_at_gamma_with_real_states_is_a_gram_block(si2_pairs):
         a, b = g.pair_atom[p], g.pair_natn[p]
         blk = DM[g.pair_hoff[p]:g.pair_hoff[p] + g.no[a] * g.no[b]].reshape(g.no[a], g.no[b])
         assert np.allclose(blk, D[mp[b]:mp[b + 1], mp[a]:mp[a + 1]].T, rtol=1e-13, atol=1e-13)
+
+
+def test_hs_matrix_nc_blocks_reduce_to_the_collinear_sum(si2_pairs):
+    """Each spin block of HS_matrix_NC! is linear in its real and imaginary
+    flat parts: block = hs(re) + i hs(im) at the block's position (up-up,
+    down-down, up-down), and the down-up block stays zero."""
+    g, Sf, Hf = si2_pairs
+    fsize = int(g.no.sum())
+    rng = np.random.default_rng(7)
+    hks = [Hf, Sf, rng.standard_normal(Hf.size), rng.standard_normal(Hf.size)]
+    ihks = [rng.standard_normal(Hf.size) for _ in range(3)]
+    k = [0.17, -0.29, 0.36]
+    H = O.hs_matrix_nc(g, k, hks, ihks)
+    (h0, h1, h2, h3, i0, i1, i2) = O.hs_matrix(g, k, hks + ihks)
+    f = fsize
+    scale = np.abs(H).max()
+    for blk, want in ((H[:f, :f], h0 + 1j * i0), (H[f:, f:], h1 + 1j * i1), (H[:f, f:], h2 + 1j * (h3 + i2))):
+        assert np.abs(blk - want).max() < 1e-13 * scale
+    assert np.abs(H[f:, :f]).max() == 0.0
+    # a planted swap of the up-down block's imaginary parts is caught
+    (bad,) = O.hs_matrix(g, k, [hks[3] - ihks[2]])
+    assert np.abs(H[:f, f:] - (h2 + 1j * bad)).max() > 1e-6 * scale
+
+
+def test_dm_nc_reduces_to_the_collinear_dm(si2_pairs):
+    """Calc_DM_Crystal_NonCollinear!: the spin-diagonal channels are the
+    collinear DM of each spin half; with C_down = C_up the up-down channels
+    become (collinear DM, iDM up-up), with C_down = i C_up they become
+    (-iDM up-up, collinear DM); iDM of an inter-atom pair is minus the
+    collinear DM of states whose rows of atom 1 carry a factor i."""
+    g, _, _ = si2_pairs
+    f = int(g.no.sum())
+    kpts = np.array([[0.0, 0.0, 0.0], [0.3, -0.2, 0.15], [-0.45, 0.1, 0.25]])
+    sc = 1.0 / len(kpts)
+    up, dn = _states(f, 6, len(kpts), 8), _states(f, 6, len(kpts), 9)
+    DM, iDM = O.dm_nc(g, kpts, [np.vstack([u, d]) for u, d in zip(up, dn)], sc)
+    c_up, c_dn = O.dm_collinear(g, kpts, up, sc), O.dm_collinear(g, kpts, dn, sc)
+    tol = 1e-13 * np.abs(c_up).max()
+    assert np.abs(DM[0] - c_up).max() < tol and np.abs(DM[1] - c_dn).max() < tol
+    DMs, iDMs = O.dm_nc(g, kpts, [np.vstack([u, u]) for u in up], sc)
+    assert np.abs(DMs[2] - c_up).max() < tol and np.abs(DMs[3] - iDMs[0]).max() < tol
+    assert np.abs(iDMs[1] - iDMs[0]).max() < tol
+    DMi, iDMi = O.dm_nc(g, kpts, [np.vstack([u, 1j * u]) for u in up], sc)
+    assert np.abs(DMi[3] - c_up).max() < tol and np.abs(DMi[2] + iDMs[0]).max() < tol
+    # iDM against the collinear DM of phase-shifted states (atom 1 rows times i)
+    mp = np.concatenate([[0], np.cumsum(g.no)])
+    shifted = []
+    for u in up:
+        v = u.copy()
+        v[mp[1]:mp[2]] *= 1j
+        shifted.append(v)
+    c_sh = O.dm_collinear(g, kpts, shifted, sc)
+    checked = 0
+    for p in range(g.npair):
+        a, b = g.pair_atom[p], g.pair_natn[p]
+        if a == 0 and b == 1:
+            sl = slice(g.pair_hoff[p], g.pair_hoff[p] + g.no[a] * g.no[b])
+            assert np.abs(iDMs[0][sl] + c_sh[sl]).max() < tol
+            checked += 1
+    assert checked > 0
+    assert np.abs(iDMs[0]).max() > 1e-3 * np.abs(c_up).max()  # the imaginary parts are not trivially zero

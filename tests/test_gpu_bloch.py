@@ -77,3 +77,62 @@ def test_dm_bloch_duality_gaas216_device_resident():
     print("duality", lhs, rhs, flush=True)
     assert abs(lhs - rhs) < 1e-10 * abs(rhs)
     ctx.close()
+
+
+@pytest.mark.parametrize("name", ["si2", "si64"])
+def test_nc_bloch_sum_and_nc_dm_match_oracle(name):
+    """HS_matrix_NC! and Calc_DM_Crystal_NonCollinear! (+ the iDM sums) on
+    the device against the oracle, host and device entry points; the
+    spin-polarised DM (two collinear passes) through the Python mirror."""
+    import torch
+    from lcpao_grid import grid as Gm
+    st = S.CONFIGS[name]()
+    u = st.ucell()
+    g = O.Geom(st.Latvecs, st.Gxyz, st.cutoffs(), st.norbs(), st.Ngrid)
+    ctx = GridContext(u)
+    fsize, _ = ctx.bloch_sizes()
+    rng = np.random.default_rng(21)
+    hks = [rng.standard_normal(u.myHsize) for _ in range(4)]
+    ihks = [rng.standard_normal(u.myHsize) for _ in range(3)]
+    k = [0.19, -0.31, 0.27]
+    print(name, "nc bloch", flush=True)
+    got = ctx.bloch_sum_nc(k, hks, ihks)
+    want = O.hs_matrix_nc(g, k, hks, ihks)
+    assert _rel(got, want) < 1e-13
+    Hm = np.zeros_like(got)
+    Gm.HS_matrix_NC(Hm, hks, ihks, k, ctx)
+    assert np.array_equal(Hm, got)
+    dev = torch.device("cuda:0")
+    dH = torch.empty((2 * fsize, 2 * fsize), dtype=torch.complex128, device=dev)
+    ctx.bloch_sum_nc_dev(k, [torch.from_numpy(x).to(dev) for x in hks], [torch.from_numpy(x).to(dev) for x in ihks], dH)
+    assert np.array_equal(dH.T.cpu().numpy(), got)  # column-major memory -> the matrix
+
+    kpts = np.array([[0.0, 0.0, 0.0], [0.4, -0.15, 0.2]])
+    nst = 7 if name == "si2" else 24
+    Cs = _states(2 * fsize, nst, len(kpts), 4)
+    print(name, "nc dm", flush=True)
+    DM, iDM = ctx.dm_states_nc(kpts, Cs, 0.5, with_idm=True)
+    DMo, iDMo = O.dm_nc(g, kpts, Cs, 0.5)
+    for a, b in zip(DM + iDM, DMo + iDMo):
+        assert _rel(a, b) < 1e-13
+    DMn, none = ctx.dm_states_nc(kpts, Cs, 0.5)
+    assert none is None and all(np.array_equal(a, b) for a, b in zip(DMn, DM))
+    dC = [torch.from_numpy(np.ascontiguousarray(c.T)).to(dev) for c in Cs]  # (nst, 2f) row-major = column-major C
+    dDM = [torch.empty(u.myHsize, dtype=torch.float64, device=dev) for _ in range(4)]
+    diDM = [torch.empty(u.myHsize, dtype=torch.float64, device=dev) for _ in range(2)]
+    ctx.dm_states_nc_dev(kpts, dC, nst, 0.5, dDM, diDM)
+    for a, b in zip(dDM + diDM, DM + iDM):
+        assert np.array_equal(a.cpu().numpy(), b)
+    out4 = [np.zeros(u.myHsize) for _ in range(4)]
+    Gm.Calc_DM_Crystal_NonCollinear(Cs, kpts, 2, out4, ctx)
+    out2 = [np.zeros(u.myHsize) for _ in range(2)]
+    Gm.Calc_iDM_Crystal_NonCollinear(Cs, kpts, 2, out2, ctx)
+    assert all(np.array_equal(a, b) for a, b in zip(out4 + out2, DM + iDM))
+
+    print(name, "pol dm", flush=True)
+    up, dn = _states(fsize, nst, len(kpts), 5), _states(fsize, nst, len(kpts), 6)
+    pol = [np.zeros(u.myHsize), np.zeros(u.myHsize)]
+    Gm.Calc_DM_Crystal_Collinear_pol([up, dn], kpts, 2, pol, ctx)
+    assert _rel(pol[0], O.dm_collinear(g, kpts, up, 0.5)) < 1e-13
+    assert _rel(pol[1], O.dm_collinear(g, kpts, dn, 0.5)) < 1e-13
+    ctx.close()
