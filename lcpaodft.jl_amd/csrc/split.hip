@@ -1002,6 +1002,14 @@ constexpr int RB_AREA_BYTES = RB_SUMS_OFFSET + RB_WAVES * BLK_PTS * 4;
 constexpr int RB_IMAGE_BYTES = HB_WAVES * 8 * 2 * 512 * 2;    // W images of the 8-wave (batched) form
 static_assert(RB_AREA_BYTES <= RB_IMAGE_BYTES, "the 8-wave form's LDS covers the planes and sums");
 static_assert(RB_WAVES * (16 / RB_WAVES) == HB_WAVES * 2, "an item's super-tiles fill the density waves");
+// Double-buffered form of the 16-wave kernel: two chunk-plane buffers (chunk
+// c is staged into buffer c % 2 while the other holds chunk c - 1, so one
+// barrier per chunk suffices), then the per-chunk wave sums (two buffers of
+// [wave][32 points]) and the block's 512 point sums.
+constexpr int RB_DB_BUF_BYTES = MAX_ROWS * RB_BYTES_PER_ROW;
+constexpr int RB_DB_WSUM_OFFSET = 2 * RB_DB_BUF_BYTES;
+constexpr int RB_DB_PSUM_OFFSET = RB_DB_WSUM_OFFSET + 2 * RB_WAVES * CHUNK * 4;
+constexpr int RB_DB_AREA_BYTES = RB_DB_PSUM_OFFSET + BLK_PTS * 4;
 
 // L2 warm-up of a chunk by NT threads: one dword per 128-byte line, K loads
 // per thread (lines past the chunk repeat line 0), as touch_chunk.
@@ -1032,10 +1040,14 @@ __device__ __forceinline__ void release_n(const TouchN<NT> &t) {
 
 // Density of one item: W waves, each owning PER = 16 / W of the item's
 // super-tiles.  With W = 16 a wave holds the W fragments of one super-tile
-// (64 VGPRs), so four waves fit per SIMD.
-template <int NF, int W>
+// (64 VGPRs), so four waves fit per SIMD.  DB (W = 16 only): double-buffered
+// chunk planes with one barrier per chunk; the 16 wave sums of a chunk's
+// points are added (in wave order, as in the single-buffered form) by one
+// wave after the next barrier, so both forms give the same bits.
+template <int NF, int W, bool DB>
 __device__ __forceinline__ void rblk_item(const HItem it, const BlockArgs &A, unsigned char *lds_raw) {
     constexpr int PER = 16 / W;
+    static_assert(!DB || W == RB_WAVES, "the double-buffered form has one super-tile per wave");
     const BlockDev B = as_global(A.blocks)[it.block];
     const GLOBAL_AS uint16_t *__restrict__ thi = as_global(A.thi);
     const GLOBAL_AS uint16_t *__restrict__ tlo = as_global(A.tlo);
@@ -1051,7 +1063,10 @@ __device__ __forceinline__ void rblk_item(const HItem it, const BlockArgs &A, un
     // LDS: the chunk planes, the per-wave point sums, then the block's row
     // descriptors and slot pairs
     float *rcw = reinterpret_cast<float *>(lds_raw + RB_SUMS_OFFSET);  // [wave][512 points]
-    int16_t *s_rinfo = reinterpret_cast<int16_t *>(lds_raw + (PER == 1 ? RB_AREA_BYTES : RB_IMAGE_BYTES));
+    float *wsum = reinterpret_cast<float *>(lds_raw + RB_DB_WSUM_OFFSET);  // DB: [chunk % 2][wave][32 points]
+    float *psum = reinterpret_cast<float *>(lds_raw + RB_DB_PSUM_OFFSET);  // DB: [512 points]
+    int16_t *s_rinfo =
+        reinterpret_cast<int16_t *>(lds_raw + (DB ? RB_DB_AREA_BYTES : PER == 1 ? RB_AREA_BYTES : RB_IMAGE_BYTES));
     SlotPair *s_sp = reinterpret_cast<SlotPair *>(s_rinfo + MAX_ROWS);
     for (int x = tid; x < R; x += 64 * W) s_rinfo[x] = rowinfo[B.row_off + x];
     for (int x = tid; x < B.nslot * B.nslot; x += 64 * W) s_sp[x] = spair[B.sp_off + x];
@@ -1207,16 +1222,48 @@ __device__ __forceinline__ void rblk_item(const HItem it, const BlockArgs &A, un
             fl[f] = *reinterpret_cast<const GLOBAL_AS u16x8 *>(sl + (g * 64 + lane) * 8);
         }
     };
+    // DB: chunk c + 1 is written into the other buffer while chunk c is read;
+    // the barrier of step c also orders the writes of step c + 1 after every
+    // wave's reads of chunk c - 1 (they precede that barrier), so the
+    // barrier at the top of the single-buffered step is not needed
+    auto add_wave_sums = [&](int c) {  // DB: the 16 wave sums of chunk c's points
+        if (wave == c % W && lane < CHUNK) {
+            const float *src = wsum + (c & 1) * W * CHUNK;
+            float sum = 0.f;
+            // in wave order, four loads in flight (the rest of the kernel
+            // holds 120 VGPRs: sixteen at once would spill)
+#pragma unroll
+            for (int w0 = 0; w0 < W; w0 += 4) {
+                float v[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) v[k] = src[(w0 + k) * CHUNK + lane];
+#pragma unroll
+                for (int k = 0; k < 4; k++) sum += v[k];
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            psum[c * CHUNK + lane] = sum;
+        }
+    };
     auto step = [&](int c, TouchN<64 * W> &slot) {
-        __syncthreads();
+        // the buffer offset (72 KB) exceeds the LDS instructions' immediate
+        // offsets: kept opaque in an SGPR, it costs one add per access instead
+        // of a second set of address registers
+        int bsel = DB ? (c & 1) : 0;
+        if constexpr (DB) asm volatile("" : "+s"(bsel));
+        u16x8 *BH = TH + bsel * (RB_DB_BUF_BYTES / 16);
+        u16x8 *BL = BH + G * 64;
+        if constexpr (!DB) __syncthreads();
 #pragma unroll
         for (int f = 0; f < NF; f++) {  // branch-free, as in k_hblk_bf
             const int g = frag_of(f);
-            TH[g * 64 + lane] = fh[f];
-            TL[g * 64 + lane] = fl[f];
+            BH[g * 64 + lane] = fh[f];
+            BL[g * 64 + lane] = fl[f];
         }
         release_n(slot);
         __syncthreads();
+        if constexpr (DB) {
+            if (c > 0) add_wave_sums(c - 1);
+        }
         prefetch(min(c + 1, NCHUNK - 1));
         slot = touch_chunk_n<64 * W>(thi, tlo, B.tile_off + (int64_t)min(c + 2, NCHUNK - 1) * G * FRAG, R, tid);
         // per 16-point group: a lane's partial sum over its super-tiles; the
@@ -1231,8 +1278,8 @@ __device__ __forceinline__ void rblk_item(const HItem it, const BlockArgs &A, un
 #pragma unroll
                 for (int ks = 0; ks < 2; ks++) {
                     const int fi = pt * G32 + sj[h] * 2 + ks;
-                    bh[ks] = __builtin_bit_cast(bf16x8, TH[fi * 64 + lane]);
-                    bl[ks] = __builtin_bit_cast(bf16x8, TL[fi * 64 + lane]);
+                    bh[ks] = __builtin_bit_cast(bf16x8, BH[fi * 64 + lane]);
+                    bl[ks] = __builtin_bit_cast(bf16x8, BL[fi * 64 + lane]);
                 }
 #pragma unroll
                 for (int rt = 0; rt < 4; rt++) {
@@ -1247,8 +1294,8 @@ __device__ __forceinline__ void rblk_item(const HItem it, const BlockArgs &A, un
                     // same four rows and point: 8 contiguous bytes of a point fragment
                     const int fi = pt * G32 + si[h] * 2 + (rt >> 1);
                     const int off = (fi * 64 + (2 * (rt & 1) + (q >> 1)) * 16 + l16) * 8 + 4 * (q & 1);
-                    const u16x4 ph = *reinterpret_cast<const u16x4 *>(reinterpret_cast<const uint16_t *>(TH) + off);
-                    const u16x4 pl = *reinterpret_cast<const u16x4 *>(reinterpret_cast<const uint16_t *>(TL) + off);
+                    const u16x4 ph = *reinterpret_cast<const u16x4 *>(reinterpret_cast<const uint16_t *>(BH) + off);
+                    const u16x4 pl = *reinterpret_cast<const u16x4 *>(reinterpret_cast<const uint16_t *>(BL) + off);
 #pragma unroll
                     for (int v = 0; v < 4; v++)
                         part = fmaf(bf_bits_to_f32(ph[v]) + bf_bits_to_f32(pl[v]), acc[v], part);
@@ -1260,7 +1307,10 @@ __device__ __forceinline__ void rblk_item(const HItem it, const BlockArgs &A, un
             part = __uint_as_float(s32[0]) + __uint_as_float(s32[1]);
             const auto s16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(part), __float_as_uint(part), false, false);
             part = __uint_as_float(s16[0]) + __uint_as_float(s16[1]);
-            if (q == 0) rcw[wave * BLK_PTS + c * CHUNK + pt * 16 + l16] = has[0] ? part : 0.f;
+            if (q == 0) {
+                if constexpr (DB) wsum[(c & 1) * W * CHUNK + wave * CHUNK + pt * 16 + l16] = has[0] ? part : 0.f;
+                else rcw[wave * BLK_PTS + c * CHUNK + pt * 16 + l16] = has[0] ? part : 0.f;
+            }
         }
     };
     prefetch(0);
@@ -1272,11 +1322,19 @@ __device__ __forceinline__ void rblk_item(const HItem it, const BlockArgs &A, un
         step(c + 1, t1);
     }
     __syncthreads();
+    if constexpr (DB) {
+        add_wave_sums(NCHUNK - 1);
+        __syncthreads();
+    }
     if (tid < BLK_PTS) {
         const int k = tid;
         float sum = 0.f;
+        if constexpr (DB) {
+            sum = psum[k];
+        } else {
 #pragma unroll
-        for (int w = 0; w < W; w++) sum += rcw[w * BLK_PTS + k];
+            for (int w = 0; w < W; w++) sum += rcw[w * BLK_PTS + k];
+        }
         const int x1 = B.g1 + k / (BLK * BLK), x2 = B.g2 + (k / BLK) % BLK, x3 = B.g3 + k % BLK;
         if (x1 < n1 && x2 < n2 && x3 < n3) atomicAdd(&rho[((int64_t)x1 * n2 + x2) * n3 + x3], (double)sum);
     }
@@ -1291,13 +1349,14 @@ constexpr int MAXF_RHO = (MAX_ROWS / 16 + RB_WAVES - 1) / RB_WAVES;
 template <int MODE>
 constexpr int rb_waves() { return MODE == LAUNCH_QUEUE ? HB_WAVES : RB_WAVES; }
 
-template <int NF, int MODE>
+template <int NF, int MODE, bool DB = false>
 __global__ __launch_bounds__(64 * rb_waves<MODE>()) void k_rblk_bf(const HItem *__restrict__ items, int32_t nitems,
                                                                   const BlockArgs *__restrict__ args, BlockArgs one,
                                                                   int32_t *__restrict__ queue) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    run_items<MODE>(items, nitems, args, one, queue,
-                    [&](const HItem &it, const BlockArgs &A) { rblk_item<NF, rb_waves<MODE>()>(it, A, lds_raw); });
+    run_items<MODE>(items, nitems, args, one, queue, [&](const HItem &it, const BlockArgs &A) {
+        rblk_item<NF, rb_waves<MODE>(), DB>(it, A, lds_raw);
+    });
 }
 
 BlockArgs block_args(const lcpao_ctx *ctx, const double *V, double *rho) {
@@ -1312,26 +1371,42 @@ BlockArgs block_args(const lcpao_ctx *ctx, const double *V, double *rho) {
     return a;
 }
 
-size_t block_kernel_lds(bool density, int rows, int max_slots, bool persistent) {
-    // density: chunk planes and point sums (the 8-wave form: W images first),
-    // then the block's row descriptors and slot pairs
+constexpr size_t LDS_PER_CU = 160 * 1024;
+
+bool rho_db_fits(int rows, int max_slots) {
+    // up to two prefetch fragments per wave (R <= 512): with three the
+    // double-buffered kernel spills
+    return nfrag_rho(rows) <= 2 &&
+           RB_DB_AREA_BYTES + MAX_ROWS * sizeof(int16_t) + (size_t)max_slots * max_slots * sizeof(SlotPair) <=
+               LDS_PER_CU;
+}
+
+size_t block_kernel_lds(bool density, int rows, int max_slots, bool persistent, bool db) {
+    // density: chunk planes and point sums (the 8-wave form: W images first;
+    // the double-buffered form: two plane buffers and the chunk sums), then
+    // the block's row descriptors and slot pairs
     if (density)
-        return (persistent ? RB_IMAGE_BYTES : RB_AREA_BYTES) + MAX_ROWS * sizeof(int16_t) +
+        return (persistent ? RB_IMAGE_BYTES : db ? RB_DB_AREA_BYTES : RB_AREA_BYTES) + MAX_ROWS * sizeof(int16_t) +
                (size_t)max_slots * max_slots * sizeof(SlotPair);
     return (size_t)4 * rows * CHUNK * sizeof(uint16_t) + BLK_PTS * sizeof(float);
 }
 
 hipError_t launch_block_kernel(bool density, hipStream_t stream, int rows, int max_slots, const HItem *items,
                                int32_t nitems, const BlockArgs *args, const BlockArgs &one, int32_t *queue,
-                               unsigned grid) {
+                               unsigned grid, bool db) {
     const int mode = launch_mode(args, queue);
     const bool persistent = mode == LAUNCH_QUEUE;
-    const size_t lds = block_kernel_lds(density, rows, max_slots, persistent);
+    if (db && (!density || persistent || !rho_db_fits(rows, max_slots))) return hipErrorInvalidValue;
+    const size_t lds = block_kernel_lds(density, rows, max_slots, persistent, db);
     void (*kern)(const HItem *, int32_t, const BlockArgs *, BlockArgs, int32_t *) = nullptr;
 #define BY_MODE(K, NF)                                                                             \
     kern = mode == LAUNCH_ONE ? K<NF, LAUNCH_ONE> : mode == LAUNCH_BATCH ? K<NF, LAUNCH_BATCH> \
                                                                          : K<NF, LAUNCH_QUEUE>
-    if (density && !persistent) {
+#define BY_MODE_DB(K, NF) kern = mode == LAUNCH_ONE ? K<NF, LAUNCH_ONE, true> : K<NF, LAUNCH_BATCH, true>
+    if (density && !persistent && db) {
+        if (nfrag_rho(rows) == 1) BY_MODE_DB(k_rblk_bf, 1);
+        else BY_MODE_DB(k_rblk_bf, 2);
+    } else if (density && !persistent) {
         static_assert(MAXF_RHO <= 3, "density prefetch depths instantiated: 1 .. 3");
         switch (nfrag_rho(rows)) {
             case 1: BY_MODE(k_rblk_bf, 1); break;
@@ -1356,6 +1431,7 @@ hipError_t launch_block_kernel(bool density, hipStream_t stream, int rows, int m
         }
     }
 #undef BY_MODE
+#undef BY_MODE_DB
     const int threads = 64 * (density && !persistent ? RB_WAVES : HB_WAVES);
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lds);
@@ -1389,11 +1465,47 @@ int split_density(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t 
             hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((ctx->nn + 255) / 256)), dim3(256), 0, ctx->stream, ctx->nn,
                                d_rho[s], 0.0);
         if (!P->items.empty()) {
-            KTimer kt(ctx, KID_RHO);
             const unsigned ni = (unsigned)P->items.size();
             const BlockArgs one = block_args(ctx, nullptr, d_rho[s]);
-            SPLIT_TRY(ctx, launch_block_kernel(true, ctx->stream, P->max_rows, P->max_slots, P->d_items, (int32_t)ni,
-                                               nullptr, one, nullptr, ni));
+            // candidates: 0 = single-buffered chunk planes, 1 = double-buffered
+            // (when its LDS fits the plan's largest slot count)
+            auto launch = [&](int v) {
+                return launch_block_kernel(true, ctx->stream, P->max_rows, P->max_slots, P->d_items, (int32_t)ni,
+                                           nullptr, one, nullptr, ni, v == 1);
+            };
+            const bool db_ok = rho_db_fits(P->max_rows, P->max_slots);
+            if (ctx->variant[TUNE_RHOBF] < 0) {
+                if (!db_ok) {
+                    ctx->variant[TUNE_RHOBF] = 0;
+                } else {
+                    // each candidate once untimed, once between events; they add
+                    // into rho, which is cleared again before the real launch
+                    hipEvent_t e0, e1;
+                    SPLIT_TRY(ctx, hipEventCreate(&e0));
+                    SPLIT_TRY(ctx, hipEventCreate(&e1));
+                    int best = 0;
+                    float best_ms = 0.f;
+                    for (int v = 0; v < TUNE_NCAND[TUNE_RHOBF]; v++) {
+                        SPLIT_TRY(ctx, launch(v));
+                        SPLIT_TRY(ctx, hipEventRecord(e0, ctx->stream));
+                        SPLIT_TRY(ctx, launch(v));
+                        SPLIT_TRY(ctx, hipEventRecord(e1, ctx->stream));
+                        SPLIT_TRY(ctx, hipEventSynchronize(e1));
+                        float ms = 0.f;
+                        SPLIT_TRY(ctx, hipEventElapsedTime(&ms, e0, e1));
+                        ctx->variant_ms[TUNE_RHOBF][v] = ms;
+                        if (v == 0 || ms < best_ms) { best = v; best_ms = ms; }
+                    }
+                    (void)hipEventDestroy(e0);
+                    (void)hipEventDestroy(e1);
+                    ctx->variant[TUNE_RHOBF] = best;
+                    if (ctx->nn > 0)
+                        hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((ctx->nn + 255) / 256)), dim3(256), 0,
+                                           ctx->stream, ctx->nn, d_rho[s], 0.0);
+                }
+            }
+            KTimer kt(ctx, KID_RHO);
+            SPLIT_TRY(ctx, launch(ctx->variant[TUNE_RHOBF] == 1 && db_ok ? 1 : 0));
         }
         SPLIT_TRY(ctx, hipGetLastError());
     }

@@ -136,15 +136,19 @@ struct BlockPlan {
 int plan_build(lcpao_ctx *ctx);
 // bf16x3 block kernels, shared with the batched form
 BlockArgs block_args(const lcpao_ctx *ctx, const double *V, double *rho);
-size_t block_kernel_lds(bool density, int rows, int max_slots, bool persistent);
+size_t block_kernel_lds(bool density, int rows, int max_slots, bool persistent, bool db = false);
+// whether the double-buffered 16-wave density kernel serves blocks of up to
+// `rows` rows and max_slots sphere images (LDS and register limits)
+bool rho_db_fits(int rows, int max_slots);
 void launch_fill_f64(hipStream_t stream, int64_t n, double *x, double v);
 void launch_fill_i32(hipStream_t stream, int64_t n, int32_t *x, int32_t v);
 void launch_hblk_reduce(const lcpao_ctx *ctx, hipStream_t stream, double *H);
 // args != nullptr: batched launch (item.pad indexes args, `one` unused);
-// grid == 0 with a queue: persistent grid (resident workgroups)
+// grid == 0 with a queue: persistent grid (resident workgroups); db: the
+// double-buffered density kernel (one workgroup per item; see rho_db_fits)
 hipError_t launch_block_kernel(bool density, hipStream_t stream, int rows, int max_slots, const HItem *items,
                                int32_t nitems, const BlockArgs *args, const BlockArgs &one, int32_t *queue,
-                               unsigned grid);
+                               unsigned grid, bool db = false);
 void plan_free(BlockPlan *p);
 int split_tables(lcpao_ctx *ctx, int precision);
 int split_hamiltonian(lcpao_ctx *ctx, int nch, const double *const *d_V, double *const *d_H, int precision);
