@@ -9,6 +9,9 @@
 #
 # Replaced methods (reference file:line):
 #   HS_matrix!(S, H, OLP::Vector{Float64}, Hks::Vector{Float64}, ..., kpts)   HS_matrix.jl:143
+#   HS_matrix!(S, OLP::Vector{Float64}, ..., kpts)                          HS_matrix.jl:88
+#   HS_matrix!(S, H, OLP::Vector{Float64}, Hks::Vector{Float64}, ..., natn) HS_matrix.jl:114 (no k; real S, H)
+#   HS_matrix!(S, OLP::Vector{Float64}, ..., natn)                          HS_matrix.jl:67 (no k; real S)
 #   Calc_DM_Crystal_Collinear_nopol!                                        Crystal_DFT.jl:219
 #   Calc_DM_Crystal_Collinear_pol!                                          Crystal_DFT.jl:276
 #   Calc_DM_Crystal_NonCollinear!                                           Crystal_DFT.jl:342
@@ -33,6 +36,45 @@ function HS_matrix!(S, H, OLP::Vector{Float64}, Hks::Vector{Float64}, Natom, Tot
                    (Ptr{Cvoid}, Ptr{Float64}, Int32, Ptr{Ptr{Float64}}, Ptr{Ptr{Float64}}), h.ptr, k, 2, fp, dp)
         _grid_check(rc, h.ptr)
     end
+    return nothing
+end
+
+# one to four flat arrays -> dense complex matrices (column-major ComplexF64)
+function _bloch_sum!(h, k::Vector{Float64}, flats, denses)
+    n = length(flats)
+    fp = [pointer(f) for f in flats]
+    dp = [Ptr{Float64}(pointer(d)) for d in denses]
+    GC.@preserve flats denses fp dp k begin
+        rc = ccall((:lcpao_grid_bloch_sum, LIBGRID), Cint,
+                   (Ptr{Cvoid}, Ptr{Float64}, Int32, Ptr{Ptr{Float64}}, Ptr{Ptr{Float64}}), h.ptr, k, n, fp, dp)
+        _grid_check(rc, h.ptr)
+    end
+end
+
+function HS_matrix!(S, OLP::Vector{Float64}, Natom, Total_NumOrbs, MP, FNAN, natn, ncn, atv_ijk,
+                    kpts::AbstractVector{Float64})
+    ucell, h = _kpoint_context()
+    _bloch_sum!(h, Float64[kpts[1], kpts[2], kpts[3]], [OLP], [S])
+    return nothing
+end
+
+# Without k (the cluster forms): the same sum at k = 0, where every phase is
+# exactly 1 + 0im, so the real parts are the reference's sums bit for bit.
+function HS_matrix!(S, H, OLP::Vector{Float64}, Hks::Vector{Float64}, Natom, Total_NumOrbs, MP, FNAN, natn)
+    ucell, h = _kpoint_context()
+    Sc = zeros(ComplexF64, size(S))
+    Hc = zeros(ComplexF64, size(H))
+    _bloch_sum!(h, zeros(3), [OLP, Hks], [Sc, Hc])
+    S .= real.(Sc)
+    H .= real.(Hc)
+    return nothing
+end
+
+function HS_matrix!(S, OLP::Vector{Float64}, Natom, Total_NumOrbs, MP, FNAN, natn)
+    ucell, h = _kpoint_context()
+    Sc = zeros(ComplexF64, size(S))
+    _bloch_sum!(h, zeros(3), [OLP], [Sc])
+    S .= real.(Sc)
     return nothing
 end
 

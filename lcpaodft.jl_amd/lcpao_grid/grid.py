@@ -491,6 +491,18 @@ def HS_matrix(S, H, OLP, Hks, kpts, Orbs_Grid):
     H[:, :] = Hk
 
 
+def HS_matrix_cluster(S, H, OLP, Hks, Orbs_Grid):
+    """HS_matrix!(S, H, OLP::Vector{Float64}, Hks::Vector{Float64}, Natom, ...,
+    natn) (HS_matrix.jl:114-136), the form without k: real S and H are
+    overwritten with the sums at k = 0 (every phase exactly 1), H may be None
+    for the one-matrix form (HS_matrix.jl:67-85)."""
+    flats = [OLP] if H is None else [OLP, Hks]
+    out = Orbs_Grid.bloch_sum([0.0, 0.0, 0.0], flats)
+    S[:, :] = out[0].real
+    if H is not None:
+        H[:, :] = out[1].real
+
+
 def Calc_DM_Crystal_Collinear_nopol(Cnk, MPI_kpts, AllNkpt, DM, Orbs_Grid):
     """The accumulation of Calc_DM_Crystal_Collinear_nopol! (Crystal_DFT.jl:219-270)
     for this rank's k-points (states scaled as Calc_fnkCnk! leaves them);

@@ -136,3 +136,24 @@ def test_nc_bloch_sum_and_nc_dm_match_oracle(name):
     assert _rel(pol[0], O.dm_collinear(g, kpts, up, 0.5)) < 1e-13
     assert _rel(pol[1], O.dm_collinear(g, kpts, dn, 0.5)) < 1e-13
     ctx.close()
+
+
+def test_cluster_forms_are_the_gamma_sums():
+    """HS_matrix! without k (the cluster forms): real S, H from the sum at
+    k = 0, where every phase is exactly 1, so they equal the oracle's sums
+    bit for bit; the one-matrix form gives the same S."""
+    from lcpao_grid import grid as Gm
+    st = S.si2(16)
+    u = st.ucell()
+    g = O.Geom(st.Latvecs, st.Gxyz, st.cutoffs(), st.norbs(), st.Ngrid)
+    ctx = GridContext(u)
+    fsize, _ = ctx.bloch_sizes()
+    rng = np.random.default_rng(13)
+    olp, hks = rng.standard_normal(u.myHsize), rng.standard_normal(u.myHsize)
+    Sm, Hm, S1 = (np.zeros((fsize, fsize)) for _ in range(3))
+    Gm.HS_matrix_cluster(Sm, Hm, olp, hks, ctx)
+    Gm.HS_matrix_cluster(S1, None, olp, None, ctx)
+    So, Ho = O.hs_matrix(g, [0.0, 0.0, 0.0], [olp, hks])
+    assert np.array_equal(Sm, So.real) and np.array_equal(Hm, Ho.real) and np.array_equal(S1, Sm)
+    assert np.abs(So.imag).max() == 0.0
+    ctx.close()
