@@ -153,10 +153,12 @@ def traffic_from_profile(config, kernel):
         return n.split("lcpao::")[-1].replace(" ", "")
 
     if "<" in kernel:
-        hits = [b for n, b in tab.items() if norm(n) == norm(kernel)]
-    else:  # every template instance of the kernel (one launch per row class)
-        hits = [b for n, b in tab.items() if norm(n).split("<")[0] == kernel]
-    return (sum(hits) if hits else None), os.path.relpath(path, ROOT)
+        hits = [v for n, v in tab.items() if norm(n) == norm(kernel)]
+    else:  # the template instance the timed loop ran: the most dispatches (others are autotuner candidates)
+        hits = [v for n, v in tab.items() if norm(n).split("<")[0] == kernel]
+    if not hits:
+        return None, os.path.relpath(path, ROOT)
+    return max(hits)[1], os.path.relpath(path, ROOT)
 
 
 def mfma_from_profile(config, kernel):
@@ -176,7 +178,8 @@ def mfma_from_profile(config, kernel):
     hits = [v for n, v in tab.items() if norm(n) == norm(kernel) or norm(n).split("<")[0] == kernel]
     if not hits:
         return None
-    typ, gf, tf, of_peak, busy = hits[0]
+    # the instance the timed loop ran: the most dispatches (others are autotuner candidates)
+    _, typ, gf, tf, of_peak, busy = max(hits)
     return {"type": typ, "executed_gflop_per_launch": gf, "executed_TFLOPs": tf, "of_type_peak": of_peak,
             "mfma_busy": busy, "source": os.path.relpath(path, ROOT)}
 

@@ -194,6 +194,17 @@ enum { LCPAO_TUNE_RHO64 = 0, LCPAO_TUNE_F364 = 1, LCPAO_TUNE_RHOBF = 2 };
 int lcpao_grid_tune(lcpao_ctx *ctx, int32_t op, int32_t variant);
 int lcpao_grid_tuned(const lcpao_ctx *ctx, int32_t op, int32_t *variant, float *candidate_ms, int32_t ncand);
 
+/* Autotuner cache (no counterpart in the reference): a JSON file of chosen
+ * variants keyed by the context's shape (its distinct orbital counts, the
+ * grid spacing GridVol^(1/3) and the mean pairs per atom), so that a later
+ * run on a like geometry skips the timing.  lcpao_grid_tune_load fixes every
+ * operation stored under this context's key (*loaded = how many; a missing
+ * file loads nothing and is not an error); lcpao_grid_tune_save writes the
+ * context's tuned operations into the file, replacing its earlier entries
+ * for the same key and keeping the others. */
+int lcpao_grid_tune_load(lcpao_ctx *ctx, const char *path, int32_t *loaded);
+int lcpao_grid_tune_save(lcpao_ctx *ctx, const char *path);
+
 /* ---- grid fields next to the contractions --------------------------------
  * Radial data of one species for the atomic densities: the PAO log mesh and
  * valence density (Spe_PAO_XV/RV, Spe_Atomic_Den of Read_PAO.jl:183-303, the

@@ -1517,6 +1517,98 @@ int lcpao_grid_tuned(const lcpao_ctx *ctx, int32_t op, int32_t *variant, float *
     return LCPAO_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Key of a context's shape for the tuner cache: the distinct orbital counts
+// of its atoms, the grid spacing (cube root of GridVol) and the mean number
+// of pairs per atom, which set the kernels' tile shapes and work per pair.
+std::string tune_key(const lcpao_ctx *ctx) {
+    std::vector<int> nos(ctx->no.begin(), ctx->no.end());
+    std::sort(nos.begin(), nos.end());
+    nos.erase(std::unique(nos.begin(), nos.end()), nos.end());
+    std::string k = "no=";
+    for (size_t i = 0; i < nos.size(); i++) k += (i ? "," : "") + std::to_string(nos[i]);
+    char buf[96];
+    std::snprintf(buf, sizeof buf, ";h=%.4f;ppa=%lld", std::cbrt(ctx->gridvol),
+                  (long long)(ctx->natom > 0 ? (ctx->npair + ctx->natom / 2) / ctx->natom : 0));
+    return k + buf;
+}
+
+struct TuneEntry {
+    std::string key;
+    int op, variant;
+    float ms[2];
+};
+
+// The cache is a JSON object with one entry per line:
+//   {"key": "no=19;h=0.2225;ppa=87", "op": 0, "variant": 1, "ms": [19.9, 11.9]}
+// Lines that do not hold an entry are skipped.  Returns false if the file
+// cannot be opened.
+bool read_tune_file(const char *path, std::vector<TuneEntry> &out) {
+    FILE *f = std::fopen(path, "r");
+    if (!f) return false;
+    char line[1024];
+    while (std::fgets(line, sizeof line, f)) {
+        const char *p = std::strstr(line, "{\"key\": \"");
+        if (!p) continue;
+        char key[512];
+        int op = -1, v = -1;
+        float a = 0.f, b = 0.f;
+        if (std::sscanf(p, "{\"key\": \"%511[^\"]\", \"op\": %d, \"variant\": %d, \"ms\": [%f, %f]", key, &op, &v,
+                        &a, &b) >= 3)
+            out.push_back(TuneEntry{key, op, v, {a, b}});
+    }
+    std::fclose(f);
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lcpao_grid_tune_load(lcpao_ctx *ctx, const char *path, int32_t *loaded) {
+    if (!ctx || !path) return LCPAO_EINVAL;
+    if (loaded) *loaded = 0;
+    std::vector<TuneEntry> entries;
+    if (!read_tune_file(path, entries)) return LCPAO_OK;  // no cache yet
+    const std::string key = tune_key(ctx);
+    int n = 0;
+    for (const TuneEntry &e : entries) {
+        if (e.key != key || e.op < 0 || e.op >= TUNE_NOPS || e.variant < 0 || e.variant >= TUNE_NCAND[e.op]) continue;
+        ctx->variant[e.op] = e.variant;
+        for (int v = 0; v < 2 && v < TUNE_NCAND[e.op]; v++) ctx->variant_ms[e.op][v] = e.ms[v];
+        n++;
+    }
+    if (loaded) *loaded = n;
+    return LCPAO_OK;
+}
+
+int lcpao_grid_tune_save(lcpao_ctx *ctx, const char *path) {
+    if (!ctx || !path) return LCPAO_EINVAL;
+    std::vector<TuneEntry> entries;
+    read_tune_file(path, entries);
+    const std::string key = tune_key(ctx);
+    std::vector<TuneEntry> keep;
+    for (const TuneEntry &e : entries)
+        if (!(e.key == key && e.op >= 0 && e.op < TUNE_NOPS && ctx->variant[e.op] >= 0)) keep.push_back(e);
+    for (int op = 0; op < TUNE_NOPS; op++)
+        if (ctx->variant[op] >= 0)
+            keep.push_back(TuneEntry{key, op, ctx->variant[op], {ctx->variant_ms[op][0], ctx->variant_ms[op][1]}});
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE *f = std::fopen(tmp.c_str(), "w");
+    if (!f) return set_err(ctx, LCPAO_EINVAL, "cannot write the tuner cache " + tmp);
+    std::fprintf(f, "{\"lcpao_grid_tune\": [\n");
+    for (size_t i = 0; i < keep.size(); i++)
+        std::fprintf(f, "{\"key\": \"%s\", \"op\": %d, \"variant\": %d, \"ms\": [%.4f, %.4f]}%s\n", keep[i].key.c_str(),
+                     keep[i].op, keep[i].variant, keep[i].ms[0], keep[i].ms[1], i + 1 < keep.size() ? "," : "");
+    std::fprintf(f, "]}\n");
+    const bool ok = std::fclose(f) == 0;
+    if (!ok || std::rename(tmp.c_str(), path) != 0) return set_err(ctx, LCPAO_EINVAL, std::string("cannot write ") + path);
+    return LCPAO_OK;
+}
+
 int lcpao_grid_kernel_stats(const lcpao_ctx *ctx, int32_t kernel_id, int64_t *launches, double *total_ms) {
     if (!ctx || kernel_id < 0 || kernel_id >= KID_COUNT) return LCPAO_EINVAL;
     if (launches) *launches = ctx->prof_launches[kernel_id];

@@ -52,7 +52,7 @@ EXPORTS = [
     "lcpao_grid_aden_pcc", "lcpao_grid_aden_pcc_dev", "lcpao_grid_poisson", "lcpao_grid_poisson_dev",
     "lcpao_grid_bloch_sum", "lcpao_grid_bloch_sum_dev", "lcpao_grid_bloch_sizes", "lcpao_grid_dm_states",
     "lcpao_grid_dm_states_dev", "lcpao_grid_bloch_sum_nc", "lcpao_grid_bloch_sum_nc_dev", "lcpao_grid_dm_states_nc",
-    "lcpao_grid_dm_states_nc_dev",
+    "lcpao_grid_dm_states_nc_dev", "lcpao_grid_tune_load", "lcpao_grid_tune_save",
 ]
 
 _lib = None
@@ -114,6 +114,8 @@ def load():
     L.lcpao_grid_bloch_sizes.argtypes = [vp, lp, lp]
     L.lcpao_grid_dm_states.argtypes = [vp, C.c_int32, dp, C.c_int64, dpp, C.c_double, dp]
     L.lcpao_grid_dm_states_dev.argtypes = [vp, C.c_int32, dp, C.c_int64, dpp, C.c_double, dp]
+    L.lcpao_grid_tune_load.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int32)]
+    L.lcpao_grid_tune_save.argtypes = [vp, C.c_char_p]
     L.lcpao_grid_bloch_sum_nc.argtypes = [vp, dp, dpp, dpp, dp]
     L.lcpao_grid_bloch_sum_nc_dev.argtypes = [vp, dp, dpp, dpp, dp]
     L.lcpao_grid_dm_states_nc.argtypes = [vp, C.c_int32, dp, C.c_int64, dpp, C.c_double, dpp, dpp]

@@ -359,6 +359,17 @@ class GridContext:
         self._check(self.L.lcpao_grid_tuned(self.h, self.TUNE_OPS[op], C.byref(v), ms, 4))
         return v.value, [round(float(x), 4) for x in ms[:2]]
 
+    def tune_load(self, path):
+        """Fix the variants stored for this geometry's key in the tuner cache
+        file `path` (JSON); returns how many operations were set."""
+        n = C.c_int32()
+        self._check(self.L.lcpao_grid_tune_load(self.h, str(path).encode(), C.byref(n)))
+        return n.value
+
+    def tune_save(self, path):
+        """Write this context's tuned variants into the cache file `path`."""
+        self._check(self.L.lcpao_grid_tune_save(self.h, str(path).encode()))
+
     def set_stream(self, stream):
         """Launch on `stream` (a torch.cuda.Stream, or None for the context's
         own stream).  The default stream of torch is the HIP null stream, which

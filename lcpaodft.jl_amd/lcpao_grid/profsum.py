@@ -111,7 +111,7 @@ def cmd_traffic(a):
 
 
 def read_traffic_table(path):
-    """{kernel: traffic bytes per dispatch} from a `traffic` summary."""
+    """{kernel: (dispatches, traffic bytes per dispatch)} from a `traffic` summary."""
     out = {}
     with open(path) as f:
         for line in f:
@@ -120,7 +120,7 @@ def read_traffic_table(path):
                 continue
             cols = [c.strip() for c in m.group(2).split("|")]
             if len(cols) == 6:
-                out[m.group(1)] = float(cols[5])
+                out[m.group(1)] = (int(cols[0]), float(cols[5]))
     return out
 
 
@@ -182,7 +182,7 @@ def cmd_mfma(a):
 
 
 def read_mfma_table(path):
-    """{kernel: (type, executed GFLOP per dispatch, TFLOP/s, of-peak, busy)} from an `mfma` summary."""
+    """{kernel: (dispatches, type, executed GFLOP per dispatch, TFLOP/s, of-peak, busy)} from an `mfma` summary."""
     out = {}
     with open(path) as f:
         for line in f:
@@ -191,7 +191,7 @@ def read_mfma_table(path):
                 continue
             cols = [c.strip() for c in m.group(2).split("|")]
             if len(cols) == 7:
-                out[m.group(1)] = (cols[2], float(cols[3]), float(cols[4]), float(cols[5]), float(cols[6]))
+                out[m.group(1)] = (int(cols[0]), cols[2], float(cols[3]), float(cols[4]), float(cols[5]), float(cols[6]))
     return out
 
 

@@ -285,3 +285,33 @@ def test_autotuner_picks_a_timed_candidate():
     ctx.density([c.DM])
     v, ms = ctx.tuned("rho64")
     assert v in (0, 1) and all(m > 0 for m in ms) and ms[v] == min(ms)
+
+
+def test_autotuner_cache_round_trip(tmp_path):
+    """Tuned variants saved for one geometry are loaded by a new context of
+    the same shape (no timing on its first call), not by another shape."""
+    path = tmp_path / "tune.json"
+    c = case("si8")
+    ctx = GridContext(c.u)
+    ctx.set_tables(c.tabs)
+    assert ctx.tune_load(path) == 0  # no file yet
+    ctx.density([c.DM])
+    ctx.density([c.DM], precision="bf16x3")
+    chosen = {op: ctx.tuned(op)[0] for op in ("rho64", "rhobf")}
+    assert all(v in (0, 1) for v in chosen.values())
+    ctx.tune_save(path)
+    text = path.read_text()
+    assert text.startswith('{"lcpao_grid_tune": [') and text.count('"key": ') == 2
+    ctx2 = GridContext(c.u)
+    ctx2.set_tables(c.tabs)
+    assert ctx2.tune_load(path) == 2
+    assert {op: ctx2.tuned(op)[0] for op in ("rho64", "rhobf")} == chosen
+    rho = ctx2.density([c.DM])[0]
+    Rref, M = O.density(c.g, c.tabs, [c.DM], with_abs=True)
+    ok, msg = within(rho, Rref[0], M[0])
+    assert ok, msg
+    ctx2.tune_save(path)  # same key: entries replaced, not duplicated
+    assert path.read_text().count('"key": ') == 2
+    other = case("nacl8")
+    ctx3 = GridContext(other.u)
+    assert ctx3.tune_load(path) == 0 and ctx3.tuned("rho64")[0] == -1
