@@ -5,6 +5,8 @@
          --copy-match SUBSTR --copy-bytes N [--title T] [--command C]
   python -m lcpao_grid.profsum mfma COUNTER_CSV --agent AGENT_INFO_CSV
          [--title T] [--command C]
+  python -m lcpao_grid.profsum counters COUNTER_CSV [--match SUBSTR ...]
+         [--title T] [--command C]
 
 `stats` condenses a `--kernel-trace --stats` run (kernel_stats.csv) to a
 table of calls and average/total durations.  `traffic` turns two `--pmc`
@@ -13,6 +15,11 @@ per dispatch of every kernel, each counter scaled by what it reads for a
 plain copy kernel that moves a known number of bytes (N read and N written):
 factor = N / (counter KiB * 1024).  The table that `traffic` prints is read
 back by bench.py (column "traffic B/dispatch").
+
+`counters` prints the per-dispatch average of every counter of one `--pmc`
+run for the kernels whose names contain one of the --match strings, with
+each SQ_* wave-cycle counter also as a fraction of SQ_WAVE_CYCLES when that
+counter is present.
 
 `mfma` turns one `--pmc` run of SQ_INSTS_VALU_MFMA_MOPS_{BF16,F32,F64},
 SQ_VALU_MFMA_BUSY_CYCLES and GRBM_GUI_ACTIVE into the matrix-core use of
@@ -181,6 +188,39 @@ def cmd_mfma(a):
               % (k, r["n"], r["ms"], typ, gf, tf, tf / MFMA_PEAK_TF[typ], busy))
 
 
+def cmd_counters(a):
+    acc = defaultdict(lambda: defaultdict(list))
+    with open(a.csv, newline="") as f:
+        for r in csv.DictReader(f):
+            k = short_name(r["Kernel_Name"])
+            if a.match and not any(m in k for m in a.match):
+                continue
+            acc[k][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    names = sorted({n for c in acc.values() for n in c})
+    print("# %s" % (a.title or "rocprofv3 --pmc counters per dispatch"))
+    print()
+    if a.command:
+        print("Command: `%s`" % a.command)
+        print()
+    print("Per-dispatch averages; in parentheses, SQ_WAIT_* / SQ_ACTIVE_INST_* as a fraction of SQ_WAVE_CYCLES.")
+    print()
+    print("| kernel | dispatches | " + " | ".join(names) + " |")
+    print("|---|---:|" + "---:|" * len(names))
+    for k in sorted(acc):
+        c = {n: sum(v) / len(v) for n, v in acc[k].items()}
+        wc = c.get("SQ_WAVE_CYCLES", 0.0)
+        cells = []
+        for n in names:
+            v = c.get(n)
+            if v is None:
+                cells.append("")
+            elif wc > 0 and (n.startswith("SQ_WAIT") or n.startswith("SQ_ACTIVE_INST")):
+                cells.append("%.4g (%.2f)" % (v, v / wc))
+            else:
+                cells.append("%.4g" % v)
+        print("| `%s` | %d | %s |" % (k, len(next(iter(acc[k].values()))), " | ".join(cells)))
+
+
 def read_mfma_table(path):
     """{kernel: (dispatches, type, executed GFLOP per dispatch, TFLOP/s, of-peak, busy)} from an `mfma` summary."""
     out = {}
@@ -214,8 +254,13 @@ def main(argv=None):
     m.add_argument("--agent", required=True)
     m.add_argument("--title", default="")
     m.add_argument("--command", default="")
+    c = sub.add_parser("counters")
+    c.add_argument("csv")
+    c.add_argument("--match", nargs="*", default=[])
+    c.add_argument("--title", default="")
+    c.add_argument("--command", default="")
     a = p.parse_args(argv)
-    {"stats": cmd_stats, "traffic": cmd_traffic, "mfma": cmd_mfma}[a.cmd](a)
+    {"stats": cmd_stats, "traffic": cmd_traffic, "mfma": cmd_mfma, "counters": cmd_counters}[a.cmd](a)
 
 
 if __name__ == "__main__":
