@@ -4,7 +4,6 @@
 #include <algorithm>
 #include <cstring>
 #include <thread>
-#include <type_traits>
 #include <unordered_map>
 
 #include "split.hpp"
@@ -303,16 +302,6 @@ __device__ __forceinline__ void store_partials(const f32x4 *acc, int si, int sj,
 
 inline int nfrag(int rows) { return (rows / 16 + HB_WAVES - 1) / HB_WAVES; }
 
-// f(integral_constant<int, I>) for I = B .. E - 1 in order, each I a
-// compile-time constant (the chunk loops index register arrays with it)
-template <int B, int E, class F>
-__device__ __forceinline__ void static_for(F &&f) {
-    if constexpr (B < E) {
-        f(std::integral_constant<int, B>{});
-        static_for<B + 1, E>(f);
-    }
-}
-
 __device__ __forceinline__ void tile_bf16x3(const u16x8 *__restrict__ PH, const u16x8 *__restrict__ PL,
                                             const u16x8 *__restrict__ VH, const u16x8 *__restrict__ VL, int si,
                                             int sj, int lane, f32x4 *acc) {
@@ -337,9 +326,7 @@ __device__ __forceinline__ void tile_bf16x3(const u16x8 *__restrict__ PH, const 
     }
 }
 
-// TA: chunks between a chunk's L2 touch and its staging (2 or 3; the
-// autotuner picks).  A touch is released when its chunk is staged.
-template <int NF, int TA>
+template <int NF>
 __device__ __forceinline__ void hblk_item(const HItem it, const BlockArgs &A, unsigned char *lds_raw) {
     const BlockDev B = as_global(A.blocks)[it.block];
     const GLOBAL_AS uint16_t *__restrict__ hi = as_global(A.hi);
@@ -382,9 +369,9 @@ __device__ __forceinline__ void hblk_item(const HItem it, const BlockArgs &A, un
             fl[f] = *reinterpret_cast<const GLOBAL_AS u16x8 *>(sl + (g * 64 + lane) * 8);
         }
     };
-    // chunk c is touched TA iterations before it is staged (c - TA), loaded
-    // into registers one iteration before (c - 1); touches rotate over TA
-    // register slots so that no register copy waits on a load in flight
+    // chunk c is touched two iterations before it is staged (c - 2), loaded
+    // into registers one iteration before (c - 1); touches alternate between
+    // two register slots so that no register copy waits on a load in flight
     auto step = [&](int c, Touch &slot) {
         __syncthreads();
         // V at this lane's eight points k = 32 c + 8 q + e
@@ -413,30 +400,32 @@ __device__ __forceinline__ void hblk_item(const HItem it, const BlockArgs &A, un
         // c is a constant in the unrolled loop: these branches fold away
         if (c + 1 < NCHUNK) prefetch(c + 1);
         else epi_fetch(epi, A, B, tid);  // the epilogue's tables, behind the last chunk's MFMAs
-        if (c + TA < NCHUNK) slot = touch_chunk(hi, lo, B.tile_off + (int64_t)(c + TA) * G * FRAG, R, tid);
+        if (c + 2 < NCHUNK) slot = touch_chunk(hi, lo, B.tile_off + (int64_t)(c + 2) * G * FRAG, R, tid);
         if (has0) tile_bf16x3(PH, PL, VH, VL, si0, sj0, lane, acc0);
         if (has1) tile_bf16x3(PH, PL, VH, VL, si1, sj1, lane, acc1);
     };
-    Touch ts[TA];
-    ts[0] = Touch{0u, 0u};
-    static_for<1, TA>([&](auto k) { ts[k] = touch_chunk(hi, lo, B.tile_off + (int64_t)k * G * FRAG, R, tid); });
+    Touch t0{0u, 0u}, t1 = touch_chunk(hi, lo, B.tile_off + (int64_t)G * FRAG, R, tid);
     prefetch(0);
     // fully unrolled: with no loop back-edge the compiler's waits count the
     // loads exactly and never wait for a touch issued in the step before
-    static_for<0, NCHUNK>([&](auto c) { step(c, ts[c % TA]); });
+#pragma unroll
+    for (int c = 0; c < NCHUNK; c += 2) {
+        step(c, t0);
+        step(c + 1, t1);
+    }
     // lane holds D[4q + v][l16] of each 16x16 tile
     const EpiTables T = epi_store(epi, B, lds_raw, tid);
     if (has0) store_partials(acc0, si0, sj0, lane, T, part);
     if (has1) store_partials(acc1, si1, sj1, lane, T, part);
 }
 
-template <int NF, int MODE, int TA = 2>
+template <int NF, int MODE>
 __global__ __launch_bounds__(64 * HB_WAVES) void k_hblk_bf(const HItem *__restrict__ items, int32_t nitems,
                                                           const BlockArgs *__restrict__ args, BlockArgs one,
                                                           int32_t *__restrict__ queue) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     run_items<MODE>(items, nitems, args, one, queue,
-                    [&](const HItem &it, const BlockArgs &A) { hblk_item<NF, TA>(it, A, lds_raw); });
+                    [&](const HItem &it, const BlockArgs &A) { hblk_item<NF>(it, A, lds_raw); });
 }
 
 // ---- fp32 form --------------------------------------------------------------
@@ -917,31 +906,6 @@ int split_tables(lcpao_ctx *ctx, int precision) {
     return LCPAO_OK;
 }
 
-// Autotuner for the block kernels: each candidate `launch(v)` once untimed,
-// once between events on the context stream; the fastest is kept.
-template <class F>
-hipError_t tune_launches(lcpao_ctx *ctx, int op, F launch) {
-    hipEvent_t e0, e1;
-    hipError_t e = hipEventCreate(&e0);
-    if (e != hipSuccess) return e;
-    if ((e = hipEventCreate(&e1)) != hipSuccess) { (void)hipEventDestroy(e0); return e; }
-    int best = 0;
-    float best_ms = 0.f;
-    for (int v = 0; v < TUNE_NCAND[op] && e == hipSuccess; v++) {
-        float ms = 0.f;
-        if ((e = launch(v)) != hipSuccess || (e = hipEventRecord(e0, ctx->stream)) != hipSuccess ||
-            (e = launch(v)) != hipSuccess || (e = hipEventRecord(e1, ctx->stream)) != hipSuccess ||
-            (e = hipEventSynchronize(e1)) != hipSuccess || (e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess)
-            break;
-        ctx->variant_ms[op][v] = ms;
-        if (v == 0 || ms < best_ms) { best = v; best_ms = ms; }
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (e == hipSuccess) ctx->variant[op] = best;
-    return e;
-}
-
 int split_hamiltonian(lcpao_ctx *ctx, int nch, const double *const *d_V, double *const *d_H, int precision) {
     int rc = split_tables(ctx, precision);
     if (rc) return rc;
@@ -957,14 +921,8 @@ int split_hamiltonian(lcpao_ctx *ctx, int nch, const double *const *d_V, double 
             const unsigned ni = (unsigned)P->items.size();
             if (bf) {
                 const BlockArgs one = block_args(ctx, d_V[c], nullptr);
-                // candidates: 0 = L2 touch two chunks ahead, 1 = three; both
-                // overwrite the same partials, so tuning launches are harmless
-                auto launch = [&](int v) {
-                    return launch_block_kernel(false, ctx->stream, P->max_rows, 0, P->d_items, (int32_t)ni, nullptr,
-                                               one, nullptr, ni, v);
-                };
-                if (ctx->variant[TUNE_HBF] < 0) SPLIT_TRY(ctx, tune_launches(ctx, TUNE_HBF, launch));
-                SPLIT_TRY(ctx, launch(ctx->variant[TUNE_HBF]));
+                SPLIT_TRY(ctx, launch_block_kernel(false, ctx->stream, P->max_rows, 0, P->d_items, (int32_t)ni, nullptr,
+                                                   one, nullptr, ni));
             } else {
                 const size_t lds = (size_t)P->max_rows * LROW32 * sizeof(float) + CHUNK * sizeof(float);
                 SPLIT_TRY(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(k_hblk_f32),
@@ -1435,13 +1393,10 @@ size_t block_kernel_lds(bool density, int rows, int max_slots, bool persistent, 
 
 hipError_t launch_block_kernel(bool density, hipStream_t stream, int rows, int max_slots, const HItem *items,
                                int32_t nitems, const BlockArgs *args, const BlockArgs &one, int32_t *queue,
-                               unsigned grid, int variant) {
+                               unsigned grid, bool db) {
     const int mode = launch_mode(args, queue);
     const bool persistent = mode == LAUNCH_QUEUE;
-    const bool db = density && variant == 1;      // density: double-buffered chunk planes
-    const bool ta3 = !density && variant == 1;    // H: L2 touch three chunks ahead
-    if (db && (persistent || !rho_db_fits(rows, max_slots))) return hipErrorInvalidValue;
-    if (ta3 && mode != LAUNCH_ONE) return hipErrorInvalidValue;
+    if (db && (!density || persistent || !rho_db_fits(rows, max_slots))) return hipErrorInvalidValue;
     const size_t lds = block_kernel_lds(density, rows, max_slots, persistent, db);
     void (*kern)(const HItem *, int32_t, const BlockArgs *, BlockArgs, int32_t *) = nullptr;
 #define BY_MODE(K, NF)                                                                             \
@@ -1465,14 +1420,6 @@ hipError_t launch_block_kernel(bool density, hipStream_t stream, int rows, int m
             case 3: kern = k_rblk_bf<3, LAUNCH_QUEUE>; break;
             case 4: kern = k_rblk_bf<4, LAUNCH_QUEUE>; break;
             default: kern = k_rblk_bf<MAXF, LAUNCH_QUEUE>; break;
-        }
-    } else if (ta3) {
-        switch (nfrag(rows)) {
-            case 1: kern = k_hblk_bf<1, LAUNCH_ONE, 3>; break;
-            case 2: kern = k_hblk_bf<2, LAUNCH_ONE, 3>; break;
-            case 3: kern = k_hblk_bf<3, LAUNCH_ONE, 3>; break;
-            case 4: kern = k_hblk_bf<4, LAUNCH_ONE, 3>; break;
-            default: kern = k_hblk_bf<MAXF, LAUNCH_ONE, 3>; break;
         }
     } else {
         switch (nfrag(rows)) {
@@ -1531,9 +1478,27 @@ int split_density(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t 
                 if (!db_ok) {
                     ctx->variant[TUNE_RHOBF] = 0;
                 } else {
-                    // the candidates add into rho, which is cleared again
-                    // before the real launch
-                    SPLIT_TRY(ctx, tune_launches(ctx, TUNE_RHOBF, launch));
+                    // each candidate once untimed, once between events; they add
+                    // into rho, which is cleared again before the real launch
+                    hipEvent_t e0, e1;
+                    SPLIT_TRY(ctx, hipEventCreate(&e0));
+                    SPLIT_TRY(ctx, hipEventCreate(&e1));
+                    int best = 0;
+                    float best_ms = 0.f;
+                    for (int v = 0; v < TUNE_NCAND[TUNE_RHOBF]; v++) {
+                        SPLIT_TRY(ctx, launch(v));
+                        SPLIT_TRY(ctx, hipEventRecord(e0, ctx->stream));
+                        SPLIT_TRY(ctx, launch(v));
+                        SPLIT_TRY(ctx, hipEventRecord(e1, ctx->stream));
+                        SPLIT_TRY(ctx, hipEventSynchronize(e1));
+                        float ms = 0.f;
+                        SPLIT_TRY(ctx, hipEventElapsedTime(&ms, e0, e1));
+                        ctx->variant_ms[TUNE_RHOBF][v] = ms;
+                        if (v == 0 || ms < best_ms) { best = v; best_ms = ms; }
+                    }
+                    (void)hipEventDestroy(e0);
+                    (void)hipEventDestroy(e1);
+                    ctx->variant[TUNE_RHOBF] = best;
                     if (ctx->nn > 0)
                         hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((ctx->nn + 255) / 256)), dim3(256), 0,
                                            ctx->stream, ctx->nn, d_rho[s], 0.0);

@@ -253,24 +253,6 @@ def test_density_bf16x3_buffer_variants(name):
     assert np.abs(out[0] - out[1]).max() <= 1e-13 * np.abs(out[0]).max()
 
 
-@pytest.mark.parametrize("name", ["si2", "si8", "nacl8", "si2_r1of3"])
-def test_hamiltonian_bf16x3_touch_variants(name):
-    """Both bf16x3 H kernels the autotuner may pick (L2 touch two or three
-    chunks ahead) give the same bits and meet the bar."""
-    c = case(name)
-    ctx = GridContext(c.u)
-    ctx.set_tables(c.tabs)
-    out = []
-    for v in (0, 1):
-        ctx.tune("hbf", v)
-        print(name, "H bf16x3 variant", v, flush=True)
-        out.append(ctx.hamiltonian([c.V], precision="bf16x3")[0])
-        assert ctx.tuned("hbf")[0] == v
-    assert np.array_equal(out[0], out[1])
-    Href, M = O.hamiltonian(c.g, c.tabs, [c.V], with_abs=True)
-    assert worst_ratio(out[0], Href[0], M[0]) <= 1e-4
-
-
 @pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("name", ["si2", "nacl8", "si2_r1of3"])
 def test_fp64_kernel_variants(name, variant):
