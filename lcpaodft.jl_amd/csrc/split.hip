@@ -1450,6 +1450,43 @@ hipError_t launch_block_kernel(bool density, hipStream_t stream, int rows, int m
     return hipGetLastError();
 }
 
+// Autotuner of the block kernels' candidates `launch(v)`: each once
+// untimed, then TUNE_ROUNDS timed launches per candidate in alternating
+// order, keeping each one's fastest (a single launch each is within the
+// noise of a GPU); the fastest candidate is kept in ctx->variant[op].
+constexpr int TUNE_ROUNDS = 4;
+template <class F>
+hipError_t tune_block_kernel(lcpao_ctx *ctx, int op, F launch) {
+    const int nc = TUNE_NCAND[op];
+    hipEvent_t e0, e1;
+    hipError_t e = hipEventCreate(&e0);
+    if (e != hipSuccess) return e;
+    if ((e = hipEventCreate(&e1)) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return e;
+    }
+    for (int v = 0; v < nc && e == hipSuccess; v++) e = launch(v);
+    for (int r = 0; r < TUNE_ROUNDS && e == hipSuccess; r++)
+        for (int x = 0; x < nc && e == hipSuccess; x++) {
+            const int v = r % 2 ? nc - 1 - x : x;
+            float ms = 0.f;
+            if ((e = hipEventRecord(e0, ctx->stream)) != hipSuccess || (e = launch(v)) != hipSuccess ||
+                (e = hipEventRecord(e1, ctx->stream)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess ||
+                (e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess)
+                break;
+            float &m = ctx->variant_ms[op][v];
+            if (r == 0 || ms < m) m = ms;
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return e;
+    int best = 0;
+    for (int v = 1; v < nc; v++)
+        if (ctx->variant_ms[op][v] < ctx->variant_ms[op][best]) best = v;
+    ctx->variant[op] = best;
+    return hipSuccess;
+}
+
 int split_density(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_offset, double *const *d_rho) {
     int rc = split_tables(ctx, LCPAO_BF16X3);
     if (rc) return rc;
@@ -1478,27 +1515,9 @@ int split_density(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t 
                 if (!db_ok) {
                     ctx->variant[TUNE_RHOBF] = 0;
                 } else {
-                    // each candidate once untimed, once between events; they add
-                    // into rho, which is cleared again before the real launch
-                    hipEvent_t e0, e1;
-                    SPLIT_TRY(ctx, hipEventCreate(&e0));
-                    SPLIT_TRY(ctx, hipEventCreate(&e1));
-                    int best = 0;
-                    float best_ms = 0.f;
-                    for (int v = 0; v < TUNE_NCAND[TUNE_RHOBF]; v++) {
-                        SPLIT_TRY(ctx, launch(v));
-                        SPLIT_TRY(ctx, hipEventRecord(e0, ctx->stream));
-                        SPLIT_TRY(ctx, launch(v));
-                        SPLIT_TRY(ctx, hipEventRecord(e1, ctx->stream));
-                        SPLIT_TRY(ctx, hipEventSynchronize(e1));
-                        float ms = 0.f;
-                        SPLIT_TRY(ctx, hipEventElapsedTime(&ms, e0, e1));
-                        ctx->variant_ms[TUNE_RHOBF][v] = ms;
-                        if (v == 0 || ms < best_ms) { best = v; best_ms = ms; }
-                    }
-                    (void)hipEventDestroy(e0);
-                    (void)hipEventDestroy(e1);
-                    ctx->variant[TUNE_RHOBF] = best;
+                    // the candidates add into rho, which is cleared again
+                    // before the real launch
+                    SPLIT_TRY(ctx, tune_block_kernel(ctx, TUNE_RHOBF, launch));
                     if (ctx->nn > 0)
                         hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((ctx->nn + 255) / 256)), dim3(256), 0,
                                            ctx->stream, ctx->nn, d_rho[s], 0.0);
