@@ -35,7 +35,9 @@ extern "C" {
 enum {
     LCPAO_OK = 0,
     LCPAO_EINVAL = 1,    /* bad argument / inconsistent geometry */
-    LCPAO_ERANGE = 2,    /* size beyond a supported maximum (e.g. NO > 32) */
+    LCPAO_ERANGE = 2,    /* size beyond a supported maximum (e.g. NO > 32, or more than 576
+                            orbital rows / 45 sphere images on one 8x8x8 grid block
+                            for the block forms: split-BF16x3, FP32 H) */
     LCPAO_EDEVICE = 3,   /* HIP runtime error, no device, or out of device memory */
     LCPAO_ESTATE = 4,    /* call order (e.g. contraction before tables are set) */
     LCPAO_EMESH = 5      /* radial mesh index out of range in table generation */
