@@ -157,3 +157,39 @@ def test_cluster_forms_are_the_gamma_sums():
     assert np.array_equal(Sm, So.real) and np.array_equal(Hm, Ho.real) and np.array_equal(S1, Sm)
     assert np.abs(So.imag).max() == 0.0
     ctx.close()
+
+
+def test_nc_forms_on_a_rank_share_and_edge_cases():
+    """The non-collinear Bloch sum and DM on one rank's share of the pairs
+    (rank 1 of 3) match the oracle built for the same share; zero states give
+    zero density matrices; null pointers are reported, not dereferenced."""
+    import ctypes as C
+    from lcpao_grid import _lib
+    from lcpao_grid.grid import GridError
+    st = S.si2(16)
+    u = st.ucell(3, 1)
+    g = O.Geom(st.Latvecs, st.Gxyz, st.cutoffs(), st.norbs(), st.Ngrid, (0, 0, 0), 3, 1)
+    ctx = GridContext(u)
+    fsize, _ = ctx.bloch_sizes()
+    rng = np.random.default_rng(31)
+    hks = [rng.standard_normal(u.myHsize) for _ in range(4)]
+    ihks = [rng.standard_normal(u.myHsize) for _ in range(3)]
+    k = [0.11, 0.23, -0.37]
+    assert _rel(ctx.bloch_sum_nc(k, hks, ihks), O.hs_matrix_nc(g, k, hks, ihks)) < 1e-13
+    kpts = np.array([[0.0, 0.0, 0.0], [0.2, -0.3, 0.1]])
+    Cs = _states(2 * fsize, 5, len(kpts), 32)
+    DM, iDM = ctx.dm_states_nc(kpts, Cs, 0.5, with_idm=True)
+    DMo, iDMo = O.dm_nc(g, kpts, Cs, 0.5)
+    for a, b in zip(DM + iDM, DMo + iDMo):
+        assert _rel(a, b) < 1e-13
+    # no occupied states: every channel is zero
+    Z, iZ = ctx.dm_states_nc(kpts, [c[:, :0] for c in Cs], 0.5, with_idm=True)
+    assert all(np.abs(x).max() == 0.0 for x in Z + iZ)
+    # null state pointer and null DM pointers: errors, no call into the kernels
+    kp = np.ascontiguousarray(kpts)
+    nullp = (_lib.dp * 2)(None, None)
+    with pytest.raises(GridError):
+        ctx._check(ctx.L.lcpao_grid_dm_states_nc(ctx.h, 2, kp.ctypes.data_as(_lib.dp), 5, nullp, 0.5, None, None))
+    with pytest.raises(GridError):
+        ctx._check(ctx.L.lcpao_grid_bloch_sum_nc(ctx.h, kp.ctypes.data_as(_lib.dp), None, None, None))
+    ctx.close()
