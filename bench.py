@@ -770,7 +770,7 @@ def main():
         "speedup_vs_fp64": round(timing["fp64"]["ms"] / ms, 3) if "fp64" in timing else None,
         "kernels": {f: timing[f]["kernels"] for f in timed},
         "accuracy_vs_fp64": acc,
-        "autotune": {op: {"variant": ctx.tuned(op)[0], "candidate_ms": ctx.tuned(op)[1]} for op in ("rho64", "rhobf")},
+        "autotune": {op: {"variant": ctx.tuned(op)[0], "candidate_ms": ctx.tuned(op)[1]} for op in ("rho64", "rhobf", "rho32")},
         "tflops_reference_algorithm": round(f_ref / (ms * 1e-3) / 1e12, 3),
         "setup_s": round(setup_s, 2),
         "roofline": roofline,

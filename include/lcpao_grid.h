@@ -189,10 +189,13 @@ int lcpao_grid_kernel_stats(const lcpao_ctx *ctx, int32_t kernel_id, int64_t *la
  *   LCPAO_TUNE_RHOBF  bf16x3 density (0 single-buffered chunk planes,
  *                     1 double-buffered, used only when its LDS fits;
  *                     both give the same bits)
+ *   LCPAO_TUNE_RHO32  FP32 density  (0 pair-list kernel, 1 block form on the
+ *                     FP32 matrix cores, used only when the block plan holds
+ *                     the geometry)
  * lcpao_grid_tune fixes a variant (or -1 to re-tune at the next call);
  * lcpao_grid_tuned reports the choice and, when candidate_ms is not NULL,
  * the measured milliseconds of up to ncand candidates. */
-enum { LCPAO_TUNE_RHO64 = 0, LCPAO_TUNE_F364 = 1, LCPAO_TUNE_RHOBF = 2 };
+enum { LCPAO_TUNE_RHO64 = 0, LCPAO_TUNE_F364 = 1, LCPAO_TUNE_RHOBF = 2, LCPAO_TUNE_RHO32 = 3 };
 int lcpao_grid_tune(lcpao_ctx *ctx, int32_t op, int32_t variant);
 int lcpao_grid_tuned(const lcpao_ctx *ctx, int32_t op, int32_t *variant, float *candidate_ms, int32_t ncand);
 

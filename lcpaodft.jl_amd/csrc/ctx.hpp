@@ -127,8 +127,8 @@ struct lcpao_ctx {
 };
 
 // tunable operations and their candidate counts
-enum { TUNE_RHO64 = 0, TUNE_F364 = 1, TUNE_RHOBF = 2, TUNE_NOPS = 3 };
-constexpr int TUNE_NCAND[TUNE_NOPS] = {2, 2, 2};
+enum { TUNE_RHO64 = 0, TUNE_F364 = 1, TUNE_RHOBF = 2, TUNE_RHO32 = 3, TUNE_NOPS = 4 };
+constexpr int TUNE_NCAND[TUNE_NOPS] = {2, 2, 2, 2};
 
 
 

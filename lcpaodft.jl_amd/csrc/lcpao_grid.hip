@@ -1195,6 +1195,9 @@ int autotune(lcpao_ctx *ctx, int op, F launch) {
     return LCPAO_OK;
 }
 
+int rho_dev_pairs(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_offset, double *const *d_rho,
+                  bool f32);
+
 int rho_dev(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_offset, double *const *d_rho, int precision) {
     if (!ctx->tables) return set_err(ctx, LCPAO_ESTATE, "orbital tables are not set");
     if (nspin != 1 && nspin != 2 && nspin != 4) return set_err(ctx, LCPAO_EINVAL, "nspin must be 1, 2 or 4");
@@ -1203,10 +1206,58 @@ int rho_dev(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_off
     if (rc) return rc;
     if (precision == LCPAO_BF16X3) return split_density(ctx, nspin, d_DM, dm_offset, d_rho);
     const bool f32 = precision != LCPAO_FP64;
+    for (int s = 0; s < nspin; s++)
+        if (!d_DM[s] || !d_rho[s]) return set_err(ctx, LCPAO_EINVAL, "null spin pointer");
+    // FP32 candidates: 0 = pair-list kernel (k_rho<float>), 1 = block form on
+    // the FP32 matrix cores (k_rblk_f32); a geometry the block plan cannot
+    // hold keeps the pair-list kernel
+    if (f32 && ctx->variant[TUNE_RHO32] != 0) {
+        const int r2 = split_tables(ctx, LCPAO_FP32);
+        if (r2 == LCPAO_ERANGE) {
+            ctx->variant[TUNE_RHO32] = 0;
+            ctx->err.clear();
+        } else if (r2) {
+            return r2;
+        }
+    }
+    if (f32 && ctx->variant[TUNE_RHO32] == 1) {
+        if ((rc = split_density_f32_enqueue(ctx, nspin, d_DM, dm_offset, d_rho))) return rc;
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        collect_timings(ctx);
+        return LCPAO_OK;
+    }
     if (f32 && (rc = fp32_tables(ctx, false))) return rc;
+    if (f32 && ctx->variant[TUNE_RHO32] < 0) {
+        // both candidates overwrite rho; tuned with every spin's work
+        int trc = LCPAO_OK;
+        auto cand = [&](int v) {
+            if (trc) return;
+            trc = v == 0 ? rho_dev_pairs(ctx, nspin, d_DM, dm_offset, d_rho, true)
+                         : split_density_f32_enqueue(ctx, nspin, d_DM, dm_offset, d_rho);
+        };
+        if ((rc = autotune(ctx, TUNE_RHO32, cand))) return rc;
+        if (trc) return trc;
+        if (ctx->variant[TUNE_RHO32] == 1) {
+            if ((rc = split_density_f32_enqueue(ctx, nspin, d_DM, dm_offset, d_rho))) return rc;
+            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+            collect_timings(ctx);
+            return LCPAO_OK;
+        }
+    }
+    if ((rc = rho_dev_pairs(ctx, nspin, d_DM, dm_offset, d_rho, f32))) return rc;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    collect_timings(ctx);
+    return LCPAO_OK;
+}
+
+// The pair-list density kernels (FP64 candidates or the FP32 one) for every
+// spin: fold, contraction into the per-entry scratch, gather into rho
+// (enqueue only).
+int rho_dev_pairs(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_offset, double *const *d_rho,
+                  bool f32) {
+    int rc = LCPAO_OK;
     const int nu = (int)ctx->runits.size();
     for (int s = 0; s < nspin; s++) {
-        if (!d_DM[s] || !d_rho[s]) return set_err(ctx, LCPAO_EINVAL, "null spin pointer");
         const double *D = d_DM[s] + dm_offset;
         if (ctx->ncanon > 0) {
             KTimer kt(ctx, KID_DEFF);
@@ -1246,9 +1297,7 @@ int rho_dev(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_off
         }
         HIP_TRY(ctx, hipGetLastError());
     }
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    collect_timings(ctx);
-    return LCPAO_OK;
+    return rc;
 }
 
 int f3_dev(lcpao_ctx *ctx, int nch, const double *const *d_V, const double *const *d_DM, int64_t dm_offset,

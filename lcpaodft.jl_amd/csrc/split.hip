@@ -1340,6 +1340,121 @@ __device__ __forceinline__ void rblk_item(const HItem it, const BlockArgs &A, un
     }
 }
 
+// ---- FP32 block density -----------------------------------------------------
+// rho_k = sum_{si<=sj} Phi_si^T W[si][sj] Phi_sj as in k_rblk_bf, with FP32
+// operands on the FP32 matrix cores (v_mfma_f32_16x16x4f32): one 1 024-thread
+// workgroup per item, one super-tile per wave.  The wave's W super-tile is
+// held as A fragments (lane l of fragment (rt, kt): W[16 rt + l % 16][4 kt +
+// l / 16], 64 VGPRs), gathered from the FP32 fold as the bf16x3 kernel does;
+// per 32-point chunk the block's FP32 tile (R x 32, the H kernel's tile in
+// its perm32 point order) is staged in LDS, Q = W Phi_sj is formed per
+// 16-point group and dotted with Phi_si at the same rows, and the four lanes
+// of a point are summed with the lane swaps into per-wave LDS slots, added in
+// wave order at the end of the item (one FP64 atomic per item and point).
+constexpr int RF_WAVES = 16;
+constexpr int RF_SUMS_OFFSET = MAX_ROWS * LROW32 * 4;
+constexpr int RF_AREA_BYTES = RF_SUMS_OFFSET + RF_WAVES * BLK_PTS * 4;
+
+__device__ __forceinline__ void rblk_f32_item(const HItem it, const BlockArgs &A, const float *__restrict__ t32,
+                                              unsigned char *lds_raw) {
+    const BlockDev B = A.blocks[it.block];
+    const int16_t *__restrict__ rowinfo = A.rowinfo;
+    const SlotPair *__restrict__ spair = A.spair;
+    const float *__restrict__ Deff = A.Deff;
+    double *__restrict__ rho = A.rho;
+    const int n1 = A.n1, n2 = A.n2, n3 = A.n3;
+    const int R = B.rows, S = R / ST;
+    const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63, l16 = lane & 15,
+              q = lane >> 4;
+    float *A32 = reinterpret_cast<float *>(lds_raw);
+    float *rcw = reinterpret_cast<float *>(lds_raw + RF_SUMS_OFFSET);  // [wave][512 points]
+    int16_t *s_rinfo = reinterpret_cast<int16_t *>(lds_raw + RF_AREA_BYTES);
+    SlotPair *s_sp = reinterpret_cast<SlotPair *>(s_rinfo + MAX_ROWS);
+    for (int x = tid; x < R; x += 64 * RF_WAVES) s_rinfo[x] = rowinfo[B.row_off + x];
+    for (int x = tid; x < B.nslot * B.nslot; x += 64 * RF_WAVES) s_sp[x] = spair[B.sp_off + x];
+    __syncthreads();
+    const bool has = wave < it.nst;
+    int si = 0, sj = 0;
+    if (has) st_decode(it.st0 + wave, S, si, sj);
+    // W fragments: w[rt][kt] = W[16 rt + l16][4 kt + q] of super-tile (si, sj)
+    float w[4][16];
+    {
+        const float wdiag = si == sj ? 0.5f : 1.f;
+        const float keep = has ? 1.f : 0.f;
+#pragma unroll
+        for (int rt = 0; rt < 4; rt++) {
+            const int rinf = s_rinfo[si * ST + 16 * rt + l16];
+            const int ra = rinf >> 5, ri = rinf & 31;
+#pragma unroll
+            for (int kt = 0; kt < 16; kt++) {
+                const int cinf = s_rinfo[sj * ST + 4 * kt + q];
+                const int cb = cinf >> 5, cj = cinf & 31;
+                const bool valid = rinf >= 0 && cinf >= 0;
+                const SlotPair P = s_sp[valid ? min(ra, cb) * B.nslot + max(ra, cb) : 0];
+                const bool sw = ra > cb;  // lower triangle of a diagonal super-tile: W is symmetric
+                const int i = sw ? cj : ri, j = sw ? ri : cj;
+                const bool ok = valid && P.hoff >= 0;
+                const int ld = P.meta & 63;
+                const int x1 = (P.meta >> 6) ? j * ld + i : i * ld + j;
+                // D is finite, so masking by a product keeps the load unconditional
+                w[rt][kt] = Deff[ok ? P.hoff + x1 : 0] * (ok ? wdiag * keep : 0.f);
+            }
+        }
+    }
+    for (int c = 0; c < NCHUNK; c++) {
+        __syncthreads();  // every wave is done with the previous chunk
+        const float *s32 = t32 + B.tile_off + (int64_t)c * R * CHUNK;
+        for (int e = tid * 4; e < R * CHUNK; e += 64 * RF_WAVES * 4) {
+            const int row = e / CHUNK, col = e % CHUNK;
+            *reinterpret_cast<f32x4 *>(&A32[row * LROW32 + col]) = *reinterpret_cast<const f32x4 *>(&s32[e]);
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int pt = 0; pt < 2; pt++) {
+            // the lane's point: 16 pt + l16, stored at perm32 of it
+            const int pcol = perm32(16 * pt + l16);
+            float part = 0.f;
+            if (has) {
+                f32x4 acc[4];
+#pragma unroll
+                for (int rt = 0; rt < 4; rt++) acc[rt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int kt = 0; kt < 16; kt++) {
+                    const float b = A32[(sj * ST + 4 * kt + q) * LROW32 + pcol];
+#pragma unroll
+                    for (int rt = 0; rt < 4; rt++) acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(w[rt][kt], b, acc[rt], 0, 0, 0);
+                }
+                // acc[rt][v] = Q[16 rt + 4 q + v][point 16 pt + l16]
+#pragma unroll
+                for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        part = fmaf(A32[(si * ST + 16 * rt + 4 * q + v) * LROW32 + pcol], acc[rt][v], part);
+            }
+            const auto s32s = __builtin_amdgcn_permlane32_swap(__float_as_uint(part), __float_as_uint(part), false, false);
+            part = __uint_as_float(s32s[0]) + __uint_as_float(s32s[1]);
+            const auto s16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(part), __float_as_uint(part), false, false);
+            part = __uint_as_float(s16[0]) + __uint_as_float(s16[1]);
+            if (q == 0) rcw[wave * BLK_PTS + c * CHUNK + pt * 16 + l16] = part;
+        }
+    }
+    __syncthreads();
+    if (tid < BLK_PTS) {
+        const int k = tid;
+        float sum = 0.f;
+#pragma unroll
+        for (int w2 = 0; w2 < RF_WAVES; w2++) sum += rcw[w2 * BLK_PTS + k];
+        const int x1 = B.g1 + k / (BLK * BLK), x2 = B.g2 + (k / BLK) % BLK, x3 = B.g3 + k % BLK;
+        if (x1 < n1 && x2 < n2 && x3 < n3) atomicAdd(&rho[((int64_t)x1 * n2 + x2) * n3 + x3], (double)sum);
+    }
+}
+
+__global__ __launch_bounds__(64 * RF_WAVES) void k_rblk_f32(const HItem *__restrict__ items, int32_t nitems,
+                                                            const float *__restrict__ t32, BlockArgs A) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    if ((int)blockIdx.x < nitems) rblk_f32_item(items[blockIdx.x], A, t32, lds_raw);
+}
+
 // fragments per wave and plane of the density kernel for R rows
 inline int nfrag_rho(int rows) { return (rows / 16 + RB_WAVES - 1) / RB_WAVES; }
 constexpr int MAXF_RHO = (MAX_ROWS / 16 + RB_WAVES - 1) / RB_WAVES;
@@ -1485,6 +1600,34 @@ hipError_t tune_block_kernel(lcpao_ctx *ctx, int op, F launch) {
         if (ctx->variant_ms[op][v] < ctx->variant_ms[op][best]) best = v;
     ctx->variant[op] = best;
     return hipSuccess;
+}
+
+int split_density_f32_enqueue(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_offset,
+                              double *const *d_rho) {
+    BlockPlan *P = ctx->plan;
+    if (!P || !P->f32_ready) { ctx->err = "FP32 block tiles are not built"; return LCPAO_ESTATE; }
+    const size_t lds =
+        RF_AREA_BYTES + MAX_ROWS * sizeof(int16_t) + (size_t)P->max_slots * P->max_slots * sizeof(SlotPair);
+    SPLIT_TRY(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(k_rblk_f32),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    for (int s = 0; s < nspin; s++) {
+        if (!d_DM[s] || !d_rho[s]) { ctx->err = "null spin pointer"; return LCPAO_EINVAL; }
+        if (ctx->ncanon > 0) {
+            KTimer kt(ctx, KID_DEFF);
+            launch_deff_f32(ctx, d_DM[s] + dm_offset, 1);
+        }
+        if (ctx->nn > 0)
+            hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((ctx->nn + 255) / 256)), dim3(256), 0, ctx->stream, ctx->nn,
+                               d_rho[s], 0.0);
+        if (!P->items.empty()) {
+            KTimer kt(ctx, KID_RHO);
+            const unsigned ni = (unsigned)P->items.size();
+            hipLaunchKernelGGL(k_rblk_f32, dim3(ni), dim3(64 * RF_WAVES), lds, ctx->stream, P->d_items, (int32_t)ni,
+                               P->d_t32, block_args(ctx, nullptr, d_rho[s]));
+        }
+        SPLIT_TRY(ctx, hipGetLastError());
+    }
+    return LCPAO_OK;
 }
 
 int split_density(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_offset, double *const *d_rho) {

@@ -151,6 +151,10 @@ hipError_t launch_block_kernel(bool density, hipStream_t stream, int rows, int m
                                unsigned grid, bool db = false);
 void plan_free(BlockPlan *p);
 int split_tables(lcpao_ctx *ctx, int precision);
+// FP32 density in the block form (needs split_tables(ctx, LCPAO_FP32));
+// enqueues the fold, the clear and one launch per spin, no synchronisation
+int split_density_f32_enqueue(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_offset,
+                              double *const *d_rho);
 int split_hamiltonian(lcpao_ctx *ctx, int nch, const double *const *d_V, double *const *d_H, int precision);
 int split_density(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_offset, double *const *d_rho);
 // bf16x3 Force3 of one channel: adds w * S_A into d_fsum (natom x 3, atom-major)

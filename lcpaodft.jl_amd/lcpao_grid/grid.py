@@ -345,10 +345,10 @@ class GridContext:
     def _dptrs(ts):
         return (_lib.dp * len(ts))(*[C.cast(t.data_ptr(), _lib.dp) for t in ts])
 
-    TUNE_OPS = {"rho64": 0, "f364": 1, "rhobf": 2}
+    TUNE_OPS = {"rho64": 0, "f364": 1, "rhobf": 2, "rho32": 3}
 
     def tune(self, op, variant=-1):
-        """Fix the kernel variant of a tunable operation ("rho64", "f364", "rhobf"), or
+        """Fix the kernel variant of a tunable operation ("rho64", "f364", "rhobf", "rho32"), or
         -1 to let the next call time the candidates and keep the fastest."""
         self._check(self.L.lcpao_grid_tune(self.h, self.TUNE_OPS[op], variant))
 

@@ -253,6 +253,33 @@ def test_density_bf16x3_buffer_variants(name):
     assert np.abs(out[0] - out[1]).max() <= 1e-13 * np.abs(out[0]).max()
 
 
+@pytest.mark.parametrize("name", ["si2", "si8", "nacl8", "si2_r1of3", "nacl8_r0of2"])
+def test_density_fp32_variants(name):
+    """Both FP32 density kernels the autotuner may pick (pair lists, or the
+    block form on the FP32 matrix cores) meet the FP32 bar on two spins and
+    agree with each other to FP32 round-off."""
+    c = case(name)
+    ctx = GridContext(c.u)
+    ctx.set_tables(c.tabs)
+    out = []
+    for v in (0, 1):
+        ctx.tune("rho32", v)
+        print(name, "rho fp32 variant", v, flush=True)
+        out.append(ctx.density([c.DM, c.DM2], dm_offset=c.u.MPHks_rank, precision="fp32"))
+        assert ctx.tuned("rho32")[0] == v
+    Rref, M = O.density(c.g, c.tabs, [c.DM, c.DM2], with_abs=True)
+    for R in out:
+        for k in range(2):
+            assert worst_ratio(R[k], Rref[k], M[k]) <= SPLIT_RTOL["fp32"]
+    for k in range(2):
+        assert np.abs(out[0][k] - out[1][k]).max() <= 1e-5 * np.abs(Rref[k]).max()
+    ctx.tune("rho32", -1)
+    R = ctx.density([c.DM], dm_offset=c.u.MPHks_rank, precision="fp32")[0]
+    v, ms = ctx.tuned("rho32")
+    assert v in (0, 1) and all(m > 0 for m in ms)
+    assert worst_ratio(R, Rref[0], M[0]) <= SPLIT_RTOL["fp32"]
+
+
 @pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("name", ["si2", "nacl8", "si2_r1of3"])
 def test_fp64_kernel_variants(name, variant):
