@@ -28,6 +28,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -37,6 +38,35 @@
 
 namespace {
 std::string g_create_err;
+}  // namespace
+
+// Scratch of the allocation-free FP64 Force3 entry point
+// (lcpao_grid_force3_dev_noalloc): the per-unit partial sums (3 doubles per
+// Force3 unit) followed by the per-atom sums (3 doubles per atom).  Every GPU
+// that loads the library holds one copy of this array until the process
+// ends; calls on one GPU share it.
+#define NOALLOC_SCRATCH_DOUBLES (2 * 1024 * 1024)  // 16 MiB
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+// Device address of `scratch` on each device, looked up once per device by
+// lcpao_grid_create with that device current.  The host-side name `scratch`
+// is not a device address and is never passed to a kernel or copy.
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+hipError_t g_scratch_rc[NOALLOC_MAX_DEVICES] = {};
+bool g_scratch_done[NOALLOC_MAX_DEVICES] = {};
+
+void scratch_setup(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_done[device]) return;
+    void *p = nullptr;
+    g_scratch_rc[device] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_ptr[device] = g_scratch_rc[device] == hipSuccess ? static_cast<double *>(p) : nullptr;
+    g_scratch_done[device] = true;
+}
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1358,6 +1388,63 @@ int f3_dev(lcpao_ctx *ctx, int nch, const double *const *d_V, const double *cons
     return LCPAO_OK;
 }
 
+// FP64 Force3 with its per-unit partials and per-atom sums in the
+// file-scope scratch array instead of the context's buffers: the same
+// kernels in the same order as f3_dev (so the same bits for the same kernel
+// variant), no allocation, no event timing and no autotuning (a context
+// whose FP64 Force3 variant is still open runs the vector-FMA kernel).
+int f3_dev_noalloc(lcpao_ctx *ctx, int nch, const double *const *d_V, const double *const *d_DM, int64_t dm_offset,
+                   const double *weight, double *d_force) {
+    if (!ctx->tables || !ctx->dtables) return set_err(ctx, LCPAO_ESTATE, "orbital and gradient tables must be set");
+    if (nch != 1 && nch != 2 && nch != 4) return set_err(ctx, LCPAO_EINVAL, "nch must be 1, 2 or 4");
+    if (!weight || !d_force) return set_err(ctx, LCPAO_EINVAL, "null weight or force pointer");
+    if (dm_offset < 0) return set_err(ctx, LCPAO_EINVAL, "negative dm_offset");
+    for (int c = 0; c < nch; c++)
+        if (!d_V[c] || !d_DM[c]) return set_err(ctx, LCPAO_EINVAL, "null channel pointer");
+    double *base = nullptr;
+    hipError_t src = hipErrorInvalidDevice;
+    if (ctx->device >= 0 && ctx->device < NOALLOC_MAX_DEVICES) {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (g_scratch_done[ctx->device]) { base = g_scratch_ptr[ctx->device]; src = g_scratch_rc[ctx->device]; }
+    }
+    if (!base)
+        return set_err(ctx, LCPAO_EDEVICE, std::string("scratch address lookup failed: ") + hipGetErrorString(src));
+    const int nu = (int)ctx->funits.size();
+    const int64_t fpart_n = ((3 * (int64_t)nu + 31) / 32) * 32;  // keep the per-atom sums 256-byte aligned
+    const int64_t need = fpart_n + 3 * (int64_t)ctx->natom;
+    if (need > NOALLOC_SCRATCH_DOUBLES)
+        return set_err(ctx, LCPAO_ERANGE, "geometry needs " + std::to_string(need) + " doubles of scratch (" +
+                                              std::to_string(nu) + " Force3 units, " + std::to_string(ctx->natom) +
+                                              " atoms); the allocation-free entry point holds at most " +
+                                              std::to_string((long long)NOALLOC_SCRATCH_DOUBLES));
+    double *fpart = base, *fsum = base + fpart_n;
+    const int variant = ctx->variant[TUNE_F364] >= 0 ? ctx->variant[TUNE_F364] : 1;
+    if (ctx->natom > 0)
+        hipLaunchKernelGGL(k_fill_zero, dim3((ctx->natom * 3 + 255) / 256), dim3(256), 0, ctx->stream,
+                           (int64_t)ctx->natom * 3, fsum);
+    for (int c = 0; c < nch; c++) {
+        const double *D = d_DM[c] + dm_offset;
+        if (nu > 0) {
+            if (variant == 0)
+                hipLaunchKernelGGL(k_f3_m64, dim3(nu), dim3(64), 0, ctx->stream, ctx->d_funits, ctx->d_forder, nu,
+                                   ctx->d_pairs, ctx->d_tab, ctx->d_dtab[0], ctx->d_dtab[1], ctx->d_dtab[2],
+                                   ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c], D, fpart);
+            else
+                hipLaunchKernelGGL(k_f3<double>, dim3(nu), dim3(64), 0, ctx->stream, ctx->d_funits, ctx->d_forder, nu,
+                                   ctx->d_pairs, ctx->d_tab, ctx->d_dtab[0], ctx->d_dtab[1], ctx->d_dtab[2],
+                                   ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c], D, fpart);
+            hipLaunchKernelGGL(k_f3_reduce, dim3((ctx->natom + 3) / 4), dim3(256), 0, ctx->stream, ctx->d_atom_ub,
+                               ctx->natom, fpart, weight[c], fsum);
+        }
+        HIP_TRY(ctx, hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_f3_apply, dim3((ctx->natom + 127) / 128), dim3(128), 0, ctx->stream, ctx->natom, fsum,
+                       ctx->gridvol, d_force);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return LCPAO_OK;
+}
+
 void free_all(lcpao_ctx *ctx) {
     void *ptrs[] = {ctx->d_glist, ctx->d_clist, ctx->d_gl_off, ctx->d_tab_off, ctx->d_no, ctx->d_olg1, ctx->d_olg2,
                     ctx->d_pairs, ctx->d_tab, ctx->d_dtab[0], ctx->d_dtab[1], ctx->d_dtab[2], ctx->d_hunits,
@@ -1424,6 +1511,7 @@ int lcpao_grid_create(lcpao_ctx **out, const lcpao_geom *g, int device) {
         return code;
     };
     if (hipSetDevice(device) != hipSuccess) return fail(LCPAO_EDEVICE, "hipSetDevice failed");
+    scratch_setup(device);
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return fail(LCPAO_EDEVICE, "stream creation failed");
     ctx->natom = g->natom;
     ctx->cpy = g->cpycell;
@@ -1855,6 +1943,14 @@ int lcpao_grid_force3_dev(lcpao_ctx *ctx, int32_t nch, const double *const *d_V,
     if (rc) return rc;
     if (!d_V || !d_DM) return set_err(ctx, LCPAO_EINVAL, "null argument");
     return f3_dev(ctx, nch, d_V, d_DM, dm_offset, weight, d_force, precision);
+}
+
+int lcpao_grid_force3_dev_noalloc(lcpao_ctx *ctx, int32_t nch, const double *const *d_V, const double *const *d_DM,
+                                  int64_t dm_offset, const double *weight, double *d_force) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!d_V || !d_DM) return set_err(ctx, LCPAO_EINVAL, "null argument");
+    return f3_dev_noalloc(ctx, nch, d_V, d_DM, dm_offset, weight, d_force);
 }
 
 int lcpao_grid_hamiltonian(lcpao_ctx *ctx, int32_t nch, const double *const *V, double *const *H, int32_t precision) {

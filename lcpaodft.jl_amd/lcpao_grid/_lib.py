@@ -43,7 +43,8 @@ class AtomicRadial(C.Structure):
 EXPORTS = [
     "lcpao_grid_create", "lcpao_grid_set_tables", "lcpao_grid_set_dtables", "lcpao_grid_build_tables",
     "lcpao_grid_set_positions", "lcpao_grid_get_tables", "lcpao_grid_hamiltonian", "lcpao_grid_density", "lcpao_grid_force3",
-    "lcpao_grid_hamiltonian_dev", "lcpao_grid_density_dev", "lcpao_grid_force3_dev", "lcpao_grid_set_stream",
+    "lcpao_grid_hamiltonian_dev", "lcpao_grid_density_dev", "lcpao_grid_force3_dev", "lcpao_grid_force3_dev_noalloc",
+    "lcpao_grid_set_stream",
     "lcpao_grid_profile", "lcpao_grid_kernel_stats", "lcpao_grid_tune", "lcpao_grid_tuned", "lcpao_grid_sizes",
     "lcpao_grid_last_error", "lcpao_grid_destroy", "lcpao_ucell_build", "lcpao_ucell_scalars",
     "lcpao_ucell_copy", "lcpao_ucell_last_error", "lcpao_ucell_destroy",
@@ -103,6 +104,7 @@ def load():
     L.lcpao_grid_hamiltonian_dev.argtypes = [vp, C.c_int32, dpp, dpp, C.c_int32]
     L.lcpao_grid_density_dev.argtypes = [vp, C.c_int32, dpp, C.c_int64, dpp, C.c_int32]
     L.lcpao_grid_force3_dev.argtypes = [vp, C.c_int32, dpp, dpp, C.c_int64, dp, dp, C.c_int32]
+    L.lcpao_grid_force3_dev_noalloc.argtypes = [vp, C.c_int32, dpp, dpp, C.c_int64, dp, dp]
     L.lcpao_grid_sizes.argtypes = [vp, lp, lp, lp, lp]
     ar = C.POINTER(AtomicRadial)
     L.lcpao_grid_aden_pcc.argtypes = [vp, C.c_int32, C.c_int32, ar, dp, dp, dp, dp, dpp]

@@ -403,6 +403,15 @@ class GridContext:
                                                  w.ctypes.data_as(_lib.dp), C.cast(dforce.data_ptr(), _lib.dp),
                                                  _prec(precision)))
 
+    def force3_dev_noalloc(self, dV, dDM, dforce, weights=None, dm_offset=0):
+        """FP64 Force3 of force3_dev through the allocation-free entry point
+        (its partial sums in the library's fixed per-GPU scratch array)."""
+        w = np.ascontiguousarray(weights if weights is not None else WEIGHTS_FORCE3[len(dV)], np.float64)
+        self._order()
+        self._check(self.L.lcpao_grid_force3_dev_noalloc(self.h, len(dV), self._dptrs(dV), self._dptrs(dDM),
+                                                         int(dm_offset), w.ctypes.data_as(_lib.dp),
+                                                         C.cast(dforce.data_ptr(), _lib.dp)))
+
 
 # ----------------------------------------------------------------------------
 # reference-named entry points
