@@ -180,8 +180,22 @@ def nacl_cells(reps, ngrid, disp=0.05, seed=0, a=10.66):
     return Structure("nacl%d" % len(frac), lat, frac, sp, ngrid, a, disp, seed)
 
 
+def nacl_prim_cells(n, ngrid, disp=0.05, seed=0, a=10.66):
+    """Rocksalt NaCl, n x n x n primitive fcc cells (2 atoms each: Na at 0,
+    Cl at (1/2, 1/2, 1/2) of the primitive vectors a/2 (0,1,1), a/2 (1,0,1),
+    a/2 (1,1,0))."""
+    na = _supercell(np.array([[0.0, 0.0, 0.0]]), n)
+    cl = _supercell(np.array([[0.5, 0.5, 0.5]]), n)
+    frac = np.concatenate([na, cl])
+    sp = ["Na"] * len(na) + ["Cl"] * len(cl)
+    lat = n * 0.5 * a * np.array([[0, 1, 1], [1, 0, 1], [1, 1, 0]], dtype=np.float64)
+    return Structure("nacl%d" % len(frac), lat, frac, sp, ngrid, a, disp, seed)
+
+
 # The uneven batch: Si diamond cells of 8 to 216 atoms and NaCl rocksalt
-# cells of 8 to 128 atoms, grids at ~0.21 bohr (48 ... 208 points per edge).
+# cells of 8 to 128 atoms, grids of 48 to 144 points per edge at ~0.21 bohr
+# (the 128-atom NaCl cell is the 4x4x4 supercell of the primitive cell,
+# edges of 30.2 bohr, so that it stays within 144 points per edge).
 BATCH_MENU = [
     ("si8", lambda s: si_cells((1, 1, 1), (48, 48, 48), seed=s)),
     ("si16", lambda s: si_cells((2, 1, 1), (96, 48, 48), seed=s)),
@@ -191,7 +205,7 @@ BATCH_MENU = [
     ("nacl16", lambda s: nacl_cells((2, 1, 1), (104, 52, 52), seed=s)),
     ("nacl32", lambda s: nacl_cells((2, 2, 1), (104, 104, 52), seed=s)),
     ("nacl64", lambda s: nacl_cells((2, 2, 2), (104, 104, 104), seed=s)),
-    ("nacl128", lambda s: nacl_cells((4, 2, 2), (208, 104, 104), seed=s)),
+    ("nacl128", lambda s: nacl_prim_cells(4, (144, 144, 144), seed=s)),
 ]
 
 

@@ -44,3 +44,27 @@ def test_si64_fp64_rank_share_matches_oracle(si64, rank):
     print("rank", rank, "rho worst ratio", worst_ratio(rho, Rref[0], MR[0]))
     ok, msg = within(rho, Rref[0], MR[0])
     assert ok, msg
+
+
+def test_si64_fp64_whole_pair_list_matches_cpu_restatement(si64):
+    """All 5568 pairs at once (the union of the 16 rank shares) against the
+    threaded CPU restatement of the reference blocked algorithm (pinned to
+    the single-threaded oracle in tests/test_oracle.py)."""
+    import os
+    st, V, DM = si64
+    u = st.ucell()
+    ctx = GridContext(u)
+    ctx.build_tables(st.paos())
+    tabs = ctx.get_tables(0)
+    g = O.Geom(st.Latvecs, st.Gxyz, st.cutoffs(), st.norbs(), st.Ngrid)
+    H = ctx.hamiltonian([V])[0]
+    rho = ctx.density([DM])[0]
+    nw = max(1, min(16, os.cpu_count() or 1))
+    _, done, Hc, Rc = O.cpu_baseline(g, tabs, V, DM, nworkers=nw)
+    assert done == u.MPI_size == 5568
+    _, _, MH, MR = O.cpu_baseline(g, [np.abs(t) for t in tabs], np.abs(V), np.abs(DM), nworkers=nw)
+    print("H worst ratio", worst_ratio(H, Hc, MH), "rho worst ratio", worst_ratio(rho, Rc, MR))
+    ok, msg = within(H, Hc, MH)
+    assert ok, msg
+    ok, msg = within(rho, Rc, MR)
+    assert ok, msg

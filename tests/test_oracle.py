@@ -230,6 +230,57 @@ def test_duality_hermiticity_linearity(which, request):
     assert float(np.dot(c.DM, H1)) == pytest.approx(float(g.gridvol * R.sum()), rel=1e-12)
 
 
+def _nonsymmetric_dm(st, seed):
+    """Flat DM from a random non-symmetric Gamma matrix: as the up-down
+    channels of the non-collinear form, D(B,A,-Rn) != D(A,B,Rn)^T."""
+    n = int(st.norbs().sum())
+    rng = S.SplitMix64(seed)
+    Dg = rng.normal(n * n).reshape(n, n) / n
+    return S.density_matrix_flat(st.ucell(), Dg)
+
+
+@pytest.mark.parametrize("which", ["si2s", "nacl8s"])
+def test_blocked_density_equals_bruteforce_for_nonsymmetric_dm(which, request):
+    """The blocked density restatement takes each pair's own block (no
+    symmetry assumed), which the non-collinear channels DM3/DM4 need."""
+    c = request.getfixturevalue(which)
+    D = _nonsymmetric_dm(c.st, 11)
+    R, MR = O.density(c.g, c.tabs, [D], with_abs=True)
+    _, Rb, _ = O.bruteforce(c.g, c.tabs, c.V, D)
+    ok, msg = within(Rb, R[0], MR[0])
+    assert ok, msg
+    # four spins are four independent densities
+    R4 = O.density(c.g, c.tabs, [c.DM, D, -D, 0.5 * c.DM])
+    assert np.array_equal(R4[1], R[0]) and np.array_equal(R4[2], -R[0])
+
+
+@pytest.fixture(scope="module")
+def nacl8s_r1of2():
+    return Small(S.nacl(1, 16), nprocs=2, rank=1)
+
+
+@pytest.mark.parametrize("nworkers,use_blas,which", [(1, True, "nacl8s"), (3, True, "nacl8s"), (2, False, "nacl8s"),
+                                                     (3, True, "nacl8s_r1of2")])
+def test_cpu_baseline_equals_blocked_restatement(nworkers, use_blas, which, request):
+    """The CPU baseline (the same blocked algorithm, pairs split over
+    threads, DGEMM from OpenBLAS) gives the restatement's H and rho within
+    the FP64 bar, also on a rank's share of the pairs, so it can check
+    full-size FP64 results."""
+    c = request.getfixturevalue(which)
+    H, M = O.hamiltonian(c.g, c.tabs, [c.V], with_abs=True)
+    R, MR = O.density(c.g, c.tabs, [c.DM], with_abs=True)
+    _, done, Hc, Rc = O.cpu_baseline(c.g, c.tabs, c.V, c.DM, nworkers=nworkers, use_blas=use_blas)
+    assert done == c.g.npair
+    ok, msg = within(Hc, H[0], M[0])
+    assert ok, msg
+    ok, msg = within(Rc, R[0], MR[0])
+    assert ok, msg
+    # with absolute inputs it returns the sums of absolute terms
+    _, _, Ha, Ra = O.cpu_baseline(c.g, [np.abs(t) for t in c.tabs], np.abs(c.V), np.abs(c.DM), nworkers=nworkers,
+                                  use_blas=use_blas)
+    assert np.allclose(Ha, M[0], rtol=1e-13, atol=0) and np.allclose(Ra, MR[0], rtol=1e-13, atol=0)
+
+
 def _energy(st, gxyz, V, Dg):
     s2 = S.Structure("fd", st.Latvecs, np.linalg.solve(st.Latvecs.T, np.asarray(gxyz).T).T, st.species, st.Ngrid, st.a_cell)
     g = O.Geom(s2.Latvecs, s2.Gxyz, s2.cutoffs(), s2.norbs(), s2.Ngrid)
