@@ -88,6 +88,31 @@ def work_counts(u):
     return f_ham, f_rho, f_ref
 
 
+def host_cpu():
+    """The host the CPU baseline runs on: CPU model, logical CPUs the OS
+    reports, CPUs this process may run on, and the threads used (the GPU
+    box gives one GPU's job a share of 16 host threads -- OMP_NUM_THREADS=16
+    there -- whatever nproc reports, so at most 16 workers are started)."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    share = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    return {"cpu_model": model, "nproc": os.cpu_count(), "allowed_cpus": allowed,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "threads_rule": "min(16, allowed CPUs, OMP_NUM_THREADS): the CPU share of one GPU's job on the GPU box",
+            "threads": max(1, min(16, allowed, share))}
+
+
 def cpu_baseline(struct, ctx, u, V, DM, ranks, gpu_fp64=None):
     """Reference-algorithm restatement (C + OpenBLAS 0.3.29 DGEMM) of the
     blocked H + rho assembly.  Measured twice: the whole assembly on
@@ -110,8 +135,9 @@ def cpu_baseline(struct, ctx, u, V, DM, ranks, gpu_fp64=None):
     w_r0 = (g.no[g.pair_atom].astype(np.float64) * g.no[g.pair_natn] * g.numolg).sum()
     one_core_ms = secs * 1e3 * (w_all / w_r0)
     del g
-    # all cores, the whole assembly
-    threads = max(1, min(ranks, os.cpu_count() or 1))
+    # the host's share of cores, the whole assembly
+    host = host_cpu()
+    threads = max(1, min(ranks, host["threads"]))
     t0 = time.time()
     ga = O.Geom(struct.Latvecs, struct.Gxyz, struct.cutoffs(), struct.norbs(), struct.Ngrid, (0, 0, 0), 1, 0)
     geom_s = time.time() - t0
@@ -120,7 +146,7 @@ def cpu_baseline(struct, ctx, u, V, DM, ranks, gpu_fp64=None):
     blas = O.blas_path() is not None
     out = {
         "value": round(secs_all * 1e3, 3), "unit": "ms per H+rho assembly (whole assembly, measured)",
-        "cores": threads, "kind": "port",
+        "cores": threads, "kind": "port", "host": host,
         "sample": "the whole assembly (%d pairs) on %d host threads, the reference's weighted contiguous pair split "
                   "with one BLAS thread each, reference blocked algorithm (512-point gathers + %s DGEMM), rho "
                   "partials summed after the threads (as the reference's Allreduce); list build %.1f s not timed"
@@ -146,7 +172,7 @@ def traffic_from_profile(config, kernel):
     path = os.path.join(ROOT, "profiles", "traffic_%s.md" % config)
     if not os.path.exists(path):
         return None, None
-    from lcpao_grid.profsum import read_traffic_table
+    from profsum import read_traffic_table
     tab = read_traffic_table(path)
 
     def norm(n):
@@ -169,7 +195,7 @@ def mfma_from_profile(config, kernel):
     path = os.path.join(ROOT, "profiles", "mfma_%s.md" % config)
     if not os.path.exists(path):
         return None
-    from lcpao_grid.profsum import read_mfma_table
+    from profsum import read_mfma_table
     tab = read_mfma_table(path)
 
     def norm(n):
@@ -227,6 +253,38 @@ def force3_flops(u):
     A, B = no[u.MPI_atom], no[u.MPI_natn]
     n = u.MPI_NumOLG.astype(np.int64)
     return float((n * (2 * A * B + 6 * A + 6)).sum())
+
+
+def force3_cpu_baseline(struct, ctx, u, V, DM, ranks):
+    """The oracle's restatement of _Calc_Force3_Blocked! (Force.jl:3321-3392:
+    512-point blocks, D x phi_B then the three gradient dots; plain-loop
+    products, one thread) timed on rank 0's share of a `ranks`-rank weighted
+    pair split and scaled to the whole contraction by the work ratio.  Only
+    the tables of the atoms that share's pairs touch are copied back."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    g = O.Geom(struct.Latvecs, struct.Gxyz, struct.cutoffs(), struct.norbs(), struct.Ngrid, (0, 0, 0), ranks, 0)
+    atoms = sorted(set(int(a) for a in g.pair_atom) | set(int(b) for b in g.pair_natn))
+    tabs = ctx.get_tables(0, atoms)
+    dts = [ctx.get_tables(w, atoms) for w in (1, 2, 3)]
+    t0 = time.perf_counter()
+    O.force3_sums(g, tabs, dts, V, DM)
+    secs = time.perf_counter() - t0
+    no = u.Total_NumOrbs.astype(np.float64)
+
+    def work(na, nb, n):
+        return float((n * (2.0 * na * nb + 6.0 * na + 6.0)).sum())
+
+    w_all = work(no[u.MPI_atom], no[u.MPI_natn], u.MPI_NumOLG.astype(np.float64))
+    w_r0 = work(g.no[g.pair_atom].astype(np.float64), g.no[g.pair_natn].astype(np.float64),
+                g.numolg.astype(np.float64))
+    host = host_cpu()
+    return {"value": round(secs * 1e3 * w_all / w_r0, 1), "unit": "ms per Force3 contraction (1 core, extrapolated)",
+            "cores": 1, "kind": "port", "host": host,
+            "sample": "rank 0 of a %d-rank weighted pair split (%d of %d pairs, %.1f%% of the work), the oracle's "
+                      "restatement of _Calc_Force3_Blocked! (512-point blocks, plain-loop products instead of the "
+                      "reference's BLAS mul!) timed %.2f s on 1 host core; scaled by total/sample work"
+                      % (ranks, g.npair, u.MPI_size, 100.0 * w_r0 / w_all, secs)}
 
 
 def bench_force3(args):
@@ -298,8 +356,7 @@ def bench_force3(args):
                      "kernel": {"fp32": "k_f3<float>", "fp64": "k_f3", "bf16x3": "k_fblk_bf"}[head],
                      "achieved": round(flops / kt / 1e12, 3), "peak": round(peak / 1e12, 1), "unit": "TFLOP/s",
                      "frac": round(flops / kt / peak, 4), "traffic": None, "flops_per_launch": flops},
-        "cpu_baseline": {"value": None, "reason": "not timed for Force3: the CPU restatement would need the "
-                                                  "31 GB of FP64 tables copied to the host"},
+        "cpu_baseline": None if args.no_cpu_baseline else force3_cpu_baseline(struct, ctx, u, V, DM, args.cpu_ranks),
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -728,9 +785,14 @@ def main():
     dom = max(("ham", "rho"), key=lambda k: stats.get(k, {}).get("avg_ms", 0.0))
     flops = f_ham if dom == "ham" else f_rho
     achieved = flops / (stats[dom]["avg_ms"] * 1e-3)
-    kname = {("fp64", "ham"): "k_ham_mfma64", ("fp64", "rho"): "k_rho_m64", ("bf16x3", "ham"): "k_hblk_bf",
-             ("bf16x3", "rho"): "k_rblk_bf", ("fp32", "ham"): "k_hblk_f32", ("fp32", "rho"): "k_rho<float>"}
-    kern = kname[(args.precision, dom)]
+    # the kernel the autotuner picked for this form and operation
+    rho_kernels = {"fp64": ("k_rho_m64", "k_rho<double>"), "fp32": ("k_rho<float>", "k_rblk_f32"),
+                   "bf16x3": ("k_rblk_bf", "k_rblk_bf")}
+    if dom == "ham":
+        kern = {"fp64": "k_ham_mfma64", "fp32": "k_hblk_f32", "bf16x3": "k_hblk_bf"}[args.precision]
+    else:
+        op = {"fp64": "rho64", "fp32": "rho32", "bf16x3": "rhobf"}[args.precision]
+        kern = rho_kernels[args.precision][max(0, ctx.tuned(op)[0])]
     traffic, traffic_src = traffic_from_profile(args.config, kern)
     # algorithmic bytes of the same launch: its half of the compulsory traffic
     # (tables at the form's element size, V or rho, D or H, grid lists)
@@ -742,14 +804,20 @@ def main():
     kt = stats[dom]["avg_ms"] * 1e-3
     mfma_frac = achieved / PEAK[args.precision]
     hbm_frac = alg_bytes / kt / HBM_PEAK
+    # the roofline fraction is the larger of the FLOP part and the byte part
+    # (each against its own peak); `bound` names the larger one
+    mfma_part = {"achieved": round(achieved / 1e12, 3), "peak": round(PEAK[args.precision] / 1e12, 1),
+                 "unit": "TFLOP/s", "frac": round(mfma_frac, 4), "flops_per_launch": flops,
+                 "flops_note": "exact algorithmic FLOPs of the launch (canonical pairs: each transpose pair "
+                               "computed once)"}
+    hbm_part = {"achieved": round(alg_bytes / kt / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": round(hbm_frac, 4), "algorithmic_bytes": alg_bytes}
+    top = mfma_part if mfma_frac >= hbm_frac else hbm_part
     roofline = {
-        "bound": "mfma", "kernel": kern,
-        "achieved": round(achieved / 1e12, 3), "peak": round(PEAK[args.precision] / 1e12, 1), "unit": "TFLOP/s",
-        "frac": round(mfma_frac, 4), "traffic": traffic,
-        "flops_per_launch": flops,
-        "flops_note": "exact algorithmic FLOPs of the launch (canonical pairs: each transpose pair computed once)",
-        "hbm_part": {"algorithmic_bytes": alg_bytes, "achieved_GBps": round(alg_bytes / kt / 1e9, 1),
-                     "peak_GBps": HBM_PEAK / 1e9, "frac": round(hbm_frac, 4)},
+        "bound": "mfma" if mfma_frac >= hbm_frac else "hbm", "kernel": kern,
+        "achieved": top["achieved"], "peak": top["peak"], "unit": top["unit"], "frac": top["frac"],
+        "traffic": traffic, "kernel_avg_ms": stats[dom]["avg_ms"],
+        "mfma_part": mfma_part, "hbm_part": hbm_part,
     }
     if traffic is not None:
         roofline["traffic_source"] = traffic_src

@@ -187,6 +187,17 @@ int lcpao_batch_assemble(lcpao_batch *b, const double *const *d_V, const double 
     if (!b) return LCPAO_EINVAL;
     if (!d_V || !d_DM || !d_H || !d_rho) return berr(b, LCPAO_EINVAL, "null pointer array");
     BATCH_TRY(b, hipSetDevice(b->device));
+    // a member whose tables changed since the last call (set_tables,
+    // set_dtables, build_tables clear its tiles' ready flag) gets its bf16
+    // tiles rebuilt into the same buffers, so the bound arguments and a
+    // captured graph stay valid
+    for (lcpao_ctx *c : b->ctx) {
+        if (!c->tables) return berr(b, LCPAO_ESTATE, "a member context has no orbital tables");
+        if (!c->plan || !c->plan->bf16_ready) {
+            const int rc = lcpao::split_tables(c, LCPAO_BF16X3);
+            if (rc) return berr(b, rc, "rebuilding a member's split tables: " + c->err);
+        }
+    }
     int rc = bind(b, d_V, d_DM, d_H, d_rho);
     if (rc) return rc;
     if (b->sched < 0) {  // tune the schedule: each candidate once untimed, once timed

@@ -1239,13 +1239,16 @@ int rho_dev(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_off
     for (int s = 0; s < nspin; s++)
         if (!d_DM[s] || !d_rho[s]) return set_err(ctx, LCPAO_EINVAL, "null spin pointer");
     // FP32 candidates: 0 = pair-list kernel (k_rho<float>), 1 = block form on
-    // the FP32 matrix cores (k_rblk_f32); a geometry the block plan cannot
-    // hold keeps the pair-list kernel
+    // the FP32 matrix cores (k_rblk_f32).  With the variant open, a block
+    // form that cannot be set up (a geometry the block plan cannot hold, no
+    // cell lists, or no memory for its tiles) leaves the pair-list kernel;
+    // a variant fixed to 1 reports the failure.
     if (f32 && ctx->variant[TUNE_RHO32] != 0) {
         const int r2 = split_tables(ctx, LCPAO_FP32);
-        if (r2 == LCPAO_ERANGE) {
+        if (r2 && ctx->variant[TUNE_RHO32] < 0) {
             ctx->variant[TUNE_RHO32] = 0;
             ctx->err.clear();
+            (void)hipGetLastError();  // a failed allocation leaves no sticky state behind
         } else if (r2) {
             return r2;
         }
@@ -1268,6 +1271,10 @@ int rho_dev(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_off
         if ((rc = autotune(ctx, TUNE_RHO32, cand))) return rc;
         if (trc) return trc;
         if (ctx->variant[TUNE_RHO32] == 1) {
+            // the pair-list candidate's FP32 table copy is no longer needed
+            // (fp32_tables makes it again for the FP32 Force3)
+            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+            if (ctx->d_tab32) { (void)hipFree(ctx->d_tab32); ctx->d_tab32 = nullptr; }
             if ((rc = split_density_f32_enqueue(ctx, nspin, d_DM, dm_offset, d_rho))) return rc;
             HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
             collect_timings(ctx);

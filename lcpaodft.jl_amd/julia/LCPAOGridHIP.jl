@@ -49,8 +49,31 @@ mutable struct GridHandle
     dorbs::Any
 end
 
-const GRID_CTX = IdDict{Any,GridHandle}()
-const LAST_GRID = Ref{Any}(nothing)   # (ucell, handle) of the latest context: Solve_Poisson! has no ucell
+# Context lifetime: one live device context per process, that of the latest
+# UCell.  The reference builds a new UCell per geometry (DFT.jl:219, every
+# step of a geometry optimisation); when a grid routine first sees a new
+# UCell the previous context is destroyed before the new one is created, so
+# its tables, block plan and tiles (several GB at 216 atoms) are released
+# at once.  UCell is an immutable struct, so it cannot key a WeakKeyDict;
+# the identity test is `===`.  grid_release!() frees the context explicitly.
+const LAST_GRID = Ref{Any}(nothing)   # (ucell, handle) of the live context: Solve_Poisson! has no ucell
+
+function _grid_destroy!(h::GridHandle)
+    if h.ptr != C_NULL
+        ccall((:lcpao_grid_destroy, LIBGRID), Cvoid, (Ptr{Cvoid},), h.ptr)
+        h.ptr = C_NULL
+    end
+    h.orbs = nothing
+    h.dorbs = nothing
+    return nothing
+end
+
+function grid_release!()
+    last = LAST_GRID[]
+    last === nothing || _grid_destroy!(last[2])
+    LAST_GRID[] = nothing
+    return nothing
+end
 
 function _grid_check(rc, ctx)
     rc == 0 && return nothing
@@ -59,8 +82,9 @@ function _grid_check(rc, ctx)
 end
 
 function _grid_context(ucell::UCell)
-    h = get(GRID_CTX, ucell, nothing)
-    h === nothing || return h
+    last = LAST_GRID[]
+    last !== nothing && last[1] === ucell && return last[2]
+    grid_release!()   # a new geometry: free the previous context first
     sg = ucell.system_grid
     glp = [pointer(v) for v in ucell.GridListAtom]
     clp = [pointer(v) for v in ucell.CellListAtom]
@@ -81,8 +105,7 @@ function _grid_context(ucell::UCell)
         _grid_check(rc, C_NULL)
     end
     h = GridHandle(ctxref[], nothing, nothing)
-    finalizer(x -> ccall((:lcpao_grid_destroy, LIBGRID), Cvoid, (Ptr{Cvoid},), x.ptr), h)
-    GRID_CTX[ucell] = h
+    finalizer(_grid_destroy!, h)   # at process exit; a no-op after grid_release!()
     LAST_GRID[] = (ucell, h)
     return h
 end

@@ -17,6 +17,8 @@
 #   Calc_DM_Crystal_NonCollinear!                                           Crystal_DFT.jl:342
 #   Calc_iDM_Crystal_NonCollinear!                                          Crystal_DFT.jl:413
 #   HS_matrix_NC!(H, Hks::Vector{Vector{Float64}}, iHks, ..., kpts)         HS_matrix.jl:211
+#   HS_matrix_NC!(H, Hks nested, iHks, ..., kpts)                           HS_matrix.jl:260
+#   HS_matrix_NC!(H, Hks::Vector{Vector{Float64}}, iHks, ..., natn)         HS_matrix.jl:167 (no k)
 
 function _kpoint_context()
     MPI.Comm_size(MPI.COMM_WORLD) == 1 || error("LCPAOGridHIP_kpoint.jl: one rank only")
@@ -191,4 +193,23 @@ function HS_matrix_NC!(H, Hks::Vector{Vector{Float64}}, iHks::Vector{Vector{Vect
         _grid_check(rc, h.ptr)
     end
     return nothing
+end
+
+# nested Hks[s][atom][Rn][ist][jst] (HS_matrix.jl:260): flattened in pair order
+function HS_matrix_NC!(H, Hks::Vector{Vector{Vector{Vector{Vector{Float64}}}}},
+                       iHks::Vector{Vector{Vector{Vector{Vector{Float64}}}}},
+                       Natom, Total_NumOrbs, MP, FNAN, natn, ncn, atv_ijk, kpts::AbstractVector{Float64})
+    flat = [_flatten_nested(Hks[s], Natom, FNAN, Total_NumOrbs, natn) for s in 1:4]
+    HS_matrix_NC!(H, flat, iHks, Natom, Total_NumOrbs, MP, FNAN, natn, ncn, atv_ijk, kpts)
+end
+
+# the form without k (HS_matrix.jl:167): every phase is 1, i.e. the k form at
+# k = 0.  The reference reads the fourth channel as Hks[4][ist][jst] (:199),
+# which indexes a number for jst > 1; the flat Hks[4][hst] of its k form
+# (:246) is taken here.
+function HS_matrix_NC!(H, Hks::Vector{Vector{Float64}}, iHks::Vector{Vector{Vector{Vector{Vector{Float64}}}}},
+                       Natom, Total_NumOrbs, MP, FNAN, natn)
+    ucell, _ = _kpoint_context()
+    sg = ucell.system_grid
+    HS_matrix_NC!(H, Hks, iHks, Natom, Total_NumOrbs, MP, FNAN, natn, sg.ncn, sg.atv_ijk, Float64[0.0, 0.0, 0.0])
 end

@@ -44,7 +44,7 @@ def _prec(p):
 class GridContext:
     """Device context of one geometry (lists, tables, work decomposition)."""
 
-    def __init__(self, ucell, device=0, with_lattice=True):
+    def __init__(self, ucell, device=0, with_lattice=True, with_cell_lists=True):
         L = _lib.load()
         u = ucell
         self.ucell = u
@@ -81,7 +81,9 @@ class GridContext:
         g.ngrid[:] = list(u.Ngrid)
         g.cpycell, g.gridvol = u.CpyCell, u.GridVol
         g.no, g.gridn = no.ctypes.data_as(_lib.ip), gridn.ctypes.data_as(_lib.ip)
-        g.grid_list, g.cell_list = gptr, cptr
+        g.grid_list = gptr
+        if with_cell_lists:  # optional in the C ABI (needed by the block forms and device tables)
+            g.cell_list = cptr
         g.npairs = P
         g.pair_atom, g.pair_natn, g.pair_ncn = pa.ctypes.data_as(_lib.ip), pn.ctypes.data_as(_lib.ip), pc.ctypes.data_as(_lib.ip)
         g.numolg, g.olg1, g.olg2 = nolg.ctypes.data_as(_lib.ip), o1p, o2p
@@ -313,8 +315,11 @@ class GridContext:
         g = np.ascontiguousarray(gxyz, np.float64).reshape(-1, 3)
         self._check(self.L.lcpao_grid_set_positions(self.h, g.ctypes.data_as(_lib.dp)))
 
-    def get_tables(self, which=0):
-        out = [np.zeros((int(self.gridn[a]), int(self.no[a]))) for a in range(self.natom)]
+    def get_tables(self, which=0, atoms=None):
+        """Device tables copied back (which: 0 phi, 1..3 d/dx, d/dy, d/dz);
+        with `atoms`, only those atoms' tables (the others are None)."""
+        keep = set(range(self.natom)) if atoms is None else {int(a) for a in atoms}
+        out = [np.zeros((int(self.gridn[a]), int(self.no[a]))) if a in keep else None for a in range(self.natom)]
         self._check(self.L.lcpao_grid_get_tables(self.h, which, _ptrs(out)))
         return out
 
@@ -563,3 +568,34 @@ def HS_matrix_NC(H, Hks, iHks, kpts, Orbs_Grid):
     nested arrays flattened in pair order; H (complex 2 fsize x 2 fsize) is
     overwritten (its down-up block zero, as in the reference)."""
     H[:, :] = Orbs_Grid.bloch_sum_nc(kpts, Hks, iHks)
+
+
+def flatten_nested(X, ucell):
+    """A nested per-pair array X[atom][Rn][ist][jst] (the layout of the
+    reference's nested Hks / iHks / DM_Vec, Convert_Matrix.jl:27-62) as the
+    flat pair-block array of the pairs of `ucell`, in pair order (atom, Rn,
+    ist, jst)."""
+    out = []
+    for a in range(ucell.Natom):
+        for r in range(int(ucell.nb_off[a + 1] - ucell.nb_off[a])):
+            out.append(np.asarray(X[a][r], np.float64).reshape(-1))
+    return np.concatenate(out) if out else np.zeros(0)
+
+
+def HS_matrix_NC_nested(H, Hks, iHks, kpts, Orbs_Grid):
+    """HS_matrix_NC!(H, Hks::Vector{Vector{Vector{Vector{Vector{Float64}}}}},
+    iHks, ..., kpts) (HS_matrix.jl:260-305): the four Hks channels and the
+    three iHks arrays are nested [atom][Rn][ist][jst]; flattened in pair
+    order they are the flat form's inputs.  H (complex 2 fsize x 2 fsize) is
+    overwritten."""
+    u = Orbs_Grid.ucell
+    H[:, :] = Orbs_Grid.bloch_sum_nc(kpts, [flatten_nested(x, u) for x in Hks], [flatten_nested(x, u) for x in iHks])
+
+
+def HS_matrix_NC_cluster(H, Hks, iHks, Orbs_Grid):
+    """HS_matrix_NC!(H, Hks::Vector{Vector{Float64}}, iHks, Natom, ..., natn)
+    (HS_matrix.jl:167-208), the form without k: every phase is 1, so H is the
+    flat form's sum at k = 0.  The reference reads the fourth channel as
+    Hks[4][ist][jst] (:199), which indexes a number for jst > 1; the flat
+    element Hks[4][hst] that its k form reads (:246) is taken here."""
+    H[:, :] = Orbs_Grid.bloch_sum_nc([0.0, 0.0, 0.0], Hks, iHks)

@@ -66,7 +66,7 @@ def _arr(ptr, n, dtype):
 
 
 def _dptrs(arrs):
-    return (_dp * len(arrs))(*[a.ctypes.data_as(_dp) for a in arrs])
+    return (_dp * len(arrs))(*[a.ctypes.data_as(_dp) if a is not None else None for a in arrs])
 
 
 class Pao:

@@ -1,11 +1,11 @@
 """Summaries of rocprofv3 output for the grid kernels (markdown text).
 
-  python -m lcpao_grid.profsum stats  KERNEL_STATS_CSV [--title T] [--command C]
-  python -m lcpao_grid.profsum traffic --fetch FETCH_CSV --write WRITE_CSV
+  python profsum.py stats  KERNEL_STATS_CSV [--title T] [--command C]
+  python profsum.py traffic --fetch FETCH_CSV --write WRITE_CSV
          --copy-match SUBSTR --copy-bytes N [--title T] [--command C]
-  python -m lcpao_grid.profsum mfma COUNTER_CSV --agent AGENT_INFO_CSV
+  python profsum.py mfma COUNTER_CSV --agent AGENT_INFO_CSV
          [--title T] [--command C]
-  python -m lcpao_grid.profsum counters COUNTER_CSV [--match SUBSTR ...]
+  python profsum.py counters COUNTER_CSV [--match SUBSTR ...]
          [--title T] [--command C]
 
 `stats` condenses a `--kernel-trace --stats` run (kernel_stats.csv) to a
@@ -236,7 +236,7 @@ def read_mfma_table(path):
 
 
 def main(argv=None):
-    p = argparse.ArgumentParser(prog="lcpao_grid.profsum")
+    p = argparse.ArgumentParser(prog="profsum.py")
     sub = p.add_subparsers(dest="cmd", required=True)
     s = sub.add_parser("stats")
     s.add_argument("csv")

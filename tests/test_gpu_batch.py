@@ -88,3 +88,30 @@ def test_batch_graph_replays(small_batch):
         out[0]["ctx"].hamiltonian_dev([out[0]["V"]], [scratch], "bf16x3")
         torch.cuda.synchronize()
         assert np.array_equal(scratch.cpu().numpy(), ref[0][0])
+
+
+def test_batch_follows_table_updates(small_batch):
+    """New tables on a member context after lcpao_batch_create (here the
+    tables doubled, so its H becomes 4x and its rho 4x) reach the batch's
+    next call, eager and graph replay alike, and the old tables back give
+    the per-structure result again."""
+    out, ref = small_batch
+    b = Batch([s["ctx"] for s in out])
+    b.tune(1)
+    _run(b, out, use_graph=True)
+    _check(out, ref)
+    s0 = out[1]
+    tabs = s0["ctx"].get_tables(0)
+    s0["ctx"].set_tables([2.0 * t for t in tabs])
+    for use_graph in (False, True):
+        _run(b, out, use_graph=use_graph)
+        H, R = s0["H"].cpu().numpy(), s0["rho"].cpu().numpy()
+        H0, R0 = ref[1]
+        assert np.abs(H - 4.0 * H0).max() <= 1e-3 * np.abs(H0).max()
+        assert np.abs(R - 4.0 * R0).max() <= 1e-3 * np.abs(R0).max()
+        for k, (s, (Hk, Rk)) in enumerate(zip(out, ref)):
+            if k != 1:
+                assert np.array_equal(s["H"].cpu().numpy(), Hk)
+    s0["ctx"].set_tables(tabs)
+    _run(b, out, use_graph=True)
+    _check(out, ref)

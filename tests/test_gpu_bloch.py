@@ -193,3 +193,44 @@ def test_nc_forms_on_a_rank_share_and_edge_cases():
     with pytest.raises(GridError):
         ctx._check(ctx.L.lcpao_grid_bloch_sum_nc(ctx.h, kp.ctypes.data_as(_lib.dp), None, None, None))
     ctx.close()
+
+
+def _nest(flat, u):
+    """flat pair blocks -> nested [atom][Rn][ist][jst] (the reference's layout)."""
+    out, pos = [], 0
+    for a in range(u.Natom):
+        per = []
+        na = int(u.Total_NumOrbs[a])
+        for h in range(int(u.nb_off[a]), int(u.nb_off[a + 1])):
+            nb = int(u.Total_NumOrbs[u.natn_flat[h]])
+            per.append([list(flat[pos + i * nb: pos + (i + 1) * nb]) for i in range(na)])
+            pos += na * nb
+        out.append(per)
+    assert pos == flat.size
+    return out
+
+
+def test_nc_bloch_sum_nested_and_cluster_forms():
+    """HS_matrix_NC! with nested Hks (HS_matrix.jl:260) equals the flat form
+    on the same numbers; the form without k (HS_matrix.jl:167) equals the
+    oracle's sum at k = 0."""
+    from lcpao_grid import grid as Gm
+    st = S.CONFIGS["si2"]()
+    u = st.ucell()
+    g = O.Geom(st.Latvecs, st.Gxyz, st.cutoffs(), st.norbs(), st.Ngrid)
+    ctx = GridContext(u)
+    fsize, _ = ctx.bloch_sizes()
+    rng = np.random.default_rng(5)
+    hks = [rng.standard_normal(u.myHsize) for _ in range(4)]
+    ihks = [rng.standard_normal(u.myHsize) for _ in range(3)]
+    assert np.array_equal(Gm.flatten_nested(_nest(hks[0], u), u), hks[0])
+    k = [0.11, 0.23, -0.4]
+    Hflat = np.zeros((2 * fsize, 2 * fsize), complex)
+    Gm.HS_matrix_NC(Hflat, hks, ihks, k, ctx)
+    Hn = np.zeros_like(Hflat)
+    Gm.HS_matrix_NC_nested(Hn, [_nest(x, u) for x in hks], [_nest(x, u) for x in ihks], k, ctx)
+    assert np.array_equal(Hn, Hflat)
+    Hc = np.zeros_like(Hflat)
+    Gm.HS_matrix_NC_cluster(Hc, hks, ihks, ctx)
+    want = O.hs_matrix_nc(g, [0.0, 0.0, 0.0], hks, ihks)
+    assert _rel(Hc, want) < 1e-13

@@ -1,11 +1,11 @@
 """Grid fields on the device against the oracle and the reference's vectors:
 Set_AdenPCC_Grid (lcpao_grid_aden_pcc) and Solve_Poisson!
-(lcpao_grid_poisson, rocFFT).
+(lcpao_grid_poisson, the library's own LDS-resident mixed-radix FFT passes).
 
 Atomic densities: every grid point's sum runs in the reference's atom order
 on both sides, so the device agrees with the oracle to the last few ulps of
 log/exp (bound 1e-13 of the field's maximum).  Poisson: the oracle's FFT and
-rocFFT differ in rounding only (bound 1e-13 of the maximum); the reference's
+the device FFT differ in rounding only (bound 1e-13 of the maximum); the reference's
 vector at 1e-15; at the 144^3 GaAs grid a plane wave against its closed
 form."""
 import json

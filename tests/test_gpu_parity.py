@@ -342,3 +342,23 @@ def test_autotuner_cache_round_trip(tmp_path):
     other = case("nacl8")
     ctx3 = GridContext(other.u)
     assert ctx3.tune_load(path) == 0 and ctx3.tuned("rho64")[0] == -1
+
+
+def test_fp32_density_without_cell_lists_falls_back_to_pair_lists():
+    """A context built without cell lists (allowed by the C ABI) cannot form
+    the block plan: with the FP32 density variant open it runs the pair-list
+    kernel and meets the FP32 bar; fixed to the block form it reports the
+    missing lists."""
+    from lcpao_grid import GridError
+    c = case("nacl8")
+    ctx = GridContext(c.u, with_cell_lists=False)
+    ctx.set_tables(c.tabs)
+    R = ctx.density([c.DM], precision="fp32")[0]
+    assert ctx.tuned("rho32")[0] == 0
+    Rref, M = O.density(c.g, c.tabs, [c.DM], with_abs=True)
+    assert worst_ratio(R, Rref[0], M[0]) <= SPLIT_RTOL["fp32"]
+    ctx2 = GridContext(c.u, with_cell_lists=False)
+    ctx2.set_tables(c.tabs)
+    ctx2.tune("rho32", 1)
+    with pytest.raises(GridError):
+        ctx2.density([c.DM], precision="fp32")
