@@ -35,9 +35,8 @@ extern "C" {
 enum {
     LCPAO_OK = 0,
     LCPAO_EINVAL = 1,    /* bad argument / inconsistent geometry */
-    LCPAO_ERANGE = 2,    /* size beyond a supported maximum (e.g. NO > 32, or more than 576
-                            orbital rows / 45 sphere images on one 8x8x8 grid block
-                            for the block forms: split-BF16x3, FP32 H) */
+    LCPAO_ERANGE = 2,    /* size beyond a supported maximum (more than 144 orbitals on an
+                            atom, the largest basis Read_PAO's basis string describes) */
     LCPAO_EDEVICE = 3,   /* HIP runtime error, no device, or out of device memory */
     LCPAO_ESTATE = 4,    /* call order (e.g. contraction before tables are set) */
     LCPAO_EMESH = 5      /* radial mesh index out of range in table generation */
@@ -86,7 +85,7 @@ typedef struct {
     int32_t num_basis[4];            /* Spe_Num_Basis */
     int32_t nmesh;                   /* Spe_Num_Mesh_PAO */
     const double *rv;                /* Spe_PAO_RV[nmesh] */
-    const double *const *rwf;        /* rwf[l*4 + p] -> Spe_PAO_RWF[l+1][p+1][nmesh] */
+    const double *const *rwf;        /* rwf[l*16 + p] -> Spe_PAO_RWF[l+1][p+1][nmesh] (64 entries, p < 16) */
 } lcpao_pao;
 
 /* Builds a context for one geometry: copies the lists to the device and
@@ -343,6 +342,25 @@ int lcpao_grid_dm_states_nc(lcpao_ctx *ctx, int32_t nk, const double *kpts, int6
                             double scale, double *const *DM, double *const *iDM);
 int lcpao_grid_dm_states_nc_dev(lcpao_ctx *ctx, int32_t nk, const double *kpts, int64_t nstates,
                                 const double *const *d_C, double scale, double *const *d_DM, double *const *d_iDM);
+
+/* Which kernels serve the split-precision forms (no counterpart in the
+ * reference).  The block forms (bf16x3 H, density and Force3 on the matrix
+ * cores, the FP32 H and the FP32 block density) work on 8x8x8-point grid
+ * blocks whose rows are the orbitals of every sphere image covering the
+ * block; they hold at most 32 orbitals per atom, 576 rows and 45 sphere
+ * images per block.  A geometry beyond that is served by the pair-list
+ * kernels of a wider form, which meet the narrower form's bound: the FP32
+ * density and Force3 for bf16x3, the FP64 H for FP32 and bf16x3.
+ * *state: -1 not decided yet (no split-precision call so far), 1 block
+ * forms, 0 pair-list kernels. */
+int lcpao_grid_block_forms(const lcpao_ctx *ctx, int32_t *state);
+
+/* Host-only check of the block-form limits above for a geometry (no device
+ * needed): the largest padded row count and sphere-image count over the
+ * 8x8x8 blocks and the number of blocks.  Returns LCPAO_OK when the block
+ * forms hold the geometry, LCPAO_ERANGE when the pair-list kernels will
+ * serve it, LCPAO_EINVAL for a geometry without cell lists. */
+int lcpao_grid_block_layout(const lcpao_geom *geom, int32_t *max_rows, int32_t *max_slots, int64_t *nblocks);
 
 /* Sizes of the context (rank-local H size, grid size, sums of the lists). */
 int lcpao_grid_sizes(const lcpao_ctx *ctx, int64_t *my_hsize, int64_t *nn, int64_t *total_olg,

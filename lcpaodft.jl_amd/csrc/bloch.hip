@@ -18,16 +18,22 @@
 
 namespace lcpao {
 
+// One workgroup's share of an (A, B) atom pair: the orbitals i0 .. i0+ni of
+// A and j0 .. j0+nj of B (the whole NO_A x NO_B block up to 32 orbitals per
+// atom; larger bases are cut into tiles of at most 32 x 32).
+constexpr int BL_TILE = 32;
 struct GroupDev {
     int32_t noa, nob;    // orbitals of A and B
     int32_t row_a, row_b;  // first row / column of A and B in the dense matrix (MP, 0-based)
     int32_t p0, np;      // the group's pairs: gpairs[p0 .. p0 + np)
+    int32_t i0, j0, ni, nj;  // the tile of the NO_A x NO_B block
 };
 struct GPair { int64_t hoff; int32_t ijk[3], pad; };
 
 struct BlochPlan {
     int64_t fsize = 0;
-    int32_t ngroups = 0, max_nel = 0;
+    int32_t ngroups = 0, max_nel = 0;   // ngroups: workgroup tiles
+    int64_t natom_pairs = 0;            // distinct (A, B) atom pairs
     GroupDev *d_groups = nullptr;
     GPair *d_gpairs = nullptr;
     double *stage = nullptr;  // host-pointer entry points
@@ -65,9 +71,9 @@ struct BlochArgs {
 __global__ __launch_bounds__(BL_THREADS) void k_bloch(const GroupDev *__restrict__ groups,
                                                        const GPair *__restrict__ gp, BlochArgs A) {
     const GroupDev G = groups[blockIdx.x];
-    const int nel = G.noa * G.nob;
+    const int nel = G.ni * G.nj;
     for (int e = threadIdx.x; e < nel; e += blockDim.x) {
-        const int i = e / G.nob, j = e - (e / G.nob) * G.nob;
+        const int i = G.i0 + e / G.nj, j = G.j0 + e % G.nj;
         double2 acc[BL_MAXMAT];
 #pragma unroll
         for (int m = 0; m < BL_MAXMAT; m++) acc[m] = make_double2(0.0, 0.0);
@@ -129,11 +135,11 @@ struct DmArgs {
 
 __global__ __launch_bounds__(BL_THREADS) void k_dm_states(const GroupDev *__restrict__ groups,
                                                            const GPair *__restrict__ gp, DmArgs A) {
-    __shared__ double2 ta[MAXNO][DM_CHUNK + 1];
-    __shared__ double2 tb[MAXNO][DM_CHUNK + 1];
+    __shared__ double2 ta[BL_TILE][DM_CHUNK + 1];
+    __shared__ double2 tb[BL_TILE][DM_CHUNK + 1];
     const GroupDev G = groups[blockIdx.x];
-    const int nel = G.noa * G.nob;
-    constexpr int PER = (MAXNO * MAXNO + BL_THREADS - 1) / BL_THREADS;
+    const int nel = G.ni * G.nj;
+    constexpr int PER = (BL_TILE * BL_TILE + BL_THREADS - 1) / BL_THREADS;
     for (int kk = 0; kk < A.nk; kk++) {
         const double2 *__restrict__ C = A.C[kk];
         double2 acc[PER];
@@ -142,22 +148,22 @@ __global__ __launch_bounds__(BL_THREADS) void k_dm_states(const GroupDev *__rest
         for (int64_t m0 = 0; m0 < A.nstates; m0 += DM_CHUNK) {
             const int mc = (int)min<int64_t>(DM_CHUNK, A.nstates - m0);
             __syncthreads();
-            for (int x = threadIdx.x; x < (G.noa + G.nob) * DM_CHUNK; x += blockDim.x) {
-                const int r = x % (G.noa + G.nob), mu = x / (G.noa + G.nob);
+            for (int x = threadIdx.x; x < (G.ni + G.nj) * DM_CHUNK; x += blockDim.x) {
+                const int r = x % (G.ni + G.nj), mu = x / (G.ni + G.nj);
                 double2 v = make_double2(0.0, 0.0);
                 if (mu < mc) {
-                    const int64_t row = r < G.noa ? A.ra + G.row_a + r : A.rb + G.row_b + (r - G.noa);
+                    const int64_t row = r < G.ni ? A.ra + G.row_a + G.i0 + r : A.rb + G.row_b + G.j0 + (r - G.ni);
                     v = C[row + (m0 + mu) * A.ld];
                 }
-                if (r < G.noa) ta[r][mu] = v;
-                else tb[r - G.noa][mu] = v;
+                if (r < G.ni) ta[r][mu] = v;
+                else tb[r - G.ni][mu] = v;
             }
             __syncthreads();
 #pragma unroll
             for (int t = 0; t < PER; t++) {
                 const int e = threadIdx.x + t * BL_THREADS;
                 if (e < nel) {
-                    const int i = e / G.nob, j = e - (e / G.nob) * G.nob;
+                    const int i = e / G.nj, j = e - (e / G.nj) * G.nj;
                     double re = acc[t].x, im = acc[t].y;
                     for (int mu = 0; mu < DM_CHUNK; mu++) {
                         const double2 ca = ta[i][mu], cb = tb[j][mu];
@@ -179,7 +185,7 @@ __global__ __launch_bounds__(BL_THREADS) void k_dm_states(const GroupDev *__rest
             for (int t = 0; t < PER; t++) {
                 const int e = threadIdx.x + t * BL_THREADS;
                 if (e < nel) {
-                    const int64_t x = P.hoff + e;  // e = i * nob + j
+                    const int64_t x = P.hoff + (int64_t)(G.i0 + e / G.nj) * G.nob + (G.j0 + e % G.nj);
                     if (A.DM) {
                         const double v = acc[t].x * er - acc[t].y * ei;
                         A.DM[x] = (kk == 0 ? 0.0 : A.DM[x]) + v;
@@ -218,7 +224,6 @@ int bloch_plan(lcpao_ctx *ctx, BlochPlan **out) {
     int max_nel = 0;
     for (const auto &mem : members) {
         const PairDev &P0 = ctx->pairs[mem[0]];
-        if (P0.noa > MAXNO || P0.nob > MAXNO) return ctx_error(ctx, LCPAO_ERANGE, "more than 32 orbitals on an atom");
         GroupDev g;
         g.noa = P0.noa; g.nob = P0.nob;
         g.row_a = (int32_t)mp[P0.a]; g.row_b = (int32_t)mp[P0.b];
@@ -230,12 +235,18 @@ int bloch_plan(lcpao_ctx *ctx, BlochPlan **out) {
             q.pad = 0;
             gpairs.push_back(q);
         }
-        groups.push_back(g);
+        for (int i0 = 0; i0 < g.noa; i0 += BL_TILE)
+            for (int j0 = 0; j0 < g.nob; j0 += BL_TILE) {
+                g.i0 = i0; g.j0 = j0;
+                g.ni = std::min(BL_TILE, g.noa - i0); g.nj = std::min(BL_TILE, g.nob - j0);
+                groups.push_back(g);
+            }
         max_nel = std::max(max_nel, g.noa * g.nob);
     }
     BlochPlan *b = new BlochPlan();
     b->fsize = mp[ctx->natom];
     b->ngroups = (int32_t)groups.size();
+    b->natom_pairs = (int64_t)members.size();
     b->max_nel = max_nel;
     if (hipMalloc(&b->d_groups, sizeof(GroupDev) * std::max<size_t>(1, groups.size())) != hipSuccess ||
         hipMalloc(&b->d_gpairs, sizeof(GPair) * std::max<size_t>(1, gpairs.size())) != hipSuccess ||
@@ -412,7 +423,7 @@ int lcpao_grid_bloch_sizes(lcpao_ctx *ctx, int64_t *fsize, int64_t *ngroups) {
     lcpao::BlochPlan *b = nullptr;
     if ((rc = lcpao::bloch_plan(ctx, &b))) return rc;
     if (fsize) *fsize = b->fsize;
-    if (ngroups) *ngroups = b->ngroups;
+    if (ngroups) *ngroups = b->natom_pairs;
     return LCPAO_OK;
 }
 

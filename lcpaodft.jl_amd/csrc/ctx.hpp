@@ -13,7 +13,11 @@
 #define UNIT_PTS 2048     // overlap points per H / Force3 unit
 #define RHO_CHUNK 2048    // grid-list entries per density unit
 #define HK 32             // points per LDS chunk in the H kernel
-#define MAXNO 32          // largest orbital count per atom supported
+// largest orbital count per atom: Read_PAO's basis string (Read_PAO.jl:49-76)
+// takes one decimal digit of multiplicity for each of s, p, d, f, so at
+// most 9 * (1 + 3 + 5 + 7) orbitals
+#define MAXNO 144
+#define MFMA64_NO 32      // the FP64 matrix-core density / Force3 kernels hold at most 32 orbitals
 #define RHO_WAVES 4
 
 namespace lcpao {
@@ -28,7 +32,7 @@ struct SpeciesDev {
     int32_t maxl, nmesh, no, pad;
     int32_t num_basis[4];
     const double *rv;
-    const double *rwf[4][16];
+    const double *rwf[4][16];   // multiplicity at most 16 per l
 };
 
 // Block-dense structure of the split-precision forms (built on first use).
@@ -105,6 +109,12 @@ struct lcpao_ctx {
     int32_t *d_flag = nullptr;
     // split-precision forms
     lcpao::BlockPlan *plan = nullptr;
+    // block forms: -1 not decided yet, 1 the plan holds the geometry, 0 it
+    // does not (too many orbital rows or sphere images on a grid block, or
+    // more than 32 orbitals on an atom): the split forms then run the
+    // pair-list kernels of a wider form (FP32 density and Force3, FP64 H)
+    int block_state = -1;
+    std::string block_why;
     // grid-field routines (atomic densities, Poisson)
     lcpao::FieldPlan *fields = nullptr;
     lcpao::BlochPlan *bloch = nullptr;

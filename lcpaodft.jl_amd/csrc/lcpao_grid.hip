@@ -73,110 +73,59 @@ void scratch_setup(int device) {
 // Kernels
 // ---------------------------------------------------------------------------
 
-// Hamiltonian unit kernel, FP64.  One wave (64 lanes) per unit.  Lane tiling:
-// the NO_A x NO_B block padded to multiples of 4 is cut into ntile 4x4 tiles;
-// 64/ntile "slices" of lanes split the points of each LDS chunk.
-__global__ __launch_bounds__(64) void k_ham_f64(const HUnit *__restrict__ units, const int32_t *__restrict__ order,
-                                                int32_t nunits, const PairDev *__restrict__ pairs,
-                                                const double *__restrict__ tab, const int32_t *__restrict__ olg1,
-                                                const int32_t *__restrict__ olg2, const int32_t *__restrict__ glist,
-                                                const double *__restrict__ V, double *__restrict__ part) {
-    __shared__ double As[HK * MAXNO];
-    __shared__ double Bs[HK * MAXNO];
-    __shared__ int32_t s_t1[HK], s_t2[HK];
-    __shared__ double s_v[HK];
-    const int b = blockIdx.x;
-    if (b >= nunits) return;
-    const HUnit u = units[order[b]];
-    const PairDev P = pairs[u.pair];
-    const int lane = threadIdx.x;
-    const int noa = P.noa, nob = P.nob;
-    const int tia = (noa + 3) >> 2, tjb = (nob + 3) >> 2;
-    const int sa = tia * 4, sb = tjb * 4;
-    const int ntile = tia * tjb;
-    const int nslice = 64 / ntile;
-    const int slice = lane / ntile, tl = lane - slice * ntile;
-    const int ti = tl / tjb, tj = tl - ti * tjb;
-    const bool active = slice < nslice;
-    const double *tabA = tab + P.tab_a, *tabB = tab + P.tab_b;
-    const int32_t *L1 = olg1 + P.olg + u.lo, *L2 = olg2 + P.olg + u.lo;
-    const int32_t *glA = glist + P.gl_a;
-    double acc[4][4];
-#pragma unroll
-    for (int r = 0; r < 4; r++)
-#pragma unroll
-        for (int c = 0; c < 4; c++) acc[r][c] = 0.0;
-    for (int k0 = 0; k0 < u.n; k0 += HK) {
-        const int cnt = min(HK, u.n - k0);
-        if (lane < HK) {
-            if (lane < cnt) {
-                const int t1 = L1[k0 + lane];
-                s_t1[lane] = t1;
-                s_t2[lane] = L2[k0 + lane];
-                s_v[lane] = V[glA[t1]];
-            } else {
-                s_t1[lane] = -1;
-                s_t2[lane] = -1;
-                s_v[lane] = 0.0;
-            }
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+// Hamiltonian unit kernel on the FP64 matrix cores (v_mfma_f64_16x16x4f64).
+// One wave per unit.  The NO_A x NO_B block is computed in 32 x 32 chunks,
+// each held as 2x2 MFMA tiles: one chunk for up to 32 orbitals per atom;
+// larger bases loop over the chunks and read the unit's points again for
+// each.  Lane l supplies A[l%16][l/16] = phi_A row of point l/16 and
+// B[l/16][l%16] = V*phi_B column, gathered straight from the tables; the 4
+// points of a step are consecutive overlap points.
+__device__ __forceinline__ void ham_chunk64(const double *__restrict__ tabA, const double *__restrict__ tabB,
+                                            const int32_t *__restrict__ L1, const int32_t *__restrict__ L2,
+                                            const int32_t *__restrict__ glA, const double *__restrict__ V, int n,
+                                            int noa, int nob, int r0, int c0, int lane, double *__restrict__ out) {
+    const int r16 = lane & 15, kq = lane >> 4;
+    d4_t c00 = {0, 0, 0, 0}, c01 = c00, c10 = c00, c11 = c00;
+    const bool ra0 = r0 + r16 < noa, ra1 = r0 + 16 + r16 < noa, rb0 = c0 + r16 < nob, rb1 = c0 + 16 + r16 < nob;
+    for (int g0 = 0; g0 < n; g0 += 64) {
+        // indices and potential of 64 points, one per lane
+        const int gi = g0 + lane;
+        const bool gok = gi < n;
+        const int t1g = gok ? L1[gi] : 0;
+        const int t2g = gok ? L2[gi] : 0;
+        const double vg = gok ? V[glA[t1g]] : 0.0;
+        const int nstep = min(16, (n - g0 + 3) >> 2);
+#pragma unroll 4
+        for (int st = 0; st < nstep; st++) {
+            const int src = 4 * st + kq;
+            const int t1 = __shfl(t1g, src, 64);
+            const int t2 = __shfl(t2g, src, 64);
+            const double v = __shfl(vg, src, 64);  // 0 for points past the end
+            const double *ca = tabA + (int64_t)t1 * noa + r0, *cb = tabB + (int64_t)t2 * nob + c0;
+            const double a0 = ra0 ? ca[r16] : 0.0, a1 = ra1 ? ca[16 + r16] : 0.0;
+            const double b0 = rb0 ? cb[r16] * v : 0.0, b1 = rb1 ? cb[16 + r16] * v : 0.0;
+            c00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, c00, 0, 0, 0);
+            c01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, c01, 0, 0, 0);
+            c10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, c10, 0, 0, 0);
+            c11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, c11, 0, 0, 0);
         }
-        __syncthreads();
-        for (int f = lane; f < HK * sa; f += 64) {
-            const int k = f / sa, i = f - k * sa;
-            const int t1 = s_t1[k];
-            As[f] = (t1 >= 0 && i < noa) ? tabA[(int64_t)t1 * noa + i] : 0.0;
-        }
-        for (int f = lane; f < HK * sb; f += 64) {
-            const int k = f / sb, j = f - k * sb;
-            const int t2 = s_t2[k];
-            Bs[f] = (t2 >= 0 && j < nob) ? tabB[(int64_t)t2 * nob + j] * s_v[k] : 0.0;
-        }
-        __syncthreads();
-        if (active) {
-            for (int k = slice; k < HK; k += nslice) {
-                const double2 a01 = *reinterpret_cast<const double2 *>(&As[k * sa + 4 * ti]);
-                const double2 a23 = *reinterpret_cast<const double2 *>(&As[k * sa + 4 * ti + 2]);
-                const double2 b01 = *reinterpret_cast<const double2 *>(&Bs[k * sb + 4 * tj]);
-                const double2 b23 = *reinterpret_cast<const double2 *>(&Bs[k * sb + 4 * tj + 2]);
-                const double av[4] = {a01.x, a01.y, a23.x, a23.y};
-                const double bv[4] = {b01.x, b01.y, b23.x, b23.y};
-#pragma unroll
-                for (int r = 0; r < 4; r++)
-#pragma unroll
-                    for (int c = 0; c < 4; c++) acc[r][c] = fma(av[r], bv[c], acc[r][c]);
-            }
-        }
-        __syncthreads();
     }
-    // cross-slice reduction through LDS (As is free now: 64*16 doubles fit in HK*MAXNO)
-    double *red = As;
-    if (active) {
+    // v_mfma_f64_16x16x4f64 keeps D[kq + 4v][r16] in lane (16 kq + r16), register v
+    const d4_t *cs[4] = {&c00, &c01, &c10, &c11};
 #pragma unroll
-        for (int r = 0; r < 4; r++)
+    for (int t = 0; t < 4; t++) {
+        const int rt = t >> 1, ct = t & 1;
+        const int col = c0 + ct * 16 + r16;
 #pragma unroll
-            for (int c = 0; c < 4; c++) red[(slice * ntile + tl) * 16 + r * 4 + c] = acc[r][c];
-    }
-    __syncthreads();
-    double *out = part + u.part;
-    for (int e = lane; e < ntile * 16; e += 64) {
-        const int t = e >> 4, rc = e & 15;
-        const int r = rc >> 2, c = rc & 3;
-        const int i = (t / tjb) * 4 + r, j = (t % tjb) * 4 + c;
-        if (i < noa && j < nob) {
-            double s = 0.0;
-            for (int q = 0; q < nslice; q++) s += red[(q * ntile + t) * 16 + rc];
-            out[i * nob + j] = s;
+        for (int v = 0; v < 4; v++) {
+            const int row = r0 + rt * 16 + kq + 4 * v;
+            if (row < noa && col < nob) out[row * nob + col] = (*cs[t])[v];
         }
     }
 }
 
-typedef double d4_t __attribute__((ext_vector_type(4)));
-
-// Hamiltonian unit kernel on the FP64 matrix cores (v_mfma_f64_16x16x4f64).
-// One wave per unit.  The NO_A x NO_B block is held as 2x2 MFMA tiles
-// (rows/cols padded to 32).  Lane l supplies A[l%16][l/16] = phi_A row of
-// point l/16 and B[l/16][l%16] = V*phi_B column, gathered straight from the
-// tables; the 4 points of a step are consecutive overlap points.
 __global__ __launch_bounds__(64) void k_ham_mfma64(const HUnit *__restrict__ units, const int32_t *__restrict__ order,
                                                    int32_t nunits, const PairDev *__restrict__ pairs,
                                                    const double *__restrict__ tab, const int32_t *__restrict__ olg1,
@@ -188,47 +137,15 @@ __global__ __launch_bounds__(64) void k_ham_mfma64(const HUnit *__restrict__ uni
     const PairDev P = pairs[__builtin_amdgcn_readfirstlane(u.pair)];
     const int noa = __builtin_amdgcn_readfirstlane(P.noa), nob = __builtin_amdgcn_readfirstlane(P.nob);
     const int lane = threadIdx.x;
-    const int r16 = lane & 15, kq = lane >> 4;
     const double *tabA = tab + P.tab_a, *tabB = tab + P.tab_b;
     const int32_t *L1 = olg1 + P.olg + u.lo, *L2 = olg2 + P.olg + u.lo;
     const int32_t *glA = glist + P.gl_a;
-    d4_t c00 = {0, 0, 0, 0}, c01 = c00, c10 = c00, c11 = c00;
-    const bool ra0 = r16 < noa, ra1 = 16 + r16 < noa, rb0 = r16 < nob, rb1 = 16 + r16 < nob;
-    for (int g0 = 0; g0 < u.n; g0 += 64) {
-        // indices and potential of 64 points, one per lane
-        const int gi = g0 + lane;
-        const bool gok = gi < u.n;
-        const int t1g = gok ? L1[gi] : 0;
-        const int t2g = gok ? L2[gi] : 0;
-        const double vg = gok ? V[glA[t1g]] : 0.0;
-        const int nstep = min(16, (u.n - g0 + 3) >> 2);
-#pragma unroll 4
-        for (int st = 0; st < nstep; st++) {
-            const int src = 4 * st + kq;
-            const int t1 = __shfl(t1g, src, 64);
-            const int t2 = __shfl(t2g, src, 64);
-            const double v = __shfl(vg, src, 64);  // 0 for points past the end
-            const double *ca = tabA + (int64_t)t1 * noa, *cb = tabB + (int64_t)t2 * nob;
-            const double a0 = ra0 ? ca[r16] : 0.0, a1 = ra1 ? ca[16 + r16] : 0.0;
-            const double b0 = rb0 ? cb[r16] * v : 0.0, b1 = rb1 ? cb[16 + r16] * v : 0.0;
-            c00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, c00, 0, 0, 0);
-            c01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, c01, 0, 0, 0);
-            c10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, c10, 0, 0, 0);
-            c11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, c11, 0, 0, 0);
-        }
-    }
-    // v_mfma_f64_16x16x4f64 keeps D[kq + 4v][r16] in lane (16 kq + r16), register v
     double *out = part + u.part;
-    const d4_t *cs[4] = {&c00, &c01, &c10, &c11};
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-        const int rt = t >> 1, ct = t & 1;
-        const int col = ct * 16 + r16;
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int row = rt * 16 + kq + 4 * v;
-            if (row < noa && col < nob) out[row * nob + col] = (*cs[t])[v];
-        }
+    if (noa <= 32 && nob <= 32) {
+        ham_chunk64(tabA, tabB, L1, L2, glA, V, u.n, noa, nob, 0, 0, lane, out);
+    } else {
+        for (int r0 = 0; r0 < noa; r0 += 32)
+            for (int c0 = 0; c0 < nob; c0 += 32) ham_chunk64(tabA, tabB, L1, L2, glA, V, u.n, noa, nob, r0, c0, lane, out);
     }
 }
 
@@ -1161,7 +1078,13 @@ int ham_dev(lcpao_ctx *ctx, int nch, const double *const *d_V, double *const *d_
     if (nch != 1 && nch != 2 && nch != 4) return set_err(ctx, LCPAO_EINVAL, "nch must be 1, 2 or 4");
     int rc = check_precision(ctx, precision);
     if (rc) return rc;
-    if (precision != LCPAO_FP64) return split_hamiltonian(ctx, nch, d_V, d_H, precision);
+    if (precision != LCPAO_FP64) {
+        rc = split_hamiltonian(ctx, nch, d_V, d_H, precision);
+        if (rc != LCPAO_ERANGE || ctx->block_state != 0) return rc;
+        // the block plan cannot hold this geometry: the FP64 pair-list
+        // kernels serve the FP32 and bf16x3 H (within any form's bound)
+        ctx->err.clear();
+    }
     const int nu = (int)ctx->hunits.size();
     for (int c = 0; c < nch; c++) {
         if (!d_V[c] || !d_H[c]) return set_err(ctx, LCPAO_EINVAL, "null channel pointer");
@@ -1234,7 +1157,14 @@ int rho_dev(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_off
     if (dm_offset < 0) return set_err(ctx, LCPAO_EINVAL, "negative dm_offset");
     int rc = check_precision(ctx, precision);
     if (rc) return rc;
-    if (precision == LCPAO_BF16X3) return split_density(ctx, nspin, d_DM, dm_offset, d_rho);
+    if (precision == LCPAO_BF16X3) {
+        rc = split_density(ctx, nspin, d_DM, dm_offset, d_rho);
+        if (rc != LCPAO_ERANGE || ctx->block_state != 0) return rc;
+        // no block plan for this geometry: the FP32 pair-list kernel serves
+        // the bf16x3 density (FP32 products are within the bf16x3 bound)
+        ctx->err.clear();
+        precision = LCPAO_FP32;
+    }
     const bool f32 = precision != LCPAO_FP64;
     for (int s = 0; s < nspin; s++)
         if (!d_DM[s] || !d_rho[s]) return set_err(ctx, LCPAO_EINVAL, "null spin pointer");
@@ -1245,7 +1175,7 @@ int rho_dev(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_off
     // a variant fixed to 1 reports the failure.
     if (f32 && ctx->variant[TUNE_RHO32] != 0) {
         const int r2 = split_tables(ctx, LCPAO_FP32);
-        if (r2 && ctx->variant[TUNE_RHO32] < 0) {
+        if (r2 && (ctx->variant[TUNE_RHO32] < 0 || ctx->block_state == 0)) {
             ctx->variant[TUNE_RHO32] = 0;
             ctx->err.clear();
             (void)hipGetLastError();  // a failed allocation leaves no sticky state behind
@@ -1316,6 +1246,10 @@ int rho_dev_pairs(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t 
                                    ctx->d_rorder, nu, ctx->d_segs, ctx->d_pairs, ctx->d_tab, ctx->d_olg1, ctx->d_olg2,
                                    ctx->d_deff, ctx->d_gl_off, ctx->d_rhoA);
         };
+        if (ctx->maxno > MFMA64_NO) {  // the matrix-core candidate holds at most 32 orbitals
+            if (ctx->variant[TUNE_RHO64] == 0) return set_err(ctx, LCPAO_ERANGE, "the FP64 matrix-core density kernel holds at most 32 orbitals per atom");
+            ctx->variant[TUNE_RHO64] = 1;
+        }
         if (nu > 0 && !f32 && (rc = autotune(ctx, TUNE_RHO64, rho64))) return rc;
         if (nu > 0) {
             KTimer kt(ctx, KID_RHO);
@@ -1344,8 +1278,19 @@ int f3_dev(lcpao_ctx *ctx, int nch, const double *const *d_V, const double *cons
     if (!weight || !d_force) return set_err(ctx, LCPAO_EINVAL, "null weight or force pointer");
     int rc = check_precision(ctx, precision);
     if (rc) return rc;
+    if (precision == LCPAO_BF16X3 && (rc = split_force3_ready(ctx))) {
+        if (rc != LCPAO_ERANGE) return rc;
+        // the block form cannot hold this geometry: the FP32 pair-list
+        // kernel serves the bf16x3 Force3 (FP32 products are within its bound)
+        ctx->err.clear();
+        precision = LCPAO_FP32;
+    }
     const bool bf = precision == LCPAO_BF16X3;
     const bool f32 = precision == LCPAO_FP32;
+    if (!f32 && !bf && ctx->maxno > MFMA64_NO) {  // the matrix-core candidate holds at most 32 orbitals
+        if (ctx->variant[TUNE_F364] == 0) return set_err(ctx, LCPAO_ERANGE, "the FP64 matrix-core Force3 kernel holds at most 32 orbitals per atom");
+        ctx->variant[TUNE_F364] = 1;
+    }
     if (f32 && (rc = fp32_tables(ctx, true))) return rc;
     const int nu = (int)ctx->funits.size();
     if (ctx->natom > 0)
@@ -1426,6 +1371,8 @@ int f3_dev_noalloc(lcpao_ctx *ctx, int nch, const double *const *d_V, const doub
                                               std::to_string((long long)NOALLOC_SCRATCH_DOUBLES));
     double *fpart = base, *fsum = base + fpart_n;
     const int variant = ctx->variant[TUNE_F364] >= 0 ? ctx->variant[TUNE_F364] : 1;
+    if (variant == 0 && ctx->maxno > MFMA64_NO)
+        return set_err(ctx, LCPAO_ERANGE, "the FP64 matrix-core Force3 kernel holds at most 32 orbitals per atom");
     if (ctx->natom > 0)
         hipLaunchKernelGGL(k_fill_zero, dim3((ctx->natom * 3 + 255) / 256), dim3(256), 0, ctx->stream,
                            (int64_t)ctx->natom * 3, fsum);
@@ -1531,7 +1478,7 @@ int lcpao_grid_create(lcpao_ctx **out, const lcpao_geom *g, int device) {
     ctx->gl_off.assign(g->natom + 1, 0);
     ctx->tab_off.assign(g->natom + 1, 0);
     for (int a = 0; a < g->natom; a++) {
-        if (ctx->no[a] < 0 || ctx->no[a] > MAXNO) return fail(LCPAO_ERANGE, "orbital count per atom must be in [0, 32]");
+        if (ctx->no[a] < 0 || ctx->no[a] > MAXNO) return fail(LCPAO_ERANGE, "orbital count per atom must be in [0, 144]");
         if (ctx->gridn[a] < 0 || (ctx->gridn[a] > 0 && !g->grid_list[a])) return fail(LCPAO_EINVAL, "bad grid list");
         ctx->maxno = std::max(ctx->maxno, ctx->no[a]);
         ctx->gl_off[a + 1] = ctx->gl_off[a] + ctx->gridn[a];
@@ -1753,6 +1700,12 @@ int lcpao_grid_tune_save(lcpao_ctx *ctx, const char *path) {
     return LCPAO_OK;
 }
 
+int lcpao_grid_block_forms(const lcpao_ctx *ctx, int32_t *state) {
+    if (!ctx || !state) return LCPAO_EINVAL;
+    *state = ctx->block_state;
+    return LCPAO_OK;
+}
+
 int lcpao_grid_kernel_stats(const lcpao_ctx *ctx, int32_t kernel_id, int64_t *launches, double *total_ms) {
     if (!ctx || kernel_id < 0 || kernel_id >= KID_COUNT) return LCPAO_EINVAL;
     if (launches) *launches = ctx->prof_launches[kernel_id];
@@ -1846,7 +1799,7 @@ int lcpao_grid_build_tables(lcpao_ctx *ctx, int32_t nspecies, const lcpao_pao *p
         S.rv = d;
         for (int l = 0; l <= P.maxl; l++)
             for (int p = 0; p < P.num_basis[l]; p++) {
-                const double *src = P.rwf[l * 4 + p];
+                const double *src = P.rwf[l * 16 + p];
                 if (!src) { cleanup(); return set_err(ctx, LCPAO_EINVAL, "null radial function"); }
                 double *w = nullptr;
                 if (hipMalloc(&w, sizeof(double) * P.nmesh) != hipSuccess) { cleanup(); return set_err(ctx, LCPAO_EDEVICE, "hipMalloc failed"); }

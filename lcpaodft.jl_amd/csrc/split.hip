@@ -611,8 +611,31 @@ void invalidate_split_tables(lcpao_ctx *ctx) {
         if (ctx->d_dtab32[x]) { (void)hipFree(ctx->d_dtab32[x]); ctx->d_dtab32[x] = nullptr; }
 }
 
+int plan_build_inner(lcpao_ctx *ctx);
+
+// Builds the block plan once.  A geometry the block forms cannot hold (more
+// than 32 orbitals on an atom, too many orbital rows or sphere images on a
+// grid block, index ranges) gives LCPAO_ERANGE and marks the context
+// (block_state 0), so that the callers switch to the pair-list kernels and
+// the plan is not attempted again.
 int plan_build(lcpao_ctx *ctx) {
     if (ctx->plan) return LCPAO_OK;
+    if (ctx->block_state == 0) { ctx->err = ctx->block_why; return LCPAO_ERANGE; }
+    if (ctx->maxno > 32) {
+        ctx->err = "the block forms hold at most 32 orbitals per atom (this geometry has " + std::to_string(ctx->maxno) + ")";
+    } else {
+        const int rc = plan_build_inner(ctx);
+        if (rc != LCPAO_ERANGE) {
+            if (rc == LCPAO_OK) ctx->block_state = 1;
+            return rc;
+        }
+    }
+    ctx->block_state = 0;
+    ctx->block_why = ctx->err;
+    return LCPAO_ERANGE;
+}
+
+int plan_build_inner(lcpao_ctx *ctx) {
     const int N1 = ctx->ngrid[0], N2 = ctx->ngrid[1], N3 = ctx->ngrid[2];
     if (ctx->h_clist.empty()) { ctx->err = "split-precision forms need the cell lists"; return LCPAO_ESTATE; }
     auto *P = new BlockPlan();
@@ -1996,11 +2019,21 @@ __global__ __launch_bounds__(256) void k_fblk_reduce(const int32_t *__restrict__
         for (int d = 0; d < 3; d++) fsum[3 * a + d] += w * s[d];
 }
 
-int split_force3(lcpao_ctx *ctx, const double *d_V, const double *d_DM, double w, double *d_fsum) {
+int split_force3_ready(lcpao_ctx *ctx) {
     int rc = split_tables(ctx, LCPAO_BF16X3);
     if (rc) return rc;
+    if (ctx->plan->max_slots > FB_MAX_SLOTS) {
+        ctx->err = "too many sphere images per block for the bf16x3 Force3 (" + std::to_string(ctx->plan->max_slots) +
+                   ", at most " + std::to_string(FB_MAX_SLOTS) + ")";
+        return LCPAO_ERANGE;
+    }
+    return LCPAO_OK;
+}
+
+int split_force3(lcpao_ctx *ctx, const double *d_V, const double *d_DM, double w, double *d_fsum) {
+    int rc = split_force3_ready(ctx);
+    if (rc) return rc;
     BlockPlan *P = ctx->plan;
-    if (P->max_slots > FB_MAX_SLOTS) { ctx->err = "too many sphere images per block for the bf16x3 Force3"; return LCPAO_ERANGE; }
     if (!P->dtile_ready) {
         const size_t n = (size_t)std::max<int64_t>(1, P->tile_elems) * 3;
         if (!P->d_dtile) SPLIT_TRY(ctx, hipMalloc(&P->d_dtile, n * sizeof(float)));
@@ -2044,3 +2077,48 @@ int split_force3(lcpao_ctx *ctx, const double *d_V, const double *d_DM, double w
 }
 
 }  // namespace lcpao
+
+// ---------------------------------------------------------------------------
+// Host-only layout check of the block forms (no device needed): the same
+// bucketing of the grid-list entries into 8x8x8 blocks as the plan, and the
+// same limits.
+// ---------------------------------------------------------------------------
+extern "C" int lcpao_grid_block_layout(const lcpao_geom *g, int32_t *max_rows, int32_t *max_slots, int64_t *nblocks) {
+    using namespace lcpao;
+    if (!g || g->natom <= 0 || !g->no || !g->gridn || !g->grid_list || !g->cell_list || g->ngrid[0] <= 0 ||
+        g->ngrid[1] <= 0 || g->ngrid[2] <= 0)
+        return LCPAO_EINVAL;
+    const int N1 = g->ngrid[0], N2 = g->ngrid[1], N3 = g->ngrid[2];
+    const int nb1 = (N1 + BLK - 1) / BLK, nb2 = (N2 + BLK - 1) / BLK, nb3 = (N3 + BLK - 1) / BLK;
+    const int64_t NB = (int64_t)nb1 * nb2 * nb3;
+    // (block, atom, cell) of every grid-list entry, unique per block
+    std::vector<std::pair<int64_t, int64_t>> key;
+    int maxno = 0;
+    for (int a = 0; a < g->natom; a++) {
+        maxno = std::max(maxno, (int)g->no[a]);
+        if (g->gridn[a] > 0 && (!g->grid_list[a] || !g->cell_list[a])) return LCPAO_EINVAL;
+        for (int t = 0; t < g->gridn[a]; t++) {
+            const int gi = g->grid_list[a][t];
+            if (gi < 0 || (int64_t)gi >= (int64_t)N1 * N2 * N3) return LCPAO_EINVAL;
+            const int n1 = gi / (N2 * N3), n2 = (gi / N3) % N2, n3 = gi % N3;
+            const int64_t b = ((int64_t)(n1 / BLK) * nb2 + (n2 / BLK)) * nb3 + (n3 / BLK);
+            key.push_back({b, (int64_t)a * 1000000 + g->cell_list[a][t]});
+        }
+    }
+    std::sort(key.begin(), key.end());
+    key.erase(std::unique(key.begin(), key.end()), key.end());
+    int mr = 0, ms = 0;
+    for (size_t i = 0; i < key.size();) {
+        size_t j = i;
+        int r = 0, ns = 0;
+        while (j < key.size() && key[j].first == key[i].first) { r += g->no[key[j].second / 1000000]; ns++; j++; }
+        mr = std::max(mr, std::max(ST, (r + ST - 1) / ST * ST));
+        ms = std::max(ms, ns);
+        i = j;
+    }
+    if (max_rows) *max_rows = mr;
+    if (max_slots) *max_slots = ms;
+    if (nblocks) *nblocks = NB;
+    const bool fits = maxno <= 32 && mr <= MAX_ROWS && ms * ms <= EPI_SP * 64 * HB_WAVES && ms <= FB_MAX_SLOTS;
+    return fits ? LCPAO_OK : LCPAO_ERANGE;
+}

@@ -159,6 +159,9 @@ int split_hamiltonian(lcpao_ctx *ctx, int nch, const double *const *d_V, double 
 int split_density(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_offset, double *const *d_rho);
 // bf16x3 Force3 of one channel: adds w * S_A into d_fsum (natom x 3, atom-major)
 int split_force3(lcpao_ctx *ctx, const double *d_V, const double *d_DM, double w, double *d_fsum);
+// LCPAO_OK when the bf16x3 Force3 block form holds the geometry (tiles built),
+// LCPAO_ERANGE when it does not
+int split_force3_ready(lcpao_ctx *ctx);
 int fp32_tables(lcpao_ctx *ctx, bool deriv);
 void invalidate_split_tables(lcpao_ctx *ctx);
 void launch_to_f32(hipStream_t stream, int64_t n, const double *x, float *y);

@@ -53,7 +53,8 @@ EXPORTS = [
     "lcpao_grid_aden_pcc", "lcpao_grid_aden_pcc_dev", "lcpao_grid_poisson", "lcpao_grid_poisson_dev",
     "lcpao_grid_bloch_sum", "lcpao_grid_bloch_sum_dev", "lcpao_grid_bloch_sizes", "lcpao_grid_dm_states",
     "lcpao_grid_dm_states_dev", "lcpao_grid_bloch_sum_nc", "lcpao_grid_bloch_sum_nc_dev", "lcpao_grid_dm_states_nc",
-    "lcpao_grid_dm_states_nc_dev", "lcpao_grid_tune_load", "lcpao_grid_tune_save",
+    "lcpao_grid_dm_states_nc_dev", "lcpao_grid_tune_load", "lcpao_grid_tune_save", "lcpao_grid_block_forms",
+    "lcpao_grid_block_layout",
 ]
 
 _lib = None
@@ -106,6 +107,8 @@ def load():
     L.lcpao_grid_force3_dev.argtypes = [vp, C.c_int32, dpp, dpp, C.c_int64, dp, dp, C.c_int32]
     L.lcpao_grid_force3_dev_noalloc.argtypes = [vp, C.c_int32, dpp, dpp, C.c_int64, dp, dp]
     L.lcpao_grid_sizes.argtypes = [vp, lp, lp, lp, lp]
+    L.lcpao_grid_block_forms.argtypes = [vp, ip]
+    L.lcpao_grid_block_layout.argtypes = [C.POINTER(Geom), ip, ip, lp]
     ar = C.POINTER(AtomicRadial)
     L.lcpao_grid_aden_pcc.argtypes = [vp, C.c_int32, C.c_int32, ar, dp, dp, dp, dp, dpp]
     L.lcpao_grid_aden_pcc_dev.argtypes = [vp, C.c_int32, C.c_int32, ar, dp, dp, dp, dp, dpp]

@@ -41,6 +41,62 @@ def _prec(p):
     return int(p)
 
 
+def _geom(u, with_lattice=True, with_cell_lists=True):
+    """lcpao_geom of a UCell (and the arrays it points into, to keep alive)."""
+    keep = []
+    no = np.ascontiguousarray(u.Total_NumOrbs, np.int32)
+    gridn = np.ascontiguousarray(u.GridN_Atom, np.int32)
+    glist = np.ascontiguousarray(u.glist, np.int32)
+    clist = np.ascontiguousarray(u.clist, np.int32)
+    olg1 = np.ascontiguousarray(u.olg1, np.int32)
+    olg2 = np.ascontiguousarray(u.olg2, np.int32)
+    keep += [no, gridn, glist, clist, olg1, olg2]
+    isz = glist.itemsize
+    base_g, base_c = glist.ctypes.data, clist.ctypes.data
+    gptr = (_lib.ip * u.Natom)(*[C.cast(base_g + int(u.gl_off[a]) * isz, _lib.ip) for a in range(u.Natom)])
+    cptr = (_lib.ip * u.Natom)(*[C.cast(base_c + int(u.gl_off[a]) * isz, _lib.ip) for a in range(u.Natom)])
+    b1, b2 = olg1.ctypes.data, olg2.ctypes.data
+    P = u.MPI_size
+    o1p = (_lib.ip * max(P, 1))(*[C.cast(b1 + int(u.olg_off[p]) * isz, _lib.ip) for p in range(P)])
+    o2p = (_lib.ip * max(P, 1))(*[C.cast(b2 + int(u.olg_off[p]) * isz, _lib.ip) for p in range(P)])
+    pa = np.ascontiguousarray(u.MPI_atom, np.int32)
+    pn = np.ascontiguousarray(u.MPI_natn, np.int32)
+    pc = np.ascontiguousarray(u.MPI_ncn, np.int32)
+    nolg = np.ascontiguousarray(u.MPI_NumOLG, np.int32)
+    lat = np.ascontiguousarray(u.Latvecs, np.float64)
+    org = np.ascontiguousarray(u.Grid_Origin, np.float64)
+    gx = np.ascontiguousarray(u.Gxyz, np.float64)
+    a2s = np.ascontiguousarray(u.atom2spe, np.int32)
+    keep += [gptr, cptr, o1p, o2p, pa, pn, pc, nolg, lat, org, gx, a2s]
+    g = _lib.Geom()
+    g.natom, g.atom_base = u.Natom, 0
+    g.ngrid[:] = list(u.Ngrid)
+    g.cpycell, g.gridvol = u.CpyCell, u.GridVol
+    g.no, g.gridn = no.ctypes.data_as(_lib.ip), gridn.ctypes.data_as(_lib.ip)
+    g.grid_list = gptr
+    if with_cell_lists:  # optional in the C ABI (needed by the block forms and device tables)
+        g.cell_list = cptr
+    g.npairs = P
+    g.pair_atom, g.pair_natn, g.pair_ncn = pa.ctypes.data_as(_lib.ip), pn.ctypes.data_as(_lib.ip), pc.ctypes.data_as(_lib.ip)
+    g.numolg, g.olg1, g.olg2 = nolg.ctypes.data_as(_lib.ip), o1p, o2p
+    if with_lattice:
+        g.latvecs, g.origin, g.gxyz = lat.ctypes.data_as(_lib.dp), org.ctypes.data_as(_lib.dp), gx.ctypes.data_as(_lib.dp)
+        g.atom2spe = a2s.ctypes.data_as(_lib.ip)
+    return g, keep
+
+
+def block_layout(ucell):
+    """Host-only check of the block-form limits for a geometry (no device):
+    (fits, largest padded row count, largest sphere-image count, blocks)."""
+    L = _lib.load()
+    g, keep = _geom(ucell)
+    mr, ms, nb = C.c_int32(), C.c_int32(), C.c_int64()
+    rc = L.lcpao_grid_block_layout(C.byref(g), C.byref(mr), C.byref(ms), C.byref(nb))
+    if rc not in (0, _lib.LCPAO_ERANGE):
+        raise GridError(rc, "lcpao_grid_block_layout failed")
+    return rc == 0, mr.value, ms.value, nb.value
+
+
 class GridContext:
     """Device context of one geometry (lists, tables, work decomposition)."""
 
@@ -48,48 +104,9 @@ class GridContext:
         L = _lib.load()
         u = ucell
         self.ucell = u
-        self._keep = []
-        keep = self._keep.append
+        g, self._keep = _geom(u, with_lattice, with_cell_lists)
         no = np.ascontiguousarray(u.Total_NumOrbs, np.int32)
         gridn = np.ascontiguousarray(u.GridN_Atom, np.int32)
-        glist = np.ascontiguousarray(u.glist, np.int32)
-        clist = np.ascontiguousarray(u.clist, np.int32)
-        olg1 = np.ascontiguousarray(u.olg1, np.int32)
-        olg2 = np.ascontiguousarray(u.olg2, np.int32)
-        for x in (no, gridn, glist, clist, olg1, olg2):
-            keep(x)
-        isz = glist.itemsize
-        base_g, base_c = glist.ctypes.data, clist.ctypes.data
-        gptr = (_lib.ip * u.Natom)(*[C.cast(base_g + int(u.gl_off[a]) * isz, _lib.ip) for a in range(u.Natom)])
-        cptr = (_lib.ip * u.Natom)(*[C.cast(base_c + int(u.gl_off[a]) * isz, _lib.ip) for a in range(u.Natom)])
-        b1, b2 = olg1.ctypes.data, olg2.ctypes.data
-        P = u.MPI_size
-        o1p = (_lib.ip * max(P, 1))(*[C.cast(b1 + int(u.olg_off[p]) * isz, _lib.ip) for p in range(P)])
-        o2p = (_lib.ip * max(P, 1))(*[C.cast(b2 + int(u.olg_off[p]) * isz, _lib.ip) for p in range(P)])
-        pa = np.ascontiguousarray(u.MPI_atom, np.int32)
-        pn = np.ascontiguousarray(u.MPI_natn, np.int32)
-        pc = np.ascontiguousarray(u.MPI_ncn, np.int32)
-        nolg = np.ascontiguousarray(u.MPI_NumOLG, np.int32)
-        lat = np.ascontiguousarray(u.Latvecs, np.float64)
-        org = np.ascontiguousarray(u.Grid_Origin, np.float64)
-        gx = np.ascontiguousarray(u.Gxyz, np.float64)
-        a2s = np.ascontiguousarray(u.atom2spe, np.int32)
-        for x in (gptr, cptr, o1p, o2p, pa, pn, pc, nolg, lat, org, gx, a2s):
-            keep(x)
-        g = _lib.Geom()
-        g.natom, g.atom_base = u.Natom, 0
-        g.ngrid[:] = list(u.Ngrid)
-        g.cpycell, g.gridvol = u.CpyCell, u.GridVol
-        g.no, g.gridn = no.ctypes.data_as(_lib.ip), gridn.ctypes.data_as(_lib.ip)
-        g.grid_list = gptr
-        if with_cell_lists:  # optional in the C ABI (needed by the block forms and device tables)
-            g.cell_list = cptr
-        g.npairs = P
-        g.pair_atom, g.pair_natn, g.pair_ncn = pa.ctypes.data_as(_lib.ip), pn.ctypes.data_as(_lib.ip), pc.ctypes.data_as(_lib.ip)
-        g.numolg, g.olg1, g.olg2 = nolg.ctypes.data_as(_lib.ip), o1p, o2p
-        if with_lattice:
-            g.latvecs, g.origin, g.gxyz = lat.ctypes.data_as(_lib.dp), org.ctypes.data_as(_lib.dp), gx.ctypes.data_as(_lib.dp)
-            g.atom2spe = a2s.ctypes.data_as(_lib.ip)
         self.h = C.c_void_p()
         rc = L.lcpao_grid_create(C.byref(self.h), C.byref(g), int(device))
         if rc != 0:
@@ -104,6 +121,13 @@ class GridContext:
         self.natom = u.Natom
         self.no = no.copy()
         self.gridn = gridn.copy()
+
+    def block_forms(self):
+        """-1 not decided yet, 1 the split forms run the block kernels, 0 the
+        geometry is beyond the block limits and pair-list kernels serve them."""
+        v = C.c_int32()
+        self._check(self.L.lcpao_grid_block_forms(self.h, C.byref(v)))
+        return v.value
 
     def close(self):
         if getattr(self, "h", None):
@@ -141,12 +165,12 @@ class GridContext:
             rv = np.ascontiguousarray(p.Spe_PAO_RV)
             keep.append(rv)
             arr[s].rv = rv.ctypes.data_as(_lib.dp)
-            rw = (_lib.dp * 16)()
+            rw = (_lib.dp * 64)()
             for l in range(p.Spe_MaxL_Basis + 1):
                 for q in range(p.Spe_Num_Basis[l]):
                     w = np.ascontiguousarray(p.Spe_PAO_RWF[l][q])
                     keep.append(w)
-                    rw[l * 4 + q] = w.ctypes.data_as(_lib.dp)
+                    rw[l * 16 + q] = w.ctypes.data_as(_lib.dp)
             keep.append(rw)
             arr[s].rwf = C.cast(rw, _lib.dpp)
         self._check(self.L.lcpao_grid_build_tables(self.h, len(paos), arr, int(bool(with_derivatives))))

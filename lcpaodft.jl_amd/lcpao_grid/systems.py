@@ -28,6 +28,8 @@ BASES = {
     "As": ("As", 7.0, "s3p2d2"),
     "Na": ("Na", 9.0, "s3p2d1"),
     "Cl": ("Cl", 7.0, "s2p2d1"),
+    # a basis beyond 32 orbitals per atom (3 + 9 + 15 + 14 = 41), f shells included
+    "Sif": ("Si", 7.0, "s3p3d3f2"),
 }
 
 
@@ -125,6 +127,19 @@ def si2(ngrid=32):
     a = 10.26
     lat = 0.5 * a * np.array([[0, 1, 1], [1, 0, 1], [1, 1, 0]], dtype=np.float64)
     return Structure("si2", lat, [[0, 0, 0], [.25, .25, .25]], ["Si", "Si"], (ngrid,) * 3, a)
+
+
+def si2_large_basis(ngrid=16):
+    """The Si2 primitive cell with the 41-orbital Si7.0-s3p3d3f2 basis."""
+    st = si2(ngrid)
+    return Structure("si2f", st.Latvecs, [[0, 0, 0], [.25, .25, .25]], ["Sif", "Sif"], (ngrid,) * 3, st.a_cell)
+
+
+def dense_cell(a=3.5, ngrid=14):
+    """One Si atom in a simple cubic cell of edge a (bohr), much smaller than
+    its 7 bohr cutoff: about 48 sphere images cover every 8x8x8 grid block
+    (more orbital rows and images than the block forms hold)."""
+    return Structure("dense%d" % ngrid, np.eye(3) * a, [[0.1, 0.2, 0.3]], ["Si"], (ngrid,) * 3, a)
 
 
 def si(n, ngrid, disp=0.05, seed=1, a=10.26, vacancy=False, name=None):
