@@ -393,13 +393,11 @@ def bench_batch(args):
     torch.cuda.synchronize()
     ref = [(it["H"].cpu().numpy().copy(), it["rho"].cpu().numpy().copy()) for it in items]
     loop("bf16x3")  # builds every structure's block plan and tiles
+    loop("fp32")
     torch.cuda.synchronize()
     b = Batch([it["ctx"] for it in items])
     args_b = ([it["V"] for it in items], [it["DM"] for it in items], [it["H"] for it in items],
               [it["rho"] for it in items])
-    b.assemble(*args_b, use_graph=False)  # tunes the schedule
-    b.assemble(*args_b, use_graph=True)   # captures the graph
-    torch.cuda.synchronize()
     setup_s = time.time() - t_setup
 
     def timed(fn):
@@ -412,20 +410,33 @@ def bench_batch(args):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) * 1e3 / args.steps
 
+    def accuracy():
+        # the last call's H and rho against FP64
+        dh = dr = de = 0.0
+        for it, (H0, R0) in zip(items, ref):
+            H, R = it["H"].cpu().numpy(), it["rho"].cpu().numpy()
+            dh = max(dh, float(np.abs(H - H0).max() / np.abs(H0).max()))
+            dr = max(dr, float(np.abs(R - R0).max() / np.abs(R0).max()))
+            e0 = 2.0 * float(np.dot(it["DMh"], H0))
+            de = max(de, 1e6 * abs(2.0 * float(np.dot(it["DMh"], H)) - e0) / it["st"].Natom)
+        return {"max_abs_dH_over_max_H": dh, "max_abs_drho_over_max_rho": dr, "max_dE_DH_uHa_per_atom": de}
+
     ms_loop64 = timed(lambda: loop("fp64"))
-    ms_loop = timed(lambda: loop("bf16x3"))
-    ms_eager = timed(lambda: b.assemble(*args_b, use_graph=False))
-    ms_graph = timed(lambda: b.assemble(*args_b, use_graph=True))
-    # accuracy of the graph-replayed batch against FP64
-    dh = dr = de = 0.0
-    natoms = 0
-    for it, (H0, R0) in zip(items, ref):
-        H, R = it["H"].cpu().numpy(), it["rho"].cpu().numpy()
-        dh = max(dh, float(np.abs(H - H0).max() / np.abs(H0).max()))
-        dr = max(dr, float(np.abs(R - R0).max() / np.abs(R0).max()))
-        e0 = 2.0 * float(np.dot(it["DMh"], H0))
-        de = max(de, 1e6 * abs(2.0 * float(np.dot(it["DMh"], H)) - e0) / it["st"].Natom)
-        natoms += it["st"].Natom
+    forms, acc_f, sched = {}, {}, {}
+    for f in ("bf16x3", "fp32"):
+        b.set_precision(f)
+        b.assemble(*args_b, use_graph=False)  # builds the form's tiles, tunes the schedule
+        b.assemble(*args_b, use_graph=True)   # captures the graph
+        torch.cuda.synchronize()
+        forms["per_structure_loop_" + f] = round(timed(lambda: loop(f)), 4)
+        forms["batch_eager_" + f] = round(timed(lambda: b.assemble(*args_b, use_graph=False)), 4)
+        forms["batch_graph_" + f] = round(timed(lambda: b.assemble(*args_b, use_graph=True)), 4)
+        acc_f[f] = accuracy()
+        sched[f] = {"chosen": b.tuned()[0], "candidate_ms": b.tuned()[1]}
+    forms["per_structure_loop_fp64"] = round(ms_loop64, 4)
+    ms_graph = forms["batch_graph_bf16x3"]
+    ms_loop = forms["per_structure_loop_bf16x3"]
+    natoms = sum(it["st"].Natom for it in items)
     from collections import Counter
     names = Counter(it["st"].name for it in items)
     nstruct, ncls, nitems = b.sizes()
@@ -439,14 +450,12 @@ def bench_batch(args):
                                "classes" % (nstruct, ", ".join("%dx %s" % (v, k) for k, v in sorted(names.items())),
                                             natoms, sum(it["u"].NN for it in items), nitems, ncls),
                    "precision": "bf16x3"},
-        "forms_ms": {"batch_graph": round(ms_graph, 4), "batch_eager": round(ms_eager, 4),
-                     "per_structure_loop_bf16x3": round(ms_loop, 4), "per_structure_loop_fp64": round(ms_loop64, 4)},
-        "speedup_vs_loop": round(ms_loop / ms_graph, 3),
+        "forms_ms": forms,
+        "speedup_vs_loop": {f: round(forms["per_structure_loop_" + f] / forms["batch_graph_" + f], 3)
+                            for f in ("bf16x3", "fp32")},
         "speedup_vs_fp64_loop": round(ms_loop64 / ms_graph, 3),
-        "schedule": {"chosen": b.tuned()[0], "candidate_ms": b.tuned()[1],
-                     "names": ["persistent work queue", "one workgroup per item"]},
-        "accuracy_vs_fp64": {"max_abs_dH_over_max_H": dh, "max_abs_drho_over_max_rho": dr,
-                             "max_dE_DH_uHa_per_atom": de},
+        "schedule": dict(sched, names=["persistent work queue", "one workgroup per item"]),
+        "accuracy_vs_fp64": acc_f,
         "setup_s": round(setup_s, 2),
     }
     print(json.dumps(out), flush=True)

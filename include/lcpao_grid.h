@@ -381,7 +381,7 @@ void lcpao_grid_destroy(lcpao_ctx *ctx);
 typedef struct lcpao_ucell lcpao_ucell;
 
 /* ---- batched form (no counterpart in the reference) ------------------------
- * The split-BF16x3 grid H (1 channel) and density (1 spin) of many
+ * The split-BF16x3 (or FP32) grid H (1 channel) and density (1 spin) of many
  * structures in one go: the block items of all structures are merged into
  * one list per operation, run as one launch (persistent, its workgroups
  * pulling items from a work queue, or one workgroup per item); the whole
@@ -403,6 +403,11 @@ typedef struct lcpao_batch lcpao_batch;
 int lcpao_batch_create(lcpao_batch **out, int32_t n, lcpao_ctx *const *ctxs);
 int lcpao_batch_assemble(lcpao_batch *b, const double *const *d_V, const double *const *d_DM, double *const *d_H,
                          double *const *d_rho, int32_t use_graph);
+/* Precision of the batched assembly: LCPAO_BF16X3 (default) or LCPAO_FP32
+ * (the FP32 block kernels: H on the FP32 matrix cores, the FP32 block
+ * density; one workgroup per item).  A change drops the captured graph; the
+ * next lcpao_batch_assemble builds the FP32 tiles of every member. */
+int lcpao_batch_set_precision(lcpao_batch *b, int32_t precision);
 int lcpao_batch_tune(lcpao_batch *b, int32_t sched);
 int lcpao_batch_tuned(const lcpao_batch *b, int32_t *sched, float *candidate_ms);
 int lcpao_batch_sizes(const lcpao_batch *b, int32_t *nstruct, int32_t *nclasses, int64_t *nitems);

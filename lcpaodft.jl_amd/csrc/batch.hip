@@ -33,7 +33,16 @@ struct lcpao_batch {
     BlockArgs *d_args_h = nullptr, *d_args_r = nullptr;
     int32_t *d_queue = nullptr;   // one counter per launch: 2 per class
     int max_slots = 0;
-    // schedule: 0 persistent work queue, 1 one workgroup per item (-1: tune)
+    // per-structure tables of the merged folds, fills and reductions
+    lcpao::BatchFold *d_fold = nullptr;
+    lcpao::BatchRed *d_red = nullptr;
+    lcpao::BatchFill *d_fill = nullptr;
+    int max_canon = 0, red_threads = 64, max_rows = 0;
+    int64_t max_nn = 0;
+    // LCPAO_BF16X3 (default) or LCPAO_FP32
+    int32_t precision = LCPAO_BF16X3;
+    // schedule of the bf16x3 launches: 0 persistent work queue, 1 one
+    // workgroup per item (-1: tune); the FP32 launches run one workgroup per item
     int32_t sched = -1;
     float sched_ms[2] = {0.f, 0.f};
     // arguments of the last call and the graph captured with them
@@ -72,44 +81,61 @@ int bind(lcpao_batch *b, const double *const *V, const double *const *DM, double
     }
     if (now == b->bound) return LCPAO_OK;
     std::vector<BlockArgs> ah(n), ar(n);
+    std::vector<lcpao::BatchFold> fold(n);
+    std::vector<lcpao::BatchRed> red(n);
+    std::vector<lcpao::BatchFill> fill(n);
     for (size_t s = 0; s < n; s++) {
-        ah[s] = lcpao::block_args(b->ctx[s], V[s], nullptr);
-        ar[s] = lcpao::block_args(b->ctx[s], nullptr, rho[s]);
+        lcpao_ctx *c = b->ctx[s];
+        ah[s] = lcpao::block_args(c, V[s], nullptr);
+        ar[s] = lcpao::block_args(c, nullptr, rho[s]);
+        fold[s] = {c->d_hcanon, c->d_pairs, DM[s], c->d_deff32, (int32_t)c->ncanon, 0};
+        red[s] = {c->d_hcanon, c->plan->d_contrib_off, c->plan->d_contrib, c->d_pairs, c->plan->d_part, H[s],
+                  c->gridvol, (int32_t)c->ncanon, 0};
+        fill[s] = {rho[s], c->nn};
     }
     BATCH_TRY(b, hipMemcpy(b->d_args_h, ah.data(), n * sizeof(BlockArgs), hipMemcpyHostToDevice));
     BATCH_TRY(b, hipMemcpy(b->d_args_r, ar.data(), n * sizeof(BlockArgs), hipMemcpyHostToDevice));
+    BATCH_TRY(b, hipMemcpy(b->d_fold, fold.data(), n * sizeof(lcpao::BatchFold), hipMemcpyHostToDevice));
+    BATCH_TRY(b, hipMemcpy(b->d_red, red.data(), n * sizeof(lcpao::BatchRed), hipMemcpyHostToDevice));
+    BATCH_TRY(b, hipMemcpy(b->d_fill, fill.data(), n * sizeof(lcpao::BatchFill), hipMemcpyHostToDevice));
     b->bound = now;
     drop_graph(b);  // the graph's small launches carry the old pointers
     return LCPAO_OK;
 }
 
 // Enqueues one batched H + density on b->stream (no synchronisation, no
-// allocation, no memset: safe to capture).
-int enqueue(lcpao_batch *b, const double *const *DM, double *const *H, double *const *rho, int sched) {
-    const size_t n = b->ctx.size();
-    for (size_t s = 0; s < n; s++) {  // folded pair blocks (on-site blocks symmetrised) and zeroed densities
-        lcpao_ctx *c = b->ctx[s];
-        hipStream_t keep = c->stream;
-        c->stream = b->stream;
-        if (c->ncanon > 0) lcpao::launch_deff_f32(c, DM[s], 1);
-        c->stream = keep;
-        lcpao::launch_fill_f64(b->stream, c->nn, rho[s], 0.0);
-    }
-    const bool queue = sched == 0;
-    if (queue) lcpao::launch_fill_i32(b->stream, 2 * (int64_t)b->classes.size(), b->d_queue, 0);
+// allocation, no memset: safe to capture): one launch each for the folds of
+// every structure's density matrix (on-site blocks symmetrised), the cleared
+// densities, the merged H items, the per-pair H reductions and the merged
+// density items.
+int enqueue(lcpao_batch *b, int sched) {
+    const int n = (int)b->ctx.size();
+    lcpao::launch_deff_f32_batch(b->stream, n, b->d_fold, b->max_canon);
+    lcpao::launch_fill_batch(b->stream, n, b->d_fill, b->max_nn);
     const BlockArgs none{};
+    const bool f32 = b->precision == LCPAO_FP32;
+    const bool queue = !f32 && sched == 0;
+    if (queue) lcpao::launch_fill_i32(b->stream, 2 * (int64_t)b->classes.size(), b->d_queue, 0);
     for (size_t k = 0; k < b->classes.size(); k++) {
         const auto &cl = b->classes[k];
-        BATCH_TRY(b, lcpao::launch_block_kernel(false, b->stream, cl.rows, 0, b->d_items + cl.first, cl.count,
-                                                b->d_args_h, none, queue ? b->d_queue + 2 * k : nullptr,
-                                                queue ? 0u : (unsigned)cl.count));
+        if (f32)
+            BATCH_TRY(b, lcpao::launch_f32_block_kernel(false, b->stream, cl.rows, b->max_slots, b->d_items + cl.first,
+                                                        cl.count, b->d_args_h, none));
+        else
+            BATCH_TRY(b, lcpao::launch_block_kernel(false, b->stream, cl.rows, 0, b->d_items + cl.first, cl.count,
+                                                    b->d_args_h, none, queue ? b->d_queue + 2 * k : nullptr,
+                                                    queue ? 0u : (unsigned)cl.count));
     }
-    for (size_t s = 0; s < n; s++) lcpao::launch_hblk_reduce(b->ctx[s], b->stream, H[s]);
+    lcpao::launch_hblk_reduce_batch(b->stream, n, b->d_red, b->max_canon, b->red_threads);
     for (size_t k = 0; k < b->classes.size(); k++) {
         const auto &cl = b->classes[k];
-        BATCH_TRY(b, lcpao::launch_block_kernel(true, b->stream, cl.rows, b->max_slots, b->d_items + cl.first,
-                                                cl.count, b->d_args_r, none, queue ? b->d_queue + 2 * k + 1 : nullptr,
-                                                queue ? 0u : (unsigned)cl.count));
+        if (f32)
+            BATCH_TRY(b, lcpao::launch_f32_block_kernel(true, b->stream, cl.rows, b->max_slots, b->d_items + cl.first,
+                                                        cl.count, b->d_args_r, none));
+        else
+            BATCH_TRY(b, lcpao::launch_block_kernel(true, b->stream, cl.rows, b->max_slots, b->d_items + cl.first,
+                                                    cl.count, b->d_args_r, none, queue ? b->d_queue + 2 * k + 1 : nullptr,
+                                                    queue ? 0u : (unsigned)cl.count));
     }
     BATCH_TRY(b, hipGetLastError());
     return LCPAO_OK;
@@ -152,6 +178,9 @@ int lcpao_batch_create(lcpao_batch **out, int32_t n, lcpao_ctx *const *ctxs) {
         const lcpao::BlockPlan *P = c->plan;
         b->max_slots = std::max(b->max_slots, P->max_slots);
         max_rows = std::max(max_rows, P->max_rows);
+        b->max_canon = std::max<int>(b->max_canon, (int)c->ncanon);
+        b->red_threads = std::max(b->red_threads, P->red_threads);
+        b->max_nn = std::max(b->max_nn, c->nn);
         for (HItem it : P->items) {
             it.pad = s;
             items.push_back(it);
@@ -173,8 +202,12 @@ int lcpao_batch_create(lcpao_batch **out, int32_t n, lcpao_ctx *const *ctxs) {
     if (hipMalloc(&b->d_items, ni * sizeof(HItem)) != hipSuccess ||
         hipMalloc(&b->d_args_h, n * sizeof(BlockArgs)) != hipSuccess ||
         hipMalloc(&b->d_args_r, n * sizeof(BlockArgs)) != hipSuccess ||
-        hipMalloc(&b->d_queue, std::max<size_t>(2, 2 * b->classes.size()) * sizeof(int32_t)) != hipSuccess)
+        hipMalloc(&b->d_queue, std::max<size_t>(2, 2 * b->classes.size()) * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc(&b->d_fold, n * sizeof(lcpao::BatchFold)) != hipSuccess ||
+        hipMalloc(&b->d_red, n * sizeof(lcpao::BatchRed)) != hipSuccess ||
+        hipMalloc(&b->d_fill, n * sizeof(lcpao::BatchFill)) != hipSuccess)
         return fail(LCPAO_EDEVICE);
+    b->max_rows = max_rows;
     if (!items.empty() &&
         hipMemcpy(b->d_items, items.data(), items.size() * sizeof(HItem), hipMemcpyHostToDevice) != hipSuccess)
         return fail(LCPAO_EDEVICE);
@@ -193,22 +226,26 @@ int lcpao_batch_assemble(lcpao_batch *b, const double *const *d_V, const double 
     // captured graph stay valid
     for (lcpao_ctx *c : b->ctx) {
         if (!c->tables) return berr(b, LCPAO_ESTATE, "a member context has no orbital tables");
-        if (!c->plan || !c->plan->bf16_ready) {
-            const int rc = lcpao::split_tables(c, LCPAO_BF16X3);
+        const bool ready = c->plan && (b->precision == LCPAO_FP32 ? c->plan->f32_ready : c->plan->bf16_ready);
+        if (!ready) {
+            const bool had_t32 = c->plan && c->plan->d_t32;
+            const int rc = lcpao::split_tables(c, b->precision);
             if (rc) return berr(b, rc, "rebuilding a member's split tables: " + c->err);
+            if (b->precision == LCPAO_FP32 && !had_t32) b->bound.clear();  // new tile buffer: bind again
         }
     }
     int rc = bind(b, d_V, d_DM, d_H, d_rho);
     if (rc) return rc;
+    if (b->precision == LCPAO_FP32 && b->sched < 0) b->sched = 1;  // the FP32 launches: one workgroup per item
     if (b->sched < 0) {  // tune the schedule: each candidate once untimed, once timed
         hipEvent_t e0, e1;
         BATCH_TRY(b, hipEventCreate(&e0));
         BATCH_TRY(b, hipEventCreate(&e1));
         int best = 0;
         for (int v = 0; v < 2; v++) {
-            if ((rc = enqueue(b, d_DM, d_H, d_rho, v))) return rc;
+            if ((rc = enqueue(b, v))) return rc;
             BATCH_TRY(b, hipEventRecord(e0, b->stream));
-            if ((rc = enqueue(b, d_DM, d_H, d_rho, v))) return rc;
+            if ((rc = enqueue(b, v))) return rc;
             BATCH_TRY(b, hipEventRecord(e1, b->stream));
             BATCH_TRY(b, hipEventSynchronize(e1));
             BATCH_TRY(b, hipEventElapsedTime(&b->sched_ms[v], e0, e1));
@@ -219,11 +256,11 @@ int lcpao_batch_assemble(lcpao_batch *b, const double *const *d_V, const double 
         b->sched = best;
     }
     if (!use_graph) {
-        if ((rc = enqueue(b, d_DM, d_H, d_rho, b->sched))) return rc;
+        if ((rc = enqueue(b, b->sched))) return rc;
     } else {
         if (!b->exec) {
             BATCH_TRY(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-            rc = enqueue(b, d_DM, d_H, d_rho, b->sched);
+            rc = enqueue(b, b->sched);
             hipGraph_t g = nullptr;
             const hipError_t e = hipStreamEndCapture(b->stream, &g);
             if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -234,6 +271,20 @@ int lcpao_batch_assemble(lcpao_batch *b, const double *const *d_V, const double 
         BATCH_TRY(b, hipGraphLaunch(b->exec, b->stream));
     }
     BATCH_TRY(b, hipStreamSynchronize(b->stream));
+    return LCPAO_OK;
+}
+
+int lcpao_batch_set_precision(lcpao_batch *b, int32_t precision) {
+    if (!b) return LCPAO_EINVAL;
+    if (precision != LCPAO_BF16X3 && precision != LCPAO_FP32)
+        return berr(b, LCPAO_EINVAL, "the batched form runs LCPAO_BF16X3 or LCPAO_FP32");
+    if (precision != b->precision) {
+        drop_graph(b);
+        b->bound.clear();
+        b->precision = precision;
+        b->sched = precision == LCPAO_FP32 ? 1 : -1;
+        b->sched_ms[0] = b->sched_ms[1] = 0.f;
+    }
     return LCPAO_OK;
 }
 
@@ -269,7 +320,7 @@ void lcpao_batch_destroy(lcpao_batch *b) {
     if (!b) return;
     (void)hipSetDevice(b->device);
     drop_graph(b);
-    void *ptrs[] = {b->d_items, b->d_args_h, b->d_args_r, b->d_queue};
+    void *ptrs[] = {b->d_items, b->d_args_h, b->d_args_r, b->d_queue, b->d_fold, b->d_red, b->d_fill};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);

@@ -182,10 +182,9 @@ __global__ void k_ham_reduce(const int32_t *__restrict__ canon, const int32_t *_
 // sym_self != 0: an on-site pair (its own transpose) gets D_P + D_P^T, the
 // form the block-dense density reads (one load per W element).
 template <typename T>
-__global__ void k_deff(const int32_t *__restrict__ canon, int64_t ncanon, const PairDev *__restrict__ pairs,
-                       const double *__restrict__ D, T *__restrict__ Deff, int sym_self = 0) {
-    const int64_t c = blockIdx.x;
-    if (c >= ncanon) return;
+__device__ __forceinline__ void deff_pair(int64_t c, const int32_t *__restrict__ canon,
+                                          const PairDev *__restrict__ pairs, const double *__restrict__ D,
+                                          T *__restrict__ Deff, int sym_self) {
     const int p = canon[c];
     const PairDev P = pairs[p];
     const int ne = P.noa * P.nob;
@@ -201,6 +200,23 @@ __global__ void k_deff(const int32_t *__restrict__ canon, int64_t ncanon, const 
         }
         Deff[P.hoff + e] = (T)v;
     }
+}
+
+template <typename T>
+__global__ void k_deff(const int32_t *__restrict__ canon, int64_t ncanon, const PairDev *__restrict__ pairs,
+                       const double *__restrict__ D, T *__restrict__ Deff, int sym_self = 0) {
+    const int64_t c = blockIdx.x;
+    if (c >= ncanon) return;
+    deff_pair<T>(c, canon, pairs, D, Deff, sym_self);
+}
+
+// the FP32 folds (on-site blocks symmetrised) of every structure of a batch
+// in one launch: blockIdx.y is the structure, blockIdx.x its computed pair
+__global__ void k_deff_batch(const lcpao::BatchFold *__restrict__ fold) {
+    const lcpao::BatchFold F = fold[blockIdx.y];
+    const int64_t c = blockIdx.x;
+    if (c >= F.ncanon) return;
+    deff_pair<float>(c, F.canon, F.pairs, F.D, F.Deff, 1);
 }
 
 // ---- density: point work of one segment -------------------------------------
@@ -1111,6 +1127,10 @@ int ham_dev(lcpao_ctx *ctx, int nch, const double *const *d_V, double *const *d_
 void lcpao::launch_deff_f64(lcpao_ctx *ctx, const double *D, int sym_self) {
     hipLaunchKernelGGL(k_deff<double>, dim3((unsigned)ctx->ncanon), dim3(128), 0, ctx->stream, ctx->d_hcanon, ctx->ncanon,
                        ctx->d_pairs, D, ctx->d_deff, sym_self);
+}
+void lcpao::launch_deff_f32_batch(hipStream_t stream, int nstruct, const lcpao::BatchFold *d_fold, int max_canon) {
+    if (nstruct > 0 && max_canon > 0)
+        hipLaunchKernelGGL(k_deff_batch, dim3((unsigned)max_canon, (unsigned)nstruct), dim3(128), 0, stream, d_fold);
 }
 void lcpao::launch_deff_f32(lcpao_ctx *ctx, const double *D, int sym_self) {
     hipLaunchKernelGGL(k_deff<float>, dim3((unsigned)ctx->ncanon), dim3(128), 0, ctx->stream, ctx->d_hcanon, ctx->ncanon,

@@ -453,12 +453,10 @@ __device__ __forceinline__ void tile_f32(const float *__restrict__ A32, const fl
     }
 }
 
-__global__ __launch_bounds__(64 * HB_WAVES) void k_hblk_f32(const HItem *__restrict__ items,
-                                                           const float *__restrict__ t32, BlockArgs A) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    const HItem it = items[blockIdx.x];
-    const BlockDev B = A.blocks[it.block];
-    const double *__restrict__ V = A.V;
+__device__ __forceinline__ void hblk_f32_item(const HItem it, const BlockArgs &A, unsigned char *lds_raw) {
+    const BlockDev B = as_global(A.blocks)[it.block];
+    const GLOBAL_AS float *__restrict__ t32 = as_global(A.t32);
+    const GLOBAL_AS double *__restrict__ V = as_global(A.V);
     GLOBAL_AS float *__restrict__ part = as_global(A.part);
     const int n1 = A.n1, n2 = A.n2, n3 = A.n3;
     const int R = B.rows, S = R / ST;
@@ -477,7 +475,7 @@ __global__ __launch_bounds__(64 * HB_WAVES) void k_hblk_f32(const HItem *__restr
     }
     for (int c = 0; c < NCHUNK; c++) {
         __syncthreads();
-        const float *s32 = t32 + B.tile_off + (int64_t)c * R * CHUNK;
+        const GLOBAL_AS float *s32 = t32 + B.tile_off + (int64_t)c * R * CHUNK;
         for (int e = tid * 4; e < R * CHUNK; e += 64 * HB_WAVES * 4) {
             const int row = e / CHUNK, col = e % CHUNK;
             *reinterpret_cast<f32x4 *>(&A32[row * LROW32 + col]) = *reinterpret_cast<const f32x4 *>(&s32[e]);
@@ -497,6 +495,14 @@ __global__ __launch_bounds__(64 * HB_WAVES) void k_hblk_f32(const HItem *__restr
     if (has1) store_partials(acc1, si1, sj1, lane, T, part);
 }
 
+template <int MODE>
+__global__ __launch_bounds__(64 * HB_WAVES) void k_hblk_f32(const HItem *__restrict__ items, int32_t nitems,
+                                                           const BlockArgs *__restrict__ args, BlockArgs one) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    run_items<MODE>(items, nitems, args, one, nullptr,
+                    [&](const HItem &it, const BlockArgs &A) { hblk_f32_item(it, A, lds_raw); });
+}
+
 // per computed pair: FP64 sum of its compact partials over the blocks, in
 // block order (deterministic), x GridVol; writes H[P] and H[P']^T.  The
 // pair's contribution records are staged in LDS so that every thread issues
@@ -504,15 +510,13 @@ __global__ __launch_bounds__(64 * HB_WAVES) void k_hblk_f32(const HItem *__restr
 constexpr int RED_BATCH = 256;
 // One workgroup per canonical pair with one thread per element of its block
 // (the launch's width covers the largest NO_A x NO_B, at most 32 x 32).
-__global__ __launch_bounds__(1024) void k_hblk_reduce(const int32_t *__restrict__ canon, int64_t ncanon,
-                                                      const int32_t *__restrict__ coff,
-                                                      const Contrib *__restrict__ contrib,
-                                                      const PairDev *__restrict__ pairs, const float *__restrict__ part,
-                                                      double gridvol, double *__restrict__ H) {
+__device__ __forceinline__ void hblk_reduce_pair(int64_t ci, const int32_t *__restrict__ canon,
+                                                 const int32_t *__restrict__ coff,
+                                                 const Contrib *__restrict__ contrib,
+                                                 const PairDev *__restrict__ pairs, const float *__restrict__ part,
+                                                 double gridvol, double *__restrict__ H) {
     __shared__ int64_t s_base[RED_BATCH];
     __shared__ int32_t s_tr[RED_BATCH];
-    const int64_t ci = blockIdx.x;
-    if (ci >= ncanon) return;
     const int p = canon[ci];
     const PairDev P = pairs[p];
     const int ne = P.noa * P.nob;
@@ -542,6 +546,31 @@ __global__ __launch_bounds__(1024) void k_hblk_reduce(const int32_t *__restrict_
         H[P.hoff + e] = s;
         if (P.partner >= 0 && P.partner != p) H[pairs[P.partner].hoff + (int64_t)j * P.noa + i] = s;
     }
+}
+
+__global__ __launch_bounds__(1024) void k_hblk_reduce(const int32_t *__restrict__ canon, int64_t ncanon,
+                                                      const int32_t *__restrict__ coff,
+                                                      const Contrib *__restrict__ contrib,
+                                                      const PairDev *__restrict__ pairs, const float *__restrict__ part,
+                                                      double gridvol, double *__restrict__ H) {
+    const int64_t ci = blockIdx.x;
+    if (ci >= ncanon) return;
+    hblk_reduce_pair(ci, canon, coff, contrib, pairs, part, gridvol, H);
+}
+
+// the reductions of every structure of a batch in one launch: blockIdx.y is
+// the structure, blockIdx.x its computed pair (beyond its count: no work)
+__global__ __launch_bounds__(1024) void k_hblk_reduce_batch(const BatchRed *__restrict__ red) {
+    const BatchRed R = red[blockIdx.y];
+    const int64_t ci = blockIdx.x;
+    if (ci >= R.ncanon) return;
+    hblk_reduce_pair(ci, R.canon, R.coff, R.contrib, R.pairs, R.part, R.gridvol, R.H);
+}
+
+__global__ void k_fill_batch(const BatchFill *__restrict__ fill) {
+    const BatchFill F = fill[blockIdx.y];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < F.n) F.x[i] = 0.0;
 }
 
 // ----------------------------------------------------------------------------
@@ -947,11 +976,8 @@ int split_hamiltonian(lcpao_ctx *ctx, int nch, const double *const *d_V, double 
                 SPLIT_TRY(ctx, launch_block_kernel(false, ctx->stream, P->max_rows, 0, P->d_items, (int32_t)ni, nullptr,
                                                    one, nullptr, ni));
             } else {
-                const size_t lds = (size_t)P->max_rows * LROW32 * sizeof(float) + CHUNK * sizeof(float);
-                SPLIT_TRY(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(k_hblk_f32),
-                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                hipLaunchKernelGGL(k_hblk_f32, dim3(ni), dim3(64 * HB_WAVES), lds, ctx->stream, P->d_items, P->d_t32,
-                                   block_args(ctx, d_V[c], nullptr));
+                SPLIT_TRY(ctx, launch_f32_block_kernel(false, ctx->stream, P->max_rows, P->max_slots, P->d_items,
+                                                       (int32_t)ni, nullptr, block_args(ctx, d_V[c], nullptr)));
             }
         }
         if (ctx->ncanon > 0) {
@@ -1010,6 +1036,16 @@ void launch_fill_f64(hipStream_t stream, int64_t n, double *x, double v) {
 }
 void launch_fill_i32(hipStream_t stream, int64_t n, int32_t *x, int32_t v) {
     if (n > 0) hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, n, x, v);
+}
+void launch_hblk_reduce_batch(hipStream_t stream, int nstruct, const BatchRed *d_red, int max_canon, int threads) {
+    if (nstruct > 0 && max_canon > 0)
+        hipLaunchKernelGGL(k_hblk_reduce_batch, dim3((unsigned)max_canon, (unsigned)nstruct), dim3(threads), 0, stream,
+                           d_red);
+}
+void launch_fill_batch(hipStream_t stream, int nstruct, const BatchFill *d_fill, int64_t max_n) {
+    if (nstruct > 0 && max_n > 0)
+        hipLaunchKernelGGL(k_fill_batch, dim3((unsigned)((max_n + 255) / 256), (unsigned)nstruct), dim3(256), 0, stream,
+                           d_fill);
 }
 void launch_hblk_reduce(const lcpao_ctx *ctx, hipStream_t stream, double *H) {
     const BlockPlan *P = ctx->plan;
@@ -1378,13 +1414,13 @@ constexpr int RF_WAVES = 16;
 constexpr int RF_SUMS_OFFSET = MAX_ROWS * LROW32 * 4;
 constexpr int RF_AREA_BYTES = RF_SUMS_OFFSET + RF_WAVES * BLK_PTS * 4;
 
-__device__ __forceinline__ void rblk_f32_item(const HItem it, const BlockArgs &A, const float *__restrict__ t32,
-                                              unsigned char *lds_raw) {
-    const BlockDev B = A.blocks[it.block];
-    const int16_t *__restrict__ rowinfo = A.rowinfo;
-    const SlotPair *__restrict__ spair = A.spair;
-    const float *__restrict__ Deff = A.Deff;
-    double *__restrict__ rho = A.rho;
+__device__ __forceinline__ void rblk_f32_item(const HItem it, const BlockArgs &A, unsigned char *lds_raw) {
+    const BlockDev B = as_global(A.blocks)[it.block];
+    const GLOBAL_AS float *__restrict__ t32 = as_global(A.t32);
+    const GLOBAL_AS int16_t *__restrict__ rowinfo = as_global(A.rowinfo);
+    const GLOBAL_AS SlotPair *__restrict__ spair = as_global(A.spair);
+    const GLOBAL_AS float *__restrict__ Deff = as_global(A.Deff);
+    GLOBAL_AS double *__restrict__ rho = as_global(A.rho);
     const int n1 = A.n1, n2 = A.n2, n3 = A.n3;
     const int R = B.rows, S = R / ST;
     const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63, l16 = lane & 15,
@@ -1426,7 +1462,7 @@ __device__ __forceinline__ void rblk_f32_item(const HItem it, const BlockArgs &A
     }
     for (int c = 0; c < NCHUNK; c++) {
         __syncthreads();  // every wave is done with the previous chunk
-        const float *s32 = t32 + B.tile_off + (int64_t)c * R * CHUNK;
+        const GLOBAL_AS float *s32 = t32 + B.tile_off + (int64_t)c * R * CHUNK;
         for (int e = tid * 4; e < R * CHUNK; e += 64 * RF_WAVES * 4) {
             const int row = e / CHUNK, col = e % CHUNK;
             *reinterpret_cast<f32x4 *>(&A32[row * LROW32 + col]) = *reinterpret_cast<const f32x4 *>(&s32[e]);
@@ -1472,10 +1508,40 @@ __device__ __forceinline__ void rblk_f32_item(const HItem it, const BlockArgs &A
     }
 }
 
+template <int MODE>
 __global__ __launch_bounds__(64 * RF_WAVES) void k_rblk_f32(const HItem *__restrict__ items, int32_t nitems,
-                                                            const float *__restrict__ t32, BlockArgs A) {
+                                                            const BlockArgs *__restrict__ args, BlockArgs one) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    if ((int)blockIdx.x < nitems) rblk_f32_item(items[blockIdx.x], A, t32, lds_raw);
+    run_items<MODE>(items, nitems, args, one, nullptr,
+                    [&](const HItem &it, const BlockArgs &A) { rblk_f32_item(it, A, lds_raw); });
+}
+
+hipError_t launch_f32_block_kernel(bool density, hipStream_t stream, int rows, int max_slots, const HItem *items,
+                                   int32_t nitems, const BlockArgs *args, const BlockArgs &one) {
+    if (nitems <= 0) return hipSuccess;
+    const bool batch = args != nullptr;
+    const void *kern;
+    size_t lds;
+    if (density) {
+        kern = batch ? reinterpret_cast<const void *>(k_rblk_f32<LAUNCH_BATCH>)
+                     : reinterpret_cast<const void *>(k_rblk_f32<LAUNCH_ONE>);
+        lds = RF_AREA_BYTES + MAX_ROWS * sizeof(int16_t) + (size_t)max_slots * max_slots * sizeof(SlotPair);
+    } else {
+        kern = batch ? reinterpret_cast<const void *>(k_hblk_f32<LAUNCH_BATCH>)
+                     : reinterpret_cast<const void *>(k_hblk_f32<LAUNCH_ONE>);
+        lds = (size_t)rows * LROW32 * sizeof(float) + CHUNK * sizeof(float);
+    }
+    hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    const unsigned threads = density ? 64 * RF_WAVES : 64 * HB_WAVES;
+    if (density) {
+        if (batch) hipLaunchKernelGGL(k_rblk_f32<LAUNCH_BATCH>, dim3(nitems), dim3(threads), lds, stream, items, nitems, args, one);
+        else hipLaunchKernelGGL(k_rblk_f32<LAUNCH_ONE>, dim3(nitems), dim3(threads), lds, stream, items, nitems, args, one);
+    } else {
+        if (batch) hipLaunchKernelGGL(k_hblk_f32<LAUNCH_BATCH>, dim3(nitems), dim3(threads), lds, stream, items, nitems, args, one);
+        else hipLaunchKernelGGL(k_hblk_f32<LAUNCH_ONE>, dim3(nitems), dim3(threads), lds, stream, items, nitems, args, one);
+    }
+    return hipGetLastError();
 }
 
 // fragments per wave and plane of the density kernel for R rows
@@ -1504,7 +1570,7 @@ BlockArgs block_args(const lcpao_ctx *ctx, const double *V, double *rho) {
     a.hi = P->d_hi; a.lo = P->d_lo; a.thi = P->d_thi; a.tlo = P->d_tlo;
     a.rowinfo = P->d_rowinfo; a.spair = P->d_spair;
     a.V = V; a.Deff = ctx->d_deff32; a.part = P->d_part; a.rho = rho;
-    a.cbase = P->d_cbase; a.slotno = P->d_slotno;
+    a.cbase = P->d_cbase; a.slotno = P->d_slotno; a.t32 = P->d_t32;
     a.n1 = ctx->ngrid[0]; a.n2 = ctx->ngrid[1]; a.n3 = ctx->ngrid[2];
     return a;
 }
@@ -1629,10 +1695,6 @@ int split_density_f32_enqueue(lcpao_ctx *ctx, int nspin, const double *const *d_
                               double *const *d_rho) {
     BlockPlan *P = ctx->plan;
     if (!P || !P->f32_ready) { ctx->err = "FP32 block tiles are not built"; return LCPAO_ESTATE; }
-    const size_t lds =
-        RF_AREA_BYTES + MAX_ROWS * sizeof(int16_t) + (size_t)P->max_slots * P->max_slots * sizeof(SlotPair);
-    SPLIT_TRY(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(k_rblk_f32),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     for (int s = 0; s < nspin; s++) {
         if (!d_DM[s] || !d_rho[s]) { ctx->err = "null spin pointer"; return LCPAO_EINVAL; }
         if (ctx->ncanon > 0) {
@@ -1645,8 +1707,8 @@ int split_density_f32_enqueue(lcpao_ctx *ctx, int nspin, const double *const *d_
         if (!P->items.empty()) {
             KTimer kt(ctx, KID_RHO);
             const unsigned ni = (unsigned)P->items.size();
-            hipLaunchKernelGGL(k_rblk_f32, dim3(ni), dim3(64 * RF_WAVES), lds, ctx->stream, P->d_items, (int32_t)ni,
-                               P->d_t32, block_args(ctx, nullptr, d_rho[s]));
+            SPLIT_TRY(ctx, launch_f32_block_kernel(true, ctx->stream, P->max_rows, P->max_slots, P->d_items,
+                                                   (int32_t)ni, nullptr, block_args(ctx, nullptr, d_rho[s])));
         }
         SPLIT_TRY(ctx, hipGetLastError());
     }

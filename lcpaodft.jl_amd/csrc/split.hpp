@@ -68,6 +68,7 @@ struct BlockArgs {
     double *rho;                   // density: output
     const int32_t *cbase;          // H: compact partial slot of each slot pair (-1: none)
     const int16_t *slotno;         // H: orbital count of each slot
+    const float *t32;              // FP32 tiles (the FP32 forms)
     int32_t n1, n2, n3, pad;
 };
 
@@ -149,6 +150,36 @@ void launch_hblk_reduce(const lcpao_ctx *ctx, hipStream_t stream, double *H);
 hipError_t launch_block_kernel(bool density, hipStream_t stream, int rows, int max_slots, const HItem *items,
                                int32_t nitems, const BlockArgs *args, const BlockArgs &one, int32_t *queue,
                                unsigned grid, bool db = false);
+// FP32 block kernels (H on the FP32 matrix cores, and the FP32 block
+// density): args != nullptr launches the merged items of a batch (item.pad
+// indexes args), else one structure (`one`); one workgroup per item
+hipError_t launch_f32_block_kernel(bool density, hipStream_t stream, int rows, int max_slots, const HItem *items,
+                                   int32_t nitems, const BlockArgs *args, const BlockArgs &one);
+// batched fills and reductions of a batch (one launch each for all structures)
+struct BatchRed {
+    const int32_t *canon;   // computed pairs of the structure
+    const int32_t *coff;    // contribution offsets per computed pair
+    const Contrib *contrib;
+    const PairDev *pairs;
+    const float *part;
+    double *H;
+    double gridvol;
+    int32_t ncanon, pad;
+};
+struct BatchFill {
+    double *x;
+    int64_t n;
+};
+struct BatchFold {
+    const int32_t *canon;
+    const PairDev *pairs;
+    const double *D;        // the structure's flat DM
+    float *Deff;            // its FP32 fold
+    int32_t ncanon, pad;
+};
+void launch_deff_f32_batch(hipStream_t stream, int nstruct, const BatchFold *d_fold, int max_canon);
+void launch_hblk_reduce_batch(hipStream_t stream, int nstruct, const BatchRed *d_red, int max_canon, int threads);
+void launch_fill_batch(hipStream_t stream, int nstruct, const BatchFill *d_fill, int64_t max_n);
 void plan_free(BlockPlan *p);
 int split_tables(lcpao_ctx *ctx, int precision);
 // FP32 density in the block form (needs split_tables(ctx, LCPAO_FP32));

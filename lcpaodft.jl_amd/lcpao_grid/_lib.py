@@ -48,7 +48,7 @@ EXPORTS = [
     "lcpao_grid_profile", "lcpao_grid_kernel_stats", "lcpao_grid_tune", "lcpao_grid_tuned", "lcpao_grid_sizes",
     "lcpao_grid_last_error", "lcpao_grid_destroy", "lcpao_ucell_build", "lcpao_ucell_scalars",
     "lcpao_ucell_copy", "lcpao_ucell_last_error", "lcpao_ucell_destroy",
-    "lcpao_batch_create", "lcpao_batch_assemble", "lcpao_batch_tune", "lcpao_batch_tuned", "lcpao_batch_sizes",
+    "lcpao_batch_create", "lcpao_batch_assemble", "lcpao_batch_set_precision", "lcpao_batch_tune", "lcpao_batch_tuned", "lcpao_batch_sizes",
     "lcpao_batch_last_error", "lcpao_batch_destroy",
     "lcpao_grid_aden_pcc", "lcpao_grid_aden_pcc_dev", "lcpao_grid_poisson", "lcpao_grid_poisson_dev",
     "lcpao_grid_bloch_sum", "lcpao_grid_bloch_sum_dev", "lcpao_grid_bloch_sizes", "lcpao_grid_dm_states",
@@ -92,6 +92,7 @@ def load():
     L.lcpao_batch_create.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(vp)]
     L.lcpao_batch_assemble.argtypes = [vp, dpp, dpp, dpp, dpp, C.c_int32]
     L.lcpao_batch_tune.argtypes = [vp, C.c_int32]
+    L.lcpao_batch_set_precision.argtypes = [vp, C.c_int32]
     L.lcpao_batch_tuned.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_float)]
     L.lcpao_batch_sizes.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), lp]
     L.lcpao_batch_last_error.argtypes = [vp]

@@ -1,6 +1,7 @@
-"""Batched split-BF16x3 grid H + density over many structures (C ABI
-lcpao_batch_*): one merged, persistent launch per row class for all
-structures, optionally replayed from a captured HIP graph."""
+"""Batched split-BF16x3 or FP32 grid H + density over many structures (C ABI
+lcpao_batch_*): one merged launch per operation for all structures
+(persistent or one workgroup per item), optionally replayed from a captured
+HIP graph."""
 import ctypes as C
 
 from . import _lib
@@ -37,6 +38,10 @@ class Batch:
         self.contexts[0]._order()  # the batch stream does not follow torch's stream
         self._check(self.L.lcpao_batch_assemble(self.h, self._ptrs(dV), self._ptrs(dDM), self._ptrs(dH),
                                                 self._ptrs(drho), int(bool(use_graph))))
+
+    def set_precision(self, precision="bf16x3"):
+        """"bf16x3" (default) or "fp32": the form of the next assemble calls."""
+        self._check(self.L.lcpao_batch_set_precision(self.h, _lib.PRECISIONS[precision]))
 
     def tune(self, sched=-1):
         """0: persistent work queue, 1: one workgroup per item, -1: time both at the next call."""

@@ -115,3 +115,27 @@ def test_batch_follows_table_updates(small_batch):
     s0["ctx"].set_tables(tabs)
     _run(b, out, use_graph=True)
     _check(out, ref)
+
+
+def test_fp32_batch_matches_per_structure(small_batch):
+    """The FP32 batch (merged items of the FP32 block kernels) against the
+    per-structure FP32 block form: H from the same partials in the same
+    order (bit for bit), rho up to the order of its FP64 atomic adds; two
+    graph replays and a switch back to bf16x3."""
+    out, ref = small_batch
+    ref32 = []
+    for s in out:
+        s["ctx"].tune("rho32", 1)
+        s["ctx"].hamiltonian_dev([s["V"]], [s["H"]], "fp32")
+        s["ctx"].density_dev([s["DM"]], [s["rho"]], 0, "fp32")
+        ref32.append((s["H"].cpu().numpy().copy(), s["rho"].cpu().numpy().copy()))
+    b = Batch([s["ctx"] for s in out])
+    b.set_precision("fp32")
+    for use_graph in (False, True, True):
+        _run(b, out, use_graph=use_graph)
+        for s, (H0, R0) in zip(out, ref32):
+            assert np.array_equal(s["H"].cpu().numpy(), H0), s["st"].name
+            assert np.abs(s["rho"].cpu().numpy() - R0).max() <= 1e-13 * np.abs(R0).max(), s["st"].name
+    b.set_precision("bf16x3")
+    _run(b, out, use_graph=True)
+    _check(out, ref)
