@@ -795,7 +795,9 @@ def main():
     flops = f_ham if dom == "ham" else f_rho
     achieved = flops / (stats[dom]["avg_ms"] * 1e-3)
     # the kernel the autotuner picked for this form and operation
-    rho_kernels = {"fp64": ("k_rho_m64", "k_rho<double>"), "fp32": ("k_rho<float>", "k_rblk_f32"),
+    rho_kernels = {"fp64": ("k_rho_m64", "k_rho<double, 4, 2048>", "k_rho<double, 4, 1024>", "k_rho<double, 8, 512>",
+                            "k_rho<double, 2, 1024>", "k_rho<double, 4, 512>"),
+                   "fp32": ("k_rho<float, 4, 2048>", "k_rblk_f32", "k_rho<float, 4, 1024>", "k_rho<float, 8, 512>"),
                    "bf16x3": ("k_rblk_bf", "k_rblk_bf")}
     if dom == "ham":
         kern = {"fp64": "k_ham_mfma64", "fp32": "k_hblk_f32", "bf16x3": "k_hblk_bf"}[args.precision]

@@ -199,17 +199,22 @@ int lcpao_grid_profile(lcpao_ctx *ctx, int32_t enable);
 int lcpao_grid_kernel_stats(const lcpao_ctx *ctx, int32_t kernel_id, int64_t *launches, double *total_ms);
 
 /* Autotuner (no counterpart in the reference).  Some contractions have
- * several kernels that give the same result within the form's tolerance;
- * by default (variant -1) the first call on a context times every candidate
- * once on the context's own geometry and keeps the fastest.  op:
- *   LCPAO_TUNE_RHO64  FP64 density  (0 matrix-core kernel, 1 vector-FMA kernel)
+ * several kernels or work-unit shapes that give the same result within the
+ * form's tolerance; by default (variant -1) the first call on a context
+ * times every candidate once on the context's own geometry and keeps the
+ * fastest (a candidate that cannot serve the context is not run and reads
+ * -1 ms).  op:
+ *   LCPAO_TUNE_RHO64  FP64 density  (0 matrix-core kernel; 1..5 vector-FMA
+ *                     kernel with units of (waves per workgroup x grid-list
+ *                     entries) 4 x 2048, 4 x 1024, 8 x 512, 2 x 1024, 4 x 512)
  *   LCPAO_TUNE_F364   FP64 Force3   (0 matrix-core kernel, 1 vector-FMA kernel)
  *   LCPAO_TUNE_RHOBF  bf16x3 density (0 single-buffered chunk planes,
  *                     1 double-buffered, used only when its LDS fits;
  *                     both give the same bits)
- *   LCPAO_TUNE_RHO32  FP32 density  (0 pair-list kernel, 1 block form on the
- *                     FP32 matrix cores, used only when the block plan holds
- *                     the geometry)
+ *   LCPAO_TUNE_RHO32  FP32 density  (0 pair-list kernel 4 x 2048, 1 block
+ *                     form on the FP32 matrix cores, used only when the block
+ *                     plan holds the geometry, 2 pair lists 4 x 1024, 3 pair
+ *                     lists 8 x 512)
  * lcpao_grid_tune fixes a variant (or -1 to re-tune at the next call);
  * lcpao_grid_tuned reports the choice and, when candidate_ms is not NULL,
  * the measured milliseconds of up to ncand candidates. */

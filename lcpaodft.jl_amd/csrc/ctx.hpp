@@ -11,7 +11,7 @@
 #include "lcpao_grid.h"
 
 #define UNIT_PTS 2048     // overlap points per H / Force3 unit
-#define RHO_CHUNK 2048    // grid-list entries per density unit
+#define RHO_CHUNK 2048    // grid-list entries per density unit (the largest decomposition)
 #define HK 32             // points per LDS chunk in the H kernel
 // largest orbital count per atom: Read_PAO's basis string (Read_PAO.jl:49-76)
 // takes one decimal digit of multiplicity for each of s, p, d, f, so at
@@ -23,6 +23,12 @@
 namespace lcpao {
 
 struct HUnit { int32_t pair, lo, n, pad; int64_t part; };
+// Density work-unit decompositions: units of `chunk` consecutive grid-list
+// entries of a home atom (2048, 1024 or 512), each holding the segments of
+// the atom's computed pairs inside it.  The pair-list density kernels run on
+// one of them with a number of waves per workgroup (autotuned).
+constexpr int RHO_NDECOMP = 3;
+constexpr int RHO_DECOMP_CHUNK[RHO_NDECOMP] = {2048, 1024, 512};
 struct RhoUnit { int32_t atom, t0, nt, seg_beg, seg_end, pad; };
 struct Seg { int32_t pair, lo, hi, pad; };
 struct F3Unit { int32_t pair, lo, n, pad; };
@@ -89,11 +95,13 @@ struct lcpao_ctx {
     int32_t *d_hcanon = nullptr;
     int64_t ncanon = 0, hpart_size = 0;
     double *d_hpart = nullptr;
-    // rho
-    std::vector<RhoUnit> runits;
-    RhoUnit *d_runits = nullptr;
-    int32_t *d_rorder = nullptr;
-    Seg *d_segs = nullptr;
+    // rho: the work-unit decompositions (RHO_DECOMP_CHUNK)
+    struct RhoDecomp {
+        int64_t nunits = 0;
+        lcpao::RhoUnit *d_units = nullptr;
+        int32_t *d_order = nullptr;
+        lcpao::Seg *d_segs = nullptr;
+    } rdec[lcpao::RHO_NDECOMP];
     double *d_deff = nullptr, *d_rhoA = nullptr;
     int64_t *d_ent_off = nullptr;
     int64_t *d_ent = nullptr;
@@ -131,14 +139,19 @@ struct lcpao_ctx {
     // kernel variants chosen by the autotuner (-1: time the candidates on the
     // first call with this geometry), see lcpao_grid_tune
     int32_t variant[4] = {-1, -1, -1, -1};
-    float variant_ms[4][4] = {};
+    float variant_ms[4][8] = {};
     int64_t prof_launches[KID_COUNT] = {};
     double prof_ms[KID_COUNT] = {};
 };
 
 // tunable operations and their candidate counts
 enum { TUNE_RHO64 = 0, TUNE_F364 = 1, TUNE_RHOBF = 2, TUNE_RHO32 = 3, TUNE_NOPS = 4 };
-constexpr int TUNE_NCAND[TUNE_NOPS] = {2, 2, 2, 2};
+// FP64 density: 0 matrix cores, 1..5 vector FMA with unit shapes
+// (waves x chunk) 4 x 2048, 4 x 1024, 8 x 512, 2 x 1024, 4 x 512;
+// FP32 density: 0 pair lists 4 x 2048, 1 block form, 2 pair lists 4 x 1024,
+// 3 pair lists 8 x 512
+constexpr int TUNE_NCAND[TUNE_NOPS] = {6, 2, 2, 4};
+constexpr int TUNE_MAXCAND = 8;
 
 
 

@@ -278,21 +278,23 @@ __device__ void rho_points_any(const T *__restrict__ D, const T *__restrict__ ta
 
 #define NO_CASES(X) X(13, 13) X(13, 14) X(14, 13) X(14, 14) X(19, 19) X(13, 19) X(19, 13) X(14, 19) X(19, 14)
 
-// Density per (atom, chunk) unit; T = table/D element type (double or float).
-template <typename T>
-__global__ __launch_bounds__(64 * RHO_WAVES) void k_rho(
+// Density per (atom, chunk) unit; T = table/D element type (double or float),
+// W waves per workgroup, CH grid-list entries per unit (the unit shape: W
+// per-wave accumulators of CH doubles in LDS).
+template <typename T, int W, int CH>
+__global__ __launch_bounds__(64 * W) void k_rho(
     const RhoUnit *__restrict__ units, const int32_t *__restrict__ order, int32_t nunits,
     const Seg *__restrict__ segs, const PairDev *__restrict__ pairs, const T *__restrict__ tab,
     const int32_t *__restrict__ olg1, const int32_t *__restrict__ olg2, const T *__restrict__ Deff,
     const int64_t *__restrict__ gl_off, double *__restrict__ rhoA) {
-    __shared__ double racc[RHO_WAVES][RHO_CHUNK];
+    __shared__ double racc[W][CH];
     const int b = blockIdx.x;
     if (b >= nunits) return;
     const RhoUnit u = units[order[b]];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int t = threadIdx.x; t < RHO_WAVES * RHO_CHUNK; t += blockDim.x) (&racc[0][0])[t] = 0.0;
+    for (int t = threadIdx.x; t < W * CH; t += blockDim.x) (&racc[0][0])[t] = 0.0;
     __syncthreads();
-    for (int s = u.seg_beg + wave; s < u.seg_end; s += RHO_WAVES) {
+    for (int s = u.seg_beg + wave; s < u.seg_end; s += W) {
         const Seg sg = segs[s];
         const PairDev P = pairs[__builtin_amdgcn_readfirstlane(sg.pair)];
         const int noa = __builtin_amdgcn_readfirstlane(P.noa), nob = __builtin_amdgcn_readfirstlane(P.nob);
@@ -313,7 +315,7 @@ __global__ __launch_bounds__(64 * RHO_WAVES) void k_rho(
     for (int t = threadIdx.x; t < u.nt; t += blockDim.x) {
         double acc = racc[0][t];
 #pragma unroll
-        for (int w = 1; w < RHO_WAVES; w++) acc += racc[w][t];
+        for (int w = 1; w < W; w++) acc += racc[w][t];
         out[t] = acc;
     }
 }
@@ -392,7 +394,7 @@ __global__ __launch_bounds__(64 * RHO_WAVES) void k_rho_m64(
     const Seg *__restrict__ segs, const PairDev *__restrict__ pairs, const double *__restrict__ tab,
     const int32_t *__restrict__ olg1, const int32_t *__restrict__ olg2, const double *__restrict__ Deff,
     const int64_t *__restrict__ gl_off, double *__restrict__ rhoA) {
-    __shared__ double racc[RHO_WAVES][RHO_CHUNK];
+    __shared__ double racc[RHO_WAVES][RHO_CHUNK];  // the 2048-entry decomposition
     const int b = blockIdx.x;
     if (b >= nunits) return;
     const RhoUnit u = units[order[b]];
@@ -1013,33 +1015,43 @@ int build_work(lcpao_ctx *ctx, const lcpao_geom *g, const std::vector<int32_t> &
     }
     for (int a = 0; a < ctx->natom; a++) aub[a + 1] += aub[a];
     auto forder = launch_order(ctx->funits.size(), [&](size_t i) { return first_g(ctx->funits[i].pair, ctx->funits[i].lo); });
-    // density units: (atom, chunk) holding the segments of its computed pairs
-    ctx->runits.clear();
-    std::vector<Seg> segs;
+    // density units of each decomposition: (atom, chunk) holding the
+    // segments of its computed pairs
     std::vector<std::vector<int32_t>> apairs(ctx->natom);
     for (int p : canon) apairs[ctx->pairs[p].a].push_back(p);
-    for (int a = 0; a < ctx->natom; a++) {
-        std::vector<int> cur(apairs[a].size(), 0);
-        for (int t0 = 0; t0 < ctx->gridn[a]; t0 += RHO_CHUNK) {
-            RhoUnit u;
-            u.atom = a; u.t0 = t0; u.nt = std::min(RHO_CHUNK, ctx->gridn[a] - t0);
-            u.seg_beg = (int32_t)segs.size(); u.pad = 0;
-            for (size_t k = 0; k < apairs[a].size(); k++) {
-                const int p = apairs[a][k];
-                const int32_t *L1 = o1.data() + ctx->pairs[p].olg;
-                const int n = g->numolg[p];
-                const int lo = cur[k];
-                int hi = lo;
-                while (hi < n && L1[hi] < t0 + u.nt) hi++;
-                if (hi > lo) segs.push_back({p, lo, hi, 0});
-                cur[k] = hi;
-            }
-            u.seg_end = (int32_t)segs.size();
-            ctx->runits.push_back(u);
-        }
-    }
-    auto rorder = launch_order(ctx->runits.size(), [&](size_t i) { return (int64_t)glist_h[ctx->gl_off[ctx->runits[i].atom] + ctx->runits[i].t0]; });
     int rc;
+    for (int d = 0; d < RHO_NDECOMP; d++) {
+        const int CH = RHO_DECOMP_CHUNK[d];
+        std::vector<RhoUnit> runits;
+        std::vector<Seg> segs;
+        for (int a = 0; a < ctx->natom; a++) {
+            std::vector<int> cur(apairs[a].size(), 0);
+            for (int t0 = 0; t0 < ctx->gridn[a]; t0 += CH) {
+                RhoUnit u;
+                u.atom = a; u.t0 = t0; u.nt = std::min(CH, ctx->gridn[a] - t0);
+                u.seg_beg = (int32_t)segs.size(); u.pad = 0;
+                for (size_t k = 0; k < apairs[a].size(); k++) {
+                    const int p = apairs[a][k];
+                    const int32_t *L1 = o1.data() + ctx->pairs[p].olg;
+                    const int n = g->numolg[p];
+                    const int lo = cur[k];
+                    int hi = lo;
+                    while (hi < n && L1[hi] < t0 + u.nt) hi++;
+                    if (hi > lo) segs.push_back({p, lo, hi, 0});
+                    cur[k] = hi;
+                }
+                u.seg_end = (int32_t)segs.size();
+                runits.push_back(u);
+            }
+        }
+        if (segs.size() > (size_t)INT32_MAX) return set_err(ctx, LCPAO_ERANGE, "too many density segments");
+        auto rorder = launch_order(runits.size(), [&](size_t i) { return (int64_t)glist_h[ctx->gl_off[runits[i].atom] + runits[i].t0]; });
+        auto &D = ctx->rdec[d];
+        D.nunits = (int64_t)runits.size();
+        if ((rc = upload(ctx, &D.d_units, runits))) return rc;
+        if ((rc = upload(ctx, &D.d_order, rorder))) return rc;
+        if ((rc = upload(ctx, &D.d_segs, segs))) return rc;
+    }
     if ((rc = upload(ctx, &ctx->d_hunits, ctx->hunits))) return rc;
     if ((rc = upload(ctx, &ctx->d_horder, horder))) return rc;
     if ((rc = upload(ctx, &ctx->d_hcanon, canon))) return rc;
@@ -1047,9 +1059,6 @@ int build_work(lcpao_ctx *ctx, const lcpao_geom *g, const std::vector<int32_t> &
     if ((rc = upload(ctx, &ctx->d_funits, ctx->funits))) return rc;
     if ((rc = upload(ctx, &ctx->d_forder, forder))) return rc;
     if ((rc = upload(ctx, &ctx->d_atom_ub, aub))) return rc;
-    if ((rc = upload(ctx, &ctx->d_runits, ctx->runits))) return rc;
-    if ((rc = upload(ctx, &ctx->d_rorder, rorder))) return rc;
-    if ((rc = upload(ctx, &ctx->d_segs, segs))) return rc;
     if ((rc = dev_alloc(ctx, (void **)&ctx->d_hpart, (size_t)std::max<int64_t>(1, part) * sizeof(double)))) return rc;
     if ((rc = dev_alloc(ctx, (void **)&ctx->d_fpart, (size_t)std::max<size_t>(1, ctx->funits.size()) * 3 * sizeof(double)))) return rc;
     if ((rc = dev_alloc(ctx, (void **)&ctx->d_fsum, (size_t)ctx->natom * 3 * sizeof(double)))) return rc;
@@ -1142,15 +1151,17 @@ namespace {
 // Runs `launch(variant)` once per candidate (after one untimed run) with
 // events on the context stream and keeps the fastest; the caller's real
 // launch follows, so every candidate writes only scratch that it overwrites.
-template <class F>
-int autotune(lcpao_ctx *ctx, int op, F launch) {
+// Candidates with available(v) false are not run (their time reads -1).
+template <class F, class A>
+int autotune(lcpao_ctx *ctx, int op, F launch, A available) {
     if (ctx->variant[op] >= 0) return LCPAO_OK;
     hipEvent_t e0, e1;
     HIP_TRY(ctx, hipEventCreate(&e0));
     HIP_TRY(ctx, hipEventCreate(&e1));
-    int best = 0;
+    int best = -1;
     float best_ms = 0.f;
     for (int v = 0; v < TUNE_NCAND[op]; v++) {
+        if (!available(v)) { ctx->variant_ms[op][v] = -1.f; continue; }
         launch(v);
         HIP_TRY(ctx, hipEventRecord(e0, ctx->stream));
         launch(v);
@@ -1159,7 +1170,7 @@ int autotune(lcpao_ctx *ctx, int op, F launch) {
         float ms = 0.f;
         HIP_TRY(ctx, hipEventElapsedTime(&ms, e0, e1));
         ctx->variant_ms[op][v] = ms;
-        if (v == 0 || ms < best_ms) { best = v; best_ms = ms; }
+        if (best < 0 || ms < best_ms) { best = v; best_ms = ms; }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -1169,7 +1180,7 @@ int autotune(lcpao_ctx *ctx, int op, F launch) {
 }
 
 int rho_dev_pairs(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_offset, double *const *d_rho,
-                  bool f32);
+                  bool f32, int variant);
 
 int rho_dev(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_offset, double *const *d_rho, int precision) {
     if (!ctx->tables) return set_err(ctx, LCPAO_ESTATE, "orbital tables are not set");
@@ -1188,62 +1199,75 @@ int rho_dev(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_off
     const bool f32 = precision != LCPAO_FP64;
     for (int s = 0; s < nspin; s++)
         if (!d_DM[s] || !d_rho[s]) return set_err(ctx, LCPAO_EINVAL, "null spin pointer");
-    // FP32 candidates: 0 = pair-list kernel (k_rho<float>), 1 = block form on
-    // the FP32 matrix cores (k_rblk_f32).  With the variant open, a block
-    // form that cannot be set up (a geometry the block plan cannot hold, no
-    // cell lists, or no memory for its tiles) leaves the pair-list kernel;
-    // a variant fixed to 1 reports the failure.
-    if (f32 && ctx->variant[TUNE_RHO32] != 0) {
+    const int op = f32 ? TUNE_RHO32 : TUNE_RHO64;
+    // candidates that cannot serve this context: the FP32 block form when the
+    // block plan cannot be set up (a geometry beyond it, no cell lists, no
+    // memory for its tiles), the FP64 matrix-core kernel beyond 32 orbitals
+    bool block_ok = false;
+    if (f32 && ctx->variant[TUNE_RHO32] != 0 && ctx->variant[TUNE_RHO32] != 2 && ctx->variant[TUNE_RHO32] != 3) {
         const int r2 = split_tables(ctx, LCPAO_FP32);
-        if (r2 && (ctx->variant[TUNE_RHO32] < 0 || ctx->block_state == 0)) {
-            ctx->variant[TUNE_RHO32] = 0;
+        if (r2 == LCPAO_OK) {
+            block_ok = true;
+        } else if (ctx->variant[TUNE_RHO32] == 1 && ctx->block_state != 0) {
+            return r2;  // fixed to the block form, which failed
+        } else {
+            if (ctx->variant[TUNE_RHO32] == 1) ctx->variant[TUNE_RHO32] = 0;  // beyond the block limits
             ctx->err.clear();
             (void)hipGetLastError();  // a failed allocation leaves no sticky state behind
-        } else if (r2) {
-            return r2;
         }
     }
-    if (f32 && ctx->variant[TUNE_RHO32] == 1) {
-        if ((rc = split_density_f32_enqueue(ctx, nspin, d_DM, dm_offset, d_rho))) return rc;
-        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-        collect_timings(ctx);
-        return LCPAO_OK;
-    }
-    if (f32 && (rc = fp32_tables(ctx, false))) return rc;
-    if (f32 && ctx->variant[TUNE_RHO32] < 0) {
-        // both candidates overwrite rho; tuned with every spin's work
+    const bool m64_ok = ctx->maxno <= MFMA64_NO;
+    if (!f32 && ctx->variant[TUNE_RHO64] == 0 && !m64_ok)
+        return set_err(ctx, LCPAO_ERANGE, "the FP64 matrix-core density kernel holds at most 32 orbitals per atom");
+    auto available = [&](int v) { return f32 ? (v != 1 || block_ok) : (v != 0 || m64_ok); };
+    auto run = [&](int v) -> int {
+        if (f32 && v == 1) return split_density_f32_enqueue(ctx, nspin, d_DM, dm_offset, d_rho);
+        return rho_dev_pairs(ctx, nspin, d_DM, dm_offset, d_rho, f32, v);
+    };
+    if (f32 && (ctx->variant[op] != 1) && (rc = fp32_tables(ctx, false))) return rc;
+    if (ctx->variant[op] < 0) {
+        // every candidate overwrites rho; tuned with every spin's work
         int trc = LCPAO_OK;
         auto cand = [&](int v) {
-            if (trc) return;
-            trc = v == 0 ? rho_dev_pairs(ctx, nspin, d_DM, dm_offset, d_rho, true)
-                         : split_density_f32_enqueue(ctx, nspin, d_DM, dm_offset, d_rho);
+            if (!trc) trc = run(v);
         };
-        if ((rc = autotune(ctx, TUNE_RHO32, cand))) return rc;
+        if ((rc = autotune(ctx, op, cand, available))) return rc;
         if (trc) return trc;
-        if (ctx->variant[TUNE_RHO32] == 1) {
-            // the pair-list candidate's FP32 table copy is no longer needed
+        if (f32 && ctx->variant[op] == 1) {
+            // the pair-list candidates' FP32 table copy is no longer needed
             // (fp32_tables makes it again for the FP32 Force3)
             HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
             if (ctx->d_tab32) { (void)hipFree(ctx->d_tab32); ctx->d_tab32 = nullptr; }
-            if ((rc = split_density_f32_enqueue(ctx, nspin, d_DM, dm_offset, d_rho))) return rc;
-            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-            collect_timings(ctx);
-            return LCPAO_OK;
         }
     }
-    if ((rc = rho_dev_pairs(ctx, nspin, d_DM, dm_offset, d_rho, f32))) return rc;
+    if ((rc = run(ctx->variant[op]))) return rc;
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     collect_timings(ctx);
     return LCPAO_OK;
 }
 
-// The pair-list density kernels (FP64 candidates or the FP32 one) for every
-// spin: fold, contraction into the per-entry scratch, gather into rho
-// (enqueue only).
+// The pair-list density kernels (FP64 candidates 0 and 2..5 or the FP32
+// ones 0, 2, 3) for every spin: fold, contraction into the per-entry
+// scratch, gather into rho (enqueue only).
 int rho_dev_pairs(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_offset, double *const *d_rho,
-                  bool f32) {
-    int rc = LCPAO_OK;
-    const int nu = (int)ctx->runits.size();
+                  bool f32, int variant) {
+    // (kernel, waves, decomposition) of the variant
+    int kind = 1, waves = 4, dec = 0;  // kind 0: matrix cores, 1: vector FMA
+    if (f32) {
+        if (variant == 2) { dec = 1; }
+        else if (variant == 3) { waves = 8; dec = 2; }
+    } else {
+        switch (variant) {
+            case 0: kind = 0; break;
+            case 2: dec = 1; break;
+            case 3: waves = 8; dec = 2; break;
+            case 4: waves = 2; dec = 1; break;
+            case 5: dec = 2; break;
+            default: break;
+        }
+    }
+    const auto &R = ctx->rdec[dec];
+    const int nu = (int)R.nunits;
     for (int s = 0; s < nspin; s++) {
         const double *D = d_DM[s] + dm_offset;
         if (ctx->ncanon > 0) {
@@ -1255,30 +1279,36 @@ int rho_dev_pairs(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t 
                 hipLaunchKernelGGL(k_deff<double>, dim3((unsigned)ctx->ncanon), dim3(128), 0, ctx->stream, ctx->d_hcanon,
                                    ctx->ncanon, ctx->d_pairs, D, ctx->d_deff);
         }
-        // FP64 candidates: 0 = matrix cores (k_rho_m64), 1 = vector FMA (k_rho<double>)
-        auto rho64 = [&](int v) {
-            if (v == 0)
-                hipLaunchKernelGGL(k_rho_m64, dim3(nu), dim3(64 * RHO_WAVES), 0, ctx->stream, ctx->d_runits,
-                                   ctx->d_rorder, nu, ctx->d_segs, ctx->d_pairs, ctx->d_tab, ctx->d_olg1, ctx->d_olg2,
-                                   ctx->d_deff, ctx->d_gl_off, ctx->d_rhoA);
-            else
-                hipLaunchKernelGGL(k_rho<double>, dim3(nu), dim3(64 * RHO_WAVES), 0, ctx->stream, ctx->d_runits,
-                                   ctx->d_rorder, nu, ctx->d_segs, ctx->d_pairs, ctx->d_tab, ctx->d_olg1, ctx->d_olg2,
-                                   ctx->d_deff, ctx->d_gl_off, ctx->d_rhoA);
-        };
-        if (ctx->maxno > MFMA64_NO) {  // the matrix-core candidate holds at most 32 orbitals
-            if (ctx->variant[TUNE_RHO64] == 0) return set_err(ctx, LCPAO_ERANGE, "the FP64 matrix-core density kernel holds at most 32 orbitals per atom");
-            ctx->variant[TUNE_RHO64] = 1;
-        }
-        if (nu > 0 && !f32 && (rc = autotune(ctx, TUNE_RHO64, rho64))) return rc;
         if (nu > 0) {
             KTimer kt(ctx, KID_RHO);
-            if (f32)
-                hipLaunchKernelGGL(k_rho<float>, dim3(nu), dim3(64 * RHO_WAVES), 0, ctx->stream, ctx->d_runits,
-                                   ctx->d_rorder, nu, ctx->d_segs, ctx->d_pairs, ctx->d_tab32, ctx->d_olg1, ctx->d_olg2,
-                                   ctx->d_deff32, ctx->d_gl_off, ctx->d_rhoA);
-            else
-                rho64(ctx->variant[TUNE_RHO64]);
+#define RHO_ARGS R.d_units, R.d_order, nu, R.d_segs, ctx->d_pairs
+#define RHO_TAIL ctx->d_olg1, ctx->d_olg2
+            if (f32) {
+                if (waves == 8)
+                    hipLaunchKernelGGL((k_rho<float, 8, 512>), dim3(nu), dim3(64 * 8), 0, ctx->stream, RHO_ARGS, ctx->d_tab32,
+                                       RHO_TAIL, ctx->d_deff32, ctx->d_gl_off, ctx->d_rhoA);
+                else if (dec == 1)
+                    hipLaunchKernelGGL((k_rho<float, 4, 1024>), dim3(nu), dim3(64 * 4), 0, ctx->stream, RHO_ARGS, ctx->d_tab32,
+                                       RHO_TAIL, ctx->d_deff32, ctx->d_gl_off, ctx->d_rhoA);
+                else
+                    hipLaunchKernelGGL((k_rho<float, 4, 2048>), dim3(nu), dim3(64 * 4), 0, ctx->stream, RHO_ARGS, ctx->d_tab32,
+                                       RHO_TAIL, ctx->d_deff32, ctx->d_gl_off, ctx->d_rhoA);
+            } else if (kind == 0) {
+                hipLaunchKernelGGL(k_rho_m64, dim3(nu), dim3(64 * RHO_WAVES), 0, ctx->stream, RHO_ARGS, ctx->d_tab,
+                                   RHO_TAIL, ctx->d_deff, ctx->d_gl_off, ctx->d_rhoA);
+            } else {
+#define RHO64(W, CH)                                                                                              \
+    hipLaunchKernelGGL((k_rho<double, W, CH>), dim3(nu), dim3(64 * W), 0, ctx->stream, RHO_ARGS, ctx->d_tab, RHO_TAIL, \
+                       ctx->d_deff, ctx->d_gl_off, ctx->d_rhoA)
+                if (waves == 4 && dec == 0) RHO64(4, 2048);
+                else if (waves == 4 && dec == 1) RHO64(4, 1024);
+                else if (waves == 8 && dec == 2) RHO64(8, 512);
+                else if (waves == 2 && dec == 1) RHO64(2, 1024);
+                else RHO64(4, 512);
+#undef RHO64
+            }
+#undef RHO_ARGS
+#undef RHO_TAIL
         }
         const unsigned gb = (unsigned)((ctx->nn + 255) / 256);
         {
@@ -1288,7 +1318,7 @@ int rho_dev_pairs(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t 
         }
         HIP_TRY(ctx, hipGetLastError());
     }
-    return rc;
+    return LCPAO_OK;
 }
 
 int f3_dev(lcpao_ctx *ctx, int nch, const double *const *d_V, const double *const *d_DM, int64_t dm_offset,
@@ -1335,7 +1365,8 @@ int f3_dev(lcpao_ctx *ctx, int nch, const double *const *d_V, const double *cons
                                    ctx->d_pairs, ctx->d_tab, ctx->d_dtab[0], ctx->d_dtab[1], ctx->d_dtab[2],
                                    ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c], D, ctx->d_fpart);
         };
-        if (nu > 0 && !f32 && (rc = autotune(ctx, TUNE_F364, f364))) return rc;
+        if (nu > 0 && !f32 && (rc = autotune(ctx, TUNE_F364, f364, [&](int v) { return v != 0 || ctx->maxno <= MFMA64_NO; })))
+            return rc;
         if (nu > 0) {
             KTimer kt(ctx, KID_F3);
             if (f32)
@@ -1422,11 +1453,16 @@ int f3_dev_noalloc(lcpao_ctx *ctx, int nch, const double *const *d_V, const doub
 void free_all(lcpao_ctx *ctx) {
     void *ptrs[] = {ctx->d_glist, ctx->d_clist, ctx->d_gl_off, ctx->d_tab_off, ctx->d_no, ctx->d_olg1, ctx->d_olg2,
                     ctx->d_pairs, ctx->d_tab, ctx->d_dtab[0], ctx->d_dtab[1], ctx->d_dtab[2], ctx->d_hunits,
-                    ctx->d_horder, ctx->d_hpair_ub, ctx->d_hcanon, ctx->d_hpart, ctx->d_runits, ctx->d_rorder,
-                    ctx->d_segs, ctx->d_deff, ctx->d_rhoA, ctx->d_ent_off, ctx->d_ent, ctx->d_funits, ctx->d_forder,
+                    ctx->d_horder, ctx->d_hpair_ub, ctx->d_hcanon, ctx->d_hpart, ctx->d_deff, ctx->d_rhoA,
+                    ctx->d_ent_off, ctx->d_ent, ctx->d_funits, ctx->d_forder,
                     ctx->d_atom_ub, ctx->d_fpart, ctx->d_fsum, ctx->d_stage, ctx->d_flag};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    for (auto &D : ctx->rdec) {
+        if (D.d_units) (void)hipFree(D.d_units);
+        if (D.d_order) (void)hipFree(D.d_order);
+        if (D.d_segs) (void)hipFree(D.d_segs);
+    }
     invalidate_split_tables(ctx);
     if (ctx->d_deff32) (void)hipFree(ctx->d_deff32);
     plan_free(ctx->plan);
@@ -1650,11 +1686,11 @@ std::string tune_key(const lcpao_ctx *ctx) {
 struct TuneEntry {
     std::string key;
     int op, variant;
-    float ms[2];
+    std::vector<float> ms;   // measured ms of each candidate (-1: not run)
 };
 
 // The cache is a JSON object with one entry per line:
-//   {"key": "no=19;h=0.2225;ppa=87", "op": 0, "variant": 1, "ms": [19.9, 11.9]}
+//   {"key": "no=19;h=0.2225;ppa=87", "op": 0, "variant": 1, "ms": [19.9, 11.9, ...]}
 // Lines that do not hold an entry are skipped.  Returns false if the file
 // cannot be opened.
 bool read_tune_file(const char *path, std::vector<TuneEntry> &out) {
@@ -1665,11 +1701,23 @@ bool read_tune_file(const char *path, std::vector<TuneEntry> &out) {
         const char *p = std::strstr(line, "{\"key\": \"");
         if (!p) continue;
         char key[512];
-        int op = -1, v = -1;
-        float a = 0.f, b = 0.f;
-        if (std::sscanf(p, "{\"key\": \"%511[^\"]\", \"op\": %d, \"variant\": %d, \"ms\": [%f, %f]", key, &op, &v,
-                        &a, &b) >= 3)
-            out.push_back(TuneEntry{key, op, v, {a, b}});
+        int op = -1, v = -1, used = 0;
+        if (std::sscanf(p, "{\"key\": \"%511[^\"]\", \"op\": %d, \"variant\": %d%n", key, &op, &v, &used) < 3) continue;
+        TuneEntry e{key, op, v, {}};
+        const char *m = std::strstr(p + used, "\"ms\": [");
+        if (m) {
+            m += 7;
+            for (int k = 0; k < TUNE_MAXCAND; k++) {
+                char *end = nullptr;
+                const float x = std::strtof(m, &end);
+                if (end == m) break;
+                e.ms.push_back(x);
+                m = end;
+                while (*m == ' ' || *m == ',') m++;
+                if (*m == ']') break;
+            }
+        }
+        out.push_back(e);
     }
     std::fclose(f);
     return true;
@@ -1689,7 +1737,7 @@ int lcpao_grid_tune_load(lcpao_ctx *ctx, const char *path, int32_t *loaded) {
     for (const TuneEntry &e : entries) {
         if (e.key != key || e.op < 0 || e.op >= TUNE_NOPS || e.variant < 0 || e.variant >= TUNE_NCAND[e.op]) continue;
         ctx->variant[e.op] = e.variant;
-        for (int v = 0; v < 2 && v < TUNE_NCAND[e.op]; v++) ctx->variant_ms[e.op][v] = e.ms[v];
+        for (int v = 0; v < (int)e.ms.size() && v < TUNE_NCAND[e.op]; v++) ctx->variant_ms[e.op][v] = e.ms[v];
         n++;
     }
     if (loaded) *loaded = n;
@@ -1706,14 +1754,18 @@ int lcpao_grid_tune_save(lcpao_ctx *ctx, const char *path) {
         if (!(e.key == key && e.op >= 0 && e.op < TUNE_NOPS && ctx->variant[e.op] >= 0)) keep.push_back(e);
     for (int op = 0; op < TUNE_NOPS; op++)
         if (ctx->variant[op] >= 0)
-            keep.push_back(TuneEntry{key, op, ctx->variant[op], {ctx->variant_ms[op][0], ctx->variant_ms[op][1]}});
+            keep.push_back(TuneEntry{key, op, ctx->variant[op],
+                                     std::vector<float>(ctx->variant_ms[op], ctx->variant_ms[op] + TUNE_NCAND[op])});
     const std::string tmp = std::string(path) + ".tmp";
     FILE *f = std::fopen(tmp.c_str(), "w");
     if (!f) return set_err(ctx, LCPAO_EINVAL, "cannot write the tuner cache " + tmp);
     std::fprintf(f, "{\"lcpao_grid_tune\": [\n");
-    for (size_t i = 0; i < keep.size(); i++)
-        std::fprintf(f, "{\"key\": \"%s\", \"op\": %d, \"variant\": %d, \"ms\": [%.4f, %.4f]}%s\n", keep[i].key.c_str(),
-                     keep[i].op, keep[i].variant, keep[i].ms[0], keep[i].ms[1], i + 1 < keep.size() ? "," : "");
+    for (size_t i = 0; i < keep.size(); i++) {
+        std::fprintf(f, "{\"key\": \"%s\", \"op\": %d, \"variant\": %d, \"ms\": [", keep[i].key.c_str(), keep[i].op,
+                     keep[i].variant);
+        for (size_t k = 0; k < keep[i].ms.size(); k++) std::fprintf(f, "%s%.4f", k ? ", " : "", keep[i].ms[k]);
+        std::fprintf(f, "]}%s\n", i + 1 < keep.size() ? "," : "");
+    }
     std::fprintf(f, "]}\n");
     const bool ok = std::fclose(f) == 0;
     if (!ok || std::rename(tmp.c_str(), path) != 0) return set_err(ctx, LCPAO_EINVAL, std::string("cannot write ") + path);

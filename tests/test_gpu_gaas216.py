@@ -100,7 +100,8 @@ def _energies(gaas, H, rho):
 
 
 @pytest.mark.parametrize("prec,op,variant", [("bf16x3", "rhobf", 0), ("bf16x3", "rhobf", 1),
-                                             ("fp32", "rho32", 0), ("fp32", "rho32", 1)])
+                                             ("fp32", "rho32", 0), ("fp32", "rho32", 1), ("fp32", "rho32", 2),
+                                             ("fp32", "rho32", 3)])
 def test_gaas216_precision_ladder(gaas, prec, op, variant):
     u = gaas["u"]
     ctx = gaas["ctx"]
@@ -120,3 +121,19 @@ def test_gaas216_precision_ladder(gaas, prec, op, variant):
     assert rh <= RTOL[prec] and rr <= RTOL[prec]
     assert dE[0] <= 1.0 and dE[1] <= 1.0
     ctx.tune(op, -1)
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
+def test_gaas216_fp64_density_candidates_full_size(gaas, variant):
+    """Every FP64 density candidate of the autotuner (kernel and work-unit
+    shape) at full size against the CPU restatement, within 1e-12 of each
+    point's sum of absolute terms."""
+    ctx = gaas["ctx"]
+    ctx.tune("rho64", variant)
+    gaas["dR"].fill_(np.nan)
+    ctx.density_dev([gaas["dD"]], [gaas["dR"]], 0, "fp64")
+    R = gaas["dR"].cpu().numpy()
+    print("rho64 variant", variant, "worst ratio", worst_ratio(R, gaas["Rc"], gaas["MR"]), flush=True)
+    ok, msg = within(R, gaas["Rc"], gaas["MR"])
+    assert ok, msg
+    ctx.tune("rho64", -1)
