@@ -135,6 +135,13 @@ def si2_large_basis(ngrid=16):
     return Structure("si2f", st.Latvecs, [[0, 0, 0], [.25, .25, .25]], ["Sif", "Sif"], (ngrid,) * 3, st.a_cell)
 
 
+def mixed_pair_cell(ngrid=24, a=10.66):
+    """A rocksalt primitive cell holding Ga (19 orbitals, s3p2d2) and Na (14,
+    s3p2d1): pair blocks of 19 x 14, 14 x 19, 19 x 19 and 14 x 14."""
+    lat = 0.5 * a * np.array([[0, 1, 1], [1, 0, 1], [1, 1, 0]], dtype=np.float64)
+    return Structure("gana2", lat, [[0, 0, 0], [.5, .5, .5]], ["Ga", "Na"], (ngrid,) * 3, a, 0.05, 5)
+
+
 def dense_cell(a=3.5, ngrid=14):
     """One Si atom in a simple cubic cell of edge a (bohr), much smaller than
     its 7 bohr cutoff: about 48 sphere images cover every 8x8x8 grid block

@@ -43,6 +43,7 @@ def case(name):
             "nacl8": lambda: Case(S.nacl(1, 24), nwaves=16, seed=3),
             "si2_r1of3": lambda: Case(S.si2(24), nprocs=3, rank=1),
             "nacl8_r0of2": lambda: Case(S.nacl(1, 24), nprocs=2, rank=0, seed=3),
+            "gana2": lambda: Case(S.mixed_pair_cell(24), nwaves=14, seed=4),
         }[name]
         _cache[name] = mk()
     return _cache[name]
@@ -68,7 +69,7 @@ def test_tables_on_device_match_oracle(name):
             assert np.max(np.abs(got[a] - ref[a])) <= 1e-13 * scale
 
 
-@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("name", CASES + ["gana2"])
 def test_hamiltonian_fp64(name):
     c = case(name)
     c.ctx.set_tables(c.tabs)
@@ -79,7 +80,7 @@ def test_hamiltonian_fp64(name):
     assert ok, msg
 
 
-@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("name", CASES + ["gana2"])
 def test_density_fp64(name):
     c = case(name)
     c.ctx.set_tables(c.tabs)
