@@ -370,3 +370,29 @@ def test_fp32_density_without_cell_lists_falls_back_to_pair_lists():
     ctx2.tune("rho32", 1)
     with pytest.raises(GridError):
         ctx2.density([c.DM], precision="fp32")
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
+@pytest.mark.parametrize("rank", [0, 97, 173])
+def test_one_pair_per_rank(rank, prec):
+    """The smallest rank share: Si2 at 16^3 has 174 pairs, split over 174
+    ranks (the reference refuses more ranks than pairs, UCell.jl:52), so a
+    rank holds one pair -- on-site (rank 0) or an image pair whose transpose
+    partner is on another rank and is therefore not folded."""
+    c = Case(S.si2(16), nprocs=174, rank=rank, nwaves=8, nocc=4, seed=6)
+    assert c.u.MPI_size == 1
+    dt = O.dorbitals(c.g, c.opaos, c.st.atom2spe)
+    c.ctx.set_tables(c.tabs)
+    c.ctx.set_dtables(*dt)
+    bar = {"fp64": 1e-12, "fp32": 1e-5, "bf16x3": 1e-4}[prec]
+    H = c.ctx.hamiltonian([c.V], prec)[0]
+    Href, M = O.hamiltonian(c.g, c.tabs, [c.V], with_abs=True)
+    rho = c.ctx.density([c.DM], dm_offset=c.u.MPHks_rank, precision=prec)[0]
+    Rref, MR = O.density(c.g, c.tabs, [c.DM], with_abs=True)
+    F = c.ctx.force3([c.V], [c.DM], dm_offset=c.u.MPHks_rank, precision=prec)
+    Fref = O.force3(c.g, c.tabs, dt, [c.V], [c.DM])
+    r = (worst_ratio(H, Href[0], M[0]), worst_ratio(rho, Rref[0], MR[0]),
+         worst_ratio(F, Fref, _force_bound(c, dt, [c.V], [c.DM])))
+    print("rank", rank, prec, "H %.3g rho %.3g F %.3g" % r, flush=True)
+    assert max(r) <= bar
+    c.ctx.close()

@@ -64,6 +64,23 @@ def test_ucell_builder_errors():
         UCell(1, np.eye(3) * 10, 1, [0], [[0, 0, 0]], [7.0], (0, 8, 8))
 
 
+def test_ucell_one_pair_per_rank_boundary():
+    """Si2 at 16^3 has 174 pairs: 174 ranks each get exactly one, in order,
+    and 175 ranks are refused as the reference refuses them (UCell.jl:52)."""
+    st = S.si2(16)
+    full = st.ucell()
+    assert full.MPI_size == 174
+    got = []
+    for r in (0, 1, 96, 97, 172, 173):
+        u = st.ucell(174, r)
+        assert u.MPI_size == 1
+        got.append((int(u.MPI_atom[0]), int(u.MPI_natn[0]), int(u.MPI_ncn[0])))
+    ref = [(int(full.MPI_atom[r]), int(full.MPI_natn[r]), int(full.MPI_ncn[r])) for r in (0, 1, 96, 97, 172, 173)]
+    assert got == ref
+    with pytest.raises(_lib.GridError):
+        st.ucell(175, 0)
+
+
 @pytest.mark.parametrize("sym,cut,basis", [("Si", 7.0, "s2p2d1"), ("Ga", 7.0, "s3p2d2"), ("Na", 9.0, "s3p2d1")])
 def test_pao_reader_equals_oracle(sym, cut, basis):
     p, q = PAO(sym, cut, basis), O.Pao(sym, cut, basis)
