@@ -206,7 +206,11 @@ int lcpao_grid_kernel_stats(const lcpao_ctx *ctx, int32_t kernel_id, int64_t *la
  * -1 ms).  op:
  *   LCPAO_TUNE_RHO64  FP64 density  (0 matrix-core kernel; 1..5 vector-FMA
  *                     kernel with units of (waves per workgroup x grid-list
- *                     entries) 4 x 2048, 4 x 1024, 8 x 512, 2 x 1024, 4 x 512)
+ *                     entries) 4 x 2048, 4 x 1024, 8 x 512, 2 x 1024, 4 x 512;
+ *                     6 block form: vector FMAs over 4x4x4 sub-blocks with
+ *                     the orbital columns staged in LDS, 7 the same reading
+ *                     the columns from the tables; 6 and 7 are used only
+ *                     when the block plan holds the geometry)
  *   LCPAO_TUNE_F364   FP64 Force3   (0 matrix-core kernel, 1 vector-FMA kernel)
  *   LCPAO_TUNE_RHOBF  bf16x3 density (0 single-buffered chunk planes,
  *                     1 double-buffered, used only when its LDS fits;

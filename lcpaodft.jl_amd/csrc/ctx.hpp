@@ -13,6 +13,10 @@
 #define UNIT_PTS 2048     // overlap points per H / Force3 unit
 #define RHO_CHUNK 2048    // grid-list entries per density unit (the largest decomposition)
 #define HK 32             // points per LDS chunk in the H kernel
+// orbital-count pairs (NO_A, NO_B) with kernels of fixed size (the Si/Cl,
+// Na and Ga/As bases of the test systems); other counts take runtime loops
+#define NO_CASES(X) X(13, 13) X(13, 14) X(14, 13) X(14, 14) X(19, 19) X(13, 19) X(19, 13) X(14, 19) X(19, 14)
+
 // largest orbital count per atom: Read_PAO's basis string (Read_PAO.jl:49-76)
 // takes one decimal digit of multiplicity for each of s, p, d, f, so at
 // most 9 * (1 + 3 + 5 + 7) orbitals
@@ -147,10 +151,13 @@ struct lcpao_ctx {
 // tunable operations and their candidate counts
 enum { TUNE_RHO64 = 0, TUNE_F364 = 1, TUNE_RHOBF = 2, TUNE_RHO32 = 3, TUNE_NOPS = 4 };
 // FP64 density: 0 matrix cores, 1..5 vector FMA with unit shapes
-// (waves x chunk) 4 x 2048, 4 x 1024, 8 x 512, 2 x 1024, 4 x 512;
-// FP32 density: 0 pair lists 4 x 2048, 1 block form, 2 pair lists 4 x 1024,
-// 3 pair lists 8 x 512
-constexpr int TUNE_NCAND[TUNE_NOPS] = {6, 2, 2, 4};
+// (waves x chunk) 4 x 2048, 4 x 1024, 8 x 512, 2 x 1024, 4 x 512, 6 the
+// block form (vector FMA over the sub-blocks of the block plan, orbital
+// columns staged in LDS), 7 the block form reading the columns from the
+// tables; FP32 density: 0 pair lists 4 x 2048, 1 block form, 2 pair lists
+// 4 x 1024, 3 pair lists 8 x 512
+constexpr int RHO64_BLOCK = 6, RHO64_BLOCK_TAB = 7;
+constexpr int TUNE_NCAND[TUNE_NOPS] = {8, 2, 2, 4};
 constexpr int TUNE_MAXCAND = 8;
 
 

@@ -276,7 +276,6 @@ __device__ void rho_points_any(const T *__restrict__ D, const T *__restrict__ ta
     }
 }
 
-#define NO_CASES(X) X(13, 13) X(13, 14) X(14, 13) X(14, 14) X(19, 19) X(13, 19) X(19, 13) X(14, 19) X(19, 14)
 
 // Density per (atom, chunk) unit; T = table/D element type (double or float),
 // W waves per workgroup, CH grid-list entries per unit (the unit shape: W
@@ -1219,9 +1218,31 @@ int rho_dev(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_off
     const bool m64_ok = ctx->maxno <= MFMA64_NO;
     if (!f32 && ctx->variant[TUNE_RHO64] == 0 && !m64_ok)
         return set_err(ctx, LCPAO_ERANGE, "the FP64 matrix-core density kernel holds at most 32 orbitals per atom");
-    auto available = [&](int v) { return f32 ? (v != 1 || block_ok) : (v != 0 || m64_ok); };
+    // the FP64 block form (candidate 6) needs the block plan
+    bool block64_ok = false;
+    const int v64 = ctx->variant[TUNE_RHO64];
+    const bool v64_block = v64 >= RHO64_BLOCK;
+    if (!f32 && (v64 < 0 || v64_block)) {
+        const int r2 = plan_build(ctx);
+        if (r2 == LCPAO_OK) {
+            block64_ok = true;
+        } else if (v64_block && ctx->block_state != 0) {
+            return r2;  // fixed to the block form, which failed
+        } else {
+            if (v64_block) ctx->variant[TUNE_RHO64] = 1;  // beyond the block limits
+            ctx->err.clear();
+            (void)hipGetLastError();
+        }
+    }
+    auto available = [&](int v) {
+        if (f32) return v != 1 || block_ok;
+        if (v >= RHO64_BLOCK) return block64_ok;
+        return v != 0 || m64_ok;
+    };
     auto run = [&](int v) -> int {
         if (f32 && v == 1) return split_density_f32_enqueue(ctx, nspin, d_DM, dm_offset, d_rho);
+        if (!f32 && v >= RHO64_BLOCK)
+            return split_density_f64_enqueue(ctx, nspin, d_DM, dm_offset, d_rho, v == RHO64_BLOCK);
         return rho_dev_pairs(ctx, nspin, d_DM, dm_offset, d_rho, f32, v);
     };
     if (f32 && (ctx->variant[op] != 1) && (rc = fp32_tables(ctx, false))) return rc;

@@ -788,6 +788,21 @@ int plan_build_inner(lcpao_ctx *ctx) {
             const int64_t gs = sph_first[b] + (int64_t)q;
             sb[gs] = (int32_t)b; sa[gs] = sph[b][q].first; sr[gs] = rows0[b][q];
         }
+        // orbital rows present on each 4x4x4 sub-block (mask word = first
+        // point index, bit = second * 8 + third within the block)
+        int sub_rows[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (size_t q = 0; q < sph[b].size(); q++) {
+            const uint64_t *m = &mask[(size_t)(sph_first[b] + (int64_t)q) * 8];
+            for (int sc = 0; sc < 8; sc++) {
+                const int s1 = sc >> 2, s2 = (sc >> 1) & 1, s3 = sc & 1;
+                uint64_t sel = 0;
+                for (int u = 0; u < 4; u++) sel |= 0xFull << ((4 * s2 + u) * 8 + 4 * s3);
+                bool any = false;
+                for (int u = 0; u < 4; u++) any = any || (m[4 * s1 + u] & sel) != 0;
+                if (any) sub_rows[sc] += ctx->no[sph[b][q].first];
+            }
+        }
+        for (int sc = 0; sc < 8; sc++) P->max_sub_rows = std::max(P->max_sub_rows, sub_rows[sc]);
     }
     // contributions of each computed pair
     const int C = ctx->cpy, W = 2 * C + 1;
@@ -1758,6 +1773,509 @@ int split_density(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t 
     }
     SPLIT_TRY(ctx, hipStreamSynchronize(ctx->stream));
     collect_timings(ctx);
+    return LCPAO_OK;
+}
+
+// ---- FP64 block density -----------------------------------------------------
+// rho(p) = sum over the slot pairs x <= y of a sub-block of phi_x(p)^T W_xy
+// phi_y(p), W_xy the FP64 fold of the pair (D_P + D_P'^T; an on-site pair's
+// own block), in exact FP64 vector FMAs.  One 1 024-thread workgroup per
+// 4x4x4 sub-block of an 8^3 block, lane = grid point:
+//  1. the table column of every (slot, point) is read from the plan's point
+//     map into LDS; wave ballots give each slot's presence mask;
+//  2. one wave lists the slot pairs that are computed pairs and share a
+//     point of the sub-block (slot-major, in order) and the LDS row of every
+//     present slot's orbitals;
+//  3. the columns of all present slots are staged once into LDS as
+//     [orbital row][64 points] FP64 (row stride 65 doubles: the staging
+//     writes, one point's consecutive orbitals per lane group, and the
+//     reads, one row per wave, are both free of bank conflicts), reading
+//     each table column as one contiguous run;
+//  4. wave w takes the pairs w, w + 16, ...: phi_x and phi_y come from LDS,
+//     W is wave-uniform and read through the scalar path (SGPR operands);
+//  5. the sixteen waves' sums of every point are added in wave order and
+//     the point is written once: no atomics, deterministic.
+// The tables are read once per launch; the lanes compute at the pairs'
+// overlap points plus the absent points of partly covered sub-blocks (1.44x
+// the exact work on GaAs216).  What bounds it is the W reads: each W block
+// feeds only 64 points per wave, so the scalar loads (about 49 per pair)
+// mostly miss the scalar cache and the waves wait on them; sixteen waves (4
+// per SIMD) hide enough of that latency to match the pair-list kernels
+// (12.2 ms on GaAs216, against 18.3 ms with eight waves).  A sub-block with
+// more orbital rows than the launch's LDS holds reads its columns from the
+// tables instead (same sums).  Workgroups are dealt XCD-aware: workgroup w
+// runs on XCD w % 8, which takes a contiguous range of sub-blocks, so
+// neighbouring sub-blocks -- which read the same pair blocks of W -- meet in
+// one L2.
+constexpr int R64_WAVES = 16;
+constexpr int R64_LD = 65;       // LDS row stride of the staged columns (doubles)
+constexpr int R64_UNROLL = 8;    // staging loads in flight per thread
+
+struct Rho64Args {
+    const BlockDev *blocks;
+    const int32_t *sph_atom, *sph_map;
+    const int16_t *slotno;
+    const SlotPair *spair;
+    const int64_t *tab_off;
+    const double *tab, *Deff;
+    double *rho;
+    int64_t nsub;              // 8 x blocks
+    int32_t n1, n2, n3;
+    int32_t rows_cap;          // orbital rows the staged columns hold
+    int32_t max_slots;
+    int32_t pad;
+};
+
+// LDS layout of the FP64 block density (bytes)
+struct R64Lds {
+    int phi, tm, mask, roff, rslot, pairs, part, ctl, total;
+};
+__host__ __device__ inline R64Lds r64_lds(int rows_cap, int max_slots) {
+    R64Lds L;
+    L.phi = 0;
+    L.tm = L.phi + rows_cap * R64_LD * 8;
+    L.mask = L.tm + max_slots * 64 * 4;
+    L.mask = (L.mask + 7) / 8 * 8;
+    L.roff = L.mask + max_slots * 8;
+    L.rslot = L.roff + max_slots * 4;
+    L.pairs = L.rslot + rows_cap * 4;
+    L.part = L.pairs + max_slots * (max_slots + 1) / 2 * 4;
+    L.part = (L.part + 7) / 8 * 8;
+    L.ctl = L.part + R64_WAVES * 64 * 8;
+    L.total = L.ctl + 16;
+    return L;
+}
+
+// a^T M b, M row-major NA x NB (wave-uniform)
+template <int NA, int NB>
+__device__ __forceinline__ double quad64(const GLOBAL_AS double *__restrict__ M, const double (&a)[NA],
+                                         const double (&b)[NB]) {
+    double z = 0.0;
+#pragma unroll
+    for (int i = 0; i < NA; i++) {
+        double y = 0.0;
+#pragma unroll
+        for (int j = 0; j < NB; j++) y = fma(M[i * NB + j], b[j], y);
+        z = fma(a[i], y, z);
+    }
+    return z;
+}
+
+// where the lane's orbital values of a slot come from: staged LDS rows, or
+// the table column (absent point: zeros)
+struct Col64 {
+    const double *lds;               // staged: row 0 of the slot, stride R64_LD
+    const GLOBAL_AS double *tab;     // else: the slot's table
+    int t;                           // table column (-1: absent)
+};
+template <int N, bool STAGED>
+__device__ __forceinline__ void col64(const Col64 &c, int lane, double (&f)[N]) {
+    if constexpr (STAGED) {
+#pragma unroll
+        for (int i = 0; i < N; i++) f[i] = c.lds[i * R64_LD + lane];
+    } else {
+        const GLOBAL_AS double *p = c.tab + (int64_t)(c.t < 0 ? 0 : c.t) * N;
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            const double v = p[i];
+            f[i] = c.t >= 0 ? v : 0.0;
+        }
+    }
+}
+template <bool STAGED>
+__device__ __forceinline__ double col64_at(const Col64 &c, int lane, int i, int n) {
+    if constexpr (STAGED) return c.lds[i * R64_LD + lane];
+    else return c.t >= 0 ? c.tab[(int64_t)c.t * n + i] : 0.0;
+}
+
+template <int NX, int NY, bool STAGED>
+__device__ __forceinline__ double pair64(const GLOBAL_AS double *__restrict__ M, bool tr, bool self, const Col64 &cx,
+                                         const Col64 &cy, int lane) {
+    double fx[NX];
+    col64<NX, STAGED>(cx, lane, fx);
+    if constexpr (NX == NY) {
+        if (self) return quad64<NX, NX>(M, fx, fx);
+    }
+    double fy[NY];
+    col64<NY, STAGED>(cy, lane, fy);
+    return tr ? quad64<NY, NX>(M, fy, fx) : quad64<NX, NY>(M, fx, fy);
+}
+
+// any orbital counts (row-by-row loops)
+template <bool STAGED>
+__device__ double pair64_rt(const GLOBAL_AS double *__restrict__ M, bool tr, const Col64 &cx, int nx,
+                            const Col64 &cy, int ny, int lane) {
+    const Col64 &ca = tr ? cy : cx, &cb = tr ? cx : cy;
+    const int na = tr ? ny : nx, nb = tr ? nx : ny;
+    double z = 0.0;
+    for (int i = 0; i < na; i++) {
+        double y = 0.0;
+        for (int j = 0; j < nb; j++) y = fma(M[i * nb + j], col64_at<STAGED>(cb, lane, j, nb), y);
+        z = fma(col64_at<STAGED>(ca, lane, i, na), y, z);
+    }
+    return z;
+}
+
+template <bool STAGED>
+__device__ double rho64_pairs(const Rho64Args &A, const BlockDev &B, unsigned char *lds, const R64Lds &L, int lane,
+                              int wave, int k) {
+    const int32_t *tm = reinterpret_cast<const int32_t *>(lds + L.tm);
+    const int32_t *roff = reinterpret_cast<const int32_t *>(lds + L.roff);
+    const int32_t *pairs = reinterpret_cast<const int32_t *>(lds + L.pairs);
+    const double *phi = reinterpret_cast<const double *>(lds + L.phi);
+    const int np = reinterpret_cast<const int32_t *>(lds + L.ctl)[0];
+    const int ns = B.nslot;
+    const GLOBAL_AS double *Deff = as_global(A.Deff);
+    const GLOBAL_AS SlotPair *sp = as_global(A.spair) + B.sp_off;
+    const GLOBAL_AS int16_t *sno = as_global(A.slotno) + B.slot_off;
+    double z = 0.0;
+    for (int q = wave; q < np; q += R64_WAVES) {
+        const int xy = __builtin_amdgcn_readfirstlane(pairs[q]);
+        const int x = xy >> 16, y = xy & 0xFFFF;
+        const SlotPair P = sp[x * ns + y];
+        const GLOBAL_AS double *M = Deff + __builtin_amdgcn_readfirstlane(P.hoff);
+        const bool tr = (__builtin_amdgcn_readfirstlane(P.meta) >> 6) != 0, self = x == y;
+        const int nx = __builtin_amdgcn_readfirstlane(sno[x]), ny = __builtin_amdgcn_readfirstlane(sno[y]);
+        Col64 cx, cy;
+        if constexpr (STAGED) {
+            cx.lds = phi + (int64_t)__builtin_amdgcn_readfirstlane(roff[x]) * R64_LD;
+            cy.lds = phi + (int64_t)__builtin_amdgcn_readfirstlane(roff[y]) * R64_LD;
+        } else {
+            const GLOBAL_AS int32_t *sa = as_global(A.sph_atom) + B.slot_off;
+            cx.tab = as_global(A.tab) + as_global(A.tab_off)[__builtin_amdgcn_readfirstlane(sa[x])];
+            cy.tab = as_global(A.tab) + as_global(A.tab_off)[__builtin_amdgcn_readfirstlane(sa[y])];
+            cx.t = tm[x * 64 + lane];
+            cy.t = tm[y * 64 + lane];
+        }
+#define R64_CASE(NX, NY)                                                 \
+    if (nx == NX && ny == NY) {                                          \
+        z += pair64<NX, NY, STAGED>(M, tr, self, cx, cy, lane);          \
+        continue;                                                        \
+    }
+        NO_CASES(R64_CASE)
+#undef R64_CASE
+        z += pair64_rt<STAGED>(M, tr, cx, nx, cy, ny, lane);
+    }
+    return z;
+}
+
+__global__ __launch_bounds__(64 * R64_WAVES) void k_rblk64(Rho64Args A) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int64_t per = (A.nsub + 7) / 8;
+    const int64_t j = blockIdx.x / 8, s = (int64_t)(blockIdx.x % 8) * per + j;
+    if (j >= per || s >= A.nsub) return;
+    const BlockDev B = as_global(A.blocks)[s / 8];
+    const int sc = (int)(s % 8);
+    const int ns = B.nslot;
+    const R64Lds L = r64_lds(A.rows_cap, A.max_slots);
+    int32_t *tm = reinterpret_cast<int32_t *>(lds + L.tm);
+    uint64_t *mask = reinterpret_cast<uint64_t *>(lds + L.mask);
+    int32_t *roff = reinterpret_cast<int32_t *>(lds + L.roff);
+    int32_t *rslot = reinterpret_cast<int32_t *>(lds + L.rslot);
+    int32_t *pairs = reinterpret_cast<int32_t *>(lds + L.pairs);
+    double *part = reinterpret_cast<double *>(lds + L.part);
+    int32_t *ctl = reinterpret_cast<int32_t *>(lds + L.ctl);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    // the lane's point of the sub-block and its index in the block
+    const int p1 = 4 * (sc >> 2) + (lane >> 4), p2 = 4 * ((sc >> 1) & 1) + ((lane >> 2) & 3), p3 = 4 * (sc & 1) + (lane & 3);
+    const int k = (p1 * BLK + p2) * BLK + p3;
+    // 1. table columns and presence masks
+    const GLOBAL_AS int32_t *map = as_global(A.sph_map) + (int64_t)B.slot_off * BLK_PTS;
+    for (int x = wave; x < ns; x += R64_WAVES) {
+        const int t = map[(int64_t)x * BLK_PTS + k];
+        tm[x * 64 + lane] = t;
+        const uint64_t m = __ballot(t >= 0);
+        if (lane == 0) mask[x] = m;
+    }
+    __syncthreads();
+    // 2. pair list and LDS rows (one wave, in slot order)
+    const GLOBAL_AS int16_t *sno = as_global(A.slotno) + B.slot_off;
+    if (wave == 0) {
+        int rows = 0, np = 0;
+        for (int x = 0; x < ns; x++) {
+            const uint64_t mx = mask[x];
+            if (!mx) continue;
+            const int nx = sno[x];
+            if (lane == 0) roff[x] = rows;
+            if (rows + nx <= A.rows_cap)
+                for (int i = lane; i < nx; i += 64) rslot[rows + i] = x << 16 | i;
+            rows += nx;
+            for (int y0 = x; y0 < ns; y0 += 64) {
+                const int y = y0 + lane;
+                const bool act = y < ns && (mask[y] & mx) != 0 && as_global(A.spair)[B.sp_off + (int64_t)x * ns + y].hoff >= 0;
+                const uint64_t bal = __ballot(act);
+                if (act) pairs[np + __popcll(bal & ((1ull << lane) - 1))] = x << 16 | y;
+                np += __popcll(bal);
+            }
+        }
+        if (lane == 0) { ctl[0] = np; ctl[1] = rows; }
+    }
+    __syncthreads();
+    const int rows = ctl[1];
+    double z;
+    if (rows <= A.rows_cap) {
+        // 3. stage the present slots' columns: element e = point * rows + row,
+        // so a lane group reads one point's orbitals of a slot contiguously
+        double *phi = reinterpret_cast<double *>(lds + L.phi);
+        const GLOBAL_AS int32_t *sa = as_global(A.sph_atom) + B.slot_off;
+        const int tot = 64 * rows;
+        for (int e0 = tid; e0 < tot; e0 += 64 * R64_WAVES * R64_UNROLL) {
+            double v[R64_UNROLL];
+            int at[R64_UNROLL];
+#pragma unroll
+            for (int u = 0; u < R64_UNROLL; u++) {
+                const int e = e0 + u * 64 * R64_WAVES;
+                const bool ok = e < tot;
+                const int p = ok ? e / rows : 0, r = ok ? e - p * rows : 0;
+                const int rs = rslot[r], x = rs >> 16, i = rs & 0xFFFF;
+                const int t = tm[x * 64 + p];
+                const int a = sa[x];
+                const GLOBAL_AS double *col = as_global(A.tab) + as_global(A.tab_off)[a] + (int64_t)(t < 0 ? 0 : t) * sno[x];
+                const double val = col[i];
+                v[u] = t >= 0 ? val : 0.0;
+                at[u] = ok ? r * R64_LD + p : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < R64_UNROLL; u++)
+                if (at[u] >= 0) phi[at[u]] = v[u];
+        }
+        __syncthreads();
+        z = rho64_pairs<true>(A, B, lds, L, lane, wave, k);
+    } else {
+        z = rho64_pairs<false>(A, B, lds, L, lane, wave, k);
+    }
+    // 5. the waves' sums in wave order
+    part[wave * 64 + lane] = z;
+    __syncthreads();
+    if (wave == 0) {
+        double r = part[lane];
+#pragma unroll
+        for (int w = 1; w < R64_WAVES; w++) r += part[w * 64 + lane];
+        const int x1 = B.g1 + p1, x2 = B.g2 + p2, x3 = B.g3 + p3;
+        if (x1 < A.n1 && x2 < A.n2 && x3 < A.n3) as_global(A.rho)[((int64_t)x1 * A.n2 + x2) * A.n3 + x3] = r;
+    }
+}
+
+// Table form of the FP64 block density: as k_rblk64 without the staged
+// columns -- each wave reads the lanes' columns of its pairs' slots straight
+// from the tables (L2 serves the repeats) and sixteen waves share one
+// sub-block's pair list (wave w takes pairs w, w + W, ...), so the LDS holds
+// only the point map, the masks and the pair list and the workgroup starts
+// without a staging phase.  GaAs216: 10.4 ms against 12.2 ms staged and
+// 12.2 ms for the best pair-list kernel.  PPL points per lane (a region of
+// 4 x 4 x 4 PPL points) would feed PPL FMAs per W element read; two points
+// took 213 VGPRs and ran at 13.1 ms with eight waves (27 ms with sixteen,
+// spilling), so one point per lane is what is launched.  Also measured: one
+// workgroup per 8^3 block whose eight sub-block waves walk the block's pair
+// list in the same order, to share each W block in the scalar cache: 15.2
+// ms (16.5 ms with a barrier every four pairs).
+template <int NA, int NB, int PPL>
+__device__ __forceinline__ void quad64p(const GLOBAL_AS double *__restrict__ M, const double (&a)[PPL][NA],
+                                        const double (&b)[PPL][NB], double (&z)[PPL]) {
+#pragma unroll
+    for (int i = 0; i < NA; i++) {
+        double y[PPL];
+#pragma unroll
+        for (int u = 0; u < PPL; u++) y[u] = 0.0;
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            const double m = M[i * NB + j];
+#pragma unroll
+            for (int u = 0; u < PPL; u++) y[u] = fma(m, b[u][j], y[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < PPL; u++) z[u] = fma(a[u][i], y[u], z[u]);
+    }
+}
+
+template <int N, int PPL>
+__device__ __forceinline__ void cols64p(const GLOBAL_AS double *__restrict__ tab, const int (&t)[PPL],
+                                        double (&f)[PPL][N]) {
+#pragma unroll
+    for (int u = 0; u < PPL; u++) {
+        const GLOBAL_AS double *c = tab + (int64_t)(t[u] < 0 ? 0 : t[u]) * N;
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            const double v = c[i];
+            f[u][i] = t[u] >= 0 ? v : 0.0;
+        }
+    }
+}
+
+template <int NX, int NY, int PPL>
+__device__ __forceinline__ void pair64t(const GLOBAL_AS double *__restrict__ M, bool tr, bool self,
+                                        const GLOBAL_AS double *tx_tab, const int (&tx)[PPL],
+                                        const GLOBAL_AS double *ty_tab, const int (&ty)[PPL], double (&z)[PPL]) {
+    double fx[PPL][NX];
+    cols64p<NX, PPL>(tx_tab, tx, fx);
+    if constexpr (NX == NY) {
+        if (self) { quad64p<NX, NX, PPL>(M, fx, fx, z); return; }
+    }
+    double fy[PPL][NY];
+    cols64p<NY, PPL>(ty_tab, ty, fy);
+    if (tr) quad64p<NY, NX, PPL>(M, fy, fx, z);
+    else quad64p<NX, NY, PPL>(M, fx, fy, z);
+}
+
+__device__ __forceinline__ double quad64t_rt(const GLOBAL_AS double *__restrict__ M, const GLOBAL_AS double *ta_tab,
+                                             int ta, int na, const GLOBAL_AS double *tb_tab, int tb, int nb) {
+    if (ta < 0 || tb < 0) return 0.0;
+    const GLOBAL_AS double *ca = ta_tab + (int64_t)ta * na, *cb = tb_tab + (int64_t)tb * nb;
+    double z = 0.0;
+    for (int i = 0; i < na; i++) {
+        double y = 0.0;
+        for (int j = 0; j < nb; j++) y = fma(M[i * nb + j], cb[j], y);
+        z = fma(ca[i], y, z);
+    }
+    return z;
+}
+
+template <int PPL, int W>
+__global__ __launch_bounds__(64 * W) void k_rblk64t(Rho64Args A) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    constexpr int NREG = 8 / PPL;  // regions per block
+    const int64_t nreg = A.nsub / 8 * NREG;
+    const int64_t per = (nreg + 7) / 8;
+    const int64_t j = blockIdx.x / 8, s = (int64_t)(blockIdx.x % 8) * per + j;
+    if (j >= per || s >= nreg) return;
+    const BlockDev B = as_global(A.blocks)[s / NREG];
+    const int rg = (int)(s % NREG);
+    const int ns = B.nslot, ms = A.max_slots;
+    int32_t *tm = reinterpret_cast<int32_t *>(lds);                       // [slot][PPL][64]
+    uint64_t *mask = reinterpret_cast<uint64_t *>(tm + ms * 64 * PPL);    // [slot][PPL]
+    int32_t *pairs = reinterpret_cast<int32_t *>(mask + ms * PPL);
+    double *part = reinterpret_cast<double *>(lds + ((ms * 64 * PPL * 4 + ms * PPL * 8 + ms * (ms + 1) / 2 * 4 + 7) / 8 * 8));
+    int32_t *ctl = reinterpret_cast<int32_t *>(part + W * 64 * PPL);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    // the lane's points: region rg is sub-cube rg (PPL 1) or the 4 x 4 x 8
+    // column (rg >> 1, rg & 1) (PPL 2)
+    const int s1 = PPL == 1 ? rg >> 2 : rg >> 1, s2 = PPL == 1 ? (rg >> 1) & 1 : rg & 1;
+    const int p1 = 4 * s1 + (lane >> 4), p2 = 4 * s2 + ((lane >> 2) & 3);
+    int kk[PPL], p3[PPL];
+#pragma unroll
+    for (int u = 0; u < PPL; u++) {
+        p3[u] = 4 * (PPL == 1 ? (rg & 1) : u) + (lane & 3);
+        kk[u] = (p1 * BLK + p2) * BLK + p3[u];
+    }
+    const GLOBAL_AS int32_t *map = as_global(A.sph_map) + (int64_t)B.slot_off * BLK_PTS;
+    for (int x = wave; x < ns; x += W) {
+#pragma unroll
+        for (int u = 0; u < PPL; u++) {
+            const int t = map[(int64_t)x * BLK_PTS + kk[u]];
+            tm[(x * PPL + u) * 64 + lane] = t;
+            const uint64_t m = __ballot(t >= 0);
+            if (lane == 0) mask[x * PPL + u] = m;
+        }
+    }
+    __syncthreads();
+    if (wave == 0) {
+        int np = 0;
+        for (int x = 0; x < ns; x++) {
+            uint64_t any = 0;
+#pragma unroll
+            for (int u = 0; u < PPL; u++) any |= mask[x * PPL + u];
+            if (!any) continue;
+            for (int y0 = x; y0 < ns; y0 += 64) {
+                const int y = y0 + lane;
+                bool act = false;
+                if (y < ns) {
+                    uint64_t both = 0;
+#pragma unroll
+                    for (int u = 0; u < PPL; u++) both |= mask[y * PPL + u] & mask[x * PPL + u];
+                    act = both != 0 && as_global(A.spair)[B.sp_off + (int64_t)x * ns + y].hoff >= 0;
+                }
+                const uint64_t bal = __ballot(act);
+                if (act) pairs[np + __popcll(bal & ((1ull << lane) - 1))] = x << 16 | y;
+                np += __popcll(bal);
+            }
+        }
+        if (lane == 0) ctl[0] = np;
+    }
+    __syncthreads();
+    const int np = ctl[0];
+    const GLOBAL_AS double *Deff = as_global(A.Deff);
+    const GLOBAL_AS SlotPair *sp = as_global(A.spair) + B.sp_off;
+    const GLOBAL_AS int16_t *sno = as_global(A.slotno) + B.slot_off;
+    const GLOBAL_AS int32_t *sa = as_global(A.sph_atom) + B.slot_off;
+    double z[PPL];
+#pragma unroll
+    for (int u = 0; u < PPL; u++) z[u] = 0.0;
+    for (int q = wave; q < np; q += W) {
+        const int xy = __builtin_amdgcn_readfirstlane(pairs[q]);
+        const int x = xy >> 16, y = xy & 0xFFFF;
+        const SlotPair P = sp[x * ns + y];
+        const GLOBAL_AS double *M = Deff + __builtin_amdgcn_readfirstlane(P.hoff);
+        const bool tr = (__builtin_amdgcn_readfirstlane(P.meta) >> 6) != 0, self = x == y;
+        const int nx = __builtin_amdgcn_readfirstlane(sno[x]), ny = __builtin_amdgcn_readfirstlane(sno[y]);
+        const GLOBAL_AS double *xt = as_global(A.tab) + as_global(A.tab_off)[__builtin_amdgcn_readfirstlane(sa[x])];
+        const GLOBAL_AS double *yt = as_global(A.tab) + as_global(A.tab_off)[__builtin_amdgcn_readfirstlane(sa[y])];
+        int tx[PPL], ty[PPL];
+#pragma unroll
+        for (int u = 0; u < PPL; u++) {
+            tx[u] = tm[(x * PPL + u) * 64 + lane];
+            ty[u] = tm[(y * PPL + u) * 64 + lane];
+        }
+#define R64T_CASE(NX, NY)                                              \
+    if (nx == NX && ny == NY) {                                        \
+        pair64t<NX, NY, PPL>(M, tr, self, xt, tx, yt, ty, z);          \
+        continue;                                                      \
+    }
+        NO_CASES(R64T_CASE)
+#undef R64T_CASE
+#pragma unroll
+        for (int u = 0; u < PPL; u++)
+            z[u] += tr ? quad64t_rt(M, yt, ty[u], ny, xt, tx[u], nx) : quad64t_rt(M, xt, tx[u], nx, yt, ty[u], ny);
+    }
+#pragma unroll
+    for (int u = 0; u < PPL; u++) part[(wave * PPL + u) * 64 + lane] = z[u];
+    __syncthreads();
+    if (wave < PPL) {
+        const int u = wave;
+        double r = part[u * 64 + lane];
+#pragma unroll
+        for (int w = 1; w < W; w++) r += part[(w * PPL + u) * 64 + lane];
+        const int x1 = B.g1 + p1, x2 = B.g2 + p2, x3 = B.g3 + p3[u];
+        if (x1 < A.n1 && x2 < A.n2 && x3 < A.n3) as_global(A.rho)[((int64_t)x1 * A.n2 + x2) * A.n3 + x3] = r;
+    }
+}
+template <int PPL, int W>
+size_t r64t_lds(int ms) {
+    return (size_t)((ms * 64 * PPL * 4 + ms * PPL * 8 + ms * (ms + 1) / 2 * 4 + 7) / 8 * 8) + W * 64 * PPL * 8 + 16;
+}
+
+int split_density_f64_enqueue(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_offset,
+                              double *const *d_rho, bool staged) {
+    BlockPlan *P = ctx->plan;
+    if (!P) { ctx->err = "the block plan is not built"; return LCPAO_ESTATE; }
+    // LDS: the staged columns of the largest sub-block, as far as 160 KB
+    // allow (none when the columns are read from the tables)
+    const int ms = std::max(1, P->max_slots);
+    int cap = staged ? std::max(1, P->max_sub_rows) : 0;
+    while (cap > 1 && r64_lds(cap, ms).total > 160 * 1024) cap--;
+    const void *kern = staged ? reinterpret_cast<const void *>(k_rblk64)
+                              : reinterpret_cast<const void *>(k_rblk64t<1, R64_WAVES>);
+    const size_t lds = staged ? (size_t)r64_lds(cap, ms).total : r64t_lds<1, R64_WAVES>(ms);
+    SPLIT_TRY(ctx, hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const int64_t nsub = P->nblocks * 8, per = (nsub + 7) / 8;
+    for (int s = 0; s < nspin; s++) {
+        if (!d_DM[s] || !d_rho[s]) { ctx->err = "null spin pointer"; return LCPAO_EINVAL; }
+        if (ctx->ncanon > 0) {
+            KTimer kt(ctx, KID_DEFF);
+            launch_deff_f64(ctx, d_DM[s] + dm_offset, 0);
+        }
+        if (nsub > 0) {
+            KTimer kt(ctx, KID_RHO);
+            const Rho64Args A{P->d_blocks, P->d_sph_atom, P->d_sph_map, P->d_slotno, P->d_spair, ctx->d_tab_off,
+                              ctx->d_tab, ctx->d_deff, d_rho[s], nsub, ctx->ngrid[0], ctx->ngrid[1], ctx->ngrid[2],
+                              cap, ms, 0};
+            if (staged)
+                hipLaunchKernelGGL(k_rblk64, dim3((unsigned)(8 * per)), dim3(64 * R64_WAVES), lds, ctx->stream, A);
+            else
+                hipLaunchKernelGGL((k_rblk64t<1, R64_WAVES>), dim3((unsigned)(8 * per)), dim3(64 * R64_WAVES), lds,
+                                   ctx->stream, A);
+        }
+        SPLIT_TRY(ctx, hipGetLastError());
+    }
     return LCPAO_OK;
 }
 

@@ -375,13 +375,15 @@ class GridContext:
         return (_lib.dp * len(ts))(*[C.cast(t.data_ptr(), _lib.dp) for t in ts])
 
     TUNE_OPS = {"rho64": 0, "f364": 1, "rhobf": 2, "rho32": 3}
-    TUNE_NCAND = {"rho64": 6, "f364": 2, "rhobf": 2, "rho32": 4}
+    TUNE_NCAND = {"rho64": 8, "f364": 2, "rhobf": 2, "rho32": 4}
 
     def tune(self, op, variant=-1):
         """Fix the kernel variant of a tunable operation ("rho64", "f364", "rhobf", "rho32"), or
         -1 to let the next call time the candidates and keep the fastest.
         FP64 density: 0 matrix cores, 1..5 vector FMA with unit shapes
-        (waves x grid-list entries) 4x2048, 4x1024, 8x512, 2x1024, 4x512;
+        (waves x grid-list entries) 4x2048, 4x1024, 8x512, 2x1024, 4x512,
+        6 block form with the orbital columns staged in LDS, 7 block form
+        reading the columns from the tables;
         FP32 density: 0 pair lists 4x2048, 1 block form, 2 pair lists
         4x1024, 3 pair lists 8x512."""
         self._check(self.L.lcpao_grid_tune(self.h, self.TUNE_OPS[op], variant))
@@ -389,8 +391,8 @@ class GridContext:
     def tuned(self, op):
         """(chosen variant or -1, measured ms of each candidate)."""
         v = C.c_int32()
-        ms = (C.c_float * 8)()
-        self._check(self.L.lcpao_grid_tuned(self.h, self.TUNE_OPS[op], C.byref(v), ms, 8))
+        ms = (C.c_float * 16)()
+        self._check(self.L.lcpao_grid_tuned(self.h, self.TUNE_OPS[op], C.byref(v), ms, 16))
         return v.value, [round(float(x), 4) for x in ms[:self.TUNE_NCAND[op]]]
 
     def tune_load(self, path):
