@@ -126,6 +126,12 @@ __device__ __forceinline__ void ham_chunk64(const double *__restrict__ tabA, con
     }
 }
 
+// CHUNKED: any orbital count (a loop over the 32 x 32 chunks); else every
+// atom holds at most 32 orbitals and the block is one chunk -- a kernel of
+// its own, since the loop's registers (42 VGPRs against 24) cost it two
+// waves per SIMD of the occupancy that hides its gathers (GaAs216: 10.3 ms
+// against 9.3 ms).
+template <bool CHUNKED>
 __global__ __launch_bounds__(64) void k_ham_mfma64(const HUnit *__restrict__ units, const int32_t *__restrict__ order,
                                                    int32_t nunits, const PairDev *__restrict__ pairs,
                                                    const double *__restrict__ tab, const int32_t *__restrict__ olg1,
@@ -141,7 +147,7 @@ __global__ __launch_bounds__(64) void k_ham_mfma64(const HUnit *__restrict__ uni
     const int32_t *L1 = olg1 + P.olg + u.lo, *L2 = olg2 + P.olg + u.lo;
     const int32_t *glA = glist + P.gl_a;
     double *out = part + u.part;
-    if (noa <= 32 && nob <= 32) {
+    if constexpr (!CHUNKED) {
         ham_chunk64(tabA, tabB, L1, L2, glA, V, u.n, noa, nob, 0, 0, lane, out);
     } else {
         for (int r0 = 0; r0 < noa; r0 += 32)
@@ -1114,8 +1120,14 @@ int ham_dev(lcpao_ctx *ctx, int nch, const double *const *d_V, double *const *d_
         if (!d_V[c] || !d_H[c]) return set_err(ctx, LCPAO_EINVAL, "null channel pointer");
         if (nu > 0) {
             KTimer kt(ctx, KID_HAM);
-            hipLaunchKernelGGL(k_ham_mfma64, dim3(nu), dim3(64), 0, ctx->stream, ctx->d_hunits, ctx->d_horder, nu,
-                               ctx->d_pairs, ctx->d_tab, ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c], ctx->d_hpart);
+            if (ctx->maxno <= 32)
+                hipLaunchKernelGGL(k_ham_mfma64<false>, dim3(nu), dim3(64), 0, ctx->stream, ctx->d_hunits, ctx->d_horder,
+                                   nu, ctx->d_pairs, ctx->d_tab, ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c],
+                                   ctx->d_hpart);
+            else
+                hipLaunchKernelGGL(k_ham_mfma64<true>, dim3(nu), dim3(64), 0, ctx->stream, ctx->d_hunits, ctx->d_horder,
+                                   nu, ctx->d_pairs, ctx->d_tab, ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c],
+                                   ctx->d_hpart);
         }
         if (ctx->ncanon > 0) {
             KTimer kt(ctx, KID_HRED);
