@@ -1,19 +1,21 @@
 // lcpao_grid.hip -- MI355X (gfx950) kernels and C ABI for the LCPAO
 // real-space grid contractions: grid Hamiltonian, density and Force3.
 //
-// Work decomposition (all on the reference's atom-pair overlap lists):
+// Work decomposition of the pair-list kernels (on the reference's atom-pair
+// overlap lists; the block forms are in split.hip):
 //  * H  : pair-major.  Each "canonical" pair P (the one of P=(A,B,Rn) and its
 //         transpose P'=(B,A,-Rn) with the lower index, when both are local)
-//         is cut into units of <= UNIT_PTS overlap points.  A unit gathers
-//         phi_A and V*phi_B in chunks of 32 points into LDS and accumulates
-//         the NO_A x NO_B block in 4x4 FP64 register tiles; units write
-//         partial blocks that a second kernel sums per pair in a fixed order
-//         (deterministic), scales by GridVol and writes H[P] and H[P']^T.
-//  * rho: atom-owned.  Work unit = (home atom A, chunk of RHO_CHUNK entries
-//         of A's grid list).  Each wave walks the (pair, sub-range) segments
-//         of A's canonical pairs inside the chunk; a lane owns one overlap
-//         point, the pair's folded density block D_P + D_P'^T is wave-uniform
-//         and read through the scalar path.  Per-wave LDS accumulators are
+//         is cut into units of <= 2048 overlap points.  One wave per unit
+//         gathers phi_A and V*phi_B straight from the tables as the operands
+//         of FP64 MFMAs (2x2 tiles of 16x16, one 32x32 chunk per unit up to
+//         32 orbitals per atom); units write partial blocks that a second
+//         kernel sums per pair in a fixed order (deterministic), scales by
+//         GridVol and writes H[P] and H[P']^T.
+//  * rho: atom-owned.  Work unit = (home atom A, chunk of grid-list entries
+//         of A).  Each wave walks the (pair, sub-range) segments of A's
+//         canonical pairs inside the chunk; a lane owns one overlap point,
+//         the pair's folded density block D_P + D_P'^T is wave-uniform and
+//         read through the scalar path.  Per-wave LDS accumulators are
 //         summed in a fixed order; a gather kernel then sums the entries of
 //         every grid point (CSR by grid index) -- no atomics anywhere.
 //  * Force3: pair-major units over all local pairs, lane = overlap point,
