@@ -285,7 +285,11 @@ def test_density_fp32_variants(name):
 @pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7])
 @pytest.mark.parametrize("name", ["si2", "nacl8", "si2_r1of3", "gana2"])
 def test_fp64_kernel_variants(name, variant):
-    """Every candidate the autotuner may pick meets the FP64 bar."""
+    """Every candidate the autotuner may pick meets the FP64 bar.  For the
+    staged block form (6), nacl8 has 4x4x4 sub-blocks of up to 315 orbital
+    rows, beyond the 276 whose columns fit the LDS at its 32 sphere images
+    per block: those sub-blocks take the kernel's table-read branch, the
+    others are staged."""
     c = case(name)
     ctx = GridContext(c.u)
     ctx.set_tables(c.tabs)
