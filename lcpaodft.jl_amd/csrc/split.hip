@@ -326,6 +326,7 @@ __device__ __forceinline__ void tile_bf16x3(const u16x8 *__restrict__ PH, const 
     }
 }
 
+
 template <int NF>
 __device__ __forceinline__ void hblk_item(const HItem it, const BlockArgs &A, unsigned char *lds_raw) {
     const BlockDev B = as_global(A.blocks)[it.block];
@@ -429,6 +430,11 @@ __global__ __launch_bounds__(64 * HB_WAVES) void k_hblk_bf(const HItem *__restri
 }
 
 // ---- fp32 form --------------------------------------------------------------
+// DIAG: a diagonal super-tile (si == sj), whose 16x16 tiles below the
+// diagonal (rt > ct) hold only mirrored elements that the epilogue does not
+// write: their products are skipped (6 of 16 tiles).  (The bf16x3 H kernel
+// keeps one path: two inlined versions of its tile made it spill.)
+template <bool DIAG>
 __device__ __forceinline__ void tile_f32(const float *__restrict__ A32, const float *__restrict__ Vs, int si, int sj,
                                          int l16, int q, f32x4 *acc) {
     f32x8 a[4];
@@ -445,12 +451,18 @@ __device__ __forceinline__ void tile_f32(const float *__restrict__ A32, const fl
         for (int e = 0; e < 8; e++) b[e] *= Vs[q * 8 + e];
 #pragma unroll
         for (int rt = 0; rt < 4; rt++) {
+            if (DIAG && rt > ct) continue;
             f32x4 c = acc[rt * 4 + ct];
 #pragma unroll
             for (int s = 0; s < 8; s++) c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt][s], b[s], c, 0, 0, 0);
             acc[rt * 4 + ct] = c;
         }
     }
+}
+__device__ __forceinline__ void tile_f32_any(const float *__restrict__ A32, const float *__restrict__ Vs, int si, int sj,
+                                             int l16, int q, f32x4 *acc) {
+    if (si == sj) tile_f32<true>(A32, Vs, si, sj, l16, q, acc);
+    else tile_f32<false>(A32, Vs, si, sj, l16, q, acc);
 }
 
 __device__ __forceinline__ void hblk_f32_item(const HItem it, const BlockArgs &A, unsigned char *lds_raw) {
@@ -487,8 +499,8 @@ __device__ __forceinline__ void hblk_f32_item(const HItem it, const BlockArgs &A
             Vs[perm32(tid)] = v;
         }
         __syncthreads();
-        if (has0) tile_f32(A32, Vs, si0, sj0, l16, q, acc0);
-        if (has1) tile_f32(A32, Vs, si1, sj1, l16, q, acc1);
+        if (has0) tile_f32_any(A32, Vs, si0, sj0, l16, q, acc0);
+        if (has1) tile_f32_any(A32, Vs, si1, sj1, l16, q, acc1);
     }
     const EpiTables T = stage_epilogue(A, B, lds_raw);
     if (has0) store_partials(acc0, si0, sj0, lane, T, part);

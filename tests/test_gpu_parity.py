@@ -366,7 +366,8 @@ def test_fp32_density_without_cell_lists_falls_back_to_pair_lists():
     ctx = GridContext(c.u, with_cell_lists=False)
     ctx.set_tables(c.tabs)
     R = ctx.density([c.DM], precision="fp32")[0]
-    assert ctx.tuned("rho32")[0] == 0
+    v, ms = ctx.tuned("rho32")  # one of the pair-list shapes; the block form is not run
+    assert v in (0, 2, 3) and ms[1] == -1
     Rref, M = O.density(c.g, c.tabs, [c.DM], with_abs=True)
     assert worst_ratio(R, Rref[0], M[0]) <= SPLIT_RTOL["fp32"]
     ctx2 = GridContext(c.u, with_cell_lists=False)
