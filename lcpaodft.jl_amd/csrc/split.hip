@@ -1441,6 +1441,33 @@ constexpr int RF_WAVES = 16;
 constexpr int RF_SUMS_OFFSET = MAX_ROWS * LROW32 * 4;
 constexpr int RF_AREA_BYTES = RF_SUMS_OFFSET + RF_WAVES * BLK_PTS * 4;
 
+// One point per lane of a 16-point group: Q = W Phi_sj at the group's
+// points on the FP32 matrix cores, dotted with Phi_si; DIAG skips the
+// (zero) tiles below a diagonal super-tile's diagonal.
+template <bool DIAG>
+__device__ __forceinline__ float rblk_f32_point(const float *__restrict__ A32, const float (&w)[4][16], int si, int sj,
+                                                int q, int pcol) {
+    f32x4 acc[4];
+#pragma unroll
+    for (int rt = 0; rt < 4; rt++) acc[rt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kt = 0; kt < 16; kt++) {
+        const float b = A32[(sj * ST + 4 * kt + q) * LROW32 + pcol];
+#pragma unroll
+        for (int rt = 0; rt < 4; rt++) {
+            if (DIAG && rt > kt / 4) continue;
+            acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(w[rt][kt], b, acc[rt], 0, 0, 0);
+        }
+    }
+    // acc[rt][v] = Q[16 rt + 4 q + v][point 16 pt + l16]
+    float part = 0.f;
+#pragma unroll
+    for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) part = fmaf(A32[(si * ST + 16 * rt + 4 * q + v) * LROW32 + pcol], acc[rt][v], part);
+    return part;
+}
+
 __device__ __forceinline__ void rblk_f32_item(const HItem it, const BlockArgs &A, unsigned char *lds_raw) {
     const BlockDev B = as_global(A.blocks)[it.block];
     const GLOBAL_AS float *__restrict__ t32 = as_global(A.t32);
@@ -1463,9 +1490,12 @@ __device__ __forceinline__ void rblk_f32_item(const HItem it, const BlockArgs &A
     int si = 0, sj = 0;
     if (has) st_decode(it.st0 + wave, S, si, sj);
     // W fragments: w[rt][kt] = W[16 rt + l16][4 kt + q] of super-tile (si, sj)
+    // A diagonal super-tile (si == sj) holds a symmetric W: its 16x16 tiles
+    // above the diagonal carry W + W^T (factor 1 of the fold D + D^T), the
+    // diagonal tiles (D + D^T) / 2, and those below are zero and skipped.
+    const bool diag = si == sj;
     float w[4][16];
     {
-        const float wdiag = si == sj ? 0.5f : 1.f;
         const float keep = has ? 1.f : 0.f;
 #pragma unroll
         for (int rt = 0; rt < 4; rt++) {
@@ -1483,6 +1513,8 @@ __device__ __forceinline__ void rblk_f32_item(const HItem it, const BlockArgs &A
                 const int ld = P.meta & 63;
                 const int x1 = (P.meta >> 6) ? j * ld + i : i * ld + j;
                 // D is finite, so masking by a product keeps the load unconditional
+                const int ct = kt / 4;
+                const float wdiag = !diag ? 1.f : rt < ct ? 1.f : rt == ct ? 0.5f : 0.f;
                 w[rt][kt] = Deff[ok ? P.hoff + x1 : 0] * (ok ? wdiag * keep : 0.f);
             }
         }
@@ -1500,23 +1532,8 @@ __device__ __forceinline__ void rblk_f32_item(const HItem it, const BlockArgs &A
             // the lane's point: 16 pt + l16, stored at perm32 of it
             const int pcol = perm32(16 * pt + l16);
             float part = 0.f;
-            if (has) {
-                f32x4 acc[4];
-#pragma unroll
-                for (int rt = 0; rt < 4; rt++) acc[rt] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (int kt = 0; kt < 16; kt++) {
-                    const float b = A32[(sj * ST + 4 * kt + q) * LROW32 + pcol];
-#pragma unroll
-                    for (int rt = 0; rt < 4; rt++) acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(w[rt][kt], b, acc[rt], 0, 0, 0);
-                }
-                // acc[rt][v] = Q[16 rt + 4 q + v][point 16 pt + l16]
-#pragma unroll
-                for (int rt = 0; rt < 4; rt++)
-#pragma unroll
-                    for (int v = 0; v < 4; v++)
-                        part = fmaf(A32[(si * ST + 16 * rt + 4 * q + v) * LROW32 + pcol], acc[rt][v], part);
-            }
+            if (has) part = diag ? rblk_f32_point<true>(A32, w, si, sj, q, pcol)
+                                 : rblk_f32_point<false>(A32, w, si, sj, q, pcol);
             const auto s32s = __builtin_amdgcn_permlane32_swap(__float_as_uint(part), __float_as_uint(part), false, false);
             part = __uint_as_float(s32s[0]) + __uint_as_float(s32s[1]);
             const auto s16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(part), __float_as_uint(part), false, false);
