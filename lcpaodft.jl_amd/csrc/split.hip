@@ -28,6 +28,7 @@ typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
 constexpr int MAX_ROWS = 576;    // R limit (bf16x3 H: four R x 32 bf16 planes + V in 160 KB of LDS)
 constexpr int LROW32 = 36;       // LDS row stride of fp32 tiles (32 + 4 pad)
 constexpr int FB_WAVES = 8;      // waves of the Force3 workgroup, two super-tiles of one row each
+constexpr int MAX_SLOT_PAIRS = 2048;  // slot pairs of a block (tables staged in LDS by the block kernels)
 
 __device__ __forceinline__ int st_index(int si, int sj, int S) { return si * S - (si * (si - 1)) / 2 + (sj - si); }
 
@@ -190,114 +191,24 @@ inline int launch_mode(const BlockArgs *args, const int32_t *queue) {
     return args == nullptr ? LAUNCH_ONE : queue == nullptr ? LAUNCH_BATCH : LAUNCH_QUEUE;
 }
 
-// Writes the super-tile accumulators of a wave into the compact pair
-// partials.  The block's row descriptors, slot-pair partial bases and slot
-// orbital counts are staged in LDS first (the chunk buffers are free by
-// then).  Element (r, c) of C_b, r <= c, with slots a <= b and orbitals i, j
-// goes to part[base(a, b) + i * NO_b + j] (row-major slot: the 16 lanes of a
-// row write consecutive floats); an on-site block (a == b) also gets the
-// mirror element, so every element of every slot is written exactly once.
-struct EpiTables {
-    const int16_t *rinfo;
-    const int32_t *cb;
-    const int16_t *no;
-    int ns;
-};
-__device__ __forceinline__ EpiTables epi_layout(const BlockDev &B, unsigned char *lds) {
-    int32_t *cb = reinterpret_cast<int32_t *>(lds);
-    int16_t *rinfo = reinterpret_cast<int16_t *>(cb + B.nslot * B.nslot);
-    return EpiTables{rinfo, cb, rinfo + B.rows, B.nslot};
-}
-__device__ __forceinline__ EpiTables stage_epilogue(const BlockArgs &A, const BlockDev &B, unsigned char *lds) {
-    const EpiTables T = epi_layout(B, lds);
-    __syncthreads();  // every wave is done with the chunk buffers
-    const GLOBAL_AS int32_t *cbase = as_global(A.cbase);
-    const GLOBAL_AS int16_t *rowinfo = as_global(A.rowinfo), *slotno = as_global(A.slotno);
-    for (int x = threadIdx.x; x < B.nslot * B.nslot; x += blockDim.x) const_cast<int32_t *>(T.cb)[x] = cbase[B.sp_off + x];
-    for (int x = threadIdx.x; x < B.rows; x += blockDim.x) const_cast<int16_t *>(T.rinfo)[x] = rowinfo[B.row_off + x];
-    for (int x = threadIdx.x; x < B.nslot; x += blockDim.x) const_cast<int16_t *>(T.no)[x] = slotno[B.slot_off + x];
-    __syncthreads();
-    return T;
-}
-// The same tables fetched into registers while the last chunk is multiplied
-// (512 threads: up to 4 slot-pair entries, 2 rows and 1 slot each), written
-// to LDS after it.
-constexpr int EPI_SP = 4, EPI_ROWS = 2;
-struct EpiRegs {
-    int32_t cb[EPI_SP];
-    int32_t ri[EPI_ROWS];
-    int32_t no;
-};
-__device__ __forceinline__ void epi_fetch(EpiRegs &E, const BlockArgs &A, const BlockDev &B, int tid) {
-    const int nsp = B.nslot * B.nslot;
-    const GLOBAL_AS int32_t *cbase = as_global(A.cbase);
-    const GLOBAL_AS int16_t *rowinfo = as_global(A.rowinfo), *slotno = as_global(A.slotno);
+// Writes a wave's super-tile accumulators as they stand (the H partial layout
+// of part_index): tile t = 4 rt + ct of global super-tile st at
+// part[(st * 16 + t) * 256 + lane * 4], one 16-byte store per lane, so each
+// store instruction writes 1 KB of consecutive bytes.  Tiles that no pair
+// reads are skipped: those below the diagonal of a diagonal super-tile (the
+// reduction reads their mirror) and those whose rows or columns are all
+// padding.  The conditions are uniform across the wave.
+__device__ __forceinline__ void store_tiles(const f32x4 *acc, int si, int sj, int64_t st, int rused, int lane,
+                                            GLOBAL_AS float *__restrict__ part) {
+    GLOBAL_AS f32x4 *dst = reinterpret_cast<GLOBAL_AS f32x4 *>(part) + st * 16 * 64 + lane;
 #pragma unroll
-    for (int k = 0; k < EPI_SP; k++) {
-        const int x = tid + k * 64 * HB_WAVES;
-        E.cb[k] = x < nsp ? cbase[B.sp_off + x] : -1;
-    }
+    for (int rt = 0; rt < 4; rt++)
 #pragma unroll
-    for (int k = 0; k < EPI_ROWS; k++) {
-        const int x = tid + k * 64 * HB_WAVES;
-        E.ri[k] = x < B.rows ? rowinfo[B.row_off + x] : -1;
-    }
-    E.no = tid < B.nslot ? slotno[B.slot_off + tid] : 0;
-}
-__device__ __forceinline__ EpiTables epi_store(const EpiRegs &E, const BlockDev &B, unsigned char *lds, int tid) {
-    const EpiTables T = epi_layout(B, lds);
-    __syncthreads();  // every wave is done with the chunk buffers
-    const int nsp = B.nslot * B.nslot;
-#pragma unroll
-    for (int k = 0; k < EPI_SP; k++)
-        if (tid + k * 64 * HB_WAVES < nsp) const_cast<int32_t *>(T.cb)[tid + k * 64 * HB_WAVES] = E.cb[k];
-#pragma unroll
-    for (int k = 0; k < EPI_ROWS; k++)
-        if (tid + k * 64 * HB_WAVES < B.rows) const_cast<int16_t *>(T.rinfo)[tid + k * 64 * HB_WAVES] = (int16_t)E.ri[k];
-    if (tid < B.nslot) const_cast<int16_t *>(T.no)[tid] = (int16_t)E.no;
-    __syncthreads();
-    return T;
-}
-__device__ __forceinline__ void store_partials(const f32x4 *acc, int si, int sj, int lane, const EpiTables &T,
-                                               GLOBAL_AS float *__restrict__ part) {
-    const int l16 = lane & 15, q = lane >> 4;
-    // the lane's four columns (one per 16-column tile) and their slots' sizes
-    int ci[4], nb[4];
-#pragma unroll
-    for (int ct = 0; ct < 4; ct++) {
-        ci[ct] = T.rinfo[sj * ST + ct * 16 + l16];
-        nb[ct] = ci[ct] >= 0 ? T.no[ci[ct] >> 5] : 0;
-    }
-#pragma unroll
-    for (int rt = 0; rt < 4; rt++) {
-        const int r0 = si * ST + rt * 16 + 4 * q;
-        int ri[4];
-#pragma unroll
-        for (int v = 0; v < 4; v++) ri[v] = T.rinfo[r0 + v];
-        // partial bases for the slot of row r0 (the four rows mostly share it)
-        const int a0 = ri[0] >> 5;
-        int32_t base0[4];
-#pragma unroll
-        for (int ct = 0; ct < 4; ct++)
-            base0[ct] = (ri[0] >= 0 && ci[ct] >= 0) ? T.cb[a0 * T.ns + (ci[ct] >> 5)] : -1;
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int r = r0 + v;
-            if (ri[v] < 0) continue;
-            const int a = ri[v] >> 5, i = ri[v] & 31;
-#pragma unroll
-            for (int ct = 0; ct < 4; ct++) {
-                const int c = sj * ST + ct * 16 + l16;
-                if (r > c || ci[ct] < 0) continue;  // r > c only in diagonal super-tiles: mirror of (c, r)
-                const int b = ci[ct] >> 5, j = ci[ct] & 31;
-                const int32_t base = a == a0 ? base0[ct] : T.cb[a * T.ns + b];
-                if (base < 0) continue;
-                const float x = acc[rt * 4 + ct][v];
-                part[base + i * nb[ct] + j] = x;
-                if (a == b && i != j) part[base + j * nb[ct] + i] = x;
-            }
+        for (int ct = 0; ct < 4; ct++) {
+            if (si == sj && rt > ct) continue;
+            if (si * ST + rt * 16 >= rused || sj * ST + ct * 16 >= rused) continue;
+            dst[(rt * 4 + ct) * 64] = acc[rt * 4 + ct];
         }
-    }
 }
 
 inline int nfrag(int rows) { return (rows / 16 + HB_WAVES - 1) / HB_WAVES; }
@@ -357,7 +268,6 @@ __device__ __forceinline__ void hblk_item(const HItem it, const BlockArgs &A, un
         acc0[t] = f32x4{0.f, 0.f, 0.f, 0.f};
         acc1[t] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
-    EpiRegs epi;
     u16x8 fh[NF], fl[NF];
     // chunk c: fragments g = wave + 8 f of both planes
     auto prefetch = [&](int c) {
@@ -400,7 +310,6 @@ __device__ __forceinline__ void hblk_item(const HItem it, const BlockArgs &A, un
         __syncthreads();
         // c is a constant in the unrolled loop: these branches fold away
         if (c + 1 < NCHUNK) prefetch(c + 1);
-        else epi_fetch(epi, A, B, tid);  // the epilogue's tables, behind the last chunk's MFMAs
         if (c + 2 < NCHUNK) slot = touch_chunk(hi, lo, B.tile_off + (int64_t)(c + 2) * G * FRAG, R, tid);
         if (has0) tile_bf16x3(PH, PL, VH, VL, si0, sj0, lane, acc0);
         if (has1) tile_bf16x3(PH, PL, VH, VL, si1, sj1, lane, acc1);
@@ -415,9 +324,9 @@ __device__ __forceinline__ void hblk_item(const HItem it, const BlockArgs &A, un
         step(c + 1, t1);
     }
     // lane holds D[4q + v][l16] of each 16x16 tile
-    const EpiTables T = epi_store(epi, B, lds_raw, tid);
-    if (has0) store_partials(acc0, si0, sj0, lane, T, part);
-    if (has1) store_partials(acc1, si1, sj1, lane, T, part);
+    const int64_t st = (int64_t)B.st_off + it.st0 + wave;
+    if (has0) store_tiles(acc0, si0, sj0, st, B.rused, lane, part);
+    if (has1) store_tiles(acc1, si1, sj1, st + HB_WAVES, B.rused, lane, part);
 }
 
 template <int NF, int MODE>
@@ -431,8 +340,8 @@ __global__ __launch_bounds__(64 * HB_WAVES) void k_hblk_bf(const HItem *__restri
 
 // ---- fp32 form --------------------------------------------------------------
 // DIAG: a diagonal super-tile (si == sj), whose 16x16 tiles below the
-// diagonal (rt > ct) hold only mirrored elements that the epilogue does not
-// write: their products are skipped (6 of 16 tiles).  (The bf16x3 H kernel
+// diagonal (rt > ct) hold only mirrored elements that the reduction never
+// reads: their products are skipped (6 of 16 tiles).  (The bf16x3 H kernel
 // keeps one path: two inlined versions of its tile made it spill.)
 template <bool DIAG>
 __device__ __forceinline__ void tile_f32(const float *__restrict__ A32, const float *__restrict__ Vs, int si, int sj,
@@ -502,9 +411,9 @@ __device__ __forceinline__ void hblk_f32_item(const HItem it, const BlockArgs &A
         if (has0) tile_f32_any(A32, Vs, si0, sj0, l16, q, acc0);
         if (has1) tile_f32_any(A32, Vs, si1, sj1, l16, q, acc1);
     }
-    const EpiTables T = stage_epilogue(A, B, lds_raw);
-    if (has0) store_partials(acc0, si0, sj0, lane, T, part);
-    if (has1) store_partials(acc1, si1, sj1, lane, T, part);
+    const int64_t st = (int64_t)B.st_off + it.st0 + wave;
+    if (has0) store_tiles(acc0, si0, sj0, st, B.rused, lane, part);
+    if (has1) store_tiles(acc1, si1, sj1, st + HB_WAVES, B.rused, lane, part);
 }
 
 template <int MODE>
@@ -515,10 +424,11 @@ __global__ __launch_bounds__(64 * HB_WAVES) void k_hblk_f32(const HItem *__restr
                     [&](const HItem &it, const BlockArgs &A) { hblk_f32_item(it, A, lds_raw); });
 }
 
-// per computed pair: FP64 sum of its compact partials over the blocks, in
+// per computed pair: FP64 sum of its sub-blocks of C_b over the blocks, in
 // block order (deterministic), x GridVol; writes H[P] and H[P']^T.  The
-// pair's contribution records are staged in LDS so that every thread issues
-// independent loads of contiguous partial slots.
+// pair's contribution records are staged in LDS; each thread then gathers
+// its element from every block's super-tile partials (part_index), so the
+// lanes of a row read consecutive 16-byte lane slots of the same tiles.
 constexpr int RED_BATCH = 256;
 // One workgroup per canonical pair with one thread per element of its block
 // (the launch's width covers the largest NO_A x NO_B, at most 32 x 32).
@@ -528,14 +438,13 @@ __device__ __forceinline__ void hblk_reduce_pair(int64_t ci, const int32_t *__re
                                                  const PairDev *__restrict__ pairs, const float *__restrict__ part,
                                                  double gridvol, double *__restrict__ H) {
     __shared__ int64_t s_base[RED_BATCH];
-    __shared__ int32_t s_tr[RED_BATCH];
+    __shared__ int32_t s_rxy[RED_BATCH], s_st[RED_BATCH];
     const int p = canon[ci];
     const PairDev P = pairs[p];
     const int ne = P.noa * P.nob;
     const int k0 = coff[ci], k1 = coff[ci + 1];
     const int e = threadIdx.x;
     const int i = e / P.nob, j = e - i * P.nob;
-    const int et = j * P.noa + i;  // the element in a slot oriented (B, A)
     double acc = 0.0;
     for (int kb = k0; kb < k1; kb += RED_BATCH) {
         const int nb = min(RED_BATCH, k1 - kb);
@@ -543,13 +452,20 @@ __device__ __forceinline__ void hblk_reduce_pair(int64_t ci, const int32_t *__re
         for (int x = threadIdx.x; x < nb; x += blockDim.x) {  // the width may be below RED_BATCH
             const Contrib C = contrib[kb + x];
             s_base[x] = C.base;
-            s_tr[x] = C.trans;
+            s_rxy[x] = (int32_t)C.rx | ((int32_t)C.ry << 16);
+            s_st[x] = (int32_t)C.S | ((int32_t)C.trans << 16);
         }
         __syncthreads();
         if (e < ne) {
             double s = 0.0;
 #pragma unroll 8
-            for (int k = 0; k < nb; k++) s += (double)part[s_base[k] + (s_tr[k] ? et : e)];
+            for (int k = 0; k < nb; k++) {
+                const int rxy = s_rxy[k], sst = s_st[k];
+                const bool tr = (sst >> 16) != 0;
+                int r = (rxy & 0xffff) + (tr ? j : i), c = (rxy >> 16) + (tr ? i : j);
+                if (r > c) { const int t = r; r = c; c = t; }  // on-site block: the mirror element
+                s += (double)part[part_index(s_base[k], sst & 0xffff, r, c)];
+            }
             acc += s;
         }
     }
@@ -638,7 +554,7 @@ void plan_free(BlockPlan *p) {
     if (!p) return;
     void *ptrs[] = {p->d_blocks, p->d_items, p->d_sph_block, p->d_sph_atom, p->d_sph_row0, p->d_sph_map,
                     p->d_contrib, p->d_contrib_off, p->d_hi, p->d_lo, p->d_t32, p->d_part, p->d_thi, p->d_tlo,
-                    p->d_rowinfo, p->d_spair, p->d_contrib_pair, p->d_cbase, p->d_slotno, p->d_fpair,
+                    p->d_rowinfo, p->d_spair, p->d_contrib_pair, p->d_slotno, p->d_fpair,
                     p->d_fitems, p->d_fbase, p->d_fout, p->d_fatom_off, p->d_fatom_idx, p->d_dtile};
     for (void *x : ptrs)
         if (x) (void)hipFree(x);
@@ -741,9 +657,10 @@ int plan_build_inner(lcpao_ctx *ctx) {
         const int b1 = (int)(b / ((int64_t)P->nb[1] * P->nb[2])), b2 = (int)((b / P->nb[2]) % P->nb[1]), b3 = (int)(b % P->nb[2]);
         const int ns = (int)sph[b].size();
         if (roff + R > INT32_MAX) { ctx->err = "too many grid blocks for the split-precision plan"; delete P; return LCPAO_ERANGE; }
-        P->blocks[b] = {toff, spoff, R, b1 * BLK, b2 * BLK, b3 * BLK, ns, (int32_t)roff, (int32_t)soff, 0};
+        if (poff + nst > INT32_MAX) { ctx->err = "too many super-tiles for the split-precision plan"; delete P; return LCPAO_ERANGE; }
+        P->blocks[b] = {toff, spoff, R, b1 * BLK, b2 * BLK, b3 * BLK, ns, (int32_t)roff, (int32_t)soff, (int32_t)poff, r, 0};
         P->max_slots = std::max(P->max_slots, ns);
-        if (ns * ns > EPI_SP * 64 * HB_WAVES) {
+        if (ns * ns > MAX_SLOT_PAIRS) {
             ctx->err = "a grid block is covered by more sphere images than the split-precision kernels hold";
             delete P;
             return LCPAO_ERANGE;
@@ -782,7 +699,6 @@ int plan_build_inner(lcpao_ctx *ctx) {
     std::vector<int16_t> slotno(std::max<int64_t>(1, soff), 0);
     for (int64_t b = 0; b < NB; b++)
         for (size_t x = 0; x < sph[b].size(); x++) slotno[P->blocks[b].slot_off + x] = (int16_t)ctx->no[sph[b][x].first];
-    std::vector<int32_t> cbase(std::max<int64_t>(1, spoff), -1);
     std::vector<SlotPair> spairs(std::max<int64_t>(1, spoff), SlotPair{-1, 0});
     std::vector<SlotPair> fpairs(std::max<int64_t>(1, spoff), SlotPair{-1, 0});  // blocks own disjoint ranges
     // sphere tables and point maps
@@ -888,22 +804,20 @@ int plan_build_inner(lcpao_ctx *ctx) {
     std::vector<Contrib> contrib(all.size());
     std::vector<int32_t> cpair(all.size());
     std::vector<int32_t> coff(ctx->canon.size() + 1, 0);
-    // compact partial slots: contribution k holds NO_x * NO_y floats, in
-    // canonical-pair order so that a pair's slots are contiguous
-    int64_t pel = 0;
+    // contribution records, in canonical-pair order and block order within a
+    // pair: where the pair's sub-block lies in the block's super-tile partials
     for (size_t i = 0; i < all.size(); i++) {
-        const BlockDev &B = P->blocks[all[i].block];
+        const int64_t b = all[i].block;
+        const BlockDev &B = P->blocks[b];
         cpair[i] = ctx->canon[all[i].cpos];
-        contrib[i] = Contrib{pel, all[i].block, all[i].trans};
+        contrib[i] = Contrib{(int64_t)B.st_off * ST * ST, (int16_t)rows0[b][all[i].x], (int16_t)rows0[b][all[i].y],
+                             (int16_t)(B.rows / ST), (int16_t)all[i].trans};
         coff[all[i].cpos + 1]++;
         const PairDev &Q = ctx->pairs[cpair[i]];
         const int64_t sp = B.sp_off + (int64_t)all[i].x * B.nslot + all[i].y;
         spairs[sp] = SlotPair{(int32_t)Q.hoff, Q.nob | (all[i].trans << 6)};
-        cbase[sp] = (int32_t)pel;
-        pel += (int64_t)slotno[B.slot_off + all[i].x] * slotno[B.slot_off + all[i].y];
     }
-    P->part_elems = pel;
-    if (pel > INT32_MAX) { ctx->err = "H partials exceed the split-precision index range"; delete P; return LCPAO_ERANGE; }
+    P->part_elems = P->part_tiles * ST * ST;
     P->ncontrib = (int64_t)all.size();
     for (int32_t p : ctx->canon)
         P->red_threads = std::max(P->red_threads, (ctx->pairs[p].noa * ctx->pairs[p].nob + 63) / 64 * 64);
@@ -938,7 +852,7 @@ int plan_build_inner(lcpao_ctx *ctx) {
         (rc = up(ctx, &P->d_sph_block, sb)) || (rc = up(ctx, &P->d_sph_atom, sa)) || (rc = up(ctx, &P->d_sph_row0, sr)) ||
         (rc = up(ctx, &P->d_sph_map, smap)) || (rc = up(ctx, &P->d_contrib, contrib)) || (rc = up(ctx, &P->d_contrib_off, coff)) ||
         (rc = up(ctx, &P->d_contrib_pair, cpair)) || (rc = up(ctx, &P->d_rowinfo, rowinfo)) ||
-        (rc = up(ctx, &P->d_spair, spairs)) || (rc = up(ctx, &P->d_cbase, cbase)) ||
+        (rc = up(ctx, &P->d_spair, spairs)) ||
         (rc = up(ctx, &P->d_slotno, slotno)) || (rc = up(ctx, &P->d_fpair, fpairs)) ||
         (rc = up(ctx, &P->d_fitems, P->fitems)) ||
         (rc = up(ctx, &P->d_fbase, fbase)) || (rc = up(ctx, &P->d_fatom_off, fatom_off)) ||
@@ -1614,7 +1528,7 @@ BlockArgs block_args(const lcpao_ctx *ctx, const double *V, double *rho) {
     a.hi = P->d_hi; a.lo = P->d_lo; a.thi = P->d_thi; a.tlo = P->d_tlo;
     a.rowinfo = P->d_rowinfo; a.spair = P->d_spair;
     a.V = V; a.Deff = ctx->d_deff32; a.part = P->d_part; a.rho = rho;
-    a.cbase = P->d_cbase; a.slotno = P->d_slotno; a.t32 = P->d_t32;
+    a.slotno = P->d_slotno; a.t32 = P->d_t32;
     a.n1 = ctx->ngrid[0]; a.n2 = ctx->ngrid[1]; a.n3 = ctx->ngrid[2];
     return a;
 }
@@ -2728,6 +2642,6 @@ extern "C" int lcpao_grid_block_layout(const lcpao_geom *g, int32_t *max_rows, i
     if (max_rows) *max_rows = mr;
     if (max_slots) *max_slots = ms;
     if (nblocks) *nblocks = NB;
-    const bool fits = maxno <= 32 && mr <= MAX_ROWS && ms * ms <= EPI_SP * 64 * HB_WAVES && ms <= FB_MAX_SLOTS;
+    const bool fits = maxno <= 32 && mr <= MAX_ROWS && ms * ms <= MAX_SLOT_PAIRS && ms <= FB_MAX_SLOTS;
     return fits ? LCPAO_OK : LCPAO_ERANGE;
 }

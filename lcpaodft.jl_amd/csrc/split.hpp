@@ -38,6 +38,8 @@ struct BlockDev {
     int32_t nslot;      // sphere images covering the block
     int32_t row_off;    // offset of the block's row descriptors (R entries)
     int32_t slot_off;   // offset of the block's slot orbital counts (nslot entries)
+    int32_t st_off;     // index of the block's first upper super-tile among all blocks' (H partials)
+    int32_t rused;      // orbital rows in use (rows rused .. R - 1 are padding)
     int32_t pad;
 };
 
@@ -66,8 +68,7 @@ struct BlockArgs {
     const float *Deff;             // density: folded pair blocks (FP32)
     float *part;                   // H: super-tile partials
     double *rho;                   // density: output
-    const int32_t *cbase;          // H: compact partial slot of each slot pair (-1: none)
-    const int16_t *slotno;         // H: orbital count of each slot
+    const int16_t *slotno;         // orbital count of each slot
     const float *t32;              // FP32 tiles (the FP32 forms)
     int32_t n1, n2, n3, pad;
 };
@@ -77,13 +78,26 @@ struct ItemClass {
     int32_t rows, first, count;
 };
 
-// one (pair, block) sub-block of C_b, kept as a compact partial of
-// NO_x * NO_y floats at `base` in the block's slot orientation (x <= y):
-// element (i, j) of H[pair] is partial[i * NO_B + j], or partial[j * NO_A + i]
-// when trans != 0 (the pair's atom A is slot y)
+// H partials: every block's upper 64x64 super-tiles of C_b as the matrix
+// cores leave them, super-tile st of block b at float offset
+// (st_off(b) + st) * 4096; inside it, 16x16 tile t = 4 rt + ct at t * 256,
+// lane l = 16 (r / 4) + c holding rows 4 (l / 16) .. + 3 of column l % 16 as
+// four consecutive floats (r, c: row and column inside the tile).
+__host__ __device__ __forceinline__ int64_t part_index(int64_t base, int S, int r, int c) {
+    const int si = r >> 6, sj = c >> 6;
+    const int st = si * S - (si * (si - 1)) / 2 + (sj - si);
+    return base + (int64_t)st * 4096 + ((((r & 63) >> 4) * 4 + ((c & 63) >> 4)) * 64 + ((r & 15) >> 2) * 16 + (c & 15)) * 4 +
+           (r & 3);
+}
+
+// one (pair, block) sub-block of C_b: slots x <= y of the block, whose rows
+// start at rx and ry; element (i, j) of H[pair] is C_b[rx + i][ry + j], or
+// C_b[rx + j][ry + i] when trans != 0 (the pair's atom A is slot y); an
+// element below the diagonal (on-site blocks, x == y) is read from its
+// mirror.  base = st_off(block) * 4096, S = super-tiles per block row.
 struct Contrib {
     int64_t base;
-    int32_t block, trans;
+    int16_t rx, ry, S, trans;
 };
 
 struct BlockPlan {
@@ -116,9 +130,8 @@ struct BlockPlan {
     int32_t red_threads = 64;      // reduce width: the largest NO_A x NO_B of a computed pair, rounded to 64
     float *d_t32 = nullptr;                     // fp32 tiles (permuted point order)
     bool bf16_ready = false, f32_ready = false;
-    float *d_part = nullptr;       // compact partials (part_elems floats)
+    float *d_part = nullptr;       // H partials: part_tiles super-tiles of 4096 floats
     int64_t part_elems = 0;
-    int32_t *d_cbase = nullptr;    // per block slot pair
     int16_t *d_slotno = nullptr;   // per block slot
     // bf16x3 Force3: the ordered pair of every slot pair (hoff of D(pair),
     // meta = orbitals of the second slot; hoff < 0: no pair), items over all
