@@ -214,7 +214,9 @@ int lcpao_grid_kernel_stats(const lcpao_ctx *ctx, int32_t kernel_id, int64_t *la
  *   LCPAO_TUNE_F364   FP64 Force3   (0 matrix-core kernel, 1 vector-FMA kernel)
  *   LCPAO_TUNE_RHOBF  bf16x3 density (0 single-buffered chunk planes,
  *                     1 double-buffered, used only when its LDS fits;
- *                     both give the same bits)
+ *                     2 double-buffered with the chunks loaded straight
+ *                     into LDS, used only for blocks of at most 512 rows;
+ *                     all three form each point's sum in the same order)
  *   LCPAO_TUNE_RHO32  FP32 density  (0 pair-list kernel 4 x 2048, 1 block
  *                     form on the FP32 matrix cores, used only when the block
  *                     plan holds the geometry, 2 pair lists 4 x 1024, 3 pair

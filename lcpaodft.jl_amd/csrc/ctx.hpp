@@ -154,10 +154,11 @@ enum { TUNE_RHO64 = 0, TUNE_F364 = 1, TUNE_RHOBF = 2, TUNE_RHO32 = 3, TUNE_NOPS 
 // (waves x chunk) 4 x 2048, 4 x 1024, 8 x 512, 2 x 1024, 4 x 512, 6 the
 // block form (vector FMA over the sub-blocks of the block plan, orbital
 // columns staged in LDS), 7 the block form reading the columns from the
-// tables; FP32 density: 0 pair lists 4 x 2048, 1 block form, 2 pair lists
-// 4 x 1024, 3 pair lists 8 x 512
+// tables; bf16x3 density: 0 single-buffered chunk planes, 1 double-buffered,
+// 2 chunks loaded straight into LDS; FP32 density: 0 pair lists 4 x 2048,
+// 1 block form, 2 pair lists 4 x 1024, 3 pair lists 8 x 512
 constexpr int RHO64_BLOCK = 6, RHO64_BLOCK_TAB = 7;
-constexpr int TUNE_NCAND[TUNE_NOPS] = {8, 2, 2, 4};
+constexpr int TUNE_NCAND[TUNE_NOPS] = {8, 2, 3, 4};
 constexpr int TUNE_MAXCAND = 8;
 
 

@@ -1038,6 +1038,64 @@ __device__ __forceinline__ void release_n(const TouchN<NT> &t) {
     for (int k = 0; k < TouchN<NT>::K; k++) asm volatile("" ::"v"(t.v[k]));
 }
 
+// W fragments of one super-tile (si, sj) of a block for the 16-wave density
+// kernels, gathered straight into the A-operand layout: fragment (rt, ks) of
+// lane l holds W[16 rt + l % 16][32 ks + 8 (l / 16) + e], e < 8, split into
+// bf16 hi and lo.  W element = mask * Deff[one index] (on-site blocks hold
+// D + D^T, so every element is a single load), halved on a diagonal
+// super-tile (whose mirrored half is not computed), zero for padding rows,
+// absent pairs or a wave without a super-tile (keep = false); 32 loads in
+// flight per batch, the batches kept apart so that the compiler does not
+// hoist all 64 loads.
+__device__ __forceinline__ void gather_w16(const GLOBAL_AS float *__restrict__ Deff, const int16_t *s_rinfo,
+                                           const SlotPair *s_sp, int nslot, int si, int sj, bool keep, int l16, int q,
+                                           bf16x8 (&wh)[4][2], bf16x8 (&wl)[4][2]) {
+    constexpr int GB = 4;
+    const float wdiag = si == sj ? 0.5f : 1.f;
+    const float kf = keep ? 1.f : 0.f;
+#pragma unroll
+    for (int b2 = 0; b2 < 8; b2 += GB) {
+        float dv[GB][8];
+        uint32_t okbits = 0;
+#pragma unroll
+        for (int r2 = 0; r2 < GB; r2++) {
+            const int rt = (b2 + r2) >> 1, ks = (b2 + r2) & 1;
+            const int rinf = s_rinfo[si * ST + 16 * rt + l16];
+            const int ra = rinf >> 5, ri = rinf & 31;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int cinf = s_rinfo[sj * ST + 32 * ks + 8 * q + e];
+                const int cb = cinf >> 5, cj = cinf & 31;
+                const bool valid = rinf >= 0 && cinf >= 0;
+                const SlotPair P = s_sp[valid ? min(ra, cb) * nslot + max(ra, cb) : 0];
+                const bool sw = ra > cb;  // lower triangle of a diagonal super-tile: W is symmetric
+                const int i = sw ? cj : ri, j = sw ? ri : cj;
+                const bool ok = valid && P.hoff >= 0;
+                const int ld = P.meta & 63;
+                const int x1 = (P.meta >> 6) ? j * ld + i : i * ld + j;
+                dv[r2][e] = Deff[ok ? P.hoff + x1 : 0];  // unconditional: loads issue back to back
+                okbits |= (ok ? 1u : 0u) << (r2 * 8 + e);
+            }
+        }
+#pragma unroll
+        for (int r2 = 0; r2 < GB; r2++) {
+            const int rt = (b2 + r2) >> 1, ks = (b2 + r2) & 1;
+            u16x8 wh8, wl8;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                // D is finite, so masking by a product keeps the load unconditional
+                const float w = dv[r2][e] * (((okbits >> (r2 * 8 + e)) & 1u) ? wdiag * kf : 0.f);
+                const __bf16 hb = (__bf16)w;
+                wh8[e] = __builtin_bit_cast(uint16_t, hb);
+                wl8[e] = __builtin_bit_cast(uint16_t, (__bf16)(w - (float)hb));
+            }
+            wh[rt][ks] = __builtin_bit_cast(bf16x8, wh8);
+            wl[rt][ks] = __builtin_bit_cast(bf16x8, wl8);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
 // Density of one item: W waves, each owning PER = 16 / W of the item's
 // super-tiles.  With W = 16 a wave holds the W fragments of one super-tile
 // (64 VGPRs), so four waves fit per SIMD.  DB (W = 16 only): double-buffered
@@ -1085,55 +1143,7 @@ __device__ __forceinline__ void rblk_item(const HItem it, const BlockArgs &A, un
         if (has[h]) st_decode(it.st0 + wave + h * W, S, si[h], sj[h]);
     }
     if constexpr (PER == 1) {
-        // W gathered straight into the A-operand fragments: fragment (rt, ks)
-        // of lane l holds W[16 rt + l % 16][32 ks + 8 (l / 16) + e], e < 8.
-        // W element = mask * Deff[one index] (on-site blocks hold D + D^T, so
-        // every element is a single load); 32 loads in flight per batch, the
-        // batches kept apart so that the compiler does not hoist all 64 loads.
-        constexpr int GB = 4;
-        const float wdiag = si[0] == sj[0] ? 0.5f : 1.f;
-        const float keep = has[0] ? 1.f : 0.f;
-#pragma unroll
-        for (int b2 = 0; b2 < 8; b2 += GB) {
-            float dv[GB][8];
-            uint32_t okbits = 0;
-#pragma unroll
-            for (int r2 = 0; r2 < GB; r2++) {
-                const int rt = (b2 + r2) >> 1, ks = (b2 + r2) & 1;
-                const int rinf = s_rinfo[si[0] * ST + 16 * rt + l16];
-                const int ra = rinf >> 5, ri = rinf & 31;
-#pragma unroll
-                for (int e = 0; e < 8; e++) {
-                    const int cinf = s_rinfo[sj[0] * ST + 32 * ks + 8 * q + e];
-                    const int cb = cinf >> 5, cj = cinf & 31;
-                    const bool valid = rinf >= 0 && cinf >= 0;
-                    const SlotPair P = s_sp[valid ? min(ra, cb) * B.nslot + max(ra, cb) : 0];
-                    const bool sw = ra > cb;  // lower triangle of a diagonal super-tile: W is symmetric
-                    const int i = sw ? cj : ri, j = sw ? ri : cj;
-                    const bool ok = valid && P.hoff >= 0;
-                    const int ld = P.meta & 63;
-                    const int x1 = (P.meta >> 6) ? j * ld + i : i * ld + j;
-                    dv[r2][e] = Deff[ok ? P.hoff + x1 : 0];  // unconditional: loads issue back to back
-                    okbits |= (ok ? 1u : 0u) << (r2 * 8 + e);
-                }
-            }
-#pragma unroll
-            for (int r2 = 0; r2 < GB; r2++) {
-                const int rt = (b2 + r2) >> 1, ks = (b2 + r2) & 1;
-                u16x8 wh8, wl8;
-#pragma unroll
-                for (int e = 0; e < 8; e++) {
-                    // D is finite, so masking by a product keeps the load unconditional
-                    const float w = dv[r2][e] * (((okbits >> (r2 * 8 + e)) & 1u) ? wdiag * keep : 0.f);
-                    const __bf16 hb = (__bf16)w;
-                    wh8[e] = __builtin_bit_cast(uint16_t, hb);
-                    wl8[e] = __builtin_bit_cast(uint16_t, (__bf16)(w - (float)hb));
-                }
-                wh[0][rt][ks] = __builtin_bit_cast(bf16x8, wh8);
-                wl[0][rt][ks] = __builtin_bit_cast(bf16x8, wl8);
-            }
-            __builtin_amdgcn_sched_barrier(0);
-        }
+        gather_w16(Deff, s_rinfo, s_sp, B.nslot, si[0], sj[0], has[0], l16, q, wh[0], wl[0]);
     } else {
         // two super-tiles per wave: W of each is gathered into a per-wave LDS
         // image laid out as the A-operand fragments [rt * 2 + ks][hi, lo][lane][8]
@@ -1340,6 +1350,183 @@ __device__ __forceinline__ void rblk_item(const HItem it, const BlockArgs &A, un
     }
 }
 
+// ---- bf16x3 density, chunks loaded straight into LDS -------------------------
+// The 16-wave density with the chunk planes brought in by LDS-DMA loads
+// (global_load_lds_dwordx4: lane l of a wave writes its 16 bytes at lane * 16
+// of a 1 KB fragment), double-buffered: while chunk c is multiplied out of
+// one buffer, the loads of chunk c + 1 fill the other, with no prefetch
+// registers and one barrier per chunk.  The two buffers are distinct LDS
+// variables, so the compiler can tell the loads in flight into one from the
+// reads of the other.  The registers this frees hold one accumulator per
+// 16-row group: the products of the four groups are issued term by term
+// across the groups (each accumulator sees the terms in the order of
+// k_rblk_bf, so the sums are the same bits) and their chains no longer wait
+// on one another, and the dots of a 16-point group follow its products.
+// Limits: R <= DMA_ROWS (two fragments per wave and plane).
+constexpr int DMA_ROWS = 512;
+// fragments per wave and plane of the 16-wave density kernels
+inline int nfrag_rho(int rows) { return (rows / 16 + RB_WAVES - 1) / RB_WAVES; }
+constexpr int DMA_PLANE = DMA_ROWS * CHUNK;  // bf16 elements of one plane of one buffer
+
+struct DmaLds {
+    uint16_t *hi0, *lo0, *hi1, *lo1;  // chunk buffers 0 and 1 (point-fragment order)
+    float *wsum;                      // [chunk % 2][wave][32 points]
+    float *psum;                      // [512 points]
+    int16_t *rinfo;
+    SlotPair *sp;
+};
+
+__device__ __forceinline__ void dma_frag(const GLOBAL_AS uint16_t *src, uint16_t *dst) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_global_load_lds((const GLOBAL_AS void *)src, (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
+#endif
+}
+
+template <int NF>
+__device__ __forceinline__ void rblk_item_dma(const HItem it, const BlockArgs &A, const DmaLds &L) {
+    constexpr int W = RB_WAVES;
+    const BlockDev B = as_global(A.blocks)[it.block];
+    const GLOBAL_AS uint16_t *__restrict__ thi = as_global(A.thi);
+    const GLOBAL_AS uint16_t *__restrict__ tlo = as_global(A.tlo);
+    const GLOBAL_AS int16_t *__restrict__ rowinfo = as_global(A.rowinfo);
+    const GLOBAL_AS SlotPair *__restrict__ spair = as_global(A.spair);
+    const GLOBAL_AS float *__restrict__ Deff = as_global(A.Deff);
+    double *__restrict__ rho = A.rho;
+    const int n1 = A.n1, n2 = A.n2, n3 = A.n3;
+    const int R = B.rows, S = R / ST, G = R / 16, G32 = R / 32;
+    const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63, l16 = lane & 15,
+              q = lane >> 4;
+    for (int x = tid; x < R; x += 64 * W) L.rinfo[x] = rowinfo[B.row_off + x];
+    for (int x = tid; x < B.nslot * B.nslot; x += 64 * W) L.sp[x] = spair[B.sp_off + x];
+    // chunk c's fragments g = wave + 16 f of both planes into buffer c % 2
+    auto load_chunk = [&](int c, uint16_t *bh, uint16_t *bl) {
+        const GLOBAL_AS uint16_t *sh = thi + B.tile_off + (int64_t)c * G * FRAG;
+        const GLOBAL_AS uint16_t *sl = tlo + B.tile_off + (int64_t)c * G * FRAG;
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            const int g = wave + f * W;
+            if (g < G) {  // uniform across the wave
+                dma_frag(sh + (g * 64 + lane) * 8, bh + g * FRAG);
+                dma_frag(sl + (g * 64 + lane) * 8, bl + g * FRAG);
+            }
+        }
+    };
+    load_chunk(0, L.hi0, L.lo0);
+    TouchN<64 * W> t0 = touch_chunk_n<64 * W>(thi, tlo, B.tile_off + (int64_t)G * FRAG, R, tid);
+    TouchN<64 * W> t1 = touch_chunk_n<64 * W>(thi, tlo, B.tile_off + (int64_t)2 * G * FRAG, R, tid);
+    __syncthreads();  // the row descriptors and slot pairs
+    const bool has = wave < it.nst;
+    int si = 0, sj = 0;
+    if (has) st_decode(it.st0 + wave, S, si, sj);
+    bf16x8 wh[4][2], wl[4][2];
+    gather_w16(Deff, L.rinfo, L.sp, B.nslot, si, sj, has, l16, q, wh, wl);
+    auto add_wave_sums = [&](int c) {  // the 16 wave sums of chunk c's points, in wave order
+        if (wave == c % W && lane < CHUNK) {
+            const float *src = L.wsum + (c & 1) * W * CHUNK;
+            float sum = 0.f;
+#pragma unroll
+            for (int w0 = 0; w0 < W; w0 += 4) {
+                float v[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) v[k] = src[(w0 + k) * CHUNK + lane];
+#pragma unroll
+                for (int k = 0; k < 4; k++) sum += v[k];
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            L.psum[c * CHUNK + lane] = sum;
+        }
+    };
+    auto step = [&](int c, const uint16_t *BHs, const uint16_t *BLs, uint16_t *NH, uint16_t *NL, TouchN<64 * W> &slot) {
+        // this wave's loads of chunk c (and its touches) are complete, then
+        // every wave's: chunk c is in LDS and chunk c - 1's buffer is free
+        __builtin_amdgcn_s_waitcnt(0x3f70);  // vmcnt(0)
+        release_n(slot);
+        __syncthreads();
+        if (c > 0) add_wave_sums(c - 1);
+        if (c + 1 < NCHUNK) load_chunk(c + 1, NH, NL);
+        if (c + 3 < NCHUNK) slot = touch_chunk_n<64 * W>(thi, tlo, B.tile_off + (int64_t)(c + 3) * G * FRAG, R, tid);
+        const u16x8 *BH = reinterpret_cast<const u16x8 *>(BHs);
+        const u16x8 *BL = reinterpret_cast<const u16x8 *>(BLs);
+#pragma unroll 1
+        for (int pt = 0; pt < 2; pt++) {
+            f32x4 acc[4];
+#pragma unroll
+            for (int rt = 0; rt < 4; rt++) acc[rt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < 2; ks++) {
+                const int fi = pt * G32 + sj * 2 + ks;
+                const bf16x8 bh = __builtin_bit_cast(bf16x8, BH[fi * 64 + lane]);
+                const bf16x8 bl = __builtin_bit_cast(bf16x8, BL[fi * 64 + lane]);
+#pragma unroll
+                for (int rt = 0; rt < 4; rt++) acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl[rt][ks], bh, acc[rt], 0, 0, 0);
+#pragma unroll
+                for (int rt = 0; rt < 4; rt++) acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[rt][ks], bl, acc[rt], 0, 0, 0);
+#pragma unroll
+                for (int rt = 0; rt < 4; rt++) acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[rt][ks], bh, acc[rt], 0, 0, 0);
+            }
+            // acc[rt][v] = Q[row si*64 + 16 rt + 4 q + v][point 16 pt + l16]; Phi at the
+            // same four rows and point: 8 contiguous bytes of a point fragment
+            float part = 0.f;
+#pragma unroll
+            for (int rt = 0; rt < 4; rt++) {
+                const int fi = pt * G32 + si * 2 + (rt >> 1);
+                const int off = (fi * 64 + (2 * (rt & 1) + (q >> 1)) * 16 + l16) * 8 + 4 * (q & 1);
+                const u16x4 ph = *reinterpret_cast<const u16x4 *>(BHs + off);
+                const u16x4 pl = *reinterpret_cast<const u16x4 *>(BLs + off);
+#pragma unroll
+                for (int v = 0; v < 4; v++) part = fmaf(bf_bits_to_f32(ph[v]) + bf_bits_to_f32(pl[v]), acc[rt][v], part);
+            }
+            const auto s32 = __builtin_amdgcn_permlane32_swap(__float_as_uint(part), __float_as_uint(part), false, false);
+            part = __uint_as_float(s32[0]) + __uint_as_float(s32[1]);
+            const auto s16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(part), __float_as_uint(part), false, false);
+            part = __uint_as_float(s16[0]) + __uint_as_float(s16[1]);
+            if (q == 0) L.wsum[(c & 1) * W * CHUNK + wave * CHUNK + pt * 16 + l16] = has ? part : 0.f;
+        }
+    };
+    // fully unrolled: the buffers alternate as distinct variables
+#pragma unroll
+    for (int c = 0; c < NCHUNK; c += 2) {
+        step(c, L.hi0, L.lo0, L.hi1, L.lo1, t0);
+        step(c + 1, L.hi1, L.lo1, L.hi0, L.lo0, t1);
+    }
+    __syncthreads();
+    add_wave_sums(NCHUNK - 1);
+    __syncthreads();
+    if (tid < BLK_PTS) {
+        const int k = tid;
+        const float sum = L.psum[k];
+        const int x1 = B.g1 + k / (BLK * BLK), x2 = B.g2 + (k / BLK) % BLK, x3 = B.g3 + k % BLK;
+        if (x1 < n1 && x2 < n2 && x3 < n3) atomicAdd(&rho[((int64_t)x1 * n2 + x2) * n3 + x3], (double)sum);
+    }
+}
+
+template <int NF, int MODE>
+__global__ __launch_bounds__(64 * RB_WAVES) void k_rblk_dma(const HItem *__restrict__ items, int32_t nitems,
+                                                           const BlockArgs *__restrict__ args, BlockArgs one,
+                                                           int32_t *__restrict__ queue) {
+    __shared__ __align__(16) uint16_t s_hi0[DMA_PLANE], s_lo0[DMA_PLANE], s_hi1[DMA_PLANE], s_lo1[DMA_PLANE];
+    __shared__ __align__(16) float s_wsum[2 * RB_WAVES * CHUNK], s_psum[BLK_PTS];
+    __shared__ __align__(16) int16_t s_rinfo[MAX_ROWS];
+    __shared__ __align__(16) SlotPair s_sp[MAX_SLOT_PAIRS];
+    const DmaLds L{s_hi0, s_lo0, s_hi1, s_lo1, s_wsum, s_psum, s_rinfo, s_sp};
+    run_items<MODE>(items, nitems, args, one, queue,
+                    [&](const HItem &it, const BlockArgs &A) { rblk_item_dma<NF>(it, A, L); });
+}
+
+bool rho_dma_fits(int rows) { return rows <= DMA_ROWS; }
+
+hipError_t launch_rblk_dma(hipStream_t stream, int rows, const HItem *items, int32_t nitems, const BlockArgs *args,
+                           const BlockArgs &one) {
+    if (!rho_dma_fits(rows)) return hipErrorInvalidValue;
+    const int mode = launch_mode(args, nullptr);
+    void (*kern)(const HItem *, int32_t, const BlockArgs *, BlockArgs, int32_t *) = nullptr;
+    if (nfrag_rho(rows) == 1) kern = mode == LAUNCH_ONE ? k_rblk_dma<1, LAUNCH_ONE> : k_rblk_dma<1, LAUNCH_BATCH>;
+    else kern = mode == LAUNCH_ONE ? k_rblk_dma<2, LAUNCH_ONE> : k_rblk_dma<2, LAUNCH_BATCH>;
+    if (nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(kern, dim3((unsigned)nitems), dim3(64 * RB_WAVES), 0, stream, items, nitems, args, one, nullptr);
+    return hipGetLastError();
+}
+
 // ---- FP32 block density -----------------------------------------------------
 // rho_k = sum_{si<=sj} Phi_si^T W[si][sj] Phi_sj as in k_rblk_bf, with FP32
 // operands on the FP32 matrix cores (v_mfma_f32_16x16x4f32): one 1 024-thread
@@ -1503,7 +1690,6 @@ hipError_t launch_f32_block_kernel(bool density, hipStream_t stream, int rows, i
 }
 
 // fragments per wave and plane of the density kernel for R rows
-inline int nfrag_rho(int rows) { return (rows / 16 + RB_WAVES - 1) / RB_WAVES; }
 constexpr int MAXF_RHO = (MAX_ROWS / 16 + RB_WAVES - 1) / RB_WAVES;
 
 // waves of the density workgroup: 16 for one structure; the batched
@@ -1612,14 +1798,16 @@ hipError_t launch_block_kernel(bool density, hipStream_t stream, int rows, int m
     return hipGetLastError();
 }
 
-// Autotuner of the block kernels' candidates `launch(v)`: each once
-// untimed, then TUNE_ROUNDS timed launches per candidate in alternating
-// order, keeping each one's fastest (a single launch each is within the
-// noise of a GPU); the fastest candidate is kept in ctx->variant[op].
+// Autotuner of the block kernels' candidates `launch(v)` for which fits(v)
+// holds: each once untimed, then TUNE_ROUNDS timed launches per candidate in
+// alternating order, keeping each one's fastest (a single launch each is
+// within the noise of a GPU); the fastest candidate is kept in
+// ctx->variant[op], and a candidate that does not fit reads -1 ms.
 constexpr int TUNE_ROUNDS = 4;
-template <class F>
-hipError_t tune_block_kernel(lcpao_ctx *ctx, int op, F launch) {
+template <class F, class FITS>
+hipError_t tune_block_kernel(lcpao_ctx *ctx, int op, F launch, FITS fits) {
     const int nc = TUNE_NCAND[op];
+    for (int v = 0; v < nc; v++) ctx->variant_ms[op][v] = -1.f;
     hipEvent_t e0, e1;
     hipError_t e = hipEventCreate(&e0);
     if (e != hipSuccess) return e;
@@ -1627,10 +1815,12 @@ hipError_t tune_block_kernel(lcpao_ctx *ctx, int op, F launch) {
         (void)hipEventDestroy(e0);
         return e;
     }
-    for (int v = 0; v < nc && e == hipSuccess; v++) e = launch(v);
+    for (int v = 0; v < nc && e == hipSuccess; v++)
+        if (fits(v)) e = launch(v);
     for (int r = 0; r < TUNE_ROUNDS && e == hipSuccess; r++)
         for (int x = 0; x < nc && e == hipSuccess; x++) {
             const int v = r % 2 ? nc - 1 - x : x;
+            if (!fits(v)) continue;
             float ms = 0.f;
             if ((e = hipEventRecord(e0, ctx->stream)) != hipSuccess || (e = launch(v)) != hipSuccess ||
                 (e = hipEventRecord(e1, ctx->stream)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess ||
@@ -1644,7 +1834,7 @@ hipError_t tune_block_kernel(lcpao_ctx *ctx, int op, F launch) {
     if (e != hipSuccess) return e;
     int best = 0;
     for (int v = 1; v < nc; v++)
-        if (ctx->variant_ms[op][v] < ctx->variant_ms[op][best]) best = v;
+        if (fits(v) && ctx->variant_ms[op][v] < ctx->variant_ms[op][best]) best = v;
     ctx->variant[op] = best;
     return hipSuccess;
 }
@@ -1691,26 +1881,27 @@ int split_density(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t 
             const unsigned ni = (unsigned)P->items.size();
             const BlockArgs one = block_args(ctx, nullptr, d_rho[s]);
             // candidates: 0 = single-buffered chunk planes, 1 = double-buffered
-            // (when its LDS fits the plan's largest slot count)
+            // (when its LDS fits the plan's largest slot count), 2 = chunks
+            // loaded straight into LDS (R <= DMA_ROWS)
+            auto fits = [&](int v) {
+                return v == 0 || (v == 1 && rho_db_fits(P->max_rows, P->max_slots)) ||
+                       (v == 2 && rho_dma_fits(P->max_rows));
+            };
             auto launch = [&](int v) {
+                if (v == 2) return launch_rblk_dma(ctx->stream, P->max_rows, P->d_items, (int32_t)ni, nullptr, one);
                 return launch_block_kernel(true, ctx->stream, P->max_rows, P->max_slots, P->d_items, (int32_t)ni,
                                            nullptr, one, nullptr, ni, v == 1);
             };
-            const bool db_ok = rho_db_fits(P->max_rows, P->max_slots);
             if (ctx->variant[TUNE_RHOBF] < 0) {
-                if (!db_ok) {
-                    ctx->variant[TUNE_RHOBF] = 0;
-                } else {
-                    // the candidates add into rho, which is cleared again
-                    // before the real launch
-                    SPLIT_TRY(ctx, tune_block_kernel(ctx, TUNE_RHOBF, launch));
-                    if (ctx->nn > 0)
-                        hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((ctx->nn + 255) / 256)), dim3(256), 0,
-                                           ctx->stream, ctx->nn, d_rho[s], 0.0);
-                }
+                // the candidates add into rho, which is cleared again before
+                // the real launch
+                SPLIT_TRY(ctx, tune_block_kernel(ctx, TUNE_RHOBF, launch, fits));
+                if (ctx->nn > 0)
+                    hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((ctx->nn + 255) / 256)), dim3(256), 0, ctx->stream,
+                                       ctx->nn, d_rho[s], 0.0);
             }
             KTimer kt(ctx, KID_RHO);
-            SPLIT_TRY(ctx, launch(ctx->variant[TUNE_RHOBF] == 1 && db_ok ? 1 : 0));
+            SPLIT_TRY(ctx, launch(fits(ctx->variant[TUNE_RHOBF]) ? ctx->variant[TUNE_RHOBF] : 0));
         }
         SPLIT_TRY(ctx, hipGetLastError());
     }

@@ -99,7 +99,7 @@ def _energies(gaas, H, rho):
     return 2.0 * float(np.dot(gaas["DM"], H)), 2.0 * u.GridVol * float(np.dot(gaas["V"], rho))
 
 
-@pytest.mark.parametrize("prec,op,variant", [("bf16x3", "rhobf", 0), ("bf16x3", "rhobf", 1),
+@pytest.mark.parametrize("prec,op,variant", [("bf16x3", "rhobf", 0), ("bf16x3", "rhobf", 1), ("bf16x3", "rhobf", 2),
                                              ("fp32", "rho32", 0), ("fp32", "rho32", 1), ("fp32", "rho32", 2),
                                              ("fp32", "rho32", 3)])
 def test_gaas216_precision_ladder(gaas, prec, op, variant):
