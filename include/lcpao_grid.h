@@ -216,7 +216,10 @@ int lcpao_grid_kernel_stats(const lcpao_ctx *ctx, int32_t kernel_id, int64_t *la
  *                     1 double-buffered, used only when its LDS fits;
  *                     2 double-buffered with the chunks loaded straight
  *                     into LDS, used only for blocks of at most 512 rows;
- *                     all three form each point's sum in the same order)
+ *                     3 items of up to 8 super-tiles, two workgroups per
+ *                     CU, blocks of at most 384 rows; 0-2 form each
+ *                     point's sum in the same order, 3 adds the FP32 sums
+ *                     of a block's items in FP64)
  *   LCPAO_TUNE_RHO32  FP32 density  (0 pair-list kernel 4 x 2048, 1 block
  *                     form on the FP32 matrix cores, used only when the block
  *                     plan holds the geometry, 2 pair lists 4 x 1024, 3 pair

@@ -67,6 +67,20 @@ enum {
     KID_ADEN = 7, KID_FFT = 8, KID_PGRID = 9, KID_BLOCH = 10, KID_DMK = 11, KID_COUNT = 12
 };
 
+// tunable operations and their candidate counts
+enum { TUNE_RHO64 = 0, TUNE_F364 = 1, TUNE_RHOBF = 2, TUNE_RHO32 = 3, TUNE_NOPS = 4 };
+// FP64 density: 0 matrix cores, 1..5 vector FMA with unit shapes
+// (waves x chunk) 4 x 2048, 4 x 1024, 8 x 512, 2 x 1024, 4 x 512, 6 the
+// block form (vector FMA over the sub-blocks of the block plan, orbital
+// columns staged in LDS), 7 the block form reading the columns from the
+// tables; bf16x3 density: 0 single-buffered chunk planes, 1 double-buffered,
+// 2 chunks loaded straight into LDS, 3 8-wave items (two workgroups per CU);
+// FP32 density: 0 pair lists 4 x 2048, 1 block form, 2 pair lists 4 x 1024,
+// 3 pair lists 8 x 512
+constexpr int RHO64_BLOCK = 6, RHO64_BLOCK_TAB = 7;
+constexpr int TUNE_NCAND[TUNE_NOPS] = {8, 2, 4, 4};
+constexpr int TUNE_MAXCAND = 8;
+
 struct lcpao_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -142,24 +156,12 @@ struct lcpao_ctx {
     std::vector<hipEvent_t> event_pool;
     // kernel variants chosen by the autotuner (-1: time the candidates on the
     // first call with this geometry), see lcpao_grid_tune
-    int32_t variant[4] = {-1, -1, -1, -1};
-    float variant_ms[4][8] = {};
+    int32_t variant[TUNE_NOPS] = {-1, -1, -1, -1};
+    float variant_ms[TUNE_NOPS][TUNE_MAXCAND] = {};
     int64_t prof_launches[KID_COUNT] = {};
     double prof_ms[KID_COUNT] = {};
 };
 
-// tunable operations and their candidate counts
-enum { TUNE_RHO64 = 0, TUNE_F364 = 1, TUNE_RHOBF = 2, TUNE_RHO32 = 3, TUNE_NOPS = 4 };
-// FP64 density: 0 matrix cores, 1..5 vector FMA with unit shapes
-// (waves x chunk) 4 x 2048, 4 x 1024, 8 x 512, 2 x 1024, 4 x 512, 6 the
-// block form (vector FMA over the sub-blocks of the block plan, orbital
-// columns staged in LDS), 7 the block form reading the columns from the
-// tables; bf16x3 density: 0 single-buffered chunk planes, 1 double-buffered,
-// 2 chunks loaded straight into LDS; FP32 density: 0 pair lists 4 x 2048,
-// 1 block form, 2 pair lists 4 x 1024, 3 pair lists 8 x 512
-constexpr int RHO64_BLOCK = 6, RHO64_BLOCK_TAB = 7;
-constexpr int TUNE_NCAND[TUNE_NOPS] = {8, 2, 3, 4};
-constexpr int TUNE_MAXCAND = 8;
 
 
 

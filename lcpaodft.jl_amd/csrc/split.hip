@@ -29,6 +29,7 @@ constexpr int MAX_ROWS = 576;    // R limit (bf16x3 H: four R x 32 bf16 planes +
 constexpr int LROW32 = 36;       // LDS row stride of fp32 tiles (32 + 4 pad)
 constexpr int FB_WAVES = 8;      // waves of the Force3 workgroup, two super-tiles of one row each
 constexpr int MAX_SLOT_PAIRS = 2048;  // slot pairs of a block (tables staged in LDS by the block kernels)
+constexpr int RH_WAVES = 8;           // waves of the two-per-CU density workgroup, one super-tile each
 
 __device__ __forceinline__ int st_index(int si, int sj, int S) { return si * S - (si * (si - 1)) / 2 + (sj - si); }
 
@@ -153,6 +154,33 @@ __device__ __forceinline__ Touch touch_chunk(const GLOBAL_AS uint16_t *__restric
     return t;
 }
 __device__ __forceinline__ void release(const Touch &t) { asm volatile("" ::"v"(t.a), "v"(t.b)); }
+
+// L2 warm-up of a chunk by NT threads: one dword per 128-byte line, K loads
+// per thread (lines past the chunk repeat line 0), as touch_chunk.
+template <int NT>
+struct TouchN {
+    static constexpr int K = (MAX_ROWS + NT - 1) / NT;
+    uint32_t v[K];
+};
+template <int NT>
+__device__ __forceinline__ TouchN<NT> touch_chunk_n(const GLOBAL_AS uint16_t *__restrict__ hi,
+                                                    const GLOBAL_AS uint16_t *__restrict__ lo, int64_t off, int R,
+                                                    int tid) {
+    TouchN<NT> t;
+    const int half = R / 2;
+#pragma unroll
+    for (int k = 0; k < TouchN<NT>::K; k++) {
+        const int x = tid + k * NT < R ? tid + k * NT : 0;
+        t.v[k] = *reinterpret_cast<const GLOBAL_AS uint32_t *>((x < half ? hi : lo) + off +
+                                                               (int64_t)(x < half ? x : x - half) * 64);
+    }
+    return t;
+}
+template <int NT>
+__device__ __forceinline__ void release_n(const TouchN<NT> &t) {
+#pragma unroll
+    for (int k = 0; k < TouchN<NT>::K; k++) asm volatile("" ::"v"(t.v[k]));
+}
 
 // Work distribution of the block kernels (MODE, a template argument, so that
 // each schedule compiles to a kernel of its own and the one-pass forms carry
@@ -552,7 +580,7 @@ void deal_xcd(const BlockPlan *P, std::vector<HItem> &items, int n1) {
 
 void plan_free(BlockPlan *p) {
     if (!p) return;
-    void *ptrs[] = {p->d_blocks, p->d_items, p->d_sph_block, p->d_sph_atom, p->d_sph_row0, p->d_sph_map,
+    void *ptrs[] = {p->d_blocks, p->d_items, p->d_items8, p->d_sph_block, p->d_sph_atom, p->d_sph_row0, p->d_sph_map,
                     p->d_contrib, p->d_contrib_off, p->d_hi, p->d_lo, p->d_t32, p->d_part, p->d_thi, p->d_tlo,
                     p->d_rowinfo, p->d_spair, p->d_contrib_pair, p->d_slotno, p->d_fpair,
                     p->d_fitems, p->d_fbase, p->d_fout, p->d_fatom_off, p->d_fatom_idx, p->d_dtile};
@@ -672,6 +700,14 @@ int plan_build_inner(lcpao_ctx *ctx) {
         roff += R;
         for (int st0 = 0; st0 < nst; st0 += HB_WAVES * ST_PER_WAVE)
             P->items.push_back({(int32_t)b, st0, std::min(HB_WAVES * ST_PER_WAVE, nst - st0), 0});
+        {
+            const int n8 = (nst + RH_WAVES - 1) / RH_WAVES;
+            for (int k = 0, st0 = 0; k < n8; k++) {
+                const int cnt = nst / n8 + (k < nst % n8 ? 1 : 0);
+                P->items8.push_back({(int32_t)b, st0, cnt, 0});
+                st0 += cnt;
+            }
+        }
         sph_first[b + 1] = sph_first[b] + (int64_t)sph[b].size();
     }
     P->tile_elems = toff;
@@ -689,6 +725,9 @@ int plan_build_inner(lcpao_ctx *ctx) {
         P->classes.back().count++;
     }
     deal_xcd(P, P->items, ctx->ngrid[0]);
+    // a block's items are consecutive in its slab, so they run one after the
+    // other on one XCD and its L2 serves the second its tiles
+    deal_xcd(P, P->items8, ctx->ngrid[0]);
     // row descriptors: slot * 32 + orbital (-1 for padding rows)
     std::vector<int16_t> rowinfo(std::max<int64_t>(1, roff), -1);
     for (int64_t b = 0; b < NB; b++)
@@ -849,6 +888,7 @@ int plan_build_inner(lcpao_ctx *ctx) {
     int rc;
     ctx->plan = P;
     if ((rc = up(ctx, &P->d_blocks, P->blocks)) || (rc = up(ctx, &P->d_items, P->items)) ||
+        (rc = up(ctx, &P->d_items8, P->items8)) ||
         (rc = up(ctx, &P->d_sph_block, sb)) || (rc = up(ctx, &P->d_sph_atom, sa)) || (rc = up(ctx, &P->d_sph_row0, sr)) ||
         (rc = up(ctx, &P->d_sph_map, smap)) || (rc = up(ctx, &P->d_contrib, contrib)) || (rc = up(ctx, &P->d_contrib_off, coff)) ||
         (rc = up(ctx, &P->d_contrib_pair, cpair)) || (rc = up(ctx, &P->d_rowinfo, rowinfo)) ||
@@ -897,6 +937,47 @@ int split_tables(lcpao_ctx *ctx, int precision) {
     SPLIT_TRY(ctx, hipStreamSynchronize(ctx->stream));
     (bf ? P->bf16_ready : P->f32_ready) = true;
     return LCPAO_OK;
+}
+
+// Autotuner of the block kernels' candidates `launch(v)` for which fits(v)
+// holds: each once untimed, then TUNE_ROUNDS timed launches per candidate in
+// alternating order, keeping each one's fastest (a single launch each is
+// within the noise of a GPU); the fastest candidate is kept in
+// ctx->variant[op], and a candidate that does not fit reads -1 ms.
+constexpr int TUNE_ROUNDS = 4;
+template <class F, class FITS>
+hipError_t tune_block_kernel(lcpao_ctx *ctx, int op, F launch, FITS fits) {
+    const int nc = TUNE_NCAND[op];
+    for (int v = 0; v < nc; v++) ctx->variant_ms[op][v] = -1.f;
+    hipEvent_t e0, e1;
+    hipError_t e = hipEventCreate(&e0);
+    if (e != hipSuccess) return e;
+    if ((e = hipEventCreate(&e1)) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return e;
+    }
+    for (int v = 0; v < nc && e == hipSuccess; v++)
+        if (fits(v)) e = launch(v);
+    for (int r = 0; r < TUNE_ROUNDS && e == hipSuccess; r++)
+        for (int x = 0; x < nc && e == hipSuccess; x++) {
+            const int v = r % 2 ? nc - 1 - x : x;
+            if (!fits(v)) continue;
+            float ms = 0.f;
+            if ((e = hipEventRecord(e0, ctx->stream)) != hipSuccess || (e = launch(v)) != hipSuccess ||
+                (e = hipEventRecord(e1, ctx->stream)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess ||
+                (e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess)
+                break;
+            float &m = ctx->variant_ms[op][v];
+            if (r == 0 || ms < m) m = ms;
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return e;
+    int best = 0;
+    for (int v = 1; v < nc; v++)
+        if (fits(v) && ctx->variant_ms[op][v] < ctx->variant_ms[op][best]) best = v;
+    ctx->variant[op] = best;
+    return hipSuccess;
 }
 
 int split_hamiltonian(lcpao_ctx *ctx, int nch, const double *const *d_V, double *const *d_H, int precision) {
@@ -1011,32 +1092,6 @@ constexpr int RB_DB_WSUM_OFFSET = 2 * RB_DB_BUF_BYTES;
 constexpr int RB_DB_PSUM_OFFSET = RB_DB_WSUM_OFFSET + 2 * RB_WAVES * CHUNK * 4;
 constexpr int RB_DB_AREA_BYTES = RB_DB_PSUM_OFFSET + BLK_PTS * 4;
 
-// L2 warm-up of a chunk by NT threads: one dword per 128-byte line, K loads
-// per thread (lines past the chunk repeat line 0), as touch_chunk.
-template <int NT>
-struct TouchN {
-    static constexpr int K = (MAX_ROWS + NT - 1) / NT;
-    uint32_t v[K];
-};
-template <int NT>
-__device__ __forceinline__ TouchN<NT> touch_chunk_n(const GLOBAL_AS uint16_t *__restrict__ hi,
-                                                    const GLOBAL_AS uint16_t *__restrict__ lo, int64_t off, int R,
-                                                    int tid) {
-    TouchN<NT> t;
-    const int half = R / 2;
-#pragma unroll
-    for (int k = 0; k < TouchN<NT>::K; k++) {
-        const int x = tid + k * NT < R ? tid + k * NT : 0;
-        t.v[k] = *reinterpret_cast<const GLOBAL_AS uint32_t *>((x < half ? hi : lo) + off +
-                                                               (int64_t)(x < half ? x : x - half) * 64);
-    }
-    return t;
-}
-template <int NT>
-__device__ __forceinline__ void release_n(const TouchN<NT> &t) {
-#pragma unroll
-    for (int k = 0; k < TouchN<NT>::K; k++) asm volatile("" ::"v"(t.v[k]));
-}
 
 // W fragments of one super-tile (si, sj) of a block for the 16-wave density
 // kernels, gathered straight into the A-operand layout: fragment (rt, ks) of
@@ -1444,11 +1499,14 @@ __device__ __forceinline__ void rblk_item_dma(const HItem it, const BlockArgs &A
         __syncthreads();
         if (c > 0) add_wave_sums(c - 1);
         if (c + 1 < NCHUNK) load_chunk(c + 1, NH, NL);
-        if (c + 3 < NCHUNK) slot = touch_chunk_n<64 * W>(thi, tlo, B.tile_off + (int64_t)(c + 3) * G * FRAG, R, tid);
+        // chunk c + 2 into L2 one step before its loads (further ahead, L2 thrashes)
+        if (c + 2 < NCHUNK) slot = touch_chunk_n<64 * W>(thi, tlo, B.tile_off + (int64_t)(c + 2) * G * FRAG, R, tid);
         const u16x8 *BH = reinterpret_cast<const u16x8 *>(BHs);
         const u16x8 *BL = reinterpret_cast<const u16x8 *>(BLs);
 #pragma unroll 1
         for (int pt = 0; pt < 2; pt++) {
+            float part = 0.f;
+            if (has) {  // a wave without a super-tile only joins the sums and barriers
             f32x4 acc[4];
 #pragma unroll
             for (int rt = 0; rt < 4; rt++) acc[rt] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -1466,7 +1524,6 @@ __device__ __forceinline__ void rblk_item_dma(const HItem it, const BlockArgs &A
             }
             // acc[rt][v] = Q[row si*64 + 16 rt + 4 q + v][point 16 pt + l16]; Phi at the
             // same four rows and point: 8 contiguous bytes of a point fragment
-            float part = 0.f;
 #pragma unroll
             for (int rt = 0; rt < 4; rt++) {
                 const int fi = pt * G32 + si * 2 + (rt >> 1);
@@ -1476,11 +1533,12 @@ __device__ __forceinline__ void rblk_item_dma(const HItem it, const BlockArgs &A
 #pragma unroll
                 for (int v = 0; v < 4; v++) part = fmaf(bf_bits_to_f32(ph[v]) + bf_bits_to_f32(pl[v]), acc[rt][v], part);
             }
+            }
             const auto s32 = __builtin_amdgcn_permlane32_swap(__float_as_uint(part), __float_as_uint(part), false, false);
             part = __uint_as_float(s32[0]) + __uint_as_float(s32[1]);
             const auto s16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(part), __float_as_uint(part), false, false);
             part = __uint_as_float(s16[0]) + __uint_as_float(s16[1]);
-            if (q == 0) L.wsum[(c & 1) * W * CHUNK + wave * CHUNK + pt * 16 + l16] = has ? part : 0.f;
+            if (q == 0) L.wsum[(c & 1) * W * CHUNK + wave * CHUNK + pt * 16 + l16] = part;
         }
     };
     // fully unrolled: the buffers alternate as distinct variables
@@ -1524,6 +1582,157 @@ hipError_t launch_rblk_dma(hipStream_t stream, int rows, const HItem *items, int
     else kern = mode == LAUNCH_ONE ? k_rblk_dma<2, LAUNCH_ONE> : k_rblk_dma<2, LAUNCH_BATCH>;
     if (nitems <= 0) return hipSuccess;
     hipLaunchKernelGGL(kern, dim3((unsigned)nitems), dim3(64 * RB_WAVES), 0, stream, items, nitems, args, one, nullptr);
+    return hipGetLastError();
+}
+
+// ---- bf16x3 density, two workgroups per CU -----------------------------------
+// The density of k_rblk_bf in items of at most RH_WAVES super-tiles (one per
+// wave, 8 waves, at most 128 VGPRs) with 68 KB of LDS, so that two
+// workgroups share a CU: their barriers are independent, and the multiplies
+// of one run while the other waits for its tiles or gathers its W.  The point
+// fragments of a chunk are two independent 16-point halves (each is all that
+// the products and dots of its points read), so the tiles stream through two
+// half-chunk buffers (distinct LDS variables) filled by LDS-DMA loads: half h
+// is multiplied out of one while half h + 1 lands in the other, one barrier
+// per half.  A block's items follow one another on one XCD, whose L2 then
+// serves the second its tiles.  Each point's sum over a wave's super-tile and
+// over the waves (in wave order, by one wave after the next barrier) as in
+// k_rblk_bf, and one FP64 atomic per item and point.  Limits: R <= RH_ROWS.
+constexpr int RH_ROWS = 384;
+constexpr int RH_HALF = RH_ROWS * 16;  // bf16 elements of one plane of one half-chunk buffer
+
+struct RhLds {
+    uint16_t *hi0, *lo0, *hi1, *lo1;  // half-chunk buffers 0 and 1
+    float *wsum;                      // [half % 2][wave][16 points]
+    float *psum;                      // [512 points]
+    int16_t *rinfo;
+    SlotPair *sp;
+};
+
+__device__ __forceinline__ void rblk_item_w8(const HItem it, const BlockArgs &A, const RhLds &L) {
+    constexpr int W = RH_WAVES, NH = 2 * NCHUNK, HP = CHUNK / 2;
+    const BlockDev B = as_global(A.blocks)[it.block];
+    const GLOBAL_AS uint16_t *__restrict__ thi = as_global(A.thi);
+    const GLOBAL_AS uint16_t *__restrict__ tlo = as_global(A.tlo);
+    const GLOBAL_AS int16_t *__restrict__ rowinfo = as_global(A.rowinfo);
+    const GLOBAL_AS SlotPair *__restrict__ spair = as_global(A.spair);
+    const GLOBAL_AS float *__restrict__ Deff = as_global(A.Deff);
+    double *__restrict__ rho = A.rho;
+    const int n1 = A.n1, n2 = A.n2, n3 = A.n3;
+    const int R = B.rows, S = R / ST, G32 = R / 32;
+    const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63, l16 = lane & 15,
+              q = lane >> 4;
+    for (int x = tid; x < R; x += 64 * W) L.rinfo[x] = rowinfo[B.row_off + x];
+    for (int x = tid; x < B.nslot * B.nslot; x += 64 * W) L.sp[x] = spair[B.sp_off + x];
+    // half h (chunk h / 2, points 16 (h % 2) ..): fragments g = wave + 8 f < G32 of both planes
+    auto load_half = [&](int h, uint16_t *bh, uint16_t *bl) {
+        const GLOBAL_AS uint16_t *sh = thi + B.tile_off + (int64_t)h * G32 * FRAG;
+        const GLOBAL_AS uint16_t *sl = tlo + B.tile_off + (int64_t)h * G32 * FRAG;
+#pragma unroll
+        for (int f = 0; f < (RH_ROWS / 32 + W - 1) / W; f++) {
+            const int g = wave + f * W;
+            if (g < G32) {  // uniform across the wave
+                dma_frag(sh + (g * 64 + lane) * 8, bh + g * FRAG);
+                dma_frag(sl + (g * 64 + lane) * 8, bl + g * FRAG);
+            }
+        }
+    };
+    load_half(0, L.hi0, L.lo0);
+    __syncthreads();  // the row descriptors and slot pairs
+    const bool has = wave < it.nst;
+    int si = 0, sj = 0;
+    if (has) st_decode(it.st0 + wave, S, si, sj);
+    bf16x8 wh[4][2], wl[4][2];
+    gather_w16(Deff, L.rinfo, L.sp, B.nslot, si, sj, has, l16, q, wh, wl);
+    auto add_wave_sums = [&](int h) {  // the 8 wave sums of half h's points, in wave order
+        if (wave == h % W && lane < HP) {
+            const float *src = L.wsum + (h & 1) * W * HP;
+            float sum = 0.f;
+#pragma unroll
+            for (int w = 0; w < W; w++) sum += src[w * HP + lane];
+            L.psum[h * HP + lane] = sum;
+        }
+    };
+    auto step = [&](int h, const uint16_t *BHs, const uint16_t *BLs, uint16_t *NH_, uint16_t *NL_) {
+        // this wave's loads of half h are complete, then every wave's: half h
+        // is in LDS and the other buffer (half h - 1) is free
+        __builtin_amdgcn_s_waitcnt(0x3f70);  // vmcnt(0)
+        __syncthreads();
+        if (h > 0) add_wave_sums(h - 1);
+        if (h + 1 < NH) load_half(h + 1, NH_, NL_);
+        const u16x8 *BH = reinterpret_cast<const u16x8 *>(BHs);
+        const u16x8 *BL = reinterpret_cast<const u16x8 *>(BLs);
+        float part = 0.f;
+        if (has) {
+            bf16x8 bh[2], bl[2];
+#pragma unroll
+            for (int ks = 0; ks < 2; ks++) {
+                bh[ks] = __builtin_bit_cast(bf16x8, BH[(sj * 2 + ks) * 64 + lane]);
+                bl[ks] = __builtin_bit_cast(bf16x8, BL[(sj * 2 + ks) * 64 + lane]);
+            }
+#pragma unroll
+            for (int rt = 0; rt < 4; rt++) {
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int ks = 0; ks < 2; ks++) {
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl[rt][ks], bh[ks], acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[rt][ks], bl[ks], acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[rt][ks], bh[ks], acc, 0, 0, 0);
+                }
+                // acc[v] = Q[row si*64 + 16 rt + 4 q + v][point l16 of the half]
+                const int fi = si * 2 + (rt >> 1);
+                const int off = (fi * 64 + (2 * (rt & 1) + (q >> 1)) * 16 + l16) * 8 + 4 * (q & 1);
+                const u16x4 ph = *reinterpret_cast<const u16x4 *>(BHs + off);
+                const u16x4 pl = *reinterpret_cast<const u16x4 *>(BLs + off);
+#pragma unroll
+                for (int v = 0; v < 4; v++) part = fmaf(bf_bits_to_f32(ph[v]) + bf_bits_to_f32(pl[v]), acc[v], part);
+            }
+        }
+        const auto s32 = __builtin_amdgcn_permlane32_swap(__float_as_uint(part), __float_as_uint(part), false, false);
+        part = __uint_as_float(s32[0]) + __uint_as_float(s32[1]);
+        const auto s16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(part), __float_as_uint(part), false, false);
+        part = __uint_as_float(s16[0]) + __uint_as_float(s16[1]);
+        if (q == 0) L.wsum[(h & 1) * W * HP + wave * HP + l16] = part;
+    };
+    // fully unrolled: the buffers alternate as distinct variables
+#pragma unroll
+    for (int h = 0; h < NH; h += 2) {
+        step(h, L.hi0, L.lo0, L.hi1, L.lo1);
+        step(h + 1, L.hi1, L.lo1, L.hi0, L.lo0);
+    }
+    __syncthreads();
+    add_wave_sums(NH - 1);
+    __syncthreads();
+    {
+        const int k = tid;  // 512 threads, one point each
+        const float sum = L.psum[k];
+        const int x1 = B.g1 + k / (BLK * BLK), x2 = B.g2 + (k / BLK) % BLK, x3 = B.g3 + k % BLK;
+        if (x1 < n1 && x2 < n2 && x3 < n3) atomicAdd(&rho[((int64_t)x1 * n2 + x2) * n3 + x3], (double)sum);
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(64 * RH_WAVES) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_rblk_w8(
+    const HItem *__restrict__ items, int32_t nitems, const BlockArgs *__restrict__ args, BlockArgs one,
+    int32_t *__restrict__ queue) {
+    __shared__ __align__(16) uint16_t s_hi0[RH_HALF], s_lo0[RH_HALF], s_hi1[RH_HALF], s_lo1[RH_HALF];
+    __shared__ __align__(16) float s_wsum[2 * RH_WAVES * CHUNK / 2], s_psum[BLK_PTS];
+    __shared__ __align__(16) int16_t s_rinfo[RH_ROWS];
+    __shared__ __align__(16) SlotPair s_sp[MAX_SLOT_PAIRS];
+    const RhLds L{s_hi0, s_lo0, s_hi1, s_lo1, s_wsum, s_psum, s_rinfo, s_sp};
+    run_items<MODE>(items, nitems, args, one, queue, [&](const HItem &it, const BlockArgs &A) { rblk_item_w8(it, A, L); });
+}
+
+bool rho_w8_fits(int rows) { return rows <= RH_ROWS; }
+
+hipError_t launch_rblk_w8(hipStream_t stream, int rows, const HItem *items, int32_t nitems, const BlockArgs *args,
+                          const BlockArgs &one) {
+    if (!rho_w8_fits(rows)) return hipErrorInvalidValue;
+    const int mode = launch_mode(args, nullptr);
+    void (*kern)(const HItem *, int32_t, const BlockArgs *, BlockArgs, int32_t *) =
+        mode == LAUNCH_ONE ? k_rblk_w8<LAUNCH_ONE> : k_rblk_w8<LAUNCH_BATCH>;
+    if (nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(kern, dim3((unsigned)nitems), dim3(64 * RH_WAVES), 0, stream, items, nitems, args, one, nullptr);
     return hipGetLastError();
 }
 
@@ -1798,47 +2007,6 @@ hipError_t launch_block_kernel(bool density, hipStream_t stream, int rows, int m
     return hipGetLastError();
 }
 
-// Autotuner of the block kernels' candidates `launch(v)` for which fits(v)
-// holds: each once untimed, then TUNE_ROUNDS timed launches per candidate in
-// alternating order, keeping each one's fastest (a single launch each is
-// within the noise of a GPU); the fastest candidate is kept in
-// ctx->variant[op], and a candidate that does not fit reads -1 ms.
-constexpr int TUNE_ROUNDS = 4;
-template <class F, class FITS>
-hipError_t tune_block_kernel(lcpao_ctx *ctx, int op, F launch, FITS fits) {
-    const int nc = TUNE_NCAND[op];
-    for (int v = 0; v < nc; v++) ctx->variant_ms[op][v] = -1.f;
-    hipEvent_t e0, e1;
-    hipError_t e = hipEventCreate(&e0);
-    if (e != hipSuccess) return e;
-    if ((e = hipEventCreate(&e1)) != hipSuccess) {
-        (void)hipEventDestroy(e0);
-        return e;
-    }
-    for (int v = 0; v < nc && e == hipSuccess; v++)
-        if (fits(v)) e = launch(v);
-    for (int r = 0; r < TUNE_ROUNDS && e == hipSuccess; r++)
-        for (int x = 0; x < nc && e == hipSuccess; x++) {
-            const int v = r % 2 ? nc - 1 - x : x;
-            if (!fits(v)) continue;
-            float ms = 0.f;
-            if ((e = hipEventRecord(e0, ctx->stream)) != hipSuccess || (e = launch(v)) != hipSuccess ||
-                (e = hipEventRecord(e1, ctx->stream)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess ||
-                (e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess)
-                break;
-            float &m = ctx->variant_ms[op][v];
-            if (r == 0 || ms < m) m = ms;
-        }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (e != hipSuccess) return e;
-    int best = 0;
-    for (int v = 1; v < nc; v++)
-        if (fits(v) && ctx->variant_ms[op][v] < ctx->variant_ms[op][best]) best = v;
-    ctx->variant[op] = best;
-    return hipSuccess;
-}
-
 int split_density_f32_enqueue(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_offset,
                               double *const *d_rho) {
     BlockPlan *P = ctx->plan;
@@ -1882,13 +2050,17 @@ int split_density(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t 
             const BlockArgs one = block_args(ctx, nullptr, d_rho[s]);
             // candidates: 0 = single-buffered chunk planes, 1 = double-buffered
             // (when its LDS fits the plan's largest slot count), 2 = chunks
-            // loaded straight into LDS (R <= DMA_ROWS)
+            // loaded straight into LDS (R <= DMA_ROWS), 3 = 8-wave items, two
+            // workgroups per CU (R <= RH_ROWS)
             auto fits = [&](int v) {
                 return v == 0 || (v == 1 && rho_db_fits(P->max_rows, P->max_slots)) ||
-                       (v == 2 && rho_dma_fits(P->max_rows));
+                       (v == 2 && rho_dma_fits(P->max_rows)) || (v == 3 && rho_w8_fits(P->max_rows));
             };
             auto launch = [&](int v) {
                 if (v == 2) return launch_rblk_dma(ctx->stream, P->max_rows, P->d_items, (int32_t)ni, nullptr, one);
+                if (v == 3)
+                    return launch_rblk_w8(ctx->stream, P->max_rows, P->d_items8, (int32_t)P->items8.size(), nullptr,
+                                          one);
                 return launch_block_kernel(true, ctx->stream, P->max_rows, P->max_slots, P->d_items, (int32_t)ni,
                                            nullptr, one, nullptr, ni, v == 1);
             };

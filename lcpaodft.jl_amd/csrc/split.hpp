@@ -116,6 +116,10 @@ struct BlockPlan {
     // device
     BlockDev *d_blocks = nullptr;
     HItem *d_items = nullptr;
+    // the same super-tiles in items of at most RH_WAVES (the 8-wave density
+    // kernel, two workgroups per CU), split evenly per block, XCD-dealt
+    std::vector<HItem> items8;
+    HItem *d_items8 = nullptr;
     int32_t *d_sph_block = nullptr, *d_sph_atom = nullptr, *d_sph_row0 = nullptr;
     int32_t *d_sph_map = nullptr;  // nsph x BLK_PTS table columns (-1: not covered)
     Contrib *d_contrib = nullptr;

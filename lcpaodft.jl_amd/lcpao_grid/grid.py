@@ -375,7 +375,7 @@ class GridContext:
         return (_lib.dp * len(ts))(*[C.cast(t.data_ptr(), _lib.dp) for t in ts])
 
     TUNE_OPS = {"rho64": 0, "f364": 1, "rhobf": 2, "rho32": 3}
-    TUNE_NCAND = {"rho64": 8, "f364": 2, "rhobf": 3, "rho32": 4}
+    TUNE_NCAND = {"rho64": 8, "f364": 2, "rhobf": 4, "rho32": 4}
 
     def tune(self, op, variant=-1):
         """Fix the kernel variant of a tunable operation ("rho64", "f364", "rhobf", "rho32"), or
@@ -385,7 +385,8 @@ class GridContext:
         6 block form with the orbital columns staged in LDS, 7 block form
         reading the columns from the tables;
         bf16x3 density: 0 single-buffered chunk planes, 1 double-buffered,
-        2 double-buffered with the chunks loaded straight into LDS;
+        2 double-buffered with the chunks loaded straight into LDS, 3 items
+        of up to 8 super-tiles, two workgroups per CU;
         FP32 density: 0 pair lists 4x2048, 1 block form, 2 pair lists
         4x1024, 3 pair lists 8x512."""
         self._check(self.L.lcpao_grid_tune(self.h, self.TUNE_OPS[op], variant))

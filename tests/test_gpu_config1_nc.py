@@ -107,7 +107,7 @@ def test_config1_si2_h_rho_force3(prec):
 def _variants(prec):
     # every kernel the autotuner may pick for the density of this form
     return {"fp64": [("rho64", v) for v in range(8)], "fp32": [("rho32", v) for v in range(4)],
-            "bf16x3": [("rhobf", v) for v in range(3)]}[prec]
+            "bf16x3": [("rhobf", v) for v in range(4)]}[prec]
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])

@@ -100,6 +100,7 @@ def _energies(gaas, H, rho):
 
 
 @pytest.mark.parametrize("prec,op,variant", [("bf16x3", "rhobf", 0), ("bf16x3", "rhobf", 1), ("bf16x3", "rhobf", 2),
+                                             ("bf16x3", "rhobf", 3),
                                              ("fp32", "rho32", 0), ("fp32", "rho32", 1), ("fp32", "rho32", 2),
                                              ("fp32", "rho32", 3)])
 def test_gaas216_precision_ladder(gaas, prec, op, variant):

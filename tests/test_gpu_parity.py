@@ -235,15 +235,18 @@ def test_density_bf16x3_block_form(name):
 
 @pytest.mark.parametrize("name", ["si2", "si8", "nacl8", "si2_r1of3", "nacl8_r0of2"])
 def test_density_bf16x3_buffer_variants(name):
-    """The three bf16x3 density kernels the autotuner may pick (single- and
-    double-buffered chunk planes, chunks loaded straight into LDS) meet the
-    bar, and they agree to the round-off of the FP64 atomic adds (their
-    point sums are formed in the same order)."""
+    """The four bf16x3 density kernels the autotuner may pick (single- and
+    double-buffered chunk planes, chunks loaded straight into LDS, 8-wave
+    items two per CU) meet the bar.  The first three form each point's sum
+    over up to 16 super-tiles in the same FP32 order, so they agree to the
+    round-off of the FP64 atomic adds; the 8-wave kernel sums a block's
+    super-tiles in items of at most 8 and adds the items' FP32 sums in FP64,
+    so it agrees with them to FP32 round-off of the point sums."""
     c = case(name)
     ctx = GridContext(c.u)
     ctx.set_tables(c.tabs)
     out = []
-    for v in (0, 1, 2):
+    for v in (0, 1, 2, 3):
         ctx.tune("rhobf", v)
         print(name, "rho bf16x3 variant", v, flush=True)
         out.append(ctx.density([c.DM], dm_offset=c.u.MPHks_rank, precision="bf16x3")[0])
@@ -251,8 +254,9 @@ def test_density_bf16x3_buffer_variants(name):
     Rref, M = O.density(c.g, c.tabs, [c.DM], with_abs=True)
     for r in out:
         assert worst_ratio(r, Rref[0], M[0]) <= 1e-4
-    for r in out[1:]:
+    for r in out[1:3]:
         assert np.abs(out[0] - r).max() <= 1e-13 * np.abs(out[0]).max()
+    assert np.abs(out[0] - out[3]).max() <= 1e-6 * np.abs(out[0]).max()
 
 
 @pytest.mark.parametrize("name", ["si2", "si8", "nacl8", "si2_r1of3", "nacl8_r0of2"])
