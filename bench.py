@@ -454,7 +454,8 @@ def bench_batch(args):
         "speedup_vs_loop": {f: round(forms["per_structure_loop_" + f] / forms["batch_graph_" + f], 3)
                             for f in ("bf16x3", "fp32")},
         "speedup_vs_fp64_loop": round(ms_loop64 / ms_graph, 3),
-        "schedule": dict(sched, names=["persistent work queue", "one workgroup per item"]),
+        "schedule": dict(sched, names=["persistent work queue", "one workgroup per item",
+                                       "one workgroup per item, density two workgroups per CU"]),
         "accuracy_vs_fp64": acc_f,
         "setup_s": round(setup_s, 2),
     }

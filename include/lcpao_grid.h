@@ -217,7 +217,8 @@ int lcpao_grid_kernel_stats(const lcpao_ctx *ctx, int32_t kernel_id, int64_t *la
  *                     2 double-buffered with the chunks loaded straight
  *                     into LDS, used only for blocks of at most 512 rows;
  *                     3 items of up to 8 super-tiles, two workgroups per
- *                     CU, blocks of at most 384 rows; 0-2 form each
+ *                     CU, for the blocks of at most 384 rows (the larger
+ *                     blocks as in 0); 0-2 form each
  *                     point's sum in the same order, 3 adds the FP32 sums
  *                     of a block's items in FP64)
  *   LCPAO_TUNE_RHO32  FP32 density  (0 pair-list kernel 4 x 2048, 1 block
@@ -410,9 +411,12 @@ typedef struct lcpao_ucell lcpao_ucell;
  * blocks, Total_Hsize_s), outputs d_H[s] (scaled by GridVol) and d_rho[s]
  * (overwritten); all device pointers.  use_graph != 0 replays a graph
  * captured at the first call with these pointers (re-captured when they
- * change).  The first call also times the two schedules (0 persistent work
- * queue, 1 one workgroup per item) and keeps the faster (lcpao_batch_tune
- * fixes one, -1 re-tunes).  Returns after the stream has been synchronised. */
+ * change).  The first call also times the schedules (0 persistent work
+ * queue, 1 one workgroup per item, 2 as 1 with the density of the blocks
+ * of at most 384 rows in items of at most 8 super-tiles, two workgroups per
+ * CU) and keeps the fastest (lcpao_batch_tune fixes one, -1 re-tunes;
+ * lcpao_batch_tuned writes the three candidates' milliseconds).  Returns
+ * after the stream has been synchronised. */
 typedef struct lcpao_batch lcpao_batch;
 int lcpao_batch_create(lcpao_batch **out, int32_t n, lcpao_ctx *const *ctxs);
 int lcpao_batch_assemble(lcpao_batch *b, const double *const *d_V, const double *const *d_DM, double *const *d_H,

@@ -30,6 +30,10 @@ struct lcpao_batch {
     };
     std::vector<Cls> classes;
     HItem *d_items = nullptr;
+    // density schedule 2: every structure's items of at most 8 super-tiles
+    // (blocks of at most 384 rows) and the items of its larger blocks
+    HItem *d_items8 = nullptr, *d_items_big = nullptr;
+    int32_t nitems8 = 0, nitems_big = 0, max_rows_big = 0;
     BlockArgs *d_args_h = nullptr, *d_args_r = nullptr;
     int32_t *d_queue = nullptr;   // one counter per launch: 2 per class
     int max_slots = 0;
@@ -42,9 +46,11 @@ struct lcpao_batch {
     // LCPAO_BF16X3 (default) or LCPAO_FP32
     int32_t precision = LCPAO_BF16X3;
     // schedule of the bf16x3 launches: 0 persistent work queue, 1 one
-    // workgroup per item (-1: tune); the FP32 launches run one workgroup per item
+    // workgroup per item, 2 as 1 with the density in items of at most 8
+    // super-tiles, two workgroups per CU (-1: tune); the FP32 launches run
+    // one workgroup per item
     int32_t sched = -1;
-    float sched_ms[2] = {0.f, 0.f};
+    float sched_ms[3] = {0.f, 0.f, 0.f};
     // arguments of the last call and the graph captured with them
     std::vector<const void *> bound;
     hipGraph_t graph = nullptr;
@@ -127,6 +133,15 @@ int enqueue(lcpao_batch *b, int sched) {
                                                     queue ? 0u : (unsigned)cl.count));
     }
     lcpao::launch_hblk_reduce_batch(b->stream, n, b->d_red, b->max_canon, b->red_threads);
+    if (!f32 && sched == 2) {
+        BATCH_TRY(b, lcpao::launch_rblk_w8(b->stream, std::min(b->max_rows, 384), b->d_items8, b->nitems8, b->d_args_r,
+                                           none));
+        if (b->nitems_big > 0)
+            BATCH_TRY(b, lcpao::launch_block_kernel(true, b->stream, b->max_rows_big, b->max_slots, b->d_items_big,
+                                                    b->nitems_big, b->d_args_r, none, nullptr, (unsigned)b->nitems_big));
+        BATCH_TRY(b, hipGetLastError());
+        return LCPAO_OK;
+    }
     for (size_t k = 0; k < b->classes.size(); k++) {
         const auto &cl = b->classes[k];
         if (f32)
@@ -166,7 +181,7 @@ int lcpao_batch_create(lcpao_batch **out, int32_t n, lcpao_ctx *const *ctxs) {
     // every structure's block plan and bf16 tiles; all items in one list,
     // largest (rows, then super-tiles) first, run as one launch per operation
     // (the kernels hold one workgroup per CU whatever R is)
-    std::vector<HItem> items;
+    std::vector<HItem> items, items8, items_big;
     std::vector<int32_t> item_rows;
     int32_t max_rows = 0;
     for (int s = 0; s < n; s++) {
@@ -186,6 +201,17 @@ int lcpao_batch_create(lcpao_batch **out, int32_t n, lcpao_ctx *const *ctxs) {
             items.push_back(it);
             item_rows.push_back(P->blocks[it.block].rows);
         }
+        // each structure's XCD-dealt order is kept (a block's items run back
+        // to back on one XCD)
+        for (HItem it : P->items8) {
+            it.pad = s;
+            items8.push_back(it);
+        }
+        for (HItem it : P->items_big) {
+            it.pad = s;
+            items_big.push_back(it);
+        }
+        b->max_rows_big = std::max(b->max_rows_big, P->max_rows_big);
     }
     std::vector<size_t> order(items.size());
     for (size_t i = 0; i < order.size(); i++) order[i] = i;
@@ -200,6 +226,8 @@ int lcpao_batch_create(lcpao_batch **out, int32_t n, lcpao_ctx *const *ctxs) {
     b->classes.push_back({max_rows, 0, (int32_t)items.size()});
     const size_t ni = std::max<size_t>(1, items.size());
     if (hipMalloc(&b->d_items, ni * sizeof(HItem)) != hipSuccess ||
+        hipMalloc(&b->d_items8, std::max<size_t>(1, items8.size()) * sizeof(HItem)) != hipSuccess ||
+        hipMalloc(&b->d_items_big, std::max<size_t>(1, items_big.size()) * sizeof(HItem)) != hipSuccess ||
         hipMalloc(&b->d_args_h, n * sizeof(BlockArgs)) != hipSuccess ||
         hipMalloc(&b->d_args_r, n * sizeof(BlockArgs)) != hipSuccess ||
         hipMalloc(&b->d_queue, std::max<size_t>(2, 2 * b->classes.size()) * sizeof(int32_t)) != hipSuccess ||
@@ -208,8 +236,16 @@ int lcpao_batch_create(lcpao_batch **out, int32_t n, lcpao_ctx *const *ctxs) {
         hipMalloc(&b->d_fill, n * sizeof(lcpao::BatchFill)) != hipSuccess)
         return fail(LCPAO_EDEVICE);
     b->max_rows = max_rows;
+    b->nitems8 = (int32_t)items8.size();
+    b->nitems_big = (int32_t)items_big.size();
     if (!items.empty() &&
         hipMemcpy(b->d_items, items.data(), items.size() * sizeof(HItem), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(LCPAO_EDEVICE);
+    if (!items8.empty() &&
+        hipMemcpy(b->d_items8, items8.data(), items8.size() * sizeof(HItem), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(LCPAO_EDEVICE);
+    if (!items_big.empty() &&
+        hipMemcpy(b->d_items_big, items_big.data(), items_big.size() * sizeof(HItem), hipMemcpyHostToDevice) != hipSuccess)
         return fail(LCPAO_EDEVICE);
     *out = b;
     return LCPAO_OK;
@@ -242,7 +278,7 @@ int lcpao_batch_assemble(lcpao_batch *b, const double *const *d_V, const double 
         BATCH_TRY(b, hipEventCreate(&e0));
         BATCH_TRY(b, hipEventCreate(&e1));
         int best = 0;
-        for (int v = 0; v < 2; v++) {
+        for (int v = 0; v < 3; v++) {
             if ((rc = enqueue(b, v))) return rc;
             BATCH_TRY(b, hipEventRecord(e0, b->stream));
             if ((rc = enqueue(b, v))) return rc;
@@ -283,13 +319,13 @@ int lcpao_batch_set_precision(lcpao_batch *b, int32_t precision) {
         b->bound.clear();
         b->precision = precision;
         b->sched = precision == LCPAO_FP32 ? 1 : -1;
-        b->sched_ms[0] = b->sched_ms[1] = 0.f;
+        b->sched_ms[0] = b->sched_ms[1] = b->sched_ms[2] = 0.f;
     }
     return LCPAO_OK;
 }
 
 int lcpao_batch_tune(lcpao_batch *b, int32_t sched) {
-    if (!b || sched < -1 || sched > 1) return LCPAO_EINVAL;
+    if (!b || sched < -1 || sched > 2) return LCPAO_EINVAL;
     if (sched != b->sched) drop_graph(b);
     b->sched = sched;
     return LCPAO_OK;
@@ -298,7 +334,8 @@ int lcpao_batch_tune(lcpao_batch *b, int32_t sched) {
 int lcpao_batch_tuned(const lcpao_batch *b, int32_t *sched, float *candidate_ms) {
     if (!b || !sched) return LCPAO_EINVAL;
     *sched = b->sched;
-    if (candidate_ms) { candidate_ms[0] = b->sched_ms[0]; candidate_ms[1] = b->sched_ms[1]; }
+    if (candidate_ms)
+        for (int v = 0; v < 3; v++) candidate_ms[v] = b->sched_ms[v];
     return LCPAO_OK;
 }
 
@@ -320,7 +357,7 @@ void lcpao_batch_destroy(lcpao_batch *b) {
     if (!b) return;
     (void)hipSetDevice(b->device);
     drop_graph(b);
-    void *ptrs[] = {b->d_items, b->d_args_h, b->d_args_r, b->d_queue, b->d_fold, b->d_red, b->d_fill};
+    void *ptrs[] = {b->d_items, b->d_items8, b->d_items_big, b->d_args_h, b->d_args_r, b->d_queue, b->d_fold, b->d_red, b->d_fill};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);

@@ -30,6 +30,7 @@ constexpr int LROW32 = 36;       // LDS row stride of fp32 tiles (32 + 4 pad)
 constexpr int FB_WAVES = 8;      // waves of the Force3 workgroup, two super-tiles of one row each
 constexpr int MAX_SLOT_PAIRS = 2048;  // slot pairs of a block (tables staged in LDS by the block kernels)
 constexpr int RH_WAVES = 8;           // waves of the two-per-CU density workgroup, one super-tile each
+constexpr int RH_ROWS = 384;          // the largest block that workgroup holds
 
 __device__ __forceinline__ int st_index(int si, int sj, int S) { return si * S - (si * (si - 1)) / 2 + (sj - si); }
 
@@ -580,7 +581,7 @@ void deal_xcd(const BlockPlan *P, std::vector<HItem> &items, int n1) {
 
 void plan_free(BlockPlan *p) {
     if (!p) return;
-    void *ptrs[] = {p->d_blocks, p->d_items, p->d_items8, p->d_sph_block, p->d_sph_atom, p->d_sph_row0, p->d_sph_map,
+    void *ptrs[] = {p->d_blocks, p->d_items, p->d_items8, p->d_items_big, p->d_sph_block, p->d_sph_atom, p->d_sph_row0, p->d_sph_map,
                     p->d_contrib, p->d_contrib_off, p->d_hi, p->d_lo, p->d_t32, p->d_part, p->d_thi, p->d_tlo,
                     p->d_rowinfo, p->d_spair, p->d_contrib_pair, p->d_slotno, p->d_fpair,
                     p->d_fitems, p->d_fbase, p->d_fout, p->d_fatom_off, p->d_fatom_idx, p->d_dtile};
@@ -700,13 +701,17 @@ int plan_build_inner(lcpao_ctx *ctx) {
         roff += R;
         for (int st0 = 0; st0 < nst; st0 += HB_WAVES * ST_PER_WAVE)
             P->items.push_back({(int32_t)b, st0, std::min(HB_WAVES * ST_PER_WAVE, nst - st0), 0});
-        {
+        if (R <= RH_ROWS) {
             const int n8 = (nst + RH_WAVES - 1) / RH_WAVES;
             for (int k = 0, st0 = 0; k < n8; k++) {
                 const int cnt = nst / n8 + (k < nst % n8 ? 1 : 0);
                 P->items8.push_back({(int32_t)b, st0, cnt, 0});
                 st0 += cnt;
             }
+        } else {
+            for (int st0 = 0; st0 < nst; st0 += HB_WAVES * ST_PER_WAVE)
+                P->items_big.push_back({(int32_t)b, st0, std::min(HB_WAVES * ST_PER_WAVE, nst - st0), 0});
+            P->max_rows_big = std::max(P->max_rows_big, R);
         }
         sph_first[b + 1] = sph_first[b] + (int64_t)sph[b].size();
     }
@@ -888,7 +893,7 @@ int plan_build_inner(lcpao_ctx *ctx) {
     int rc;
     ctx->plan = P;
     if ((rc = up(ctx, &P->d_blocks, P->blocks)) || (rc = up(ctx, &P->d_items, P->items)) ||
-        (rc = up(ctx, &P->d_items8, P->items8)) ||
+        (rc = up(ctx, &P->d_items8, P->items8)) || (rc = up(ctx, &P->d_items_big, P->items_big)) ||
         (rc = up(ctx, &P->d_sph_block, sb)) || (rc = up(ctx, &P->d_sph_atom, sa)) || (rc = up(ctx, &P->d_sph_row0, sr)) ||
         (rc = up(ctx, &P->d_sph_map, smap)) || (rc = up(ctx, &P->d_contrib, contrib)) || (rc = up(ctx, &P->d_contrib_off, coff)) ||
         (rc = up(ctx, &P->d_contrib_pair, cpair)) || (rc = up(ctx, &P->d_rowinfo, rowinfo)) ||
@@ -1597,8 +1602,8 @@ hipError_t launch_rblk_dma(hipStream_t stream, int rows, const HItem *items, int
 // per half.  A block's items follow one another on one XCD, whose L2 then
 // serves the second its tiles.  Each point's sum over a wave's super-tile and
 // over the waves (in wave order, by one wave after the next barrier) as in
-// k_rblk_bf, and one FP64 atomic per item and point.  Limits: R <= RH_ROWS.
-constexpr int RH_ROWS = 384;
+// k_rblk_bf, and one FP64 atomic per item and point.  Limits: R <= RH_ROWS
+// (the plan gives the larger blocks to the 16-wave kernel).
 constexpr int RH_HALF = RH_ROWS * 16;  // bf16 elements of one plane of one half-chunk buffer
 
 struct RhLds {
@@ -2051,16 +2056,23 @@ int split_density(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t 
             // candidates: 0 = single-buffered chunk planes, 1 = double-buffered
             // (when its LDS fits the plan's largest slot count), 2 = chunks
             // loaded straight into LDS (R <= DMA_ROWS), 3 = 8-wave items, two
-            // workgroups per CU (R <= RH_ROWS)
+            // workgroups per CU (blocks of at most RH_ROWS rows; the larger
+            // ones in a second launch of the 16-wave kernel)
             auto fits = [&](int v) {
-                return v == 0 || (v == 1 && rho_db_fits(P->max_rows, P->max_slots)) ||
-                       (v == 2 && rho_dma_fits(P->max_rows)) || (v == 3 && rho_w8_fits(P->max_rows));
+                return v == 0 || v == 3 || (v == 1 && rho_db_fits(P->max_rows, P->max_slots)) ||
+                       (v == 2 && rho_dma_fits(P->max_rows));
             };
             auto launch = [&](int v) {
                 if (v == 2) return launch_rblk_dma(ctx->stream, P->max_rows, P->d_items, (int32_t)ni, nullptr, one);
-                if (v == 3)
-                    return launch_rblk_w8(ctx->stream, P->max_rows, P->d_items8, (int32_t)P->items8.size(), nullptr,
-                                          one);
+                if (v == 3) {
+                    hipError_t e = launch_rblk_w8(ctx->stream, std::min(P->max_rows, RH_ROWS), P->d_items8,
+                                                  (int32_t)P->items8.size(), nullptr, one);
+                    if (e == hipSuccess && !P->items_big.empty())
+                        e = launch_block_kernel(true, ctx->stream, P->max_rows_big, P->max_slots, P->d_items_big,
+                                                (int32_t)P->items_big.size(), nullptr, one, nullptr,
+                                                (unsigned)P->items_big.size());
+                    return e;
+                }
                 return launch_block_kernel(true, ctx->stream, P->max_rows, P->max_slots, P->d_items, (int32_t)ni,
                                            nullptr, one, nullptr, ni, v == 1);
             };

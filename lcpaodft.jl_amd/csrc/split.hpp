@@ -116,10 +116,13 @@ struct BlockPlan {
     // device
     BlockDev *d_blocks = nullptr;
     HItem *d_items = nullptr;
-    // the same super-tiles in items of at most RH_WAVES (the 8-wave density
-    // kernel, two workgroups per CU), split evenly per block, XCD-dealt
-    std::vector<HItem> items8;
-    HItem *d_items8 = nullptr;
+    // the density split by block size: the super-tiles of the blocks of at
+    // most RH_ROWS rows in items of at most RH_WAVES (the 8-wave kernel, two
+    // workgroups per CU), split evenly per block, XCD-dealt; the items of
+    // the larger blocks (the 16-wave kernel)
+    std::vector<HItem> items8, items_big;
+    HItem *d_items8 = nullptr, *d_items_big = nullptr;
+    int32_t max_rows_big = 0;
     int32_t *d_sph_block = nullptr, *d_sph_atom = nullptr, *d_sph_row0 = nullptr;
     int32_t *d_sph_map = nullptr;  // nsph x BLK_PTS table columns (-1: not covered)
     Contrib *d_contrib = nullptr;
@@ -173,6 +176,11 @@ hipError_t launch_block_kernel(bool density, hipStream_t stream, int rows, int m
 // indexes args), else one structure (`one`); one workgroup per item
 hipError_t launch_f32_block_kernel(bool density, hipStream_t stream, int rows, int max_slots, const HItem *items,
                                    int32_t nitems, const BlockArgs *args, const BlockArgs &one);
+// bf16x3 density in items of at most 8 super-tiles, two workgroups per CU
+// (blocks of at most 384 rows)
+bool rho_w8_fits(int rows);
+hipError_t launch_rblk_w8(hipStream_t stream, int rows, const HItem *items, int32_t nitems, const BlockArgs *args,
+                          const BlockArgs &one);
 // batched fills and reductions of a batch (one launch each for all structures)
 struct BatchRed {
     const int32_t *canon;   // computed pairs of the structure

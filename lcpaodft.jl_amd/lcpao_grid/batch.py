@@ -44,12 +44,13 @@ class Batch:
         self._check(self.L.lcpao_batch_set_precision(self.h, _lib.PRECISIONS[precision]))
 
     def tune(self, sched=-1):
-        """0: persistent work queue, 1: one workgroup per item, -1: time both at the next call."""
+        """0: persistent work queue, 1: one workgroup per item, 2: as 1 with the density in
+        items of at most 8 super-tiles, two workgroups per CU; -1: time them at the next call."""
         self._check(self.L.lcpao_batch_tune(self.h, sched))
 
     def tuned(self):
         s = C.c_int32()
-        ms = (C.c_float * 2)()
+        ms = (C.c_float * 3)()
         self._check(self.L.lcpao_batch_tuned(self.h, C.byref(s), ms))
         return s.value, [round(float(x), 4) for x in ms]
 

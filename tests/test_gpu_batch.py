@@ -61,7 +61,7 @@ def _run(b, out, use_graph):
                use_graph=use_graph)
 
 
-@pytest.mark.parametrize("sched", [0, 1])
+@pytest.mark.parametrize("sched", [0, 1, 2])
 def test_batch_matches_per_structure(small_batch, sched):
     out, ref = small_batch
     b = Batch([s["ctx"] for s in out])
@@ -78,7 +78,7 @@ def test_batch_graph_replays(small_batch):
     out, ref = small_batch
     b = Batch([s["ctx"] for s in out])
     _run(b, out, use_graph=False)  # first call: tunes the schedule
-    assert b.tuned()[0] in (0, 1)
+    assert b.tuned()[0] in (0, 1, 2)
     _check(out, ref)
     first = _snapshot(out)
     for rep in range(3):
