@@ -797,7 +797,8 @@ def main():
     achieved = flops / (stats[dom]["avg_ms"] * 1e-3)
     # the kernel the autotuner picked for this form and operation
     rho_kernels = {"fp64": ("k_rho_m64", "k_rho<double, 4, 2048>", "k_rho<double, 4, 1024>", "k_rho<double, 8, 512>",
-                            "k_rho<double, 2, 1024>", "k_rho<double, 4, 512>", "k_rblk64", "k_rblk64t<1, 16>"),
+                            "k_rho<double, 2, 1024>", "k_rho<double, 4, 512>", "k_rblk64", "k_rblk64t<1, 16>",
+                            "k_rblk64m"),
                    "fp32": ("k_rho<float, 4, 2048>", "k_rblk_f32", "k_rho<float, 4, 1024>", "k_rho<float, 8, 512>"),
                    "bf16x3": ("k_rblk_bf", "k_rblk_bf", "k_rblk_dma", "k_rblk_w8")}
     if dom == "ham":

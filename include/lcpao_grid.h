@@ -209,8 +209,11 @@ int lcpao_grid_kernel_stats(const lcpao_ctx *ctx, int32_t kernel_id, int64_t *la
  *                     entries) 4 x 2048, 4 x 1024, 8 x 512, 2 x 1024, 4 x 512;
  *                     6 block form: vector FMAs over 4x4x4 sub-blocks with
  *                     the orbital columns staged in LDS, 7 the same reading
- *                     the columns from the tables; 6 and 7 are used only
- *                     when the block plan holds the geometry)
+ *                     the columns from the tables; 8 block form on the
+ *                     FP64 matrix cores: per sub-block and slot y, the W
+ *                     blocks of y's pairs stacked row-wise times the staged
+ *                     orbital columns of y; 6, 7 and 8 are used only when
+ *                     the block plan holds the geometry)
  *   LCPAO_TUNE_F364   FP64 Force3   (0 matrix-core kernel, 1 vector-FMA kernel)
  *   LCPAO_TUNE_RHOBF  bf16x3 density (0 single-buffered chunk planes,
  *                     1 double-buffered, used only when its LDS fits;

@@ -10,7 +10,11 @@
 
 #include "lcpao_grid.h"
 
-#define UNIT_PTS 2048     // overlap points per H / Force3 unit
+#define UNIT_PTS 2048     // overlap points per Force3 unit
+// overlap points per FP64 H unit: the units in flight (one wave each) then
+// touch few enough grid planes that the tables they gather stay in the
+// Infinity Cache (GaAs216: 9.2 ms at 2048 points, see k_ham_mfma64)
+#define HUNIT_PTS 512
 #define RHO_CHUNK 2048    // grid-list entries per density unit (the largest decomposition)
 #define HK 32             // points per LDS chunk in the H kernel
 // orbital-count pairs (NO_A, NO_B) with kernels of fixed size (the Si/Cl,
@@ -77,9 +81,9 @@ enum { TUNE_RHO64 = 0, TUNE_F364 = 1, TUNE_RHOBF = 2, TUNE_RHO32 = 3, TUNE_NOPS 
 // 2 chunks loaded straight into LDS, 3 8-wave items (two workgroups per CU);
 // FP32 density: 0 pair lists 4 x 2048, 1 block form, 2 pair lists 4 x 1024,
 // 3 pair lists 8 x 512
-constexpr int RHO64_BLOCK = 6, RHO64_BLOCK_TAB = 7;
-constexpr int TUNE_NCAND[TUNE_NOPS] = {8, 2, 4, 4};
-constexpr int TUNE_MAXCAND = 8;
+constexpr int RHO64_BLOCK = 6, RHO64_BLOCK_TAB = 7, RHO64_BLOCK_MFMA = 8;
+constexpr int TUNE_NCAND[TUNE_NOPS] = {9, 2, 4, 4};
+constexpr int TUNE_MAXCAND = 9;
 
 struct lcpao_ctx {
     int device = 0;
@@ -112,6 +116,7 @@ struct lcpao_ctx {
     int32_t *d_hpair_ub = nullptr;  // canonical pair list + unit ranges
     int32_t *d_hcanon = nullptr;
     int64_t ncanon = 0, hpart_size = 0;
+    int32_t hred_threads = 64;   // FP64 H reduce width: the largest computed pair block, rounded to 64 (<= 1024)
     double *d_hpart = nullptr;
     // rho: the work-unit decompositions (RHO_DECOMP_CHUNK)
     struct RhoDecomp {

@@ -173,9 +173,22 @@ __global__ void k_ham_reduce(const int32_t *__restrict__ canon, const int32_t *_
     const PairDev P = pairs[p];
     const int ne = P.noa * P.nob;
     const int u0 = ub[c], u1 = ub[c + 1];
+    // a pair's units are consecutive and so are their partial blocks (a pair
+    // without overlap points has none: its H block is zero)
+    const int nu = u1 - u0;
+    const double *pp = part + (nu > 0 ? units[u0].part : 0);
     for (int e = threadIdx.x; e < ne; e += blockDim.x) {
         double s = 0.0;
-        for (int u = u0; u < u1; u++) s += part[units[u].part + e];
+        int u = 0;
+        for (; u + 4 <= nu; u += 4) {  // four loads in flight, added in unit order
+            const double v0 = pp[(int64_t)u * ne + e], v1 = pp[(int64_t)(u + 1) * ne + e];
+            const double v2 = pp[(int64_t)(u + 2) * ne + e], v3 = pp[(int64_t)(u + 3) * ne + e];
+            s += v0;
+            s += v1;
+            s += v2;
+            s += v3;
+        }
+        for (; u < nu; u++) s += pp[(int64_t)u * ne + e];
         s *= gridvol;
         H[P.hoff + e] = s;
         if (P.partner >= 0 && P.partner != p) {
@@ -998,15 +1011,18 @@ int build_work(lcpao_ctx *ctx, const lcpao_geom *g, const std::vector<int32_t> &
     ctx->hunits.clear();
     std::vector<int32_t> hub(1, 0);
     int64_t part = 0;
+    int max_ne = 1;
     for (int p : canon) {
         const PairDev &P = ctx->pairs[p];
         const int n = g->numolg[p];
-        for (int lo = 0; lo < n; lo += UNIT_PTS) {
-            ctx->hunits.push_back({p, lo, std::min(UNIT_PTS, n - lo), 0, part});
+        for (int lo = 0; lo < n; lo += HUNIT_PTS) {
+            ctx->hunits.push_back({p, lo, std::min(HUNIT_PTS, n - lo), 0, part});
             part += (int64_t)P.noa * P.nob;
         }
+        max_ne = std::max(max_ne, P.noa * P.nob);
         hub.push_back((int32_t)ctx->hunits.size());
     }
+    ctx->hred_threads = std::min(1024, (max_ne + 63) / 64 * 64);
     ctx->hpart_size = part;
     auto horder = launch_order(ctx->hunits.size(), [&](size_t i) { return first_g(ctx->hunits[i].pair, ctx->hunits[i].lo); });
     // Force3 units over all pairs; pairs are grouped by home atom
@@ -1133,7 +1149,7 @@ int ham_dev(lcpao_ctx *ctx, int nch, const double *const *d_V, double *const *d_
         }
         if (ctx->ncanon > 0) {
             KTimer kt(ctx, KID_HRED);
-            hipLaunchKernelGGL(k_ham_reduce, dim3((unsigned)ctx->ncanon), dim3(128), 0, ctx->stream, ctx->d_hcanon,
+            hipLaunchKernelGGL(k_ham_reduce, dim3((unsigned)ctx->ncanon), dim3(ctx->hred_threads), 0, ctx->stream, ctx->d_hcanon,
                                ctx->d_hpair_ub, ctx->ncanon, ctx->d_hunits, ctx->d_pairs, ctx->d_hpart, ctx->gridvol,
                                d_H[c]);
         }
@@ -1256,7 +1272,7 @@ int rho_dev(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_off
     auto run = [&](int v) -> int {
         if (f32 && v == 1) return split_density_f32_enqueue(ctx, nspin, d_DM, dm_offset, d_rho);
         if (!f32 && v >= RHO64_BLOCK)
-            return split_density_f64_enqueue(ctx, nspin, d_DM, dm_offset, d_rho, v == RHO64_BLOCK);
+            return split_density_f64_enqueue(ctx, nspin, d_DM, dm_offset, d_rho, v - RHO64_BLOCK);
         return rho_dev_pairs(ctx, nspin, d_DM, dm_offset, d_rho, f32, v);
     };
     if (f32 && (ctx->variant[op] != 1) && (rc = fp32_tables(ctx, false))) return rc;

@@ -2561,18 +2561,356 @@ size_t r64t_lds(int ms) {
     return (size_t)((ms * 64 * PPL * 4 + ms * PPL * 8 + ms * (ms + 1) / 2 * 4 + 7) / 8 * 8) + W * 64 * PPL * 8 + 16;
 }
 
+// Matrix-core form of the FP64 block density.  Per 4x4x4 sub-block: the point
+// map and presence masks in LDS and the present slots' columns staged in LDS
+// ([row][point], stride R64_LD).  The slot pairs (x <= y) that share a point
+// of the sub-block are listed y-major, and for each slot y the W blocks of its
+// pairs are stacked row-wise (row (x, i): orbital i of x, for every listed x
+// in slot order), so that
+//   Q_y = W[stacked rows][orbitals of y] . Phi_y[orbitals of y][64 points]
+// is one product on v_mfma_f64_16x16x4f64 whose 16-row tiles hold the rows
+// of several pairs (each pair alone would pad its 13-19 rows to 32), and
+//   rho(p) += sum over the stacked rows (x, i) of phi_{x,i}(p) Q_y[(x, i)][p]
+// on the vector units.  The lists are built by all waves (one slot y per wave
+// and pass, lane scans for the offsets); the work items are the 16-row tiles
+// (y, tile), dealt round robin to the waves, each wave loading the W operands
+// of its next tile while it multiplies the current one.  A lane holds the
+// output rows kq, kq + 4, kq + 8, kq + 12 of a tile (kq = lane / 16) for the
+// points lane % 16 + 16 nt and sums them over its tiles; the four lanes of a
+// point and then the waves are added in a fixed order (deterministic).  A
+// sub-block whose rows or stacked rows exceed the launch's LDS takes a vector
+// path (one point per lane, exact FMAs, columns read from the tables).
+typedef double d4_64 __attribute__((ext_vector_type(4)));
+
+struct R64mLds {
+    int phi, tm, mask, amask, tbase, sno, roff, yinfo, pairs, psp, rslot, items, stk, ctl, total;
+    int item_cap, stack_cap, npair_cap;
+};
+__host__ __device__ inline R64mLds r64m_lds(int rows_cap, int ms, int stack_cap) {
+    R64mLds L;
+    L.stack_cap = stack_cap;
+    L.item_cap = stack_cap / 16 + ms;
+    L.npair_cap = ms * (ms + 1) / 2;
+    const int phi = rows_cap * R64_LD * 8, part = R64_WAVES * 64 * 8;  // the wave sums reuse the column space
+    L.phi = 0;
+    L.tm = (phi > part ? phi : part);
+    L.mask = L.tm + ms * 64 * 4;
+    L.mask = (L.mask + 7) / 8 * 8;
+    L.amask = L.mask + ms * 8;
+    L.tbase = L.amask + ms * 8;
+    L.sno = L.tbase + ms * 8;
+    L.roff = L.sno + ms * 4;
+    L.yinfo = L.roff + ms * 4;                 // per slot y: stacked rows, first stacked row, first item, first pair
+    L.pairs = L.yinfo + ms * 4 * 4;            // x << 16 | y, y-major
+    L.psp = L.pairs + L.npair_cap * 4;         // the slot pair of each listed pair
+    L.psp = (L.psp + 7) / 8 * 8;
+    L.rslot = L.psp + L.npair_cap * 8;         // staged row: x << 16 | i
+    L.items = L.rslot + rows_cap * 4;          // y << 8 | tile
+    L.stk = L.items + L.item_cap * 4;          // stacked row: listed pair << 15 | i << 10 | staged row
+    L.ctl = L.stk + stack_cap * 4;
+    L.ctl = (L.ctl + 15) / 16 * 16;
+    L.total = L.ctl + 16;
+    return L;
+}
+
+// exclusive prefix sum over the 64 lanes of a wave (and the total)
+__device__ __forceinline__ int wave_scan_excl(int v, int lane, int &total) {
+    int x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    total = __shfl(x, 63, 64);
+    return x - v;
+}
+
+// W operands of one 16-row tile: lane (kq, r16) holds W[row 16 mt + r16][4 ks + kq],
+// and the staged rows of its output rows 16 mt + kq + 4 v (-1: past the end)
+struct Tile64 {
+    int y, s0, nr, ny, mt;
+};
+__device__ __forceinline__ Tile64 tile64(const int32_t *items, const int32_t *yinfo, const int32_t *sno, int q) {
+    Tile64 T;
+    const int it = __builtin_amdgcn_readfirstlane(items[q]);
+    T.y = it >> 8;
+    T.mt = it & 255;
+    T.nr = __builtin_amdgcn_readfirstlane(yinfo[4 * T.y]);
+    T.s0 = __builtin_amdgcn_readfirstlane(yinfo[4 * T.y + 1]);
+    T.ny = __builtin_amdgcn_readfirstlane(sno[T.y]);
+    return T;
+}
+__device__ __forceinline__ void tile64_w(const Tile64 &T, const int32_t *stk, const SlotPair *psp,
+                                         const GLOBAL_AS double *__restrict__ Deff, int kq, int r16, double (&a)[8],
+                                         int (&sr)[4]) {
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int rr = 16 * T.mt + kq + 4 * v;
+        sr[v] = rr < T.nr ? (stk[T.s0 + rr] & 1023) : -1;
+    }
+    const int r = 16 * T.mt + r16;
+    const bool rv = r < T.nr;
+    const int ev = rv ? stk[T.s0 + r] : 0;
+    const SlotPair P = psp[ev >> 15];
+    const int ia = (ev >> 10) & 31, ld = P.meta & 63;
+    const bool tr = (P.meta >> 6) != 0;
+    const GLOBAL_AS double *M = Deff + (rv ? P.hoff : 0);
+    const int kt = (T.ny + 3) >> 2;
+#pragma unroll
+    for (int ks = 0; ks < 8; ks++) {
+        const int j = 4 * ks + kq;
+        a[ks] = 0.0;
+        if (ks < kt && rv && j < T.ny) a[ks] = M[tr ? j * ld + ia : ia * ld + j];
+    }
+}
+
+__global__ __launch_bounds__(64 * R64_WAVES) void k_rblk64m(Rho64Args A, int stack_cap) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int64_t per = (A.nsub + 7) / 8;
+    const int64_t jb = blockIdx.x / 8, s = (int64_t)(blockIdx.x % 8) * per + jb;
+    if (jb >= per || s >= A.nsub) return;
+    const BlockDev B = as_global(A.blocks)[s / 8];
+    const int sc = (int)(s % 8);
+    const int ns = B.nslot;  // <= 45 (MAX_SLOT_PAIRS): one lane per slot
+    const R64mLds L = r64m_lds(A.rows_cap, A.max_slots, stack_cap);
+    double *phi = reinterpret_cast<double *>(lds + L.phi);
+    double *part = reinterpret_cast<double *>(lds + L.phi);
+    int32_t *tm = reinterpret_cast<int32_t *>(lds + L.tm);
+    uint64_t *mask = reinterpret_cast<uint64_t *>(lds + L.mask);
+    uint64_t *amask = reinterpret_cast<uint64_t *>(lds + L.amask);
+    int64_t *tbase = reinterpret_cast<int64_t *>(lds + L.tbase);
+    int32_t *sno = reinterpret_cast<int32_t *>(lds + L.sno);
+    int32_t *roff = reinterpret_cast<int32_t *>(lds + L.roff);
+    int32_t *yinfo = reinterpret_cast<int32_t *>(lds + L.yinfo);
+    int32_t *pairs = reinterpret_cast<int32_t *>(lds + L.pairs);
+    SlotPair *psp = reinterpret_cast<SlotPair *>(lds + L.psp);
+    int32_t *rslot = reinterpret_cast<int32_t *>(lds + L.rslot);
+    int32_t *items = reinterpret_cast<int32_t *>(lds + L.items);
+    int32_t *stk = reinterpret_cast<int32_t *>(lds + L.stk);
+    int32_t *ctl = reinterpret_cast<int32_t *>(lds + L.ctl);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int p1 = 4 * (sc >> 2) + (lane >> 4), p2 = 4 * ((sc >> 1) & 1) + ((lane >> 2) & 3), p3 = 4 * (sc & 1) + (lane & 3);
+    const int k = (p1 * BLK + p2) * BLK + p3;
+    const GLOBAL_AS SlotPair *sp = as_global(A.spair) + B.sp_off;
+    // 1. table columns (point index = lane), presence masks, orbital counts, table bases
+    const GLOBAL_AS int32_t *map = as_global(A.sph_map) + (int64_t)B.slot_off * BLK_PTS;
+    for (int x = wave; x < ns; x += R64_WAVES) {
+        const int t = map[(int64_t)x * BLK_PTS + k];
+        tm[x * 64 + lane] = t;
+        const uint64_t m = __ballot(t >= 0);
+        if (lane == 0) mask[x] = m;
+    }
+    if (wave == R64_WAVES - 1 && lane < ns) {
+        sno[lane] = as_global(A.slotno)[B.slot_off + lane];
+        tbase[lane] = as_global(A.tab_off)[as_global(A.sph_atom)[B.slot_off + lane]];
+    }
+    __syncthreads();
+    // 2. per slot y (one wave each, lane = x): the slots x <= y of its listed pairs
+    for (int y = wave; y < ns; y += R64_WAVES) {
+        const uint64_t my = mask[y];
+        const int x = lane;
+        bool act = false;
+        if (my != 0 && x <= y) act = (mask[x] & my) != 0 && sp[(int64_t)x * ns + y].hoff >= 0;
+        const uint64_t bal = __ballot(act);
+        int nr;
+        wave_scan_excl(act ? sno[x] : 0, lane, nr);
+        if (lane == 0) { amask[y] = bal; yinfo[4 * y] = nr; }
+    }
+    __syncthreads();
+    // 3. offsets (one wave, lane = slot): staged rows, stacked rows, items, pairs
+    if (wave == 0) {
+        const int y = lane;
+        const bool in = y < ns;
+        int rows, nst, nit, npr;
+        const int ro = wave_scan_excl(in && mask[in ? y : 0] ? sno[in ? y : 0] : 0, lane, rows);
+        const int nr = in ? yinfo[4 * y] : 0;
+        const int st0 = wave_scan_excl(nr, lane, nst);
+        const int it0 = wave_scan_excl((nr + 15) >> 4, lane, nit);
+        const int pp0 = wave_scan_excl(in ? __popcll(amask[y]) : 0, lane, npr);
+        if (in) {
+            roff[y] = ro;
+            yinfo[4 * y + 1] = st0;
+            yinfo[4 * y + 2] = it0;
+            yinfo[4 * y + 3] = pp0;
+        }
+        if (lane == 0) {
+            ctl[0] = npr;
+            ctl[1] = rows;
+            ctl[2] = nit;
+            ctl[3] = (rows <= A.rows_cap && nst <= stack_cap && nit <= L.item_cap) ? 1 : 0;
+        }
+    }
+    __syncthreads();
+    const int rows = ctl[1];
+    const bool mm = ctl[3] != 0;
+    // 4. the lists (one slot y per wave and pass) and the staged rows
+    for (int y = wave; y < ns; y += R64_WAVES) {
+        const uint64_t am = amask[y];
+        if (!am) continue;
+        const int nr = yinfo[4 * y], st0 = yinfo[4 * y + 1], it0 = yinfo[4 * y + 2], pp0 = yinfo[4 * y + 3];
+        const int x = lane;
+        const bool act = x < ns && ((am >> x) & 1) != 0;
+        int tot;
+        const int cum = wave_scan_excl(act ? sno[x] : 0, lane, tot);
+        const int q = pp0 + __popcll(am & ((1ull << lane) - 1));
+        if (act) {
+            pairs[q] = x << 16 | y;
+            psp[q] = SlotPair{sp[(int64_t)x * ns + y].hoff, sp[(int64_t)x * ns + y].meta};
+            if (mm) {
+                const int nx = sno[x], r0 = roff[x];
+                for (int i = 0; i < nx; i++) stk[st0 + cum + i] = q << 15 | i << 10 | (r0 + i);
+            }
+        }
+        if (mm)
+            for (int t = lane; t < ((nr + 15) >> 4); t += 64) items[it0 + t] = y << 8 | t;
+    }
+    if (mm && tid < ns && mask[tid]) {
+        const int nx = sno[tid], r0 = roff[tid];
+        for (int i = 0; i < nx; i++) rslot[r0 + i] = tid << 16 | i;
+    }
+    __syncthreads();
+    const GLOBAL_AS double *Deff = as_global(A.Deff);
+    double z = 0.0;
+    if (mm) {
+        // 5. stage the present slots' columns: thread = (point, row lane), so
+        // 16 lanes read 16 orbitals of one point (contiguous in the table)
+        {
+            const int p = tid >> 4, rl = tid & 15;
+            const GLOBAL_AS double *tab = as_global(A.tab);
+            for (int r0 = 0; r0 < rows; r0 += 16 * R64_UNROLL) {
+                double v[R64_UNROLL];
+#pragma unroll
+                for (int u = 0; u < R64_UNROLL; u++) {
+                    const int r = r0 + 16 * u + rl;
+                    v[u] = 0.0;
+                    if (r < rows) {
+                        const int rs = rslot[r], x = rs >> 16, i = rs & 0xFFFF;
+                        const int t = tm[x * 64 + p];
+                        if (t >= 0) v[u] = tab[tbase[x] + (int64_t)t * sno[x] + i];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < R64_UNROLL; u++) {
+                    const int r = r0 + 16 * u + rl;
+                    if (r < rows) phi[r * R64_LD + p] = v[u];
+                }
+            }
+        }
+        __syncthreads();
+        // 6. the tiles on the matrix cores
+        const int nit = ctl[2];
+        const int kq = lane >> 4, r16 = lane & 15;
+        double zs[4] = {0.0, 0.0, 0.0, 0.0};
+        double a[8];
+        int sr[4];
+        Tile64 T{0, 0, 0, 0, 0};
+        if (wave < nit) {
+            T = tile64(items, yinfo, sno, wave);
+            tile64_w(T, stk, psp, Deff, kq, r16, a, sr);
+        }
+        for (int q = wave; q < nit; q += R64_WAVES) {
+            const Tile64 C = T;
+            double ac[8];
+            int cr[4];
+#pragma unroll
+            for (int ks = 0; ks < 8; ks++) ac[ks] = a[ks];
+#pragma unroll
+            for (int v = 0; v < 4; v++) cr[v] = sr[v];
+            // the next tile's W operands load under this tile's products
+            if (q + R64_WAVES < nit) {
+                T = tile64(items, yinfo, sno, q + R64_WAVES);
+                tile64_w(T, stk, psp, Deff, kq, r16, a, sr);
+            }
+            const int kt = (C.ny + 3) >> 2;
+            d4_64 acc[4];
+#pragma unroll
+            for (int nt = 0; nt < 4; nt++) acc[nt] = d4_64{0.0, 0.0, 0.0, 0.0};
+            const double *py = phi + (int64_t)__builtin_amdgcn_readfirstlane(roff[C.y]) * R64_LD + r16;
+#pragma unroll
+            for (int ks = 0; ks < 8; ks++) {
+                if (ks < kt) {
+                    const int j = 4 * ks + kq;
+                    const bool jv = j < C.ny;
+#pragma unroll
+                    for (int nt = 0; nt < 4; nt++) {
+                        const double b = jv ? py[j * R64_LD + 16 * nt] : 0.0;
+                        acc[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[ks], b, acc[nt], 0, 0, 0);
+                    }
+                }
+            }
+            // the lane's output rows 16 mt + kq + 4 v against phi_x at its points
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                if (cr[v] >= 0) {
+                    const double *px = phi + cr[v] * R64_LD + r16;
+#pragma unroll
+                    for (int nt = 0; nt < 4; nt++) zs[nt] = fma(px[16 * nt], acc[nt][v], zs[nt]);
+                }
+            }
+        }
+        // the four lanes of a point: rows kq = 0..3 (the same sums in every lane)
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            zs[nt] += __shfl_xor(zs[nt], 16, 64);
+            zs[nt] += __shfl_xor(zs[nt], 32, 64);
+        }
+        z = kq == 0 ? zs[0] : kq == 1 ? zs[1] : kq == 2 ? zs[2] : zs[3];  // point 16 kq + r16 = lane
+        __syncthreads();  // the columns are no longer read: their space takes the wave sums
+    } else {
+        // one point per lane, columns read from the tables, loops over the
+        // orbitals (exact FMAs; only sub-blocks beyond the launch's LDS)
+        const int np = ctl[0];
+        const GLOBAL_AS double *tab = as_global(A.tab);
+        for (int q = wave; q < np; q += R64_WAVES) {
+            const int xy = __builtin_amdgcn_readfirstlane(pairs[q]);
+            const int x = xy >> 16, y = xy & 0xFFFF;
+            const SlotPair P = psp[q];
+            const GLOBAL_AS double *M = Deff + __builtin_amdgcn_readfirstlane(P.hoff);
+            const bool tr = (__builtin_amdgcn_readfirstlane(P.meta) >> 6) != 0;
+            const int nx = __builtin_amdgcn_readfirstlane(sno[x]), ny = __builtin_amdgcn_readfirstlane(sno[y]);
+            const GLOBAL_AS double *xt = tab + tbase[x];
+            const GLOBAL_AS double *yt = tab + tbase[y];
+            const int tx = tm[x * 64 + lane], ty = tm[y * 64 + lane];
+            z += tr ? quad64t_rt(M, yt, ty, ny, xt, tx, nx) : quad64t_rt(M, xt, tx, nx, yt, ty, ny);
+        }
+    }
+    // 7. the waves' sums in wave order
+    part[wave * 64 + lane] = z;
+    __syncthreads();
+    if (wave == 0) {
+        double r = part[lane];
+#pragma unroll
+        for (int w = 1; w < R64_WAVES; w++) r += part[w * 64 + lane];
+        const int x1 = B.g1 + p1, x2 = B.g2 + p2, x3 = B.g3 + p3;
+        if (x1 < A.n1 && x2 < A.n2 && x3 < A.n3) as_global(A.rho)[((int64_t)x1 * A.n2 + x2) * A.n3 + x3] = r;
+    }
+}
+
 int split_density_f64_enqueue(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_offset,
-                              double *const *d_rho, bool staged) {
+                              double *const *d_rho, int mode) {
     BlockPlan *P = ctx->plan;
     if (!P) { ctx->err = "the block plan is not built"; return LCPAO_ESTATE; }
+    if (mode < 0 || mode > 2) { ctx->err = "unknown FP64 block density form"; return LCPAO_EINVAL; }
+    const bool staged = mode == 0, mfma = mode == 2;
     // LDS: the staged columns of the largest sub-block, as far as 160 KB
-    // allow (none when the columns are read from the tables)
+    // allow (none when the columns are read from the tables); the
+    // matrix-core form also holds the stacked rows (up to 2 bytes each)
     const int ms = std::max(1, P->max_slots);
-    int cap = staged ? std::max(1, P->max_sub_rows) : 0;
-    while (cap > 1 && r64_lds(cap, ms).total > 160 * 1024) cap--;
+    int cap = staged || mfma ? std::max(1, P->max_sub_rows) : 0;
+    int scap = std::min(ms * (ms + 1) / 2 * 32, 16384);
+    if (mfma) {
+        while (scap > 2048 && r64m_lds(cap, ms, scap).total > 160 * 1024) scap -= 64;
+        while (cap > 1 && r64m_lds(cap, ms, scap).total > 160 * 1024) cap--;
+    } else {
+        while (cap > 1 && r64_lds(cap, ms).total > 160 * 1024) cap--;
+    }
     const void *kern = staged ? reinterpret_cast<const void *>(k_rblk64)
+                       : mfma ? reinterpret_cast<const void *>(k_rblk64m)
                               : reinterpret_cast<const void *>(k_rblk64t<1, R64_WAVES>);
-    const size_t lds = staged ? (size_t)r64_lds(cap, ms).total : r64t_lds<1, R64_WAVES>(ms);
+    const size_t lds = staged ? (size_t)r64_lds(cap, ms).total
+                       : mfma ? (size_t)r64m_lds(cap, ms, scap).total
+                              : r64t_lds<1, R64_WAVES>(ms);
+    if (lds > 160 * 1024) { ctx->err = "the FP64 block density does not fit the LDS for this geometry"; return LCPAO_ERANGE; }
     SPLIT_TRY(ctx, hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const int64_t nsub = P->nblocks * 8, per = (nsub + 7) / 8;
     for (int s = 0; s < nspin; s++) {
@@ -2588,6 +2926,9 @@ int split_density_f64_enqueue(lcpao_ctx *ctx, int nspin, const double *const *d_
                               cap, ms, 0};
             if (staged)
                 hipLaunchKernelGGL(k_rblk64, dim3((unsigned)(8 * per)), dim3(64 * R64_WAVES), lds, ctx->stream, A);
+            else if (mfma)
+                hipLaunchKernelGGL(k_rblk64m, dim3((unsigned)(8 * per)), dim3(64 * R64_WAVES), lds, ctx->stream, A,
+                                   scap);
             else
                 hipLaunchKernelGGL((k_rblk64t<1, R64_WAVES>), dim3((unsigned)(8 * per)), dim3(64 * R64_WAVES), lds,
                                    ctx->stream, A);

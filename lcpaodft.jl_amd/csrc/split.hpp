@@ -213,12 +213,13 @@ int split_tables(lcpao_ctx *ctx, int precision);
 int split_density_f32_enqueue(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_offset,
                               double *const *d_rho);
 int split_hamiltonian(lcpao_ctx *ctx, int nch, const double *const *d_V, double *const *d_H, int precision);
-// FP64 density in the block form, exact vector FMAs (needs the block plan,
-// plan_build); staged: the orbital columns of each sub-block staged in LDS
-// (k_rblk64), else read from the tables (k_rblk64t); enqueues the fold and
-// one launch per spin, no synchronisation
+// FP64 density in the block form (needs the block plan, plan_build); mode 0:
+// exact vector FMAs with the orbital columns of each sub-block staged in LDS
+// (k_rblk64), 1: the same reading the columns from the tables (k_rblk64t),
+// 2: the stacked W tiles of each sub-block on the FP64 matrix cores
+// (k_rblk64m); enqueues the fold and one launch per spin, no synchronisation
 int split_density_f64_enqueue(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_offset,
-                              double *const *d_rho, bool staged);
+                              double *const *d_rho, int mode);
 int split_density(lcpao_ctx *ctx, int nspin, const double *const *d_DM, int64_t dm_offset, double *const *d_rho);
 // bf16x3 Force3 of one channel: adds w * S_A into d_fsum (natom x 3, atom-major)
 int split_force3(lcpao_ctx *ctx, const double *d_V, const double *d_DM, double w, double *d_fsum);

@@ -375,7 +375,7 @@ class GridContext:
         return (_lib.dp * len(ts))(*[C.cast(t.data_ptr(), _lib.dp) for t in ts])
 
     TUNE_OPS = {"rho64": 0, "f364": 1, "rhobf": 2, "rho32": 3}
-    TUNE_NCAND = {"rho64": 8, "f364": 2, "rhobf": 4, "rho32": 4}
+    TUNE_NCAND = {"rho64": 9, "f364": 2, "rhobf": 4, "rho32": 4}
 
     def tune(self, op, variant=-1):
         """Fix the kernel variant of a tunable operation ("rho64", "f364", "rhobf", "rho32"), or

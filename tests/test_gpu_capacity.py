@@ -113,10 +113,10 @@ def test_large_basis_fp64_matrix_core_candidates_refuse():
         ctx.force3([c.V], [c.DM])
     ctx.tune("rho64", -1)
     ctx.density([c.DM])
-    # the matrix-core kernel (0) and the block forms (6, 7: no block plan beyond
-    # 32 orbitals) are not run; one of the vector-FMA pair-list shapes wins
+    # the matrix-core kernel (0) and the block forms (6, 7, 8: no block plan
+    # beyond 32 orbitals) are not run; one of the vector-FMA pair-list shapes wins
     v, ms = ctx.tuned("rho64")
-    assert v in range(1, 6) and ms[0] == -1 and ms[6] == -1 and ms[7] == -1 and all(m > 0 for m in ms[1:6])
+    assert v in range(1, 6) and ms[0] == -1 and ms[6] == ms[7] == ms[8] == -1 and all(m > 0 for m in ms[1:6])
 
 
 def test_large_basis_bloch_sum_and_dm_in_tiles():

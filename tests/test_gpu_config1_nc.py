@@ -106,7 +106,7 @@ def test_config1_si2_h_rho_force3(prec):
 
 def _variants(prec):
     # every kernel the autotuner may pick for the density of this form
-    return {"fp64": [("rho64", v) for v in range(8)], "fp32": [("rho32", v) for v in range(4)],
+    return {"fp64": [("rho64", v) for v in range(9)], "fp32": [("rho32", v) for v in range(4)],
             "bf16x3": [("rhobf", v) for v in range(4)]}[prec]
 
 

@@ -124,7 +124,7 @@ def test_gaas216_precision_ladder(gaas, prec, op, variant):
     ctx.tune(op, -1)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8])
 def test_gaas216_fp64_density_candidates_full_size(gaas, variant):
     """Every FP64 density candidate of the autotuner (kernel and work-unit
     shape) at full size against the CPU restatement, within 1e-12 of each

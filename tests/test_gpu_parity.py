@@ -287,7 +287,7 @@ def test_density_fp32_variants(name):
     assert worst_ratio(R, Rref[0], M[0]) <= SPLIT_RTOL["fp32"]
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8])
 @pytest.mark.parametrize("name", ["si2", "nacl8", "si2_r1of3", "gana2"])
 def test_fp64_kernel_variants(name, variant):
     """Every candidate the autotuner may pick meets the FP64 bar.  For the
@@ -324,7 +324,7 @@ def test_autotuner_picks_a_timed_candidate():
     assert ctx.tuned("rho64")[0] == -1
     ctx.density([c.DM])
     v, ms = ctx.tuned("rho64")
-    assert v in range(8) and all(m > 0 for m in ms) and ms[v] == min(ms)
+    assert v in range(9) and all(m > 0 for m in ms) and ms[v] == min(ms)
 
 
 def test_autotuner_cache_round_trip(tmp_path):
@@ -340,7 +340,7 @@ def test_autotuner_cache_round_trip(tmp_path):
     chosen = {op: ctx.tuned(op)[0] for op in ("rho64", "rhobf")}
     assert all(0 <= v < GridContext.TUNE_NCAND[op] for op, v in chosen.items())
     times = {op: ctx.tuned(op)[1] for op in ("rho64", "rhobf")}
-    assert len(times["rho64"]) == 8 and all(m > 0 for m in times["rho64"])
+    assert len(times["rho64"]) == 9 and all(m > 0 for m in times["rho64"])
     ctx.tune_save(path)
     text = path.read_text()
     assert text.startswith('{"lcpao_grid_tune": [') and text.count('"key": ') == 2
