@@ -10,11 +10,12 @@
 
 #include "lcpao_grid.h"
 
-#define UNIT_PTS 2048     // overlap points per Force3 unit
-// overlap points per FP64 H unit: the units in flight (one wave each) then
-// touch few enough grid planes that the tables they gather stay in the
-// Infinity Cache (GaAs216: 9.2 ms at 2048 points, see k_ham_mfma64)
-#define HUNIT_PTS 512
+// overlap points per H / Force3 unit of the pair-list kernels: the units in
+// flight (one wave each, launched in grid order) then touch few enough grid
+// planes that the tables they gather stay in the Infinity Cache (GaAs216
+// FP64 H 8.2 ms against 9.2 ms at 2048 points; Si511 FP64 Force3 44.6 ms
+// against 54.8 ms)
+#define UNIT_PTS 512
 #define RHO_CHUNK 2048    // grid-list entries per density unit (the largest decomposition)
 #define HK 32             // points per LDS chunk in the H kernel
 // orbital-count pairs (NO_A, NO_B) with kernels of fixed size (the Si/Cl,

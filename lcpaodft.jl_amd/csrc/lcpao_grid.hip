@@ -47,7 +47,7 @@ std::string g_create_err;
 // Force3 unit) followed by the per-atom sums (3 doubles per atom).  Every GPU
 // that loads the library holds one copy of this array until the process
 // ends; calls on one GPU share it.
-#define NOALLOC_SCRATCH_DOUBLES (2 * 1024 * 1024)  // 16 MiB
+#define NOALLOC_SCRATCH_DOUBLES (8 * 1024 * 1024)  // 64 MiB
 static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
 
 namespace {
@@ -1015,8 +1015,8 @@ int build_work(lcpao_ctx *ctx, const lcpao_geom *g, const std::vector<int32_t> &
     for (int p : canon) {
         const PairDev &P = ctx->pairs[p];
         const int n = g->numolg[p];
-        for (int lo = 0; lo < n; lo += HUNIT_PTS) {
-            ctx->hunits.push_back({p, lo, std::min(HUNIT_PTS, n - lo), 0, part});
+        for (int lo = 0; lo < n; lo += UNIT_PTS) {
+            ctx->hunits.push_back({p, lo, std::min(UNIT_PTS, n - lo), 0, part});
             part += (int64_t)P.noa * P.nob;
         }
         max_ne = std::max(max_ne, P.noa * P.nob);
