@@ -802,7 +802,8 @@ def main():
                    "fp32": ("k_rho<float, 4, 2048>", "k_rblk_f32", "k_rho<float, 4, 1024>", "k_rho<float, 8, 512>"),
                    "bf16x3": ("k_rblk_bf", "k_rblk_bf", "k_rblk_dma", "k_rblk_w8")}
     if dom == "ham":
-        kern = {"fp64": "k_ham_mfma64", "fp32": "k_hblk_f32", "bf16x3": "k_hblk_bf"}[args.precision]
+        kern = {"fp64": "k_ham_m4" if int(np.max(u.Total_NumOrbs)) <= 20 else "k_ham_mfma64", "fp32": "k_hblk_f32",
+                "bf16x3": "k_hblk_bf"}[args.precision]
     else:
         op = {"fp64": "rho64", "fp32": "rho32", "bf16x3": "rhobf"}[args.precision]
         kern = rho_kernels[args.precision][max(0, ctx.tuned(op)[0])]

@@ -128,6 +128,101 @@ __device__ __forceinline__ void ham_chunk64(const double *__restrict__ tabA, con
     }
 }
 
+// Hamiltonian unit kernel on the 4x4x4 FP64 matrix-core form
+// (v_mfma_f64_4x4x4f64: four independent 4x4x4 products per instruction;
+// on gfx950 it issues 512 FLOP per ~18 cycles against 2048 per ~105 for
+// the 16x16x4 form, 1.45x the rate).  One wave per unit; the NO_A x NO_B
+// block as NTA x NTB tiles of 4 x 4 (19 orbitals: 5 x 5 tiles, 90 % useful
+// against 35 % for 32 x 32), each tile accumulated in one register per lane.
+// Block b of an instruction takes 4 of every 16 overlap points: lane
+// l = 16 k + 4 b + r supplies A[r][k] = phi_A[4 ti + r](point 4 b + k) and
+// B[k][r] = V phi_B[4 tj + r](point 4 b + k), and D[m][n] of block b lands
+// in lane 16 m + 4 b + n; the four blocks' partial tiles are added with
+// lane swaps at the end of the unit (fixed order).  Splitting a unit's tile
+// rows over two waves (eight waves per SIMD instead of four, each loading
+// the V phi_B operands again) ran at 10.5 ms against 7.6 ms on GaAs216.
+template <int NTA, int NTB>
+__device__ __forceinline__ void ham_unit_m4(const double *__restrict__ tabA, const double *__restrict__ tabB,
+                                            const int32_t *__restrict__ L1, const int32_t *__restrict__ L2,
+                                            const int32_t *__restrict__ glA, const double *__restrict__ V, int n,
+                                            int noa, int nob, int lane, double *__restrict__ out) {
+    const int r = lane & 3, pb = ((lane >> 2) & 3) * 4 + (lane >> 4);  // row/column in a tile, point in a group of 16
+    double acc[NTA][NTB];
+#pragma unroll
+    for (int ti = 0; ti < NTA; ti++)
+#pragma unroll
+        for (int tj = 0; tj < NTB; tj++) acc[ti][tj] = 0.0;
+    for (int g0 = 0; g0 < n; g0 += 64) {
+        // indices and potential of 64 points, one per lane
+        const int gi = g0 + lane;
+        const bool gok = gi < n;
+        const int t1g = gok ? L1[gi] : 0;
+        const int t2g = gok ? L2[gi] : 0;
+        const double vg = gok ? V[glA[t1g]] : 0.0;
+        const int ngrp = min(4, (n - g0 + 15) >> 4);
+        for (int st = 0; st < ngrp; st++) {
+            const int src = 16 * st + pb;
+            const int t1 = __shfl(t1g, src, 64);
+            const int t2 = __shfl(t2g, src, 64);
+            const double v = __shfl(vg, src, 64);
+            const bool ok = g0 + src < n;
+            const double *ca = tabA + (int64_t)t1 * noa, *cb = tabB + (int64_t)t2 * nob;
+            double a[NTA], b[NTB];
+#pragma unroll
+            for (int ti = 0; ti < NTA; ti++) {
+                const int i = 4 * ti + r;
+                a[ti] = ok && i < noa ? ca[i] : 0.0;
+            }
+#pragma unroll
+            for (int tj = 0; tj < NTB; tj++) {
+                const int j = 4 * tj + r;
+                b[tj] = ok && j < nob ? cb[j] * v : 0.0;
+            }
+#pragma unroll
+            for (int ti = 0; ti < NTA; ti++)
+#pragma unroll
+                for (int tj = 0; tj < NTB; tj++)
+                    acc[ti][tj] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[ti], b[tj], acc[ti][tj], 0, 0, 0);
+        }
+    }
+    // lane 16 m + 4 b + n holds block b's D[m][n]: add the blocks (b = lane bits 2-3)
+    const int m = lane >> 4, nn = lane & 3;
+#pragma unroll
+    for (int ti = 0; ti < NTA; ti++)
+#pragma unroll
+        for (int tj = 0; tj < NTB; tj++) {
+            double x = acc[ti][tj];
+            x += __shfl_xor(x, 4, 64);
+            x += __shfl_xor(x, 8, 64);
+            const int row = 4 * ti + m, col = 4 * tj + nn;
+            if (((lane >> 2) & 3) == 0 && row < noa && col < nob) out[row * nob + col] = x;
+        }
+}
+
+// the 4x4x4 form for pairs of up to 20 orbitals per atom (4 or 5 tiles a side)
+__global__ __launch_bounds__(64) void k_ham_m4(const HUnit *__restrict__ units, const int32_t *__restrict__ order,
+                                               int32_t nunits, const PairDev *__restrict__ pairs,
+                                               const double *__restrict__ tab, const int32_t *__restrict__ olg1,
+                                               const int32_t *__restrict__ olg2, const int32_t *__restrict__ glist,
+                                               const double *__restrict__ V, double *__restrict__ part) {
+    const int b = blockIdx.x;
+    if (b >= nunits) return;
+    const HUnit u = units[order[b]];
+    const PairDev P = pairs[__builtin_amdgcn_readfirstlane(u.pair)];
+    const int noa = __builtin_amdgcn_readfirstlane(P.noa), nob = __builtin_amdgcn_readfirstlane(P.nob);
+    const int lane = threadIdx.x;
+    const double *tabA = tab + P.tab_a, *tabB = tab + P.tab_b;
+    const int32_t *L1 = olg1 + P.olg + u.lo, *L2 = olg2 + P.olg + u.lo;
+    const int32_t *glA = glist + P.gl_a;
+    double *out = part + u.part;
+    const int ta = (noa + 3) >> 2, tb = (nob + 3) >> 2;
+    if (ta <= 4 && tb <= 4) ham_unit_m4<4, 4>(tabA, tabB, L1, L2, glA, V, u.n, noa, nob, lane, out);
+    else if (ta <= 4) ham_unit_m4<4, 5>(tabA, tabB, L1, L2, glA, V, u.n, noa, nob, lane, out);
+    else if (tb <= 4) ham_unit_m4<5, 4>(tabA, tabB, L1, L2, glA, V, u.n, noa, nob, lane, out);
+    else ham_unit_m4<5, 5>(tabA, tabB, L1, L2, glA, V, u.n, noa, nob, lane, out);
+}
+constexpr int M4_NO = 20;  // orbitals per atom the 4x4x4 form holds
+
 // CHUNKED: any orbital count (a loop over the 32 x 32 chunks); else every
 // atom holds at most 32 orbitals and the block is one chunk -- a kernel of
 // its own, since the loop's registers (42 VGPRs against 24) cost it two
@@ -1138,7 +1233,11 @@ int ham_dev(lcpao_ctx *ctx, int nch, const double *const *d_V, double *const *d_
         if (!d_V[c] || !d_H[c]) return set_err(ctx, LCPAO_EINVAL, "null channel pointer");
         if (nu > 0) {
             KTimer kt(ctx, KID_HAM);
-            if (ctx->maxno <= 32)
+            if (ctx->maxno <= M4_NO)
+                hipLaunchKernelGGL(k_ham_m4, dim3(nu), dim3(64), 0, ctx->stream, ctx->d_hunits, ctx->d_horder, nu,
+                                   ctx->d_pairs, ctx->d_tab, ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c],
+                                   ctx->d_hpart);
+            else if (ctx->maxno <= 32)
                 hipLaunchKernelGGL(k_ham_mfma64<false>, dim3(nu), dim3(64), 0, ctx->stream, ctx->d_hunits, ctx->d_horder,
                                    nu, ctx->d_pairs, ctx->d_tab, ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c],
                                    ctx->d_hpart);
