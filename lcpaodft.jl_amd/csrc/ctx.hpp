@@ -117,6 +117,7 @@ struct lcpao_ctx {
     int32_t *d_hpair_ub = nullptr;  // canonical pair list + unit ranges
     int32_t *d_hcanon = nullptr;
     int64_t ncanon = 0, hpart_size = 0;
+    int32_t horder_n = 0, forder_n = 0;  // entries of the XCD-dealt H / Force3 launch orders (-1: no unit)
     int32_t hred_threads = 64;   // FP64 H reduce width: the largest computed pair block, rounded to 64 (<= 1024)
     double *d_hpart = nullptr;
     // rho: the work-unit decompositions (RHO_DECOMP_CHUNK)

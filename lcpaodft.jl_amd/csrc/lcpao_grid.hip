@@ -5,12 +5,13 @@
 // overlap lists; the block forms are in split.hip):
 //  * H  : pair-major.  Each "canonical" pair P (the one of P=(A,B,Rn) and its
 //         transpose P'=(B,A,-Rn) with the lower index, when both are local)
-//         is cut into units of <= 2048 overlap points.  One wave per unit
+//         is cut into units of <= 512 overlap points.  One wave per unit
 //         gathers phi_A and V*phi_B straight from the tables as the operands
-//         of FP64 MFMAs (2x2 tiles of 16x16, one 32x32 chunk per unit up to
-//         32 orbitals per atom); units write partial blocks that a second
-//         kernel sums per pair in a fixed order (deterministic), scales by
-//         GridVol and writes H[P] and H[P']^T.
+//         of FP64 MFMAs (4x4 tiles of the 4x4x4 form up to 20 orbitals per
+//         atom; 2x2 tiles of 16x16, one 32x32 chunk per unit, up to 32;
+//         32x32 chunks in a loop beyond); units write partial blocks that a
+//         second kernel sums per pair in a fixed order (deterministic),
+//         scales by GridVol and writes H[P] and H[P']^T.
 //  * rho: atom-owned.  Work unit = (home atom A, chunk of grid-list entries
 //         of A).  Each wave walks the (pair, sub-range) segments of A's
 //         canonical pairs inside the chunk; a lane owns one overlap point,
@@ -18,11 +19,13 @@
 //         read through the scalar path.  Per-wave LDS accumulators are
 //         summed in a fixed order; a gather kernel then sums the entries of
 //         every grid point (CSR by grid index) -- no atomics anywhere.
-//  * Force3: pair-major units over all local pairs, lane = overlap point,
-//         per-unit partial sums reduced per atom in a fixed order.
+//  * Force3: pair-major units of <= 512 overlap points over all local
+//         pairs, lane = overlap point, per-unit partial sums reduced per
+//         atom in a fixed order.
 // Units are launched in order of the grid index of their first point, so
 // the units in flight touch a few grid planes and the tables they gather
-// stay in the Infinity Cache.
+// stay in the Infinity Cache (the Force3 units in contiguous ranges per
+// XCD, so that each XCD's L2 serves its own band of planes).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -206,7 +209,7 @@ __global__ __launch_bounds__(64) void k_ham_m4(const HUnit *__restrict__ units, 
                                                const int32_t *__restrict__ olg2, const int32_t *__restrict__ glist,
                                                const double *__restrict__ V, double *__restrict__ part) {
     const int b = blockIdx.x;
-    if (b >= nunits) return;
+    if (b >= nunits || order[b] < 0) return;
     const HUnit u = units[order[b]];
     const PairDev P = pairs[__builtin_amdgcn_readfirstlane(u.pair)];
     const int noa = __builtin_amdgcn_readfirstlane(P.noa), nob = __builtin_amdgcn_readfirstlane(P.nob);
@@ -235,7 +238,7 @@ __global__ __launch_bounds__(64) void k_ham_mfma64(const HUnit *__restrict__ uni
                                                    const int32_t *__restrict__ olg2, const int32_t *__restrict__ glist,
                                                    const double *__restrict__ V, double *__restrict__ part) {
     const int b = blockIdx.x;
-    if (b >= nunits) return;
+    if (b >= nunits || order[b] < 0) return;
     const HUnit u = units[order[b]];
     const PairDev P = pairs[__builtin_amdgcn_readfirstlane(u.pair)];
     const int noa = __builtin_amdgcn_readfirstlane(P.noa), nob = __builtin_amdgcn_readfirstlane(P.nob);
@@ -601,6 +604,7 @@ __global__ __launch_bounds__(64) void k_f3_m64(const F3Unit *__restrict__ units,
     const int b = blockIdx.x;
     if (b >= nunits) return;
     const int ui = order[b];
+    if (ui < 0) return;
     const F3Unit u = units[ui];
     const PairDev P = pairs[u.pair];
     const int lane = threadIdx.x;
@@ -708,6 +712,7 @@ __global__ __launch_bounds__(64) void k_f3(const F3Unit *__restrict__ units, con
     const int b = blockIdx.x;
     if (b >= nunits) return;
     const int ui = order[b];
+    if (ui < 0) return;
     const F3Unit u = units[ui];
     const PairDev P = pairs[__builtin_amdgcn_readfirstlane(u.pair)];
     const int noa = __builtin_amdgcn_readfirstlane(P.noa), nob = __builtin_amdgcn_readfirstlane(P.nob);
@@ -1072,6 +1077,21 @@ std::vector<int32_t> launch_order(size_t n, GetKey key) {
     return ord;
 }
 
+constexpr int NXCD = 8;  // XCDs of an MI355X (workgroup w runs on XCD w % 8)
+
+// Deals units in launch order to the eight XCDs in contiguous ranges:
+// workgroup w runs on XCD w % 8, so entry 8 j + k is unit j of range k and
+// each XCD walks its own band of grid planes, whose table columns the pairs
+// crossing it gather again from that XCD's L2 (-1: no unit, for ranges of
+// unequal length).
+std::vector<int32_t> xcd_deal(const std::vector<int32_t> &sorted) {
+    const size_t n = sorted.size(), per = (n + NXCD - 1) / NXCD;
+    std::vector<int32_t> out(per * NXCD, -1);
+    for (size_t k = 0; k < (size_t)NXCD; k++)
+        for (size_t j = 0; j < per && k * per + j < n; j++) out[j * NXCD + k] = sorted[k * per + j];
+    return out;
+}
+
 int build_work(lcpao_ctx *ctx, const lcpao_geom *g, const std::vector<int32_t> &o1, const std::vector<int32_t> &o2,
                const std::vector<int32_t> &glist_h) {
     const int64_t np = ctx->npair;
@@ -1119,7 +1139,10 @@ int build_work(lcpao_ctx *ctx, const lcpao_geom *g, const std::vector<int32_t> &
     }
     ctx->hred_threads = std::min(1024, (max_ne + 63) / 64 * 64);
     ctx->hpart_size = part;
+    // (dealt round robin: the H units in contiguous XCD ranges ran slower,
+    // GaAs216 8.14 against 7.64 ms)
     auto horder = launch_order(ctx->hunits.size(), [&](size_t i) { return first_g(ctx->hunits[i].pair, ctx->hunits[i].lo); });
+    ctx->horder_n = (int32_t)horder.size();
     // Force3 units over all pairs; pairs are grouped by home atom
     ctx->funits.clear();
     std::vector<int32_t> aub(ctx->natom + 1, 0);
@@ -1132,7 +1155,9 @@ int build_work(lcpao_ctx *ctx, const lcpao_geom *g, const std::vector<int32_t> &
         }
     }
     for (int a = 0; a < ctx->natom; a++) aub[a + 1] += aub[a];
-    auto forder = launch_order(ctx->funits.size(), [&](size_t i) { return first_g(ctx->funits[i].pair, ctx->funits[i].lo); });
+    // (in contiguous XCD ranges: Si511 FP64 Force3 41.7 against 44.6 ms)
+    auto forder = xcd_deal(launch_order(ctx->funits.size(), [&](size_t i) { return first_g(ctx->funits[i].pair, ctx->funits[i].lo); }));
+    ctx->forder_n = (int32_t)forder.size();
     // density units of each decomposition: (atom, chunk) holding the
     // segments of its computed pairs
     std::vector<std::vector<int32_t>> apairs(ctx->natom);
@@ -1234,16 +1259,17 @@ int ham_dev(lcpao_ctx *ctx, int nch, const double *const *d_V, double *const *d_
         if (nu > 0) {
             KTimer kt(ctx, KID_HAM);
             if (ctx->maxno <= M4_NO)
-                hipLaunchKernelGGL(k_ham_m4, dim3(nu), dim3(64), 0, ctx->stream, ctx->d_hunits, ctx->d_horder, nu,
+                hipLaunchKernelGGL(k_ham_m4, dim3(ctx->horder_n), dim3(64), 0, ctx->stream, ctx->d_hunits, ctx->d_horder,
+                                   ctx->horder_n,
                                    ctx->d_pairs, ctx->d_tab, ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c],
                                    ctx->d_hpart);
             else if (ctx->maxno <= 32)
-                hipLaunchKernelGGL(k_ham_mfma64<false>, dim3(nu), dim3(64), 0, ctx->stream, ctx->d_hunits, ctx->d_horder,
-                                   nu, ctx->d_pairs, ctx->d_tab, ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c],
+                hipLaunchKernelGGL(k_ham_mfma64<false>, dim3(ctx->horder_n), dim3(64), 0, ctx->stream, ctx->d_hunits,
+                                   ctx->d_horder, ctx->horder_n, ctx->d_pairs, ctx->d_tab, ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c],
                                    ctx->d_hpart);
             else
-                hipLaunchKernelGGL(k_ham_mfma64<true>, dim3(nu), dim3(64), 0, ctx->stream, ctx->d_hunits, ctx->d_horder,
-                                   nu, ctx->d_pairs, ctx->d_tab, ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c],
+                hipLaunchKernelGGL(k_ham_mfma64<true>, dim3(ctx->horder_n), dim3(64), 0, ctx->stream, ctx->d_hunits,
+                                   ctx->d_horder, ctx->horder_n, ctx->d_pairs, ctx->d_tab, ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c],
                                    ctx->d_hpart);
         }
         if (ctx->ncanon > 0) {
@@ -1507,11 +1533,13 @@ int f3_dev(lcpao_ctx *ctx, int nch, const double *const *d_V, const double *cons
         // FP64 candidates: 0 = matrix cores (k_f3_m64), 1 = vector FMA (k_f3<double>)
         auto f364 = [&](int v) {
             if (v == 0)
-                hipLaunchKernelGGL(k_f3_m64, dim3(nu), dim3(64), 0, ctx->stream, ctx->d_funits, ctx->d_forder, nu,
+                hipLaunchKernelGGL(k_f3_m64, dim3(ctx->forder_n), dim3(64), 0, ctx->stream, ctx->d_funits, ctx->d_forder,
+                                   ctx->forder_n,
                                    ctx->d_pairs, ctx->d_tab, ctx->d_dtab[0], ctx->d_dtab[1], ctx->d_dtab[2],
                                    ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c], D, ctx->d_fpart);
             else
-                hipLaunchKernelGGL(k_f3<double>, dim3(nu), dim3(64), 0, ctx->stream, ctx->d_funits, ctx->d_forder, nu,
+                hipLaunchKernelGGL(k_f3<double>, dim3(ctx->forder_n), dim3(64), 0, ctx->stream, ctx->d_funits, ctx->d_forder,
+                                   ctx->forder_n,
                                    ctx->d_pairs, ctx->d_tab, ctx->d_dtab[0], ctx->d_dtab[1], ctx->d_dtab[2],
                                    ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c], D, ctx->d_fpart);
         };
@@ -1520,7 +1548,8 @@ int f3_dev(lcpao_ctx *ctx, int nch, const double *const *d_V, const double *cons
         if (nu > 0) {
             KTimer kt(ctx, KID_F3);
             if (f32)
-                hipLaunchKernelGGL(k_f3<float>, dim3(nu), dim3(64), 0, ctx->stream, ctx->d_funits, ctx->d_forder, nu,
+                hipLaunchKernelGGL(k_f3<float>, dim3(ctx->forder_n), dim3(64), 0, ctx->stream, ctx->d_funits, ctx->d_forder,
+                                   ctx->forder_n,
                                    ctx->d_pairs, ctx->d_tab32, ctx->d_dtab32[0], ctx->d_dtab32[1], ctx->d_dtab32[2],
                                    ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c], ctx->d_deff32, ctx->d_fpart);
             else
@@ -1581,11 +1610,13 @@ int f3_dev_noalloc(lcpao_ctx *ctx, int nch, const double *const *d_V, const doub
         const double *D = d_DM[c] + dm_offset;
         if (nu > 0) {
             if (variant == 0)
-                hipLaunchKernelGGL(k_f3_m64, dim3(nu), dim3(64), 0, ctx->stream, ctx->d_funits, ctx->d_forder, nu,
+                hipLaunchKernelGGL(k_f3_m64, dim3(ctx->forder_n), dim3(64), 0, ctx->stream, ctx->d_funits, ctx->d_forder,
+                                   ctx->forder_n,
                                    ctx->d_pairs, ctx->d_tab, ctx->d_dtab[0], ctx->d_dtab[1], ctx->d_dtab[2],
                                    ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c], D, fpart);
             else
-                hipLaunchKernelGGL(k_f3<double>, dim3(nu), dim3(64), 0, ctx->stream, ctx->d_funits, ctx->d_forder, nu,
+                hipLaunchKernelGGL(k_f3<double>, dim3(ctx->forder_n), dim3(64), 0, ctx->stream, ctx->d_funits, ctx->d_forder,
+                                   ctx->forder_n,
                                    ctx->d_pairs, ctx->d_tab, ctx->d_dtab[0], ctx->d_dtab[1], ctx->d_dtab[2],
                                    ctx->d_olg1, ctx->d_olg2, ctx->d_glist, d_V[c], D, fpart);
             hipLaunchKernelGGL(k_f3_reduce, dim3((ctx->natom + 3) / 4), dim3(256), 0, ctx->stream, ctx->d_atom_ub,
