@@ -24,8 +24,8 @@
 //         atom in a fixed order.
 // Units are launched in order of the grid index of their first point, so
 // the units in flight touch a few grid planes and the tables they gather
-// stay in the Infinity Cache (the Force3 units in contiguous ranges per
-// XCD, so that each XCD's L2 serves its own band of planes).
+// stay in the Infinity Cache, dealt to the XCDs in contiguous ranges so
+// that each XCD's L2 serves its own band of planes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -96,12 +96,17 @@ __device__ __forceinline__ void ham_chunk64(const double *__restrict__ tabA, con
     const bool ra0 = r0 + r16 < noa, ra1 = r0 + 16 + r16 < noa, rb0 = c0 + r16 < nob, rb1 = c0 + 16 + r16 < nob;
     for (int g0 = 0; g0 < n; g0 += 64) {
         // indices and potential of 64 points, one per lane
+        // (unconditional loads from clamped indices, zeroed after, so that
+        // the gathers issue together instead of one branch each)
         const int gi = g0 + lane;
         const bool gok = gi < n;
-        const int t1g = gok ? L1[gi] : 0;
-        const int t2g = gok ? L2[gi] : 0;
-        const double vg = gok ? V[glA[t1g]] : 0.0;
+        const int gc = gok ? gi : 0;
+        const int t1g = L1[gc];
+        const int t2g = L2[gc];
+        const double vv = V[glA[t1g]];
+        const double vg = gok ? vv : 0.0;
         const int nstep = min(16, (n - g0 + 3) >> 2);
+        const int ia0 = ra0 ? r16 : 0, ia1 = ra1 ? 16 + r16 : 0, ib0 = rb0 ? r16 : 0, ib1 = rb1 ? 16 + r16 : 0;
 #pragma unroll 4
         for (int st = 0; st < nstep; st++) {
             const int src = 4 * st + kq;
@@ -109,8 +114,9 @@ __device__ __forceinline__ void ham_chunk64(const double *__restrict__ tabA, con
             const int t2 = __shfl(t2g, src, 64);
             const double v = __shfl(vg, src, 64);  // 0 for points past the end
             const double *ca = tabA + (int64_t)t1 * noa + r0, *cb = tabB + (int64_t)t2 * nob + c0;
-            const double a0 = ra0 ? ca[r16] : 0.0, a1 = ra1 ? ca[16 + r16] : 0.0;
-            const double b0 = rb0 ? cb[r16] * v : 0.0, b1 = rb1 ? cb[16 + r16] * v : 0.0;
+            const double xa0 = ca[ia0], xa1 = ca[ia1], xb0 = cb[ib0], xb1 = cb[ib1];
+            const double a0 = ra0 ? xa0 : 0.0, a1 = ra1 ? xa1 : 0.0;
+            const double b0 = rb0 ? xb0 * v : 0.0, b1 = rb1 ? xb1 * v : 0.0;
             c00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, c00, 0, 0, 0);
             c01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, c01, 0, 0, 0);
             c10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, c10, 0, 0, 0);
@@ -156,30 +162,35 @@ __device__ __forceinline__ void ham_unit_m4(const double *__restrict__ tabA, con
 #pragma unroll
         for (int tj = 0; tj < NTB; tj++) acc[ti][tj] = 0.0;
     for (int g0 = 0; g0 < n; g0 += 64) {
-        // indices and potential of 64 points, one per lane
+        // indices and potential of 64 points, one per lane; the loads are
+        // unconditional (a lane past the end reads point 0 and is zeroed
+        // after), so that they issue together instead of one branch each
         const int gi = g0 + lane;
         const bool gok = gi < n;
-        const int t1g = gok ? L1[gi] : 0;
-        const int t2g = gok ? L2[gi] : 0;
-        const double vg = gok ? V[glA[t1g]] : 0.0;
+        const int gc = gok ? gi : 0;
+        const int t1g = L1[gc];
+        const int t2g = L2[gc];
+        const double vv = V[glA[t1g]];
+        const double vg = gok ? vv : 0.0;
         const int ngrp = min(4, (n - g0 + 15) >> 4);
         for (int st = 0; st < ngrp; st++) {
             const int src = 16 * st + pb;
             const int t1 = __shfl(t1g, src, 64);
             const int t2 = __shfl(t2g, src, 64);
-            const double v = __shfl(vg, src, 64);
-            const bool ok = g0 + src < n;
+            const double v = __shfl(vg, src, 64);  // 0 past the end
             const double *ca = tabA + (int64_t)t1 * noa, *cb = tabB + (int64_t)t2 * nob;
             double a[NTA], b[NTB];
 #pragma unroll
             for (int ti = 0; ti < NTA; ti++) {
                 const int i = 4 * ti + r;
-                a[ti] = ok && i < noa ? ca[i] : 0.0;
+                const double x = ca[i < noa ? i : noa - 1];
+                a[ti] = i < noa ? x : 0.0;
             }
 #pragma unroll
             for (int tj = 0; tj < NTB; tj++) {
                 const int j = 4 * tj + r;
-                b[tj] = ok && j < nob ? cb[j] * v : 0.0;
+                const double x = cb[j < nob ? j : nob - 1];
+                b[tj] = j < nob ? x * v : 0.0;
             }
 #pragma unroll
             for (int ti = 0; ti < NTA; ti++)
@@ -1139,9 +1150,8 @@ int build_work(lcpao_ctx *ctx, const lcpao_geom *g, const std::vector<int32_t> &
     }
     ctx->hred_threads = std::min(1024, (max_ne + 63) / 64 * 64);
     ctx->hpart_size = part;
-    // (dealt round robin: the H units in contiguous XCD ranges ran slower,
-    // GaAs216 8.14 against 7.64 ms)
-    auto horder = launch_order(ctx->hunits.size(), [&](size_t i) { return first_g(ctx->hunits[i].pair, ctx->hunits[i].lo); });
+    // (in contiguous XCD ranges: GaAs216 FP64 H 4.69 against 5.81 ms)
+    auto horder = xcd_deal(launch_order(ctx->hunits.size(), [&](size_t i) { return first_g(ctx->hunits[i].pair, ctx->hunits[i].lo); }));
     ctx->horder_n = (int32_t)horder.size();
     // Force3 units over all pairs; pairs are grouped by home atom
     ctx->funits.clear();

@@ -2778,16 +2778,16 @@ __global__ __launch_bounds__(64 * R64_WAVES) void k_rblk64m(Rho64Args A, int sta
             const int p = tid >> 4, rl = tid & 15;
             const GLOBAL_AS double *tab = as_global(A.tab);
             for (int r0 = 0; r0 < rows; r0 += 16 * R64_UNROLL) {
+                // unconditional loads from clamped rows and columns, zeroed
+                // after, so that they issue together
                 double v[R64_UNROLL];
 #pragma unroll
                 for (int u = 0; u < R64_UNROLL; u++) {
                     const int r = r0 + 16 * u + rl;
-                    v[u] = 0.0;
-                    if (r < rows) {
-                        const int rs = rslot[r], x = rs >> 16, i = rs & 0xFFFF;
-                        const int t = tm[x * 64 + p];
-                        if (t >= 0) v[u] = tab[tbase[x] + (int64_t)t * sno[x] + i];
-                    }
+                    const int rs = rslot[r < rows ? r : rows - 1], x = rs >> 16, i = rs & 0xFFFF;
+                    const int t = tm[x * 64 + p];
+                    const double val = tab[tbase[x] + (int64_t)(t >= 0 ? t : 0) * sno[x] + i];
+                    v[u] = t >= 0 ? val : 0.0;
                 }
 #pragma unroll
                 for (int u = 0; u < R64_UNROLL; u++) {
@@ -2821,31 +2821,37 @@ __global__ __launch_bounds__(64 * R64_WAVES) void k_rblk64m(Rho64Args A, int sta
                 T = tile64(items, yinfo, sno, q + R64_WAVES);
                 tile64_w(T, stk, psp, Deff, kq, r16, a, sr);
             }
-            const int kt = (C.ny + 3) >> 2;
+            const int kt = __builtin_amdgcn_readfirstlane((C.ny + 3) >> 2);
             d4_64 acc[4];
 #pragma unroll
             for (int nt = 0; nt < 4; nt++) acc[nt] = d4_64{0.0, 0.0, 0.0, 0.0};
             const double *py = phi + (int64_t)__builtin_amdgcn_readfirstlane(roff[C.y]) * R64_LD + r16;
+            // B operands: unconditional LDS reads (a clamped row, zero past
+            // the slot's orbitals) so that they issue ahead of the products
 #pragma unroll
             for (int ks = 0; ks < 8; ks++) {
                 if (ks < kt) {
                     const int j = 4 * ks + kq;
                     const bool jv = j < C.ny;
+                    const double *pj = py + (jv ? j : C.ny - 1) * R64_LD;
+                    double b[4];
 #pragma unroll
                     for (int nt = 0; nt < 4; nt++) {
-                        const double b = jv ? py[j * R64_LD + 16 * nt] : 0.0;
-                        acc[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[ks], b, acc[nt], 0, 0, 0);
+                        const double bv = pj[16 * nt];
+                        b[nt] = jv ? bv : 0.0;
                     }
+#pragma unroll
+                    for (int nt = 0; nt < 4; nt++)
+                        acc[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[ks], b[nt], acc[nt], 0, 0, 0);
                 }
             }
-            // the lane's output rows 16 mt + kq + 4 v against phi_x at its points
+            // the lane's output rows 16 mt + kq + 4 v against phi_x at its
+            // points (rows past the stack are zero in Q: any staged row serves)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                if (cr[v] >= 0) {
-                    const double *px = phi + cr[v] * R64_LD + r16;
+                const double *px = phi + (cr[v] >= 0 ? cr[v] : 0) * R64_LD + r16;
 #pragma unroll
-                    for (int nt = 0; nt < 4; nt++) zs[nt] = fma(px[16 * nt], acc[nt][v], zs[nt]);
-                }
+                for (int nt = 0; nt < 4; nt++) zs[nt] = fma(px[16 * nt], acc[nt][v], zs[nt]);
             }
         }
         // the four lanes of a point: rows kq = 0..3 (the same sums in every lane)
