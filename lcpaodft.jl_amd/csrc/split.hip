@@ -2705,12 +2705,22 @@ __global__ __launch_bounds__(64 * R64_WAVES) void k_rblk64m(Rho64Args A, int sta
         tbase[lane] = as_global(A.tab_off)[as_global(A.sph_atom)[B.slot_off + lane]];
     }
     __syncthreads();
-    // 2. per slot y (one wave each, lane = x): the slots x <= y of its listed pairs
-    for (int y = wave; y < ns; y += R64_WAVES) {
+    // 2. per slot y (one wave each, lane = x): the slots x <= y of its listed
+    // pairs; the slot pairs are read once here (unconditionally, clamped x)
+    // and kept for the lists of step 4, which walks the same (wave, y) passes
+    static_assert(MAX_SLOT_PAIRS < 46 * 46, "a block holds at most 45 sphere images");
+    constexpr int YPASS = (45 + R64_WAVES - 1) / R64_WAVES;
+    SlotPair spk[YPASS];
+#pragma unroll
+    for (int jp = 0; jp < YPASS; jp++) {
+        const int y = wave + jp * R64_WAVES;
+        if (y >= ns) break;
         const uint64_t my = mask[y];
         const int x = lane;
-        bool act = false;
-        if (my != 0 && x <= y) act = (mask[x] & my) != 0 && sp[(int64_t)x * ns + y].hoff >= 0;
+        const int xc = x <= y ? x : y;
+        const SlotPair P = sp[(int64_t)xc * ns + y];
+        spk[jp] = P;
+        const bool act = my != 0 && x <= y && (mask[xc] & my) != 0 && P.hoff >= 0;
         const uint64_t bal = __ballot(act);
         int nr;
         wave_scan_excl(act ? sno[x] : 0, lane, nr);
@@ -2744,7 +2754,10 @@ __global__ __launch_bounds__(64 * R64_WAVES) void k_rblk64m(Rho64Args A, int sta
     const int rows = ctl[1];
     const bool mm = ctl[3] != 0;
     // 4. the lists (one slot y per wave and pass) and the staged rows
-    for (int y = wave; y < ns; y += R64_WAVES) {
+#pragma unroll
+    for (int jp = 0; jp < YPASS; jp++) {
+        const int y = wave + jp * R64_WAVES;
+        if (y >= ns) break;
         const uint64_t am = amask[y];
         if (!am) continue;
         const int nr = yinfo[4 * y], st0 = yinfo[4 * y + 1], it0 = yinfo[4 * y + 2], pp0 = yinfo[4 * y + 3];
@@ -2755,7 +2768,7 @@ __global__ __launch_bounds__(64 * R64_WAVES) void k_rblk64m(Rho64Args A, int sta
         const int q = pp0 + __popcll(am & ((1ull << lane) - 1));
         if (act) {
             pairs[q] = x << 16 | y;
-            psp[q] = SlotPair{sp[(int64_t)x * ns + y].hoff, sp[(int64_t)x * ns + y].meta};
+            psp[q] = spk[jp];
             if (mm) {
                 const int nx = sno[x], r0 = roff[x];
                 for (int i = 0; i < nx; i++) stk[st0 + cum + i] = q << 15 | i << 10 | (r0 + i);
