@@ -163,7 +163,7 @@ int lcpao_grid_force3_dev(lcpao_ctx *ctx, int32_t nch, const double *const *d_V,
 
 /* Allocation-free FP64 Force3: lcpao_grid_force3_dev with precision
  * LCPAO_FP64 (same arguments, same kernels, same result), except that its
- * per-unit partial sums and per-atom sums live in a fixed 64 MiB array of
+ * per-unit partial sums and per-atom sums live in a fixed 128 MiB array of
  * device memory that the library holds on each GPU for the whole process
  * instead of in context buffers, and that it allocates nothing: no
  * hipMalloc / hipFree, no event timing, no autotuning (a context whose FP64
@@ -171,9 +171,9 @@ int lcpao_grid_force3_dev(lcpao_ctx *ctx, int32_t nch, const double *const *d_V,
  * lcpao_grid_tune).  No counterpart in the reference: it is for callers that
  * cannot allocate device memory during the call (code under HIP graph
  * capture on the caller's side, tight serving loops, harnesses that forbid
- * allocation).  Limit: 3 doubles per Force3 unit (one unit per pair and 512
- * overlap points) plus 3 per atom must fit in 2^23 doubles, else
- * LCPAO_ERANGE (Si511 at 192^3 needs about 4.8 M).  Every call on one GPU
+ * allocation).  Limit: 3 doubles per Force3 unit (one unit per pair and 256
+ * overlap points) plus 3 per atom must fit in 2^24 doubles, else
+ * LCPAO_ERANGE (Si511 at 192^3 needs about 9.6 M).  Every call on one GPU
  * shares that array, so calls that overlap on one GPU (other threads, other
  * streams) race; calls in sequence are safe (the per-atom sums are cleared by
  * a kernel at the start of every call). */

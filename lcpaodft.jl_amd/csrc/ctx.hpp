@@ -14,8 +14,12 @@
 // flight (one wave each, launched in grid order) then touch few enough grid
 // planes that the tables they gather stay in the Infinity Cache (GaAs216
 // FP64 H 8.2 ms against 9.2 ms at 2048 points; Si511 FP64 Force3 44.6 ms
-// against 54.8 ms)
-#define UNIT_PTS 512
+// against 54.8 ms).  H units write a partial block each (361 doubles for 19
+// orbitals), so below 512 points their reduction costs more than the
+// smaller band saves (256: 4.58 + 0.43 ms against 4.64 + 0.22 ms); Force3
+// units write three sums and run at 256 (Si511 38.4 against 41.7 ms).
+#define UNIT_PTS 256
+#define HUNIT_PTS 512
 #define RHO_CHUNK 2048    // grid-list entries per density unit (the largest decomposition)
 #define HK 32             // points per LDS chunk in the H kernel
 // orbital-count pairs (NO_A, NO_B) with kernels of fixed size (the Si/Cl,

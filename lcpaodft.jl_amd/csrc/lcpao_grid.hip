@@ -19,7 +19,7 @@
 //         read through the scalar path.  Per-wave LDS accumulators are
 //         summed in a fixed order; a gather kernel then sums the entries of
 //         every grid point (CSR by grid index) -- no atomics anywhere.
-//  * Force3: pair-major units of <= 512 overlap points over all local
+//  * Force3: pair-major units of <= 256 overlap points over all local
 //         pairs, lane = overlap point, per-unit partial sums reduced per
 //         atom in a fixed order.
 // Units are launched in order of the grid index of their first point, so
@@ -50,7 +50,7 @@ std::string g_create_err;
 // Force3 unit) followed by the per-atom sums (3 doubles per atom).  Every GPU
 // that loads the library holds one copy of this array until the process
 // ends; calls on one GPU share it.
-#define NOALLOC_SCRATCH_DOUBLES (8 * 1024 * 1024)  // 64 MiB
+#define NOALLOC_SCRATCH_DOUBLES (16 * 1024 * 1024)  // 128 MiB
 static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
 
 namespace {
@@ -1141,8 +1141,8 @@ int build_work(lcpao_ctx *ctx, const lcpao_geom *g, const std::vector<int32_t> &
     for (int p : canon) {
         const PairDev &P = ctx->pairs[p];
         const int n = g->numolg[p];
-        for (int lo = 0; lo < n; lo += UNIT_PTS) {
-            ctx->hunits.push_back({p, lo, std::min(UNIT_PTS, n - lo), 0, part});
+        for (int lo = 0; lo < n; lo += HUNIT_PTS) {
+            ctx->hunits.push_back({p, lo, std::min(HUNIT_PTS, n - lo), 0, part});
             part += (int64_t)P.noa * P.nob;
         }
         max_ne = std::max(max_ne, P.noa * P.nob);
