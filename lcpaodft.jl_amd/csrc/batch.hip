@@ -42,6 +42,8 @@ struct lcpao_batch {
     lcpao::BatchRed *d_red = nullptr;
     lcpao::BatchFill *d_fill = nullptr;
     int max_canon = 0, red_threads = 64, max_rows = 0;
+    int32_t *d_canon_off = nullptr;  // prefix sums of the structures' computed pairs (flat fold / reduce grids)
+    int total_canon = 0;
     int64_t max_nn = 0;
     // LCPAO_BF16X3 (default) or LCPAO_FP32
     int32_t precision = LCPAO_BF16X3;
@@ -116,7 +118,7 @@ int bind(lcpao_batch *b, const double *const *V, const double *const *DM, double
 // density items.
 int enqueue(lcpao_batch *b, int sched) {
     const int n = (int)b->ctx.size();
-    lcpao::launch_deff_f32_batch(b->stream, n, b->d_fold, b->max_canon);
+    lcpao::launch_deff_f32_batch(b->stream, n, b->d_fold, b->d_canon_off, b->total_canon);
     lcpao::launch_fill_batch(b->stream, n, b->d_fill, b->max_nn);
     const BlockArgs none{};
     const bool f32 = b->precision == LCPAO_FP32;
@@ -132,7 +134,7 @@ int enqueue(lcpao_batch *b, int sched) {
                                                     b->d_args_h, none, queue ? b->d_queue + 2 * k : nullptr,
                                                     queue ? 0u : (unsigned)cl.count));
     }
-    lcpao::launch_hblk_reduce_batch(b->stream, n, b->d_red, b->max_canon, b->red_threads);
+    lcpao::launch_hblk_reduce_batch(b->stream, n, b->d_red, b->d_canon_off, b->total_canon, b->red_threads);
     if (!f32 && sched == 2) {
         BATCH_TRY(b, lcpao::launch_rblk_w8(b->stream, std::min(b->max_rows, 384), b->d_items8, b->nitems8, b->d_args_r,
                                            none));
@@ -233,8 +235,19 @@ int lcpao_batch_create(lcpao_batch **out, int32_t n, lcpao_ctx *const *ctxs) {
         hipMalloc(&b->d_queue, std::max<size_t>(2, 2 * b->classes.size()) * sizeof(int32_t)) != hipSuccess ||
         hipMalloc(&b->d_fold, n * sizeof(lcpao::BatchFold)) != hipSuccess ||
         hipMalloc(&b->d_red, n * sizeof(lcpao::BatchRed)) != hipSuccess ||
-        hipMalloc(&b->d_fill, n * sizeof(lcpao::BatchFill)) != hipSuccess)
+        hipMalloc(&b->d_fill, n * sizeof(lcpao::BatchFill)) != hipSuccess ||
+        hipMalloc(&b->d_canon_off, (n + 1) * sizeof(int32_t)) != hipSuccess)
         return fail(LCPAO_EDEVICE);
+    {
+        std::vector<int32_t> off(n + 1, 0);
+        for (int s = 0; s < n; s++) {
+            if ((int64_t)off[s] + b->ctx[s]->ncanon > INT32_MAX) return fail(LCPAO_ERANGE);
+            off[s + 1] = off[s] + (int32_t)b->ctx[s]->ncanon;
+        }
+        b->total_canon = off[n];
+        if (hipMemcpy(b->d_canon_off, off.data(), (n + 1) * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(LCPAO_EDEVICE);
+    }
     b->max_rows = max_rows;
     b->nitems8 = (int32_t)items8.size();
     b->nitems_big = (int32_t)items_big.size();
@@ -357,7 +370,8 @@ void lcpao_batch_destroy(lcpao_batch *b) {
     if (!b) return;
     (void)hipSetDevice(b->device);
     drop_graph(b);
-    void *ptrs[] = {b->d_items, b->d_items8, b->d_items_big, b->d_args_h, b->d_args_r, b->d_queue, b->d_fold, b->d_red, b->d_fill};
+    void *ptrs[] = {b->d_items, b->d_items8, b->d_items_big, b->d_args_h, b->d_args_r, b->d_queue, b->d_fold, b->d_red, b->d_fill,
+                    b->d_canon_off};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);

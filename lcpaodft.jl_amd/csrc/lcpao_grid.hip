@@ -341,10 +341,12 @@ __global__ void k_deff(const int32_t *__restrict__ canon, int64_t ncanon, const 
 }
 
 // the FP32 folds (on-site blocks symmetrised) of every structure of a batch
-// in one launch: blockIdx.y is the structure, blockIdx.x its computed pair
-__global__ void k_deff_batch(const lcpao::BatchFold *__restrict__ fold) {
-    const lcpao::BatchFold F = fold[blockIdx.y];
-    const int64_t c = blockIdx.x;
+// in one launch over the computed pairs of all structures (a flat grid)
+__global__ void k_deff_batch(const lcpao::BatchFold *__restrict__ fold, const int32_t *__restrict__ off, int nstruct) {
+    const int w = blockIdx.x;
+    const int st = lcpao::batch_struct_of(off, nstruct, w);
+    const lcpao::BatchFold F = fold[st];
+    const int64_t c = w - off[st];
     if (c >= F.ncanon) return;
     deff_pair<float>(c, F.canon, F.pairs, F.D, F.Deff, 1);
 }
@@ -1301,9 +1303,10 @@ void lcpao::launch_deff_f64(lcpao_ctx *ctx, const double *D, int sym_self) {
     hipLaunchKernelGGL(k_deff<double>, dim3((unsigned)ctx->ncanon), dim3(128), 0, ctx->stream, ctx->d_hcanon, ctx->ncanon,
                        ctx->d_pairs, D, ctx->d_deff, sym_self);
 }
-void lcpao::launch_deff_f32_batch(hipStream_t stream, int nstruct, const lcpao::BatchFold *d_fold, int max_canon) {
-    if (nstruct > 0 && max_canon > 0)
-        hipLaunchKernelGGL(k_deff_batch, dim3((unsigned)max_canon, (unsigned)nstruct), dim3(128), 0, stream, d_fold);
+void lcpao::launch_deff_f32_batch(hipStream_t stream, int nstruct, const lcpao::BatchFold *d_fold,
+                                  const int32_t *d_off, int total_canon) {
+    if (nstruct > 0 && total_canon > 0)
+        hipLaunchKernelGGL(k_deff_batch, dim3((unsigned)total_canon), dim3(128), 0, stream, d_fold, d_off, nstruct);
 }
 void lcpao::launch_deff_f32(lcpao_ctx *ctx, const double *D, int sym_self) {
     hipLaunchKernelGGL(k_deff<float>, dim3((unsigned)ctx->ncanon), dim3(128), 0, ctx->stream, ctx->d_hcanon, ctx->ncanon,

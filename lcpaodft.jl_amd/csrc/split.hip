@@ -515,11 +515,15 @@ __global__ __launch_bounds__(1024) void k_hblk_reduce(const int32_t *__restrict_
     hblk_reduce_pair(ci, canon, coff, contrib, pairs, part, gridvol, H);
 }
 
-// the reductions of every structure of a batch in one launch: blockIdx.y is
-// the structure, blockIdx.x its computed pair (beyond its count: no work)
-__global__ __launch_bounds__(1024) void k_hblk_reduce_batch(const BatchRed *__restrict__ red) {
-    const BatchRed R = red[blockIdx.y];
-    const int64_t ci = blockIdx.x;
+// the reductions of every structure of a batch in one launch over the
+// computed pairs of all structures (a flat grid: no idle workgroups for the
+// smaller structures)
+__global__ __launch_bounds__(1024) void k_hblk_reduce_batch(const BatchRed *__restrict__ red,
+                                                            const int32_t *__restrict__ off, int nstruct) {
+    const int w = blockIdx.x;
+    const int st = batch_struct_of(off, nstruct, w);
+    const BatchRed R = red[st];
+    const int64_t ci = w - off[st];
     if (ci >= R.ncanon) return;
     hblk_reduce_pair(ci, R.canon, R.coff, R.contrib, R.pairs, R.part, R.gridvol, R.H);
 }
@@ -1064,10 +1068,11 @@ void launch_fill_f64(hipStream_t stream, int64_t n, double *x, double v) {
 void launch_fill_i32(hipStream_t stream, int64_t n, int32_t *x, int32_t v) {
     if (n > 0) hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, n, x, v);
 }
-void launch_hblk_reduce_batch(hipStream_t stream, int nstruct, const BatchRed *d_red, int max_canon, int threads) {
-    if (nstruct > 0 && max_canon > 0)
-        hipLaunchKernelGGL(k_hblk_reduce_batch, dim3((unsigned)max_canon, (unsigned)nstruct), dim3(threads), 0, stream,
-                           d_red);
+void launch_hblk_reduce_batch(hipStream_t stream, int nstruct, const BatchRed *d_red, const int32_t *d_off,
+                              int total_canon, int threads) {
+    if (nstruct > 0 && total_canon > 0)
+        hipLaunchKernelGGL(k_hblk_reduce_batch, dim3((unsigned)total_canon), dim3(threads), 0, stream, d_red, d_off,
+                           nstruct);
 }
 void launch_fill_batch(hipStream_t stream, int nstruct, const BatchFill *d_fill, int64_t max_n) {
     if (nstruct > 0 && max_n > 0)

@@ -203,8 +203,23 @@ struct BatchFold {
     float *Deff;            // its FP32 fold
     int32_t ncanon, pad;
 };
-void launch_deff_f32_batch(hipStream_t stream, int nstruct, const BatchFold *d_fold, int max_canon);
-void launch_hblk_reduce_batch(hipStream_t stream, int nstruct, const BatchRed *d_red, int max_canon, int threads);
+// the folds and reductions of a batch run over the computed pairs of all
+// structures in one flat grid: workgroup w is pair w - off[s] of the
+// structure s with off[s] <= w < off[s + 1] (off: nstruct + 1 prefix sums)
+void launch_deff_f32_batch(hipStream_t stream, int nstruct, const BatchFold *d_fold, const int32_t *d_off,
+                           int total_canon);
+void launch_hblk_reduce_batch(hipStream_t stream, int nstruct, const BatchRed *d_red, const int32_t *d_off,
+                              int total_canon, int threads);
+// the structure of flat-grid workgroup w (off: nstruct + 1 prefix sums)
+__device__ __forceinline__ int batch_struct_of(const int32_t *__restrict__ off, int nstruct, int w) {
+    int lo = 0, hi = nstruct;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (__builtin_amdgcn_readfirstlane(off[mid]) <= w) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
 void launch_fill_batch(hipStream_t stream, int nstruct, const BatchFill *d_fill, int64_t max_n);
 void plan_free(BlockPlan *p);
 int split_tables(lcpao_ctx *ctx, int precision);
