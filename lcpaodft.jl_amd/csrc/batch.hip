@@ -97,7 +97,7 @@ int bind(lcpao_batch *b, const double *const *V, const double *const *DM, double
         ah[s] = lcpao::block_args(c, V[s], nullptr);
         ar[s] = lcpao::block_args(c, nullptr, rho[s]);
         fold[s] = {c->d_hcanon, c->d_pairs, DM[s], c->d_deff32, (int32_t)c->ncanon, 0};
-        red[s] = {c->d_hcanon, c->plan->d_contrib_off, c->plan->d_contrib, c->d_pairs, c->plan->d_part, H[s],
+        red[s] = {c->plan->d_red_canon, c->plan->d_contrib_off, c->plan->d_contrib, c->d_pairs, c->plan->d_part, H[s],
                   c->gridvol, (int32_t)c->ncanon, 0};
         fill[s] = {rho[s], c->nn};
     }

@@ -453,14 +453,28 @@ __global__ __launch_bounds__(64 * HB_WAVES) void k_hblk_f32(const HItem *__restr
                     [&](const HItem &it, const BlockArgs &A) { hblk_f32_item(it, A, lds_raw); });
 }
 
-// per computed pair: FP64 sum of its sub-blocks of C_b over the blocks, in
-// block order (deterministic), x GridVol; writes H[P] and H[P']^T.  The
-// pair's contribution records are staged in LDS; each thread then gathers
-// its element from every block's super-tile partials (part_index), so the
-// lanes of a row read consecutive 16-byte lane slots of the same tiles.
+// per computed pair: FP64 sum of its sub-blocks of C_b over the blocks, x
+// GridVol; writes H[P] and H[P']^T.  One workgroup per pair.  A sub-block is
+// read as the 16-byte lane slots of the tile layout (part_index: four
+// consecutive rows of one column): thread t takes column b = t mod ncol of
+// the sub-block and row group g = t / ncol of the 4-row groups its rows span,
+// so the lanes of a wave read consecutive slots.  Value q of a slot is tile
+// row 4 g + q of the span, i.e. sub-block row 4 g + q - (rx mod 4); the
+// thread's four accumulators therefore keep one meaning while the class
+// (trans, rx mod 4) stays the same.  A pair's contributions are sorted by
+// class (then by block), and at each change of class the accumulators are
+// added into the pair's block in LDS (each element by one thread, a barrier
+// between classes), so the sum order is fixed.  An on-site pair (slot x =
+// slot y) reads the diagonal and the upper triangle, which the tiles hold,
+// and adds each value above the diagonal to its mirror element as well.
 constexpr int RED_BATCH = 256;
-// One workgroup per canonical pair with one thread per element of its block
-// (the launch's width covers the largest NO_A x NO_B, at most 32 x 32).
+constexpr int RED_UNROLL = 8;
+constexpr int RED_MAXNO = 32;  // the split-precision plan holds at most 32 orbitals per atom
+// workgroup width: both orientations' column x row-group grids of the pair
+__host__ __device__ inline int hblk_reduce_width(int noa, int nob) {
+    const int w = max(nob * ((noa + 6) / 4), noa * ((nob + 6) / 4));
+    return (w + 63) / 64 * 64;
+}
 __device__ __forceinline__ void hblk_reduce_pair(int64_t ci, const int32_t *__restrict__ canon,
                                                  const int32_t *__restrict__ coff,
                                                  const Contrib *__restrict__ contrib,
@@ -468,40 +482,92 @@ __device__ __forceinline__ void hblk_reduce_pair(int64_t ci, const int32_t *__re
                                                  double gridvol, double *__restrict__ H) {
     __shared__ int64_t s_base[RED_BATCH];
     __shared__ int32_t s_rxy[RED_BATCH], s_st[RED_BATCH];
+    __shared__ double s_h[RED_MAXNO * RED_MAXNO];
     const int p = canon[ci];
     const PairDev P = pairs[p];
-    const int ne = P.noa * P.nob;
+    const int noa = P.noa, nob = P.nob, ne = noa * nob;
     const int k0 = coff[ci], k1 = coff[ci + 1];
-    const int e = threadIdx.x;
-    const int i = e / P.nob, j = e - i * P.nob;
-    double acc = 0.0;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int e = tid; e < ne; e += nt) s_h[e] = 0.0;
+    // (column, row group) of this thread with the pair's rows (trans = 0) or
+    // columns (trans = 1) along the tile rows
+    const int bN = tid % nob, gN = tid / nob, bT = tid % noa, gT = tid / noa;
+    int cls = -1;  // class of the accumulators: trans * 4 + rx mod 4
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    bool onsite = false;
+    auto flush = [&]() {
+        const int tr = cls >> 2, sh = cls & 3, nrow = tr ? nob : noa;
+        const int b = tr ? bT : bN, g = tr ? gT : gN;
+        if (g < ((sh + nrow + 3) >> 2)) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int a = 4 * g + q - sh;
+                if (a < 0 || a >= nrow) continue;
+                if (!onsite) {
+                    s_h[tr ? b * nob + a : a * nob + b] += acc[q];
+                } else {  // C_b[rx + a][rx + b], a <= b, is element (a, b) and, below the diagonal, (b, a)
+                    if (a <= b) s_h[a * nob + b] += acc[q];
+                    if (a < b) s_h[b * nob + a] += acc[q];
+                }
+            }
+        }
+    };
     for (int kb = k0; kb < k1; kb += RED_BATCH) {
         const int nb = min(RED_BATCH, k1 - kb);
         __syncthreads();
-        for (int x = threadIdx.x; x < nb; x += blockDim.x) {  // the width may be below RED_BATCH
+        for (int x = tid; x < nb; x += nt) {  // the width may be below RED_BATCH
             const Contrib C = contrib[kb + x];
             s_base[x] = C.base;
             s_rxy[x] = (int32_t)C.rx | ((int32_t)C.ry << 16);
             s_st[x] = (int32_t)C.S | ((int32_t)C.trans << 16);
         }
         __syncthreads();
-        if (e < ne) {
-            double s = 0.0;
-#pragma unroll 8
-            for (int k = 0; k < nb; k++) {
-                const int rxy = s_rxy[k], sst = s_st[k];
-                const bool tr = (sst >> 16) != 0;
-                int r = (rxy & 0xffff) + (tr ? j : i), c = (rxy >> 16) + (tr ? i : j);
-                if (r > c) { const int t = r; r = c; c = t; }  // on-site block: the mirror element
-                s += (double)part[part_index(s_base[k], sst & 0xffff, r, c)];
+        // an on-site pair (atom A with itself, no lattice shift) is the one
+        // whose sub-blocks sit on the diagonal (slot x = slot y)
+        if (kb == k0) onsite = (s_st[0] >> 16) == 0 && (s_rxy[0] & 0xffff) == (s_rxy[0] >> 16);
+        for (int k = 0; k < nb; k += RED_UNROLL) {
+            f32x4 v[RED_UNROLL];
+#pragma unroll
+            for (int u = 0; u < RED_UNROLL; u++) {
+                const int kk = min(k + u, nb - 1);
+                const int rxy = s_rxy[kk], sst = s_st[kk];
+                const int rx = rxy & 0xffff, ry = rxy >> 16, tr = sst >> 16;
+                const int nrow = tr ? nob : noa;
+                // a thread past the span reads the span's last row group
+                const int g = min(tr ? gT : gN, ((rx & 3) + nrow - 1) >> 2);
+                const int c = ry + (tr ? bT : bN);
+                int r = (rx & ~3) + 4 * g;
+                // on-site: rows wholly below the diagonal are not needed (their
+                // mirror is), and their tile may not be stored; read the
+                // diagonal's row group instead
+                if (onsite && r > c) r = c & ~3;
+                const int64_t at = part_index(s_base[kk], sst & 0xffff, r, c);
+                v[u] = *reinterpret_cast<const f32x4 *>(part + at);
             }
-            acc += s;
+#pragma unroll
+            for (int u = 0; u < RED_UNROLL; u++) {
+                if (k + u >= nb) break;
+                const int c = ((s_st[k + u] >> 16) << 2) | (s_rxy[k + u] & 3);
+                if (c != cls) {  // the same for every thread of the workgroup
+                    if (cls >= 0) flush();
+                    __syncthreads();
+                    cls = c;
+                    acc[0] = acc[1] = acc[2] = acc[3] = 0.0;
+                }
+                acc[0] += (double)v[u][0];
+                acc[1] += (double)v[u][1];
+                acc[2] += (double)v[u][2];
+                acc[3] += (double)v[u][3];
+            }
         }
     }
-    if (e < ne) {
-        const double s = acc * gridvol;
+    if (cls >= 0) flush();
+    __syncthreads();
+    for (int e = tid; e < ne; e += nt) {
+        const int i = e / nob, j = e - i * nob;
+        const double s = s_h[e] * gridvol;
         H[P.hoff + e] = s;
-        if (P.partner >= 0 && P.partner != p) H[pairs[P.partner].hoff + (int64_t)j * P.noa + i] = s;
+        if (P.partner >= 0 && P.partner != p) H[pairs[P.partner].hoff + (int64_t)j * noa + i] = s;
     }
 }
 
@@ -586,7 +652,7 @@ void deal_xcd(const BlockPlan *P, std::vector<HItem> &items, int n1) {
 void plan_free(BlockPlan *p) {
     if (!p) return;
     void *ptrs[] = {p->d_blocks, p->d_items, p->d_items8, p->d_items_big, p->d_sph_block, p->d_sph_atom, p->d_sph_row0, p->d_sph_map,
-                    p->d_contrib, p->d_contrib_off, p->d_hi, p->d_lo, p->d_t32, p->d_part, p->d_thi, p->d_tlo,
+                    p->d_contrib, p->d_contrib_off, p->d_red_canon, p->d_hi, p->d_lo, p->d_t32, p->d_part, p->d_thi, p->d_tlo,
                     p->d_rowinfo, p->d_spair, p->d_contrib_pair, p->d_slotno, p->d_fpair,
                     p->d_fitems, p->d_fbase, p->d_fout, p->d_fatom_off, p->d_fatom_idx, p->d_dtile};
     for (void *x : ptrs)
@@ -846,21 +912,37 @@ int plan_build_inner(lcpao_ctx *ctx) {
     }
     std::vector<CRec> all;
     for (auto &v : recs) all.insert(all.end(), v.begin(), v.end());
-    std::stable_sort(all.begin(), all.end(), [](const CRec &x, const CRec &y) {
-        return x.cpos != y.cpos ? x.cpos < y.cpos : x.block < y.block;
+    // the reduction's launch order: pairs with the most contributions first
+    // (on-site pairs have several times the average), so that the longest
+    // workgroups do not start last; ties in canonical order
+    const int64_t ncan = (int64_t)ctx->canon.size();
+    std::vector<int32_t> ncontrib(ncan, 0), lorder(ncan), lpos(ncan), red_canon(ncan);
+    for (const CRec &r : all) ncontrib[r.cpos]++;
+    for (int64_t c = 0; c < ncan; c++) lorder[c] = (int32_t)c;
+    std::stable_sort(lorder.begin(), lorder.end(), [&](int32_t x, int32_t y) { return ncontrib[x] > ncontrib[y]; });
+    for (int64_t n = 0; n < ncan; n++) {
+        lpos[lorder[n]] = (int32_t)n;
+        red_canon[n] = ctx->canon[lorder[n]];
+    }
+    // within a pair: by the reduction's class (trans, first row mod 4), then by block
+    auto rclass = [&](const CRec &r) { return r.trans * 4 + (rows0[r.block][r.x] & 3); };
+    std::stable_sort(all.begin(), all.end(), [&](const CRec &x, const CRec &y) {
+        if (x.cpos != y.cpos) return lpos[x.cpos] < lpos[y.cpos];
+        const int cx = rclass(x), cy = rclass(y);
+        return cx != cy ? cx < cy : x.block < y.block;
     });
     std::vector<Contrib> contrib(all.size());
     std::vector<int32_t> cpair(all.size());
     std::vector<int32_t> coff(ctx->canon.size() + 1, 0);
-    // contribution records, in canonical-pair order and block order within a
-    // pair: where the pair's sub-block lies in the block's super-tile partials
+    // contribution records, in launch order: where the pair's sub-block lies
+    // in the block's super-tile partials
     for (size_t i = 0; i < all.size(); i++) {
         const int64_t b = all[i].block;
         const BlockDev &B = P->blocks[b];
         cpair[i] = ctx->canon[all[i].cpos];
         contrib[i] = Contrib{(int64_t)B.st_off * ST * ST, (int16_t)rows0[b][all[i].x], (int16_t)rows0[b][all[i].y],
                              (int16_t)(B.rows / ST), (int16_t)all[i].trans};
-        coff[all[i].cpos + 1]++;
+        coff[lpos[all[i].cpos] + 1]++;
         const PairDev &Q = ctx->pairs[cpair[i]];
         const int64_t sp = B.sp_off + (int64_t)all[i].x * B.nslot + all[i].y;
         spairs[sp] = SlotPair{(int32_t)Q.hoff, Q.nob | (all[i].trans << 6)};
@@ -868,7 +950,7 @@ int plan_build_inner(lcpao_ctx *ctx) {
     P->part_elems = P->part_tiles * ST * ST;
     P->ncontrib = (int64_t)all.size();
     for (int32_t p : ctx->canon)
-        P->red_threads = std::max(P->red_threads, (ctx->pairs[p].noa * ctx->pairs[p].nob + 63) / 64 * 64);
+        P->red_threads = std::max(P->red_threads, hblk_reduce_width(ctx->pairs[p].noa, ctx->pairs[p].nob));
     for (size_t c = 0; c < ctx->canon.size(); c++) coff[c + 1] += coff[c];
     // Force3 items: the S x S super-tiles of a block in units of two of one
     // row ((si, 2 u), (si, 2 u + 1); U = ceil(S / 2) units per row), FB_WAVES
@@ -900,6 +982,7 @@ int plan_build_inner(lcpao_ctx *ctx) {
         (rc = up(ctx, &P->d_items8, P->items8)) || (rc = up(ctx, &P->d_items_big, P->items_big)) ||
         (rc = up(ctx, &P->d_sph_block, sb)) || (rc = up(ctx, &P->d_sph_atom, sa)) || (rc = up(ctx, &P->d_sph_row0, sr)) ||
         (rc = up(ctx, &P->d_sph_map, smap)) || (rc = up(ctx, &P->d_contrib, contrib)) || (rc = up(ctx, &P->d_contrib_off, coff)) ||
+        (rc = up(ctx, &P->d_red_canon, red_canon)) ||
         (rc = up(ctx, &P->d_contrib_pair, cpair)) || (rc = up(ctx, &P->d_rowinfo, rowinfo)) ||
         (rc = up(ctx, &P->d_spair, spairs)) ||
         (rc = up(ctx, &P->d_slotno, slotno)) || (rc = up(ctx, &P->d_fpair, fpairs)) ||
@@ -1013,7 +1096,7 @@ int split_hamiltonian(lcpao_ctx *ctx, int nch, const double *const *d_V, double 
         }
         if (ctx->ncanon > 0) {
             KTimer kt(ctx, KID_HRED);
-            hipLaunchKernelGGL(k_hblk_reduce, dim3((unsigned)ctx->ncanon), dim3(P->red_threads), 0, ctx->stream, ctx->d_hcanon,
+            hipLaunchKernelGGL(k_hblk_reduce, dim3((unsigned)ctx->ncanon), dim3(P->red_threads), 0, ctx->stream, P->d_red_canon,
                                ctx->ncanon, P->d_contrib_off, P->d_contrib, ctx->d_pairs, P->d_part, ctx->gridvol,
                                d_H[c]);
         }
@@ -1082,7 +1165,7 @@ void launch_fill_batch(hipStream_t stream, int nstruct, const BatchFill *d_fill,
 void launch_hblk_reduce(const lcpao_ctx *ctx, hipStream_t stream, double *H) {
     const BlockPlan *P = ctx->plan;
     if (ctx->ncanon > 0)
-        hipLaunchKernelGGL(k_hblk_reduce, dim3((unsigned)ctx->ncanon), dim3(P->red_threads), 0, stream, ctx->d_hcanon, ctx->ncanon,
+        hipLaunchKernelGGL(k_hblk_reduce, dim3((unsigned)ctx->ncanon), dim3(P->red_threads), 0, stream, P->d_red_canon, ctx->ncanon,
                            P->d_contrib_off, P->d_contrib, ctx->d_pairs, P->d_part, ctx->gridvol, H);
 }
 

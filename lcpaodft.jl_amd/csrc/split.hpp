@@ -126,7 +126,8 @@ struct BlockPlan {
     int32_t *d_sph_block = nullptr, *d_sph_atom = nullptr, *d_sph_row0 = nullptr;
     int32_t *d_sph_map = nullptr;  // nsph x BLK_PTS table columns (-1: not covered)
     Contrib *d_contrib = nullptr;
-    int32_t *d_contrib_off = nullptr;  // per canonical pair (ncanon + 1)
+    int32_t *d_contrib_off = nullptr;  // per computed pair in the reduction's launch order (ncanon + 1)
+    int32_t *d_red_canon = nullptr;    // the computed pairs in the reduction's launch order
     // tiles of the precision forms
     uint16_t *d_hi = nullptr, *d_lo = nullptr;    // bf16 planes, row fragments
     uint16_t *d_thi = nullptr, *d_tlo = nullptr;  // bf16 planes, point fragments
@@ -134,7 +135,7 @@ struct BlockPlan {
     SlotPair *d_spair = nullptr;
     int32_t *d_contrib_pair = nullptr;
     int64_t ncontrib = 0;
-    int32_t red_threads = 64;      // reduce width: the largest NO_A x NO_B of a computed pair, rounded to 64
+    int32_t red_threads = 64;      // reduce width: the largest hblk_reduce_width of a computed pair
     float *d_t32 = nullptr;                     // fp32 tiles (permuted point order)
     bool bf16_ready = false, f32_ready = false;
     float *d_part = nullptr;       // H partials: part_tiles super-tiles of 4096 floats
@@ -183,8 +184,8 @@ hipError_t launch_rblk_w8(hipStream_t stream, int rows, const HItem *items, int3
                           const BlockArgs &one);
 // batched fills and reductions of a batch (one launch each for all structures)
 struct BatchRed {
-    const int32_t *canon;   // computed pairs of the structure
-    const int32_t *coff;    // contribution offsets per computed pair
+    const int32_t *canon;   // computed pairs of the structure, in the reduction's launch order
+    const int32_t *coff;    // contribution offsets per launch slot
     const Contrib *contrib;
     const PairDev *pairs;
     const float *part;
