@@ -43,6 +43,7 @@ struct lcpao_batch {
     lcpao::BatchFill *d_fill = nullptr;
     int max_canon = 0, red_threads = 64, max_rows = 0;
     int32_t *d_canon_off = nullptr;  // prefix sums of the structures' computed pairs (flat fold / reduce grids)
+    int32_t *d_red_perm = nullptr;   // the reduction's flat pairs, most contributions first
     int total_canon = 0;
     int64_t max_nn = 0;
     // LCPAO_BF16X3 (default) or LCPAO_FP32
@@ -134,7 +135,8 @@ int enqueue(lcpao_batch *b, int sched) {
                                                     b->d_args_h, none, queue ? b->d_queue + 2 * k : nullptr,
                                                     queue ? 0u : (unsigned)cl.count));
     }
-    lcpao::launch_hblk_reduce_batch(b->stream, n, b->d_red, b->d_canon_off, b->total_canon, b->red_threads);
+    lcpao::launch_hblk_reduce_batch(b->stream, n, b->d_red, b->d_canon_off, b->total_canon, b->red_threads,
+                                    b->d_red_perm);
     if (!f32 && sched == 2) {
         BATCH_TRY(b, lcpao::launch_rblk_w8(b->stream, std::min(b->max_rows, 384), b->d_items8, b->nitems8, b->d_args_r,
                                            none));
@@ -246,6 +248,19 @@ int lcpao_batch_create(lcpao_batch **out, int32_t n, lcpao_ctx *const *ctxs) {
         }
         b->total_canon = off[n];
         if (hipMemcpy(b->d_canon_off, off.data(), (n + 1) * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(LCPAO_EDEVICE);
+        // longest first over every structure's pairs (each structure's plan
+        // already lists its pairs by falling contribution count)
+        std::vector<int32_t> perm(b->total_canon), cnt(b->total_canon, 0);
+        for (int s = 0; s < n; s++) {
+            const auto &rc = b->ctx[s]->plan->red_count;
+            for (int32_t k = 0; k < off[s + 1] - off[s] && k < (int32_t)rc.size(); k++) cnt[off[s] + k] = rc[k];
+        }
+        for (int32_t w = 0; w < b->total_canon; w++) perm[w] = w;
+        std::stable_sort(perm.begin(), perm.end(), [&](int32_t x, int32_t y) { return cnt[x] > cnt[y]; });
+        if (hipMalloc(&b->d_red_perm, std::max<size_t>(1, perm.size()) * sizeof(int32_t)) != hipSuccess ||
+            (!perm.empty() &&
+             hipMemcpy(b->d_red_perm, perm.data(), perm.size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess))
             return fail(LCPAO_EDEVICE);
     }
     b->max_rows = max_rows;
@@ -371,7 +386,7 @@ void lcpao_batch_destroy(lcpao_batch *b) {
     (void)hipSetDevice(b->device);
     drop_graph(b);
     void *ptrs[] = {b->d_items, b->d_items8, b->d_items_big, b->d_args_h, b->d_args_r, b->d_queue, b->d_fold, b->d_red, b->d_fill,
-                    b->d_canon_off};
+                    b->d_canon_off, b->d_red_perm};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);

@@ -486,6 +486,7 @@ __device__ __forceinline__ void hblk_reduce_pair(int64_t ci, const int32_t *__re
     const int p = canon[ci];
     const PairDev P = pairs[p];
     const int noa = P.noa, nob = P.nob, ne = noa * nob;
+    if (ne == 0) return;  // an atom without orbitals: nothing to sum (the whole workgroup leaves)
     const int k0 = coff[ci], k1 = coff[ci + 1];
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int e = tid; e < ne; e += nt) s_h[e] = 0.0;
@@ -585,8 +586,9 @@ __global__ __launch_bounds__(1024) void k_hblk_reduce(const int32_t *__restrict_
 // computed pairs of all structures (a flat grid: no idle workgroups for the
 // smaller structures)
 __global__ __launch_bounds__(1024) void k_hblk_reduce_batch(const BatchRed *__restrict__ red,
-                                                            const int32_t *__restrict__ off, int nstruct) {
-    const int w = blockIdx.x;
+                                                            const int32_t *__restrict__ off, int nstruct,
+                                                            const int32_t *__restrict__ perm) {
+    const int w = perm[blockIdx.x];
     const int st = batch_struct_of(off, nstruct, w);
     const BatchRed R = red[st];
     const int64_t ci = w - off[st];
@@ -920,9 +922,11 @@ int plan_build_inner(lcpao_ctx *ctx) {
     for (const CRec &r : all) ncontrib[r.cpos]++;
     for (int64_t c = 0; c < ncan; c++) lorder[c] = (int32_t)c;
     std::stable_sort(lorder.begin(), lorder.end(), [&](int32_t x, int32_t y) { return ncontrib[x] > ncontrib[y]; });
+    P->red_count.resize(ncan);
     for (int64_t n = 0; n < ncan; n++) {
         lpos[lorder[n]] = (int32_t)n;
         red_canon[n] = ctx->canon[lorder[n]];
+        P->red_count[n] = ncontrib[lorder[n]];
     }
     // within a pair: by the reduction's class (trans, first row mod 4), then by block
     auto rclass = [&](const CRec &r) { return r.trans * 4 + (rows0[r.block][r.x] & 3); };
@@ -1152,10 +1156,10 @@ void launch_fill_i32(hipStream_t stream, int64_t n, int32_t *x, int32_t v) {
     if (n > 0) hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, n, x, v);
 }
 void launch_hblk_reduce_batch(hipStream_t stream, int nstruct, const BatchRed *d_red, const int32_t *d_off,
-                              int total_canon, int threads) {
+                              int total_canon, int threads, const int32_t *d_perm) {
     if (nstruct > 0 && total_canon > 0)
         hipLaunchKernelGGL(k_hblk_reduce_batch, dim3((unsigned)total_canon), dim3(threads), 0, stream, d_red, d_off,
-                           nstruct);
+                           nstruct, d_perm);
 }
 void launch_fill_batch(hipStream_t stream, int nstruct, const BatchFill *d_fill, int64_t max_n) {
     if (nstruct > 0 && max_n > 0)

@@ -136,6 +136,7 @@ struct BlockPlan {
     int32_t *d_contrib_pair = nullptr;
     int64_t ncontrib = 0;
     int32_t red_threads = 64;      // reduce width: the largest hblk_reduce_width of a computed pair
+    std::vector<int32_t> red_count;  // contributions per reduction launch slot (host; batches order by it)
     float *d_t32 = nullptr;                     // fp32 tiles (permuted point order)
     bool bf16_ready = false, f32_ready = false;
     float *d_part = nullptr;       // H partials: part_tiles super-tiles of 4096 floats
@@ -206,11 +207,13 @@ struct BatchFold {
 };
 // the folds and reductions of a batch run over the computed pairs of all
 // structures in one flat grid: workgroup w is pair w - off[s] of the
-// structure s with off[s] <= w < off[s + 1] (off: nstruct + 1 prefix sums)
+// structure s with off[s] <= w < off[s + 1] (off: nstruct + 1 prefix sums);
+// the reduction's workgroup i takes flat pair perm[i] (a permutation of the
+// flat pairs: those with the most contributions, over all structures, first)
 void launch_deff_f32_batch(hipStream_t stream, int nstruct, const BatchFold *d_fold, const int32_t *d_off,
                            int total_canon);
 void launch_hblk_reduce_batch(hipStream_t stream, int nstruct, const BatchRed *d_red, const int32_t *d_off,
-                              int total_canon, int threads);
+                              int total_canon, int threads, const int32_t *d_perm);
 // the structure of flat-grid workgroup w (off: nstruct + 1 prefix sums)
 __device__ __forceinline__ int batch_struct_of(const int32_t *__restrict__ off, int nstruct, int w) {
     int lo = 0, hi = nstruct;
